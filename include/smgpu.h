@@ -1,0 +1,150 @@
+/*
+ * smgpu — C ABI of the MI355X M-scheme shell-model eigensolver.
+ *
+ * The boundary replaces the body of NuclearToolkit.jl's `main_sm` path from the
+ * basis enumeration through the Lanczos solve (reference paths relative to its
+ * repository root).  Plain C types only; every call returns an int32 status
+ * (SMGPU_OK or a negative code) and never throws; smgpu_last_error() gives the
+ * text of the last failure on the calling thread.  Handles own their host and
+ * device memory; callers own every pointer they pass.  Indices cross 0-based.
+ * Vectors use the reference's global M-scheme order:
+ *   index = tdims[b] + ip * nN[b] + in
+ * (block b by ascending proton M; proton and neutron determinants of a block in
+ * descending bit-integer order, first m-state = most significant bit).
+ */
+#ifndef SMGPU_H
+#define SMGPU_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SMGPU_OK 0
+#define SMGPU_EINVAL (-1)   /* bad argument or malformed interaction file */
+#define SMGPU_EIO (-2)      /* file not readable */
+#define SMGPU_EDEVICE (-3)  /* HIP runtime / device failure */
+#define SMGPU_ENOMEM (-4)   /* does not fit (host or HBM) */
+#define SMGPU_ENOTCONV (-5) /* iteration limit reached */
+#define SMGPU_ECANCELED (-6)
+
+#define SMGPU_FP64 0        /* FP64 H·v and Krylov basis: the plain FP64 TRL */
+#define SMGPU_MIXED 1       /* FP32 H·v and basis, FP64 recurrence/reorth/eigensolves */
+
+typedef struct smgpu_problem_s* smgpu_problem;  /* interaction + basis + host tables */
+typedef struct smgpu_ham_s* smgpu_ham;          /* device-resident tables on one GPU */
+typedef struct smgpu_server_s* smgpu_server;    /* persistent serving loop */
+
+typedef struct {
+    int32_t Z, N, A, vp, vn, twoM;
+    int32_t lp, ln, cp, cn;          /* orbits and core of the interaction file */
+    int32_t nblocks;
+    int32_t n_msps_p, n_msps_n;      /* m-states per species */
+    int64_t mdim;
+    int64_t n_psd, n_nsd;            /* proton / neutron determinants over all blocks */
+    int64_t n_pp_ops, n_nn_ops, n_pn_ops;   /* two-body operator lists (prep_Hamil_T1 / _pn) */
+    int64_t n_hop_entries;           /* one-body hop-table entries (both species) */
+} smgpu_problem_info;
+
+typedef struct {
+    int32_t num_ev;      /* lowest eigenpairs wanted (main_sm num_ev) */
+    int32_t lm;          /* Krylov vectors kept (main_sm lm, default 300) */
+    int32_t ls;          /* vectors kept at a thick restart (main_sm ls, default 20) */
+    int32_t num_history; /* Ritz-value history (main_sm num_history, default 3) */
+    double tol;          /* convergence: all(en_prev - en_new < tol) (default 1e-8) */
+    int32_t precision;   /* SMGPU_FP64 or SMGPU_MIXED */
+    int32_t refine;      /* MIXED only: FP64 Rayleigh quotient of the Ritz vectors */
+    int32_t max_iter;    /* safety limit on Lanczos iterations (0 = 100000) */
+    uint64_t seed;       /* start vector seed */
+} smgpu_solve_params;
+
+typedef struct {
+    int32_t iterations;  /* elit of the last Lanczos sweep */
+    int32_t n_hv;        /* H·v applications in the working precision */
+    int32_t n_hv64;      /* FP64 H·v used by the refinement */
+    int32_t n_restart;
+    double wall_ms;
+    double ritz[64];     /* en[1]: Ritz values of the last convergence check */
+} smgpu_solve_stats;
+
+/* library */
+int32_t smgpu_version(void);
+const char* smgpu_last_error(void);
+int32_t smgpu_device_count(int32_t* n);
+
+/* readsmsnt + getZNA + construct_msps + occ + prep_Hamil_T1/prep_2bjumps +
+ * prep_Hamil_pn/prep_pn (shellmodel_main.jl:96-147, 293-365, 373-418, 580-730,
+ * 769-848, 1115-1199).  twoM < 0 selects the main_sm default A%2 (:110).
+ * ignore_massop mirrors readsmsnt's keyword (:293).  nthreads <= 0: all cores. */
+int32_t smgpu_problem_create(const char* snt_path, const char* nucleus, int32_t twoM, int32_t ignore_massop,
+                             int32_t nthreads, smgpu_problem* out);
+int32_t smgpu_problem_destroy(smgpu_problem p);
+int32_t smgpu_problem_get_info(smgpu_problem p, smgpu_problem_info* info);
+/* Replace the interaction of an existing problem (same model space, basis and
+ * index tables) — the shared-basis loop of prepEC (eigenvector_continuation.jl:139-212). */
+int32_t smgpu_problem_set_interaction(smgpu_problem p, const char* snt_path);
+
+/* Basis export for bit-exact checks against occ (shellmodel_main.jl:769-848):
+ * tdims[nblocks+1], Mp[nblocks], Mn[nblocks], pbits/nbits concatenated per
+ * block (n_psd / n_nsd entries, block order), psd_off/nsd_off[nblocks+1]. */
+int32_t smgpu_basis_export(smgpu_problem p, int64_t* tdims, int32_t* Mp, int32_t* Mn, int64_t* pbits,
+                           int64_t* psd_off, int64_t* nbits, int64_t* nsd_off);
+
+/* One-body hop tables in a canonical form for comparison with calc_1b_jumps!
+ * (shellmodel_main.jl:1169-1199): every (species, input block, output block,
+ * input SD, output SD, creation m-state, annihilation m-state, phase bit of the
+ * reference's calc_phase_1!).  Pass NULL arrays to get the count in *n. */
+int32_t smgpu_hops_export(smgpu_problem p, int64_t* n, int32_t* species, int32_t* bi, int32_t* bf, int32_t* i,
+                          int32_t* f, int32_t* a, int32_t* c, int8_t* phase);
+
+/* Like-particle matrix entries per block (the symmetric matrix that
+ * prep_2bjumps' entries define, shellmodel_main.jl:1115-1137): species 0 = pp,
+ * 1 = nn; rows (block, f, i, value).  NULL arrays: count in *n. */
+int32_t smgpu_pair_export(smgpu_problem p, int32_t species, int64_t* n, int32_t* blk, int32_t* f, int32_t* i,
+                          double* val);
+
+/* Upload tables to a device (the "uploaded once" step). */
+int32_t smgpu_ham_create(smgpu_problem p, int32_t device, smgpu_ham* out);
+int32_t smgpu_ham_destroy(smgpu_ham h);
+/* Re-upload the coefficients after smgpu_problem_set_interaction (index tables stay). */
+int32_t smgpu_ham_update_interaction(smgpu_ham h, smgpu_problem p);
+
+/* Bytes of every array the H·v kernel streams per call (roofline accounting).
+ * names: newline-separated names written into names_buf (may be NULL). */
+int32_t smgpu_table_bytes(smgpu_ham h, int32_t precision, int32_t max_arrays, int64_t* bytes, int32_t* n_arrays,
+                          char* names_buf, int32_t names_len);
+/* FMAs per H·v: [pn terms incl. padding, pn terms exact, pp, nn, diagonal] */
+int32_t smgpu_work_terms(smgpu_ham h, int64_t* terms5);
+
+/* operate_H! (lanczos_methods.jl:153-248), as w = H v (the reference accumulates
+ * into a zeroed twf).  Device pointers (precision FP64: double, MIXED: float),
+ * on the handle's stream unless `stream` is given (hipStream_t). */
+int32_t smgpu_apply_H(smgpu_ham h, int32_t precision, const void* v_dev, void* w_dev, void* stream);
+/* Host-buffer convenience: FP64 in/out, computed in `precision` on the device. */
+int32_t smgpu_apply_H_host(smgpu_ham h, int32_t precision, const double* v, double* w);
+/* Timing: `reps` launches of H·v on resident device vectors, HIP events on the
+ * launch stream; average milliseconds per launch in *ms. */
+int32_t smgpu_time_apply_H(smgpu_ham h, int32_t precision, int32_t reps, double* ms);
+
+/* TRL + Ritz assembly (lanczos_methods.jl:1-91, 250-307; shellmodel_main.jl:199-224):
+ * evals[num_ev] gets main_sm's en[1]; evecs (num_ev * mdim, may be NULL) the
+ * normalised Ritz vectors. */
+void smgpu_default_params(smgpu_solve_params* p);
+int32_t smgpu_trl_solve(smgpu_ham h, const smgpu_solve_params* params, double* evals, double* evecs,
+                        smgpu_solve_stats* stats);
+
+/* Device memory helpers for callers that keep vectors resident. */
+int32_t smgpu_malloc(smgpu_ham h, int64_t bytes, void** dev);
+int32_t smgpu_free(smgpu_ham h, void* dev);
+int32_t smgpu_memcpy(smgpu_ham h, void* dst, const void* src, int64_t bytes, int32_t kind /*0 H2D,1 D2H,2 D2D*/);
+int32_t smgpu_synchronize(smgpu_ham h);
+
+/* Self-test of the shared tridiagonal eigen-solver (host build of the same
+ * source the device kernels use): lowest k eigenpairs of T. */
+int32_t smgpu_selftest_tridiag(const double* alpha, const double* beta, int32_t n, int32_t k, double* vals,
+                               double* vecs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

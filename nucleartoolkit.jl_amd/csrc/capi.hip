@@ -1,0 +1,427 @@
+// C ABI (include/smgpu.h).  Every entry point catches all exceptions and turns
+// them into a status code plus a thread-local message.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <thread>
+
+#include "../../include/smgpu.h"
+#include "sm_device.h"
+#include "sm_host.h"
+#include "sm_trl.h"
+#include "tridiag.h"
+
+using namespace smgpu;
+
+struct smgpu_problem_s {
+    Snt snt;
+    std::string nucleus;
+    Basis basis;
+    Tables tables;
+    int Z = 0, N = 0, vp = 0, vn = 0;
+    int nthreads = 1;
+};
+
+struct smgpu_ham_s {
+    std::unique_ptr<DeviceHam> dev;
+};
+
+template <typename TA, typename TB>
+__global__ void k_cast(const TA* a, TB* b, int64_t n) {
+    for (int64_t x = (int64_t)blockIdx.x * 256 + threadIdx.x; x < n; x += (int64_t)gridDim.x * 256) b[x] = (TB)a[x];
+}
+
+static thread_local std::string g_err;
+
+static int32_t fail(int32_t code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+template <typename F> static int32_t guard(F&& f) {
+    try {
+        return f();
+    } catch (const std::invalid_argument& e) {
+        return fail(SMGPU_EINVAL, e.what());
+    } catch (const std::bad_alloc&) {
+        return fail(SMGPU_ENOMEM, "out of host memory");
+    } catch (const std::runtime_error& e) {
+        std::string m = e.what();
+        if (m.rfind("HIP error out of memory", 0) == 0 || m.find("hipErrorOutOfMemory") != std::string::npos)
+            return fail(SMGPU_ENOMEM, m);
+        if (m.rfind("HIP error", 0) == 0) return fail(SMGPU_EDEVICE, m);
+        if (m.rfind("cannot open", 0) == 0) return fail(SMGPU_EIO, m);
+        if (m.rfind("TRL: iteration limit", 0) == 0) return fail(SMGPU_ENOTCONV, m);
+        return fail(SMGPU_EINVAL, m);
+    } catch (...) {
+        return fail(SMGPU_EINVAL, "unknown error");
+    }
+}
+
+extern "C" {
+
+int32_t smgpu_version(void) { return 100; }
+const char* smgpu_last_error(void) { return g_err.c_str(); }
+
+int32_t smgpu_device_count(int32_t* n) {
+    return guard([&]() {
+        int c = 0;
+        SM_HIP(hipGetDeviceCount(&c));
+        *n = c;
+        return SMGPU_OK;
+    });
+}
+
+int32_t smgpu_problem_create(const char* snt_path, const char* nucleus, int32_t twoM, int32_t ignore_massop,
+                             int32_t nthreads, smgpu_problem* out) {
+    return guard([&]() -> int32_t {
+        if (!snt_path || !nucleus || !out) throw std::invalid_argument("null argument");
+        *out = nullptr;
+        auto p = std::make_unique<smgpu_problem_s>();
+        p->nucleus = nucleus;
+        p->snt = read_snt(snt_path, nucleus, ignore_massop != 0);
+        std::string el;
+        for (char c : p->nucleus) if (!isdigit((unsigned char)c)) el += c;
+        int Z = element_z(el);
+        if (Z < 0) throw std::invalid_argument("unknown element in nucleus '" + p->nucleus + "'");
+        p->Z = Z;
+        p->N = p->snt.A - Z;
+        p->vp = Z - p->snt.cp;
+        p->vn = p->N - p->snt.cn;
+        if (p->vp < 0 || p->vn < 0) throw std::invalid_argument("nucleus below the core of the interaction");
+        int M = twoM < 0 ? p->snt.A % 2 : twoM;
+        if ((M - p->snt.A) % 2 != 0) throw std::invalid_argument("2M parity does not match the mass number");
+        p->nthreads = nthreads > 0 ? nthreads : (int)std::max(1u, std::thread::hardware_concurrency());
+        p->basis = build_basis(p->snt, p->vp, p->vn, M);
+        p->tables = build_tables(p->snt, p->basis, p->nthreads);
+        *out = p.release();
+        return SMGPU_OK;
+    });
+}
+
+int32_t smgpu_problem_destroy(smgpu_problem p) {
+    delete p;
+    return SMGPU_OK;
+}
+
+int32_t smgpu_problem_set_interaction(smgpu_problem p, const char* snt_path) {
+    return guard([&]() -> int32_t {
+        if (!p || !snt_path) throw std::invalid_argument("null argument");
+        Snt s = read_snt(snt_path, p->nucleus, false);
+        if (s.lp != p->snt.lp || s.ln != p->snt.ln || s.cp != p->snt.cp || s.cn != p->snt.cn)
+            throw std::invalid_argument("interaction has a different model space");
+        for (int i = 0; i < s.lp; i++)
+            if (s.p_sps[i].n != p->snt.p_sps[i].n || s.p_sps[i].l != p->snt.p_sps[i].l ||
+                s.p_sps[i].j2 != p->snt.p_sps[i].j2)
+                throw std::invalid_argument("interaction has different orbits");
+        p->snt = s;
+        build_coefficients(p->snt, p->basis, p->tables, p->nthreads);
+        return SMGPU_OK;
+    });
+}
+
+int32_t smgpu_problem_get_info(smgpu_problem p, smgpu_problem_info* info) {
+    return guard([&]() -> int32_t {
+        if (!p || !info) throw std::invalid_argument("null argument");
+        std::memset(info, 0, sizeof(*info));
+        info->Z = p->Z; info->N = p->N; info->A = p->snt.A; info->vp = p->vp; info->vn = p->vn;
+        info->twoM = p->basis.twoM;
+        info->lp = p->snt.lp; info->ln = p->snt.ln; info->cp = p->snt.cp; info->cn = p->snt.cn;
+        info->nblocks = p->basis.nb;
+        info->n_msps_p = p->basis.sp[0].L; info->n_msps_n = p->basis.sp[1].L;
+        info->mdim = p->basis.mdim;
+        for (int b = 0; b < p->basis.nb; b++) {
+            info->n_psd += (int64_t)p->basis.sp[0].sets[p->basis.pset[b]].size();
+            info->n_nsd += (int64_t)p->basis.sp[1].sets[p->basis.nset[b]].size();
+        }
+        info->n_pp_ops = p->tables.n_pp_ops; info->n_nn_ops = p->tables.n_nn_ops;
+        info->n_pn_ops = p->tables.n_pn_ops;
+        for (auto& h : p->tables.hops) info->n_hop_entries += (int64_t)h.ent.size();
+        return SMGPU_OK;
+    });
+}
+
+int32_t smgpu_basis_export(smgpu_problem p, int64_t* tdims, int32_t* Mp, int32_t* Mn, int64_t* pbits,
+                           int64_t* psd_off, int64_t* nbits, int64_t* nsd_off) {
+    return guard([&]() -> int32_t {
+        if (!p) throw std::invalid_argument("null problem");
+        const Basis& b = p->basis;
+        int64_t po = 0, no = 0;
+        for (int k = 0; k <= b.nb; k++) if (tdims) tdims[k] = b.tdims[k];
+        for (int k = 0; k < b.nb; k++) {
+            if (Mp) Mp[k] = b.Mp[k];
+            if (Mn) Mn[k] = b.Mn[k];
+            if (psd_off) psd_off[k] = po;
+            if (nsd_off) nsd_off[k] = no;
+            for (uint64_t x : b.sp[0].sets[b.pset[k]]) { if (pbits) pbits[po] = (int64_t)x; po++; }
+            for (uint64_t x : b.sp[1].sets[b.nset[k]]) { if (nbits) nbits[no] = (int64_t)x; no++; }
+        }
+        if (psd_off) psd_off[b.nb] = po;
+        if (nsd_off) nsd_off[b.nb] = no;
+        return SMGPU_OK;
+    });
+}
+
+int32_t smgpu_hops_export(smgpu_problem p, int64_t* n, int32_t* species, int32_t* bi, int32_t* bf, int32_t* i,
+                          int32_t* f, int32_t* a, int32_t* c, int8_t* phase) {
+    return guard([&]() -> int32_t {
+        if (!p || !n) throw std::invalid_argument("null argument");
+        const Basis& b = p->basis;
+        const Tables& t = p->tables;
+        std::vector<int> pblk(b.sp[0].sets.size(), -1), nblk(b.sp[1].sets.size(), -1);
+        for (int k = 0; k < b.nb; k++) { pblk[b.pset[k]] = k; nblk[b.nset[k]] = k; }
+        int64_t q = 0;
+        for (const HopTable& H : t.hops) {
+            int s = H.species;
+            int bi_ = s == 0 ? pblk[H.iset] : nblk[H.iset];
+            int bf_ = s == 0 ? pblk[H.fset] : nblk[H.fset];
+            if (bi_ < 0 || bf_ < 0) continue;
+            int k = t.ob[s].index_of_dM(H.dM);
+            for (size_t ff = 0; ff + 1 < H.rowptr.size(); ff++)
+                for (int64_t e = H.rowptr[ff]; e < H.rowptr[ff + 1]; e++) {
+                    if (species) {
+                        uint32_t x = H.ent[e];
+                        int op = (int)((x >> 20) & 0x7FFu);
+                        species[q] = s; bi[q] = bi_; bf[q] = bf_;
+                        i[q] = (int32_t)(x & 0xFFFFFu); f[q] = (int32_t)ff;
+                        a[q] = t.ob[s].ops[k][op].first; c[q] = t.ob[s].ops[k][op].second;
+                        phase[q] = (int8_t)(((x >> 31) ^ 1u) & 1u);   // the reference's +1
+                    }
+                    q++;
+                }
+        }
+        *n = q;
+        return SMGPU_OK;
+    });
+}
+
+int32_t smgpu_pair_export(smgpu_problem p, int32_t species, int64_t* n, int32_t* blk, int32_t* f, int32_t* i,
+                          double* val) {
+    return guard([&]() -> int32_t {
+        if (!p || !n || species < 0 || species > 1) throw std::invalid_argument("bad argument");
+        const Basis& b = p->basis;
+        int64_t q = 0;
+        for (int k = 0; k < b.nb; k++) {
+            const PairSym& m = species == 0 ? p->tables.pp[b.pset[k]] : p->tables.nn[b.nset[k]];
+            for (size_t r = 0; r + 1 < m.rowptr.size(); r++)
+                for (int64_t e = m.rowptr[r]; e < m.rowptr[r + 1]; e++) {
+                    if (blk) { blk[q] = k; f[q] = (int32_t)r; i[q] = m.col[e]; val[q] = m.val[e]; }
+                    q++;
+                }
+        }
+        *n = q;
+        return SMGPU_OK;
+    });
+}
+
+int32_t smgpu_ham_create(smgpu_problem p, int32_t device, smgpu_ham* out) {
+    return guard([&]() -> int32_t {
+        if (!p || !out) throw std::invalid_argument("null argument");
+        *out = nullptr;
+        if (p->basis.mdim == 0) throw std::invalid_argument("empty M-scheme space (mdim = 0)");
+        auto h = std::make_unique<smgpu_ham_s>();
+        h->dev = upload_ham(p->basis, p->tables, device);
+        *out = h.release();
+        return SMGPU_OK;
+    });
+}
+
+int32_t smgpu_ham_destroy(smgpu_ham h) {
+    return guard([&]() -> int32_t {
+        if (h && h->dev) SM_HIP(hipSetDevice(h->dev->device));
+        delete h;
+        return SMGPU_OK;
+    });
+}
+
+int32_t smgpu_ham_update_interaction(smgpu_ham h, smgpu_problem p) {
+    return guard([&]() -> int32_t {
+        if (!h || !p) throw std::invalid_argument("null argument");
+        if (h->dev->mdim != p->basis.mdim) throw std::invalid_argument("problem does not match the handle");
+        update_coefficients(*h->dev, p->basis, p->tables);
+        return SMGPU_OK;
+    });
+}
+
+int32_t smgpu_table_bytes(smgpu_ham h, int32_t precision, int32_t max_arrays, int64_t* bytes, int32_t* n_arrays,
+                          char* names_buf, int32_t names_len) {
+    return guard([&]() -> int32_t {
+        if (!h) throw std::invalid_argument("null handle");
+        auto tb = h->dev->table_bytes(precision == SMGPU_FP64 ? 0 : 1);
+        std::string names;
+        for (int k = 0; k < (int)tb.size(); k++) {
+            if (k < max_arrays && bytes) bytes[k] = tb[k].second;
+            names += tb[k].first + "\n";
+        }
+        if (n_arrays) *n_arrays = (int32_t)tb.size();
+        if (names_buf && names_len > 0) {
+            std::strncpy(names_buf, names.c_str(), names_len - 1);
+            names_buf[names_len - 1] = 0;
+        }
+        return SMGPU_OK;
+    });
+}
+
+int32_t smgpu_work_terms(smgpu_ham h, int64_t* t) {
+    return guard([&]() -> int32_t {
+        if (!h || !t) throw std::invalid_argument("null argument");
+        t[0] = h->dev->pn_terms; t[1] = h->dev->pn_terms_exact; t[2] = h->dev->pp_terms;
+        t[3] = h->dev->nn_terms; t[4] = h->dev->mdim;
+        return SMGPU_OK;
+    });
+}
+
+int32_t smgpu_apply_H(smgpu_ham h, int32_t precision, const void* v_dev, void* w_dev, void* stream) {
+    return guard([&]() -> int32_t {
+        if (!h || !v_dev || !w_dev) throw std::invalid_argument("null argument");
+        SM_HIP(hipSetDevice(h->dev->device));
+        hipStream_t s = stream ? (hipStream_t)stream : h->dev->stream;
+        launch_hv(*h->dev, precision == SMGPU_FP64 ? 0 : 1, v_dev, w_dev, s);
+        return SMGPU_OK;
+    });
+}
+
+int32_t smgpu_apply_H_host(smgpu_ham h, int32_t precision, const double* v, double* w) {
+    return guard([&]() -> int32_t {
+        if (!h || !v || !w) throw std::invalid_argument("null argument");
+        DeviceHam& d = *h->dev;
+        SM_HIP(hipSetDevice(d.device));
+        int64_t n = d.mdim;
+        DBuf dv, dw;
+        dv.alloc(n * 8);
+        dw.alloc(n * 8);
+        SM_HIP(hipMemcpyAsync(dv.p, v, n * 8, hipMemcpyHostToDevice, d.stream));
+        int g = (int)std::min<int64_t>(4096, (n + 255) / 256);
+        if (precision == SMGPU_FP64) {
+            launch_hv(d, 0, dv.p, dw.p, d.stream);
+        } else {
+            DBuf fv, fw;
+            fv.alloc(n * 4);
+            fw.alloc(n * 4);
+            hipLaunchKernelGGL((k_cast<double, float>), dim3(g), dim3(256), 0, d.stream, dv.as<double>(),
+                               fv.as<float>(), n);
+            launch_hv(d, 1, fv.p, fw.p, d.stream);
+            hipLaunchKernelGGL((k_cast<float, double>), dim3(g), dim3(256), 0, d.stream, fw.as<float>(),
+                               dw.as<double>(), n);
+            SM_HIP(hipStreamSynchronize(d.stream));
+        }
+        SM_HIP(hipMemcpyAsync(w, dw.p, n * 8, hipMemcpyDeviceToHost, d.stream));
+        SM_HIP(hipStreamSynchronize(d.stream));
+        return SMGPU_OK;
+    });
+}
+
+int32_t smgpu_time_apply_H(smgpu_ham h, int32_t precision, int32_t reps, double* ms) {
+    return guard([&]() -> int32_t {
+        if (!h || !ms || reps < 1) throw std::invalid_argument("bad argument");
+        DeviceHam& d = *h->dev;
+        SM_HIP(hipSetDevice(d.device));
+        int64_t n = d.mdim;
+        size_t s = precision == SMGPU_FP64 ? 8 : 4;
+        DBuf dv, dw;
+        dv.alloc(n * s);
+        dw.alloc(n * s);
+        std::vector<double> hv(n);
+        for (int64_t i = 0; i < n; i++) hv[i] = 1.0 / std::sqrt((double)n) * (1.0 + 0.25 * std::sin(0.37 * (double)i));
+        if (s == 8) SM_HIP(hipMemcpy(dv.p, hv.data(), n * 8, hipMemcpyHostToDevice));
+        else {
+            std::vector<float> f(hv.begin(), hv.end());
+            SM_HIP(hipMemcpy(dv.p, f.data(), n * 4, hipMemcpyHostToDevice));
+        }
+        int p = precision == SMGPU_FP64 ? 0 : 1;
+        for (int r = 0; r < 2; r++) launch_hv(d, p, dv.p, dw.p, d.stream);
+        hipEvent_t e0, e1;
+        SM_HIP(hipEventCreate(&e0));
+        SM_HIP(hipEventCreate(&e1));
+        SM_HIP(hipEventRecord(e0, d.stream));
+        for (int r = 0; r < reps; r++) launch_hv(d, p, dv.p, dw.p, d.stream);
+        SM_HIP(hipEventRecord(e1, d.stream));
+        SM_HIP(hipEventSynchronize(e1));
+        float t = 0.f;
+        SM_HIP(hipEventElapsedTime(&t, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *ms = (double)t / reps;
+        return SMGPU_OK;
+    });
+}
+
+void smgpu_default_params(smgpu_solve_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->num_ev = 5; p->lm = 300; p->ls = 20; p->num_history = 3; p->tol = 1e-8;
+    p->precision = SMGPU_FP64; p->refine = 1; p->max_iter = 100000; p->seed = 1234;
+}
+
+int32_t smgpu_trl_solve(smgpu_ham h, const smgpu_solve_params* params, double* evals, double* evecs,
+                        smgpu_solve_stats* stats) {
+    return guard([&]() -> int32_t {
+        if (!h || !params || !evals) throw std::invalid_argument("null argument");
+        smgpu_solve_params p = *params;
+        if (p.max_iter <= 0) p.max_iter = 100000;
+        if (p.num_history < 2) p.num_history = 3;
+        if (p.num_ev > 64) throw std::invalid_argument("num_ev above 64");
+        smgpu_solve_stats st;
+        std::memset(&st, 0, sizeof(st));
+        p.num_ev = (int32_t)std::min<int64_t>(p.num_ev, h->dev->mdim);
+        trl_solve(*h->dev, p, evals, evecs, st);
+        if (stats) *stats = st;
+        return SMGPU_OK;
+    });
+}
+
+int32_t smgpu_malloc(smgpu_ham h, int64_t bytes, void** dev) {
+    return guard([&]() -> int32_t {
+        if (!h || !dev || bytes < 0) throw std::invalid_argument("bad argument");
+        SM_HIP(hipSetDevice(h->dev->device));
+        SM_HIP(hipMalloc(dev, (size_t)std::max<int64_t>(bytes, 1)));
+        return SMGPU_OK;
+    });
+}
+
+int32_t smgpu_free(smgpu_ham h, void* dev) {
+    return guard([&]() -> int32_t {
+        if (!h) throw std::invalid_argument("null handle");
+        SM_HIP(hipSetDevice(h->dev->device));
+        if (dev) SM_HIP(hipFree(dev));
+        return SMGPU_OK;
+    });
+}
+
+int32_t smgpu_memcpy(smgpu_ham h, void* dst, const void* src, int64_t bytes, int32_t kind) {
+    return guard([&]() -> int32_t {
+        if (!h || !dst || !src || bytes < 0) throw std::invalid_argument("bad argument");
+        hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+        SM_HIP(hipSetDevice(h->dev->device));
+        SM_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, k, h->dev->stream));
+        SM_HIP(hipStreamSynchronize(h->dev->stream));
+        return SMGPU_OK;
+    });
+}
+
+int32_t smgpu_synchronize(smgpu_ham h) {
+    return guard([&]() -> int32_t {
+        if (!h) throw std::invalid_argument("null handle");
+        SM_HIP(hipStreamSynchronize(h->dev->stream));
+        return SMGPU_OK;
+    });
+}
+
+int32_t smgpu_selftest_tridiag(const double* alpha, const double* beta, int32_t n, int32_t k, double* vals,
+                               double* vecs) {
+    return guard([&]() -> int32_t {
+        if (!alpha || !vals || !vecs || n < 1 || k < 1 || k > n) throw std::invalid_argument("bad argument");
+        std::vector<double> work(5 * (size_t)n + 8);
+        std::vector<double> b(std::max(1, n));
+        for (int i = 0; i < n - 1; i++) b[i] = beta[i];
+        tri_lowest(alpha, b.data(), n, k, vals, vecs, work.data());
+        return SMGPU_OK;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,107 @@
+// Device-side objects shared by the HIP translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "sm_host.h"
+
+namespace smgpu {
+
+#define SM_HIP(call)                                                                  \
+    do {                                                                              \
+        hipError_t e_ = (call);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(e_) + \
+                                     " at " + __FILE__ + ":" + std::to_string(__LINE__)); \
+    } while (0)
+
+// Device buffer owned by a handle.
+struct DBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf();
+    void alloc(size_t n);
+    template <typename T> void upload(const std::vector<T>& v) {
+        alloc(v.size() * sizeof(T));
+        if (!v.empty()) SM_HIP(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    }
+    template <typename T> T* as() const { return (T*)p; }
+};
+
+// Kernel view of the Hamiltonian tables for one precision (all device pointers).
+struct HvTables {
+    int nb, R;
+    const int32_t* row_block;   // R
+    const int32_t* block_row0;  // nb
+    const int32_t* nP;          // nb
+    const int32_t* nN;          // nb
+    const int64_t* tdims;       // nb+1
+    const void* ep;             // R      (T)
+    const int64_t* en_off;      // nb
+    const void* en;             // sum nN (T)
+    const int64_t* pp_rowptr;   // R+1
+    const int32_t* pp_col;
+    const void* pp_val;         // (T)
+    const int64_t* nn_slice_base;  // nb: first slice of the block
+    const int64_t* nn_slice_off;   // entry offset of a slice
+    const int32_t* nn_slice_len;
+    const int32_t* nn_col;
+    const void* nn_val;            // (T)
+    const int32_t* pair_ptr;       // nb+1
+    const int32_t* pair_bi;
+    const int32_t* pair_dm;
+    const int32_t* pair_ptab;
+    const int32_t* pair_ntab;
+    const int64_t* ptab_rowptr_base;  // per p table: offset into ptab_rowptr
+    const int32_t* ptab_rowptr;       // relative entry offsets
+    const int64_t* ptab_ent_base;
+    const uint32_t* ptab_ent;         // i | op<<20 | neg<<31
+    const int64_t* ntab_slice_base;   // per n table: first slice
+    const int64_t* ntab_slice_off;
+    const int32_t* ntab_slice_len;
+    const uint32_t* ntab_ent;         // in | col<<20 ; col = 2*bd + neg, pad col = 2*cols
+    const int64_t* veff_off;          // per dM
+    const int32_t* veff_cols2;        // per dM: 2*(cols+1)
+    const void* veff;                 // (T) signed copies: [row][2*bd+neg]
+    int pch;        // p-hop rows staged per LDS round
+    int nN_max;
+    int cols2_max;
+    int npt;        // ceil(nN_max / 256)
+    size_t lds_bytes;
+};
+
+// Device-resident Hamiltonian (index tables shared, coefficients per precision).
+struct DeviceHam {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int64_t mdim = 0;
+    int nb = 0, R = 0;
+    // index tables
+    DBuf row_block, block_row0, nP, nN, tdims, en_off, pp_rowptr, pp_col, nn_slice_base, nn_slice_off,
+        nn_slice_len, nn_col, pair_ptr, pair_bi, pair_dm, pair_ptab, pair_ntab, ptab_rowptr_base,
+        ptab_rowptr, ptab_ent_base, ptab_ent, ntab_slice_base, ntab_slice_off, ntab_slice_len, ntab_ent,
+        veff_off, veff_cols2;
+    // coefficients (index 0: FP64, 1: FP32)
+    DBuf ep[2], en[2], pp_val[2], nn_val[2], veff[2];
+    HvTables view[2];
+    // byte accounting of every array the H·v kernel streams, per precision
+    std::vector<std::pair<std::string, int64_t>> table_bytes(int prec) const;
+    int64_t pn_terms = 0, pp_terms = 0, nn_terms = 0;   // FMAs per H·v (padded n-table entries included)
+    int64_t pn_terms_exact = 0;
+    ~DeviceHam();
+};
+
+std::unique_ptr<DeviceHam> upload_ham(const Basis& b, const Tables& t, int device);
+// w = H v for one vector; prec 0 = FP64 (double*), 1 = FP32 (float*)
+void launch_hv(const DeviceHam& h, int prec, const void* v, void* w, hipStream_t s);
+// re-upload interaction-dependent coefficients (same basis and index tables)
+void update_coefficients(DeviceHam& h, const Basis& b, const Tables& t);
+
+}  // namespace smgpu

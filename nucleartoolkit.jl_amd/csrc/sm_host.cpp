@@ -1,0 +1,563 @@
+// Host-side construction of the basis and of the device-layout Hamiltonian
+// tables.  Semantics follow NuclearToolkit.jl (paths relative to its root):
+//   readsmsnt              src/ShellModel/shellmodel_main.jl:293-365
+//   construct_msps / occ   shellmodel_main.jl:373-418, 732-848, 899-907
+//   prep_Hamil_T1          shellmodel_main.jl:580-649
+//   prep_Hamil_pn          shellmodel_main.jl:658-730
+//   hop validity / phases  shellmodel_main.jl:850-897
+// The layout (gather-form hop tables, merged symmetric pp/nn rows, V_eff
+// matrices per dM) is this library's own; see sm_host.h.
+#include "sm_host.h"
+
+#include <quadmath.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <fstream>
+#include <functional>
+#include <sstream>
+#include <stdexcept>
+#include <thread>
+#include <unordered_map>
+
+namespace smgpu {
+
+static void parallel_for(int64_t n, int nthreads, const std::function<void(int64_t)>& fn) {
+    if (nthreads <= 1 || n <= 1) {
+        for (int64_t i = 0; i < n; i++) fn(i);
+        return;
+    }
+    std::atomic<int64_t> next{0};
+    std::vector<std::thread> th;
+    int nt = (int)std::min<int64_t>(nthreads, n);
+    for (int t = 0; t < nt; t++)
+        th.emplace_back([&]() {
+            for (;;) {
+                int64_t i = next.fetch_add(1);
+                if (i >= n) break;
+                fn(i);
+            }
+        });
+    for (auto& x : th) x.join();
+}
+
+// ------------------------------------------------------------------ parsing
+static std::vector<std::string> split_ws(const std::string& s) {
+    std::istringstream is(s);
+    std::vector<std::string> out;
+    std::string t;
+    while (is >> t) out.push_back(t);
+    return out;
+}
+
+static const char* kElements[] = {
+    "H", "He", "Li", "Be", "B", "C", "N", "O", "F", "Ne", "Na", "Mg", "Al", "Si", "P", "S", "Cl", "Ar",
+    "K", "Ca", "Sc", "Ti", "V", "Cr", "Mn", "Fe", "Co", "Ni", "Cu", "Zn", "Ga", "Ge", "As", "Se", "Br",
+    "Kr", "Rb", "Sr", "Y", "Zr", "Nb", "Mo", "Tc", "Ru", "Rh", "Pd", "Ag", "Cd", "In", "Sn", "Sb", "Te",
+    "I", "Xe", "Cs", "Ba", "La", "Ce", "Pr", "Nd", "Pm", "Sm", "Eu", "Gd", "Tb", "Dy", "Ho", "Er", "Tm",
+    "Yb", "Lu", "Hf", "Ta", "W", "Re", "Os", "Ir", "Pt", "Au", "Hg", "Tl", "Pb", "Bi", "Po", "At", "Rn"};
+
+int element_z(const std::string& symbol) {
+    for (int i = 0; i < (int)(sizeof(kElements) / sizeof(kElements[0])); i++)
+        if (symbol == kElements[i]) return i + 1;
+    return -1;
+}
+
+Snt read_snt(const std::string& path, const std::string& nucleus, bool ignore_massop) {
+    Snt s;
+    // mass number = first run of digits (util_func.jl:1 `reg`)
+    std::string digits;
+    for (char c : nucleus) {
+        if (isdigit((unsigned char)c)) digits += c;
+        else if (!digits.empty()) break;
+    }
+    if (digits.empty()) throw std::runtime_error("nucleus '" + nucleus + "' has no mass number");
+    s.A = std::stoi(digits);
+    if (s.A <= 0) throw std::runtime_error("invalid nucleus " + nucleus);
+    std::ifstream f(path);
+    if (!f) throw std::runtime_error("cannot open interaction file " + path);
+    std::vector<std::string> lines;
+    std::string line;
+    while (std::getline(f, line)) {
+        size_t b = line.find_first_not_of(" \t\r\n");
+        std::string t = b == std::string::npos ? std::string() : line.substr(b);
+        if (!t.empty() && (t[0] == '!' || t[0] == '#')) continue;
+        lines.push_back(t);
+    }
+    auto need = [&](size_t i) -> std::vector<std::string> {
+        if (i >= lines.size()) throw std::runtime_error("truncated interaction file " + path);
+        return split_ws(lines[i]);
+    };
+    try {
+        auto h = need(0);
+        if (h.size() < 4) throw std::runtime_error("bad model-space header");
+        s.lp = std::stoi(h[0]); s.ln = std::stoi(h[1]); s.cp = std::stoi(h[2]); s.cn = std::stoi(h[3]);
+        if (s.lp <= 0 || s.ln <= 0 || s.lp > 16 || s.ln > 16) throw std::runtime_error("bad orbit counts");
+        s.p_sps.resize(s.lp); s.n_sps.resize(s.ln);
+        for (int i = 0; i < s.lp; i++) {
+            auto t = need(1 + i);
+            if (t.size() < 5) throw std::runtime_error("bad orbit line");
+            int ith = std::stoi(t[0]);
+            if (ith < 1 || ith > s.lp) throw std::runtime_error("bad proton orbit index");
+            s.p_sps[ith - 1] = {std::stoi(t[1]), std::stoi(t[2]), std::stoi(t[3]), std::stoi(t[4])};
+        }
+        for (int i = 0; i < s.ln; i++) {
+            auto t = need(1 + i + s.ln);
+            if (t.size() < 5) throw std::runtime_error("bad orbit line");
+            int ith = std::stoi(t[0]) - s.lp;
+            if (ith < 1 || ith > s.ln) throw std::runtime_error("bad neutron orbit index");
+            s.n_sps[ith - 1] = {std::stoi(t[1]), std::stoi(t[2]), std::stoi(t[3]), std::stoi(t[4])};
+        }
+        size_t o = 1 + s.ln + s.lp;
+        auto t = need(o);
+        int nsp = std::stoi(t[0]);
+        s.spe_p.assign(s.lp, 0.0); s.spe_n.assign(s.ln, 0.0);
+        for (int i = 0; i < nsp; i++) {
+            auto u = need(o + 1 + i);
+            if (u.size() < 3) throw std::runtime_error("bad SPE line");
+            int j = std::stoi(u[0]);
+            if (j < 1 || j > s.lp + s.ln) throw std::runtime_error("bad SPE orbit");
+            if (j <= s.lp) s.spe_p[j - 1] = std::stod(u[2]);
+            else s.spe_n[j - s.lp - 1] = std::stod(u[2]);
+        }
+        auto u = need(o + 1 + nsp);
+        if (u.size() < 3) throw std::runtime_error("TBME header needs at least 3 fields");
+        int ntbme = std::stoi(u[0]);
+        s.massop = std::stoi(u[1]);
+        s.Aref = std::stod(u[2]);
+        s.pw = u.size() > 3 ? std::stod(u[3]) : 0.0;
+        for (int ith = 1; ith <= ntbme; ith++) {
+            auto r = need(o + 1 + nsp + ith);
+            if (r.size() < 6) throw std::runtime_error("bad TBME line");
+            int i = std::stoi(r[0]), j = std::stoi(r[1]), k = std::stoi(r[2]), l = std::stoi(r[3]);
+            int J = std::stoi(r[4]);
+            int ch;
+            if (i <= s.lp && j <= s.lp) ch = 0;
+            else if (i > s.lp && j > s.lp) ch = 1;
+            else if (i <= s.lp && j > s.lp) ch = 2;
+            else throw std::runtime_error("TBME line with neutron before proton index");
+            double v = std::stod(r[5]);
+            if (s.massop == 1 && !ignore_massop) v *= std::pow((double)s.A / s.Aref, s.pw);
+            bool same = (i == j) ? (k == l && k == i) : (k != l && ((i == k && j == l)));
+            // readsmsnt compares unique([i,j]) with unique([k,l]) as ordered lists
+            std::vector<int> u1{i}, u2{k};
+            if (j != i) u1.push_back(j);
+            if (l != k) u2.push_back(l);
+            same = (u1 == u2);
+            if (!same) { s.labels[ch].push_back({k, l, i, j, J, ith}); s.tbmes[ch].push_back(v); }
+            s.labels[ch].push_back({i, j, k, l, J, ith});
+            s.tbmes[ch].push_back(v);
+        }
+    } catch (std::invalid_argument&) {
+        throw std::runtime_error("unparsable number in " + path);
+    } catch (std::out_of_range&) {
+        throw std::runtime_error("number out of range in " + path);
+    }
+    return s;
+}
+
+// ------------------------------------------------------------------ CG
+static __float128 qfact(int n) {
+    __float128 r = 1;
+    for (int i = 2; i <= n; i++) r *= i;
+    return r;
+}
+
+double clebsch_gordan(int j1, int m1, int j2, int m2, int J, int M) {
+    if (m1 + m2 != M || std::abs(m1) > j1 || std::abs(m2) > j2 || std::abs(M) > J) return 0.0;
+    if (J > j1 + j2 || J < std::abs(j1 - j2)) return 0.0;
+    if (((j1 + m1) & 1) || ((j2 + m2) & 1) || ((J + M) & 1) || ((j1 + j2 + J) & 1)) return 0.0;
+    int a = (j1 + j2 - J) / 2, b = (j1 - j2 + J) / 2, c = (-j1 + j2 + J) / 2, d = (j1 + j2 + J) / 2 + 1;
+    __float128 pref = (__float128)(J + 1) * qfact(a) * qfact(b) * qfact(c) / qfact(d);
+    pref *= qfact((j1 + m1) / 2) * qfact((j1 - m1) / 2) * qfact((j2 + m2) / 2) * qfact((j2 - m2) / 2) *
+            qfact((J + M) / 2) * qfact((J - M) / 2);
+    __float128 s = 0;
+    for (int k = 0; k <= a; k++) {
+        int e[6] = {k, a - k, (j1 - m1) / 2 - k, (j2 + m2) / 2 - k, (J - j2 + m1) / 2 + k, (J - j1 - m2) / 2 + k};
+        bool ok = true;
+        for (int q = 0; q < 6; q++) ok = ok && e[q] >= 0;
+        if (!ok) continue;
+        __float128 den = 1;
+        for (int q = 0; q < 6; q++) den *= qfact(e[q]);
+        s += ((k & 1) ? -1 : 1) / den;
+    }
+    if (s == 0) return 0.0;
+    __float128 r = sqrtq(pref) * s;
+    return (double)r;
+}
+
+// ------------------------------------------------------------------ basis
+int Species::set_of_M(int M) const {
+    auto it = std::lower_bound(Ms.begin(), Ms.end(), M);
+    if (it == Ms.end() || *it != M) return -1;
+    return (int)(it - Ms.begin());
+}
+
+int64_t Species::find(int set, uint64_t sd) const {
+    const auto& v = sets[set];
+    int64_t lo = 0, hi = (int64_t)v.size() - 1;   // descending order
+    while (lo <= hi) {
+        int64_t mid = (lo + hi) / 2;
+        if (v[mid] > sd) lo = mid + 1;
+        else if (v[mid] < sd) hi = mid - 1;
+        else return mid;
+    }
+    return -1;
+}
+
+void build_species(Species& s, const std::vector<Orbit>& sps, int nv) {
+    s.orb.clear(); s.m2.clear(); s.j2.clear();
+    for (int o = 0; o < (int)sps.size(); o++)
+        for (int m = -sps[o].j2; m <= sps[o].j2; m += 2) {
+            s.orb.push_back(o); s.m2.push_back(m); s.j2.push_back(sps[o].j2);
+        }
+    s.L = (int)s.orb.size();
+    s.nv = nv;
+    s.norb = (int)sps.size();
+    if (s.L > 63) throw std::runtime_error("more than 63 m-states per species");
+    if (nv < 0 || nv > s.L) throw std::runtime_error("valence particle count out of range");
+    // lexicographic combinations of m-state indices = descending bit integers
+    std::map<int, std::vector<uint64_t>> byM;
+    std::vector<int> comb(nv);
+    for (int i = 0; i < nv; i++) comb[i] = i;
+    for (;;) {
+        uint64_t bits = 0;
+        int M = 0;
+        for (int k : comb) { bits |= 1ull << (s.L - 1 - k); M += s.m2[k]; }
+        byM[M].push_back(bits);
+        int i = nv - 1;
+        while (i >= 0 && comb[i] == s.L - nv + i) i--;
+        if (i < 0) break;
+        comb[i]++;
+        for (int k = i + 1; k < nv; k++) comb[k] = comb[k - 1] + 1;
+    }
+    s.Ms.clear(); s.sets.clear(); s.jocc.clear();
+    for (auto& kv : byM) {
+        s.Ms.push_back(kv.first);
+        s.sets.push_back(kv.second);
+        std::vector<double> jo(kv.second.size() * s.norb, 0.0);
+        for (size_t i = 0; i < kv.second.size(); i++)
+            for (int k = 0; k < s.L; k++)
+                if (kv.second[i] >> (s.L - 1 - k) & 1ull) jo[i * s.norb + s.orb[k]] += 1.0;
+        s.jocc.push_back(std::move(jo));
+    }
+}
+
+Basis build_basis(const Snt& snt, int vp, int vn, int twoM) {
+    Basis b;
+    build_species(b.sp[0], snt.p_sps, vp);
+    build_species(b.sp[1], snt.n_sps, vn);
+    b.twoM = twoM;
+    b.tdims.push_back(0);
+    for (int i = 0; i < (int)b.sp[0].Ms.size(); i++) {     // Mp ascending (occ :802-811)
+        int Mp = b.sp[0].Ms[i];
+        int j = b.sp[1].set_of_M(twoM - Mp);
+        if (j < 0) continue;
+        b.Mp.push_back(Mp); b.Mn.push_back(twoM - Mp);
+        b.pset.push_back(i); b.nset.push_back(j);
+        b.tdims.push_back(b.tdims.back() + (int64_t)b.sp[0].sets[i].size() * (int64_t)b.sp[1].sets[j].size());
+    }
+    b.nb = (int)b.Mp.size();
+    b.mdim = b.tdims.back();
+    return b;
+}
+
+// ------------------------------------------------------------------ operator lists
+namespace {
+
+struct Op4 { uint64_t a, b, c, d; };
+struct Op4Hash {
+    size_t operator()(const std::array<uint64_t, 4>& k) const {
+        uint64_t h = k[0] * 0x9E3779B97F4A7C15ull ^ (k[1] + 0x632BE59BD9B4E019ull + (h << 6));
+        h ^= k[2] * 0xC2B2AE3D27D4EB4Full + (h >> 3);
+        h ^= k[3] * 0x165667B19E3779F9ull + (h << 7);
+        return (size_t)h;
+    }
+};
+
+struct OpList {
+    std::vector<Op4> ops;
+    std::vector<double> V;
+    std::unordered_map<std::array<uint64_t, 4>, size_t, Op4Hash> seen;
+    void add(uint64_t a, uint64_t b, uint64_t c, uint64_t d, double v) {
+        std::array<uint64_t, 4> k{a, b, c, d};
+        auto it = seen.find(k);
+        if (it == seen.end()) { seen.emplace(k, ops.size()); ops.push_back({a, b, c, d}); V.push_back(v); }
+        else V[it->second] += v;
+    }
+};
+
+// m-state indices of orbit `o` in ascending m (possible_mz, :552-565)
+std::vector<int> mstates_of(const Species& s, int o) {
+    std::vector<int> r;
+    for (int k = 0; k < s.L; k++) if (s.orb[k] == o) r.push_back(k);
+    return r;
+}
+
+inline uint64_t bit_of(const Species& s, int k) { return 1ull << (s.L - 1 - k); }
+
+// like-particle two-body operators of one species (prep_Hamil_T1 :580-649)
+OpList t1_ops(const Snt& snt, const Species& s, int vr) {
+    OpList L;
+    int lo = vr == 0 ? 0 : snt.lp;
+    for (size_t q = 0; q < snt.labels[vr].size(); q++) {
+        const auto& lab = snt.labels[vr][q];
+        double ME = snt.tbmes[vr][q];
+        int a = lab[0] - lo - 1, b = lab[1] - lo - 1, c = lab[2] - lo - 1, d = lab[3] - lo - 1;
+        int J2 = 2 * lab[4];
+        auto A = mstates_of(s, a), B = mstates_of(s, b), C = mstates_of(s, c), D = mstates_of(s, d);
+        int ja = s.j2[A[0]], jb = s.j2[B[0]], jc = s.j2[C[0]], jd = s.j2[D[0]];
+        double nfac = std::sqrt((1.0 + (a == b ? 1.0 : 0.0)) * (1.0 + (c == d ? 1.0 : 0.0)));
+        for (int kc : C)
+            for (int kd : D) {
+                int mc = s.m2[kc], md = s.m2[kd];
+                if (c == d && mc >= md) continue;
+                if (std::abs(mc + md) > J2) continue;
+                int M = mc + md;
+                for (int ka : A)
+                    for (int kb : B) {
+                        int ma = s.m2[ka], mb = s.m2[kb];
+                        if (a == b && ma >= mb) continue;
+                        if (ma + mb != M) continue;
+                        double cg1 = clebsch_gordan(ja, ma, jb, mb, J2, M);
+                        double cg2 = clebsch_gordan(jc, mc, jd, md, J2, M);
+                        double v = ME * nfac * cg1 * cg2;
+                        L.add(bit_of(s, ka), bit_of(s, kb), bit_of(s, kc), bit_of(s, kd), v);
+                    }
+            }
+    }
+    return L;
+}
+
+// pn operators per proton dM (prep_Hamil_pn :658-730); key is dM
+std::map<int, OpList> pn_ops(const Snt& snt, const Species& P, const Species& N) {
+    std::map<int, OpList> out;
+    for (size_t q = 0; q < snt.labels[2].size(); q++) {
+        const auto& lab = snt.labels[2][q];
+        double ME = snt.tbmes[2][q];
+        int a = lab[0] - 1, c = lab[2] - 1, b = lab[1] - snt.lp - 1, d = lab[3] - snt.lp - 1;
+        int J2 = 2 * lab[4];
+        auto A = mstates_of(P, a), C = mstates_of(P, c), B = mstates_of(N, b), D = mstates_of(N, d);
+        int ja = P.j2[A[0]], jc = P.j2[C[0]], jb = N.j2[B[0]], jd = N.j2[D[0]];
+        for (int kc : C) {
+            int mc = P.m2[kc];
+            for (int ka : A) {
+                int ma = P.m2[ka];
+                int Mp = ma - mc;
+                OpList& L = out[Mp];
+                for (int kd : D) {
+                    int md = N.m2[kd];
+                    if (std::abs(mc + md) > J2) continue;
+                    double cg1 = clebsch_gordan(jc, mc, jd, md, J2, mc + md);
+                    double tfac = ME * cg1;
+                    for (int kb : B) {
+                        int mb = N.m2[kb];
+                        if (mb - md + Mp != 0) continue;
+                        double fac = tfac * clebsch_gordan(ja, ma, jb, mb, J2, ma + mb);
+                        L.add(bit_of(P, ka), bit_of(N, kb), bit_of(P, kc), bit_of(N, kd), fac);
+                    }
+                }
+            }
+        }
+    }
+    return out;
+}
+
+inline int popc(uint64_t x) { return __builtin_popcountll(x); }
+inline uint64_t at_or_above(uint64_t bit) { return ~(bit - 1); }   // bits of equal or higher significance
+
+// a+ b+ d c on sd: returns false if it vanishes; sign exponent parity in *neg
+bool apply2(uint64_t sd, const Op4& o, uint64_t* out, int* neg) {
+    if ((sd & o.c) == 0 || (sd & o.d) == 0) return false;
+    uint64_t rest = sd & ~o.c & ~o.d;
+    if ((rest & o.a) || (rest & o.b)) return false;
+    int e = popc(sd & at_or_above(o.c));
+    uint64_t x = sd & ~o.c;
+    e += popc(x & at_or_above(o.d));
+    x &= ~o.d;
+    e += popc(x & at_or_above(o.b));
+    x |= o.b;
+    e += popc(x & at_or_above(o.a));
+    x |= o.a;
+    *out = x;
+    *neg = e & 1;
+    return true;
+}
+
+}  // namespace
+
+int OneBodyOps::index_of_dM(int dM) const {
+    auto it = std::lower_bound(dMs.begin(), dMs.end(), dM);
+    if (it == dMs.end() || *it != dM) return -1;
+    return (int)(it - dMs.begin());
+}
+
+static void build_onebody_ops(const Species& s, OneBodyOps& ob) {
+    std::map<int, std::vector<std::pair<int, int>>> m;
+    for (int a = 0; a < s.L; a++)
+        for (int c = 0; c < s.L; c++) m[s.m2[a] - s.m2[c]].push_back({a, c});
+    ob.dMs.clear(); ob.ops.clear();
+    for (auto& kv : m) { ob.dMs.push_back(kv.first); ob.ops.push_back(kv.second); }
+}
+
+// gather-form hop table: for each f in fset, inputs i = c+ a f  (so that a+ c i = +-f)
+static HopTable build_hops(const Species& s, const OneBodyOps& ob, int sp, int iset, int fset) {
+    HopTable h;
+    h.species = sp; h.iset = iset; h.fset = fset;
+    h.dM = s.Ms[fset] - s.Ms[iset];
+    int k = ob.index_of_dM(h.dM);
+    const auto& F = s.sets[fset];
+    h.rowptr.assign(F.size() + 1, 0);
+    if (k < 0) return h;
+    const auto& ops = ob.ops[k];
+    for (size_t f = 0; f < F.size(); f++) {
+        uint64_t sdf = F[f];
+        for (size_t q = 0; q < ops.size(); q++) {
+            uint64_t ba = bit_of(s, ops[q].first), bc = bit_of(s, ops[q].second);
+            if ((sdf & ba) == 0) continue;
+            uint64_t mid = sdf & ~ba;
+            if (mid & bc) continue;
+            uint64_t sdi = mid | bc;
+            int64_t i = s.find(iset, sdi);
+            if (i < 0) continue;
+            // sign of a+ c acting on sdi (calc_phase_1! without its constant
+            // extra factor, which cancels between the proton and neutron parts)
+            int e = popc(sdi & at_or_above(bc));
+            uint64_t x = sdi & ~bc;
+            e += popc(x & at_or_above(ba));
+            if ((uint64_t)i >= (1u << 20) || q >= (1u << 11))
+                throw std::runtime_error("hop table index exceeds the packed entry range");
+            h.ent.push_back((uint32_t)i | ((uint32_t)q << 20) | ((uint32_t)(e & 1) << 31));
+        }
+        h.rowptr[f + 1] = (int64_t)h.ent.size();
+    }
+    return h;
+}
+
+static PairSym build_pair_matrix(const Species& s, int set, const OpList& L) {
+    const auto& S = s.sets[set];
+    size_t n = S.size();
+    std::vector<std::map<int32_t, double>> rows(n);
+    for (size_t i = 0; i < n; i++)
+        for (size_t q = 0; q < L.ops.size(); q++) {
+            if (L.V[q] == 0.0) continue;
+            uint64_t fsd; int neg;
+            if (!apply2(S[i], L.ops[q], &fsd, &neg)) continue;
+            int64_t f = s.find(set, fsd);
+            if (f < 0 || (int64_t)i > f) continue;          // entries kept only for i <= f
+            double coef = L.V[q] * (neg ? -1.0 : 1.0) * ((int64_t)i == f ? 0.5 : 1.0);
+            // the reference applies every entry in both directions
+            rows[f][(int32_t)i] += coef;
+            rows[i][(int32_t)f] += coef;
+        }
+    PairSym m;
+    m.rowptr.assign(n + 1, 0);
+    for (size_t r = 0; r < n; r++) {
+        for (auto& kv : rows[r]) { m.col.push_back(kv.first); m.val.push_back(kv.second); }
+        m.rowptr[r + 1] = (int64_t)m.col.size();
+    }
+    return m;
+}
+
+void build_coefficients(const Snt& snt, const Basis& b, Tables& t, int nthreads) {
+    const Species& P = b.sp[0];
+    const Species& N = b.sp[1];
+    // one-body energies (sum over orbits in orbit order, as dot(SPEs, jocc))
+    t.ep.assign(P.sets.size(), {});
+    t.en.assign(N.sets.size(), {});
+    for (size_t k = 0; k < P.sets.size(); k++) {
+        t.ep[k].resize(P.sets[k].size());
+        for (size_t i = 0; i < P.sets[k].size(); i++) {
+            double e = 0.0;
+            for (int o = 0; o < P.norb; o++) e += snt.spe_p[o] * P.jocc[k][i * P.norb + o];
+            t.ep[k][i] = e;
+        }
+    }
+    for (size_t k = 0; k < N.sets.size(); k++) {
+        t.en[k].resize(N.sets[k].size());
+        for (size_t i = 0; i < N.sets[k].size(); i++) {
+            double e = 0.0;
+            for (int o = 0; o < N.norb; o++) e += snt.spe_n[o] * N.jocc[k][i * N.norb + o];
+            t.en[k][i] = e;
+        }
+    }
+    OpList pp = t1_ops(snt, P, 0), nn = t1_ops(snt, N, 1);
+    t.n_pp_ops = (int64_t)pp.ops.size();
+    t.n_nn_ops = (int64_t)nn.ops.size();
+    t.pp.assign(P.sets.size(), {});
+    t.nn.assign(N.sets.size(), {});
+    // only M-sets that appear in some block are needed
+    std::vector<char> pneed(P.sets.size(), 0), nneed(N.sets.size(), 0);
+    for (int k = 0; k < b.nb; k++) { pneed[b.pset[k]] = 1; nneed[b.nset[k]] = 1; }
+    int64_t np = (int64_t)P.sets.size();
+    parallel_for(np + (int64_t)N.sets.size(), nthreads, [&](int64_t q) {
+        if (q < np) { if (pneed[q]) t.pp[q] = build_pair_matrix(P, (int)q, pp); }
+        else { int64_t r = q - np; if (nneed[r]) t.nn[r] = build_pair_matrix(N, (int)r, nn); }
+    });
+    // V_eff
+    auto pn = pn_ops(snt, P, N);
+    int64_t npn = 0;
+    for (auto& kv : pn) npn += (int64_t)kv.second.ops.size();
+    t.n_pn_ops = npn;
+    const OneBodyOps& obp = t.ob[0];
+    const OneBodyOps& obn = t.ob[1];
+    size_t nd = obp.dMs.size();
+    t.veff.assign(nd, {});
+    t.veff_rows.assign(nd, 0);
+    t.veff_cols.assign(nd, 0);
+    for (size_t k = 0; k < nd; k++) {
+        int dM = obp.dMs[k];
+        int kn = obn.index_of_dM(-dM);
+        int rows = (int)obp.ops[k].size();
+        int cols = kn >= 0 ? (int)obn.ops[kn].size() : 0;
+        t.veff_rows[k] = rows;
+        t.veff_cols[k] = cols;
+        t.veff[k].assign((size_t)rows * cols, 0.0);
+        if (cols == 0) continue;
+        // lookup of the reference operator coefficient:
+        //   dM > 0 : forward operator (a', b', c', d')             (pn2 of operate_H!)
+        //   dM <= 0: transposed operator (c', d', a', b') of -dM   (pn1 of operate_H!)
+        int src = dM > 0 ? dM : -dM;
+        auto it = pn.find(src);
+        if (it == pn.end()) continue;
+        const OpList& L = it->second;
+        for (int r = 0; r < rows; r++) {
+            uint64_t pa = bit_of(P, obp.ops[k][r].first), pc = bit_of(P, obp.ops[k][r].second);
+            for (int c = 0; c < cols; c++) {
+                uint64_t nb_ = bit_of(N, obn.ops[kn][c].first), nd_ = bit_of(N, obn.ops[kn][c].second);
+                std::array<uint64_t, 4> key = dM > 0 ? std::array<uint64_t, 4>{pa, nb_, pc, nd_}
+                                                     : std::array<uint64_t, 4>{pc, nd_, pa, nb_};
+                auto f = L.seen.find(key);
+                if (f != L.seen.end()) t.veff[k][(size_t)r * cols + c] = L.V[f->second];
+            }
+        }
+    }
+}
+
+Tables build_tables(const Snt& snt, const Basis& b, int nthreads) {
+    Tables t;
+    build_onebody_ops(b.sp[0], t.ob[0]);
+    build_onebody_ops(b.sp[1], t.ob[1]);
+    // ordered pairs of blocks (output bf, input bi) coupled by a proton one-body dM
+    std::vector<std::array<int, 3>> want;   // (species, iset, fset)
+    for (int bf = 0; bf < b.nb; bf++)
+        for (int bi = 0; bi < b.nb; bi++) {
+            int dM = b.Mp[bf] - b.Mp[bi];
+            if (t.ob[0].index_of_dM(dM) < 0 || t.ob[1].index_of_dM(-dM) < 0) continue;
+            want.push_back({0, b.pset[bi], b.pset[bf]});
+            want.push_back({1, b.nset[bi], b.nset[bf]});
+        }
+    std::sort(want.begin(), want.end());
+    want.erase(std::unique(want.begin(), want.end()), want.end());
+    t.hops.resize(want.size());
+    parallel_for((int64_t)want.size(), nthreads, [&](int64_t q) {
+        const auto& w = want[q];
+        t.hops[q] = build_hops(b.sp[w[0]], t.ob[w[0]], w[0], w[1], w[2]);
+    });
+    for (size_t q = 0; q < want.size(); q++) t.hop_index[want[q]] = (int)q;
+    build_coefficients(snt, b, t, nthreads);
+    return t;
+}
+
+}  // namespace smgpu

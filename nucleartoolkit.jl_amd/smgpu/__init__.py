@@ -1,0 +1,6 @@
+"""smgpu — MI355X-native M-scheme shell-model eigensolver (host-side API).
+
+The compute path is libsmgpu.so (HIP, gfx950); see include/smgpu.h.
+"""
+from ._lib import SMGPU_FP64, SMGPU_MIXED, SmgpuError, lib  # noqa: F401
+from .shellmodel import Hamiltonian, Problem, main_sm  # noqa: F401

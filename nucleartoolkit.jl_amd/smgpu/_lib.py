@@ -1,0 +1,103 @@
+"""ctypes binding of libsmgpu.so (include/smgpu.h).
+
+The library is loaded from the in-tree build directory; a missing library is an
+error (there is no CPU fallback for any compute entry point).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_ROOT, "build", "libsmgpu.so")
+
+SMGPU_OK = 0
+SMGPU_FP64 = 0
+SMGPU_MIXED = 1
+
+ERRORS = {-1: "EINVAL", -2: "EIO", -3: "EDEVICE", -4: "ENOMEM", -5: "ENOTCONV", -6: "ECANCELED"}
+
+
+class SmgpuError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"smgpu {ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class ProblemInfo(ctypes.Structure):
+    _fields_ = [("Z", ctypes.c_int32), ("N", ctypes.c_int32), ("A", ctypes.c_int32), ("vp", ctypes.c_int32),
+                ("vn", ctypes.c_int32), ("twoM", ctypes.c_int32), ("lp", ctypes.c_int32), ("ln", ctypes.c_int32),
+                ("cp", ctypes.c_int32), ("cn", ctypes.c_int32), ("nblocks", ctypes.c_int32),
+                ("n_msps_p", ctypes.c_int32), ("n_msps_n", ctypes.c_int32), ("mdim", ctypes.c_int64),
+                ("n_psd", ctypes.c_int64), ("n_nsd", ctypes.c_int64), ("n_pp_ops", ctypes.c_int64),
+                ("n_nn_ops", ctypes.c_int64), ("n_pn_ops", ctypes.c_int64), ("n_hop_entries", ctypes.c_int64)]
+
+
+class SolveParams(ctypes.Structure):
+    _fields_ = [("num_ev", ctypes.c_int32), ("lm", ctypes.c_int32), ("ls", ctypes.c_int32),
+                ("num_history", ctypes.c_int32), ("tol", ctypes.c_double), ("precision", ctypes.c_int32),
+                ("refine", ctypes.c_int32), ("max_iter", ctypes.c_int32), ("seed", ctypes.c_uint64)]
+
+
+class SolveStats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int32), ("n_hv", ctypes.c_int32), ("n_hv64", ctypes.c_int32),
+                ("n_restart", ctypes.c_int32), ("wall_ms", ctypes.c_double), ("ritz", ctypes.c_double * 64)]
+
+
+# exported symbol -> (restype, argtypes)
+P = ctypes.c_void_p
+I32 = ctypes.c_int32
+I64 = ctypes.c_int64
+SIGNATURES = {
+    "smgpu_version": (I32, []),
+    "smgpu_last_error": (ctypes.c_char_p, []),
+    "smgpu_device_count": (I32, [P]),
+    "smgpu_problem_create": (I32, [ctypes.c_char_p, ctypes.c_char_p, I32, I32, I32, P]),
+    "smgpu_problem_destroy": (I32, [P]),
+    "smgpu_problem_get_info": (I32, [P, P]),
+    "smgpu_problem_set_interaction": (I32, [P, ctypes.c_char_p]),
+    "smgpu_basis_export": (I32, [P, P, P, P, P, P, P, P]),
+    "smgpu_hops_export": (I32, [P, P, P, P, P, P, P, P, P, P]),
+    "smgpu_pair_export": (I32, [P, I32, P, P, P, P, P]),
+    "smgpu_ham_create": (I32, [P, I32, P]),
+    "smgpu_ham_destroy": (I32, [P]),
+    "smgpu_ham_update_interaction": (I32, [P, P]),
+    "smgpu_table_bytes": (I32, [P, I32, I32, P, P, ctypes.c_char_p, I32]),
+    "smgpu_work_terms": (I32, [P, P]),
+    "smgpu_apply_H": (I32, [P, I32, P, P, P]),
+    "smgpu_apply_H_host": (I32, [P, I32, P, P]),
+    "smgpu_time_apply_H": (I32, [P, I32, I32, P]),
+    "smgpu_default_params": (None, [P]),
+    "smgpu_trl_solve": (I32, [P, P, P, P, P]),
+    "smgpu_malloc": (I32, [P, I64, P]),
+    "smgpu_free": (I32, [P, P]),
+    "smgpu_memcpy": (I32, [P, P, P, I64, I32]),
+    "smgpu_synchronize": (I32, [P]),
+    "smgpu_selftest_tridiag": (I32, [P, P, I32, I32, P, P]),
+}
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libsmgpu.so not built ({LIB_PATH}); run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def check(code):
+    if code != SMGPU_OK:
+        raise SmgpuError(code, lib().smgpu_last_error().decode(errors="replace"))
+    return code
+
+
+def ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)

@@ -1,0 +1,182 @@
+"""Host-side mirror of NuclearToolkit.jl's shell-model entry points over libsmgpu.
+
+* `main_sm(sntf, target_nuc, num_ev, target_J; lm, ls, tol, ...)` mirrors
+  src/ShellModel/shellmodel_main.jl:89-281 for the eigenvalue observable
+  (`en[1]`), with the H·v and Lanczos on the GPU.
+* `Problem` is the host part (readsmsnt -> occ -> operator and hop tables),
+  `Hamiltonian` the device-resident tables; `Hamiltonian.operate_H` mirrors
+  operate_H! (src/ShellModel/lanczos_methods.jl:153-248) as w = H v and
+  `Hamiltonian.trl` mirrors TRL (lanczos_methods.jl:1-91).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from ._lib import (SMGPU_FP64, SMGPU_MIXED, ProblemInfo, SolveParams, SolveStats, check, lib, ptr)
+
+PRECISIONS = {"fp64": SMGPU_FP64, "mixed": SMGPU_MIXED}
+
+
+def _prec(p):
+    if isinstance(p, str):
+        return PRECISIONS[p]
+    return int(p)
+
+
+class Problem:
+    """Interaction + M-scheme basis + host tables for one nucleus and 2M."""
+
+    def __init__(self, sntf, target_nuc, twoM=-1, ignore_massop=False, nthreads=0):
+        h = ctypes.c_void_p()
+        check(lib().smgpu_problem_create(os.fsencode(sntf), target_nuc.encode(), int(twoM),
+                                         int(bool(ignore_massop)), int(nthreads), ctypes.byref(h)))
+        self._h = h
+        self.sntf = sntf
+        self.nucleus = target_nuc
+        self.info = self.get_info()
+
+    def get_info(self):
+        info = ProblemInfo()
+        check(lib().smgpu_problem_get_info(self._h, ctypes.byref(info)))
+        return info
+
+    @property
+    def mdim(self):
+        return int(self.info.mdim)
+
+    def set_interaction(self, sntf):
+        check(lib().smgpu_problem_set_interaction(self._h, os.fsencode(sntf)))
+        self.sntf = sntf
+
+    def basis(self):
+        """dict(tdims, Mp, Mn, pbits[list per block], nbits[list per block])"""
+        i = self.info
+        nb = i.nblocks
+        tdims = np.zeros(nb + 1, np.int64)
+        Mp = np.zeros(nb, np.int32)
+        Mn = np.zeros(nb, np.int32)
+        pb = np.zeros(max(1, i.n_psd), np.int64)
+        nbits = np.zeros(max(1, i.n_nsd), np.int64)
+        po = np.zeros(nb + 1, np.int64)
+        no = np.zeros(nb + 1, np.int64)
+        check(lib().smgpu_basis_export(self._h, ptr(tdims), ptr(Mp), ptr(Mn), ptr(pb), ptr(po), ptr(nbits),
+                                       ptr(no)))
+        return dict(tdims=tdims, Mp=Mp, Mn=Mn,
+                    pbits=[pb[po[k]:po[k + 1]] for k in range(nb)],
+                    nbits=[nbits[no[k]:no[k + 1]] for k in range(nb)])
+
+    def hops(self):
+        n = ctypes.c_int64()
+        L = lib()
+        check(L.smgpu_hops_export(self._h, ctypes.byref(n), None, None, None, None, None, None, None, None))
+        m = n.value
+        arrs = [np.zeros(m, np.int32) for _ in range(7)]
+        ph = np.zeros(m, np.int8)
+        check(L.smgpu_hops_export(self._h, ctypes.byref(n), *[ptr(a) for a in arrs], ptr(ph)))
+        keys = ["species", "bi", "bf", "i", "f", "a", "c"]
+        out = dict(zip(keys, arrs))
+        out["phase"] = ph
+        return out
+
+    def pair_matrix(self, species):
+        n = ctypes.c_int64()
+        L = lib()
+        check(L.smgpu_pair_export(self._h, species, ctypes.byref(n), None, None, None, None))
+        m = n.value
+        b = np.zeros(m, np.int32); f = np.zeros(m, np.int32); i = np.zeros(m, np.int32)
+        v = np.zeros(m, np.float64)
+        check(L.smgpu_pair_export(self._h, species, ctypes.byref(n), ptr(b), ptr(f), ptr(i), ptr(v)))
+        return b, f, i, v
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().smgpu_problem_destroy(h)
+            self._h = None
+
+
+class Hamiltonian:
+    """Device-resident H for a Problem on one GPU."""
+
+    def __init__(self, problem: Problem, device=0):
+        h = ctypes.c_void_p()
+        check(lib().smgpu_ham_create(problem._h, int(device), ctypes.byref(h)))
+        self._h = h
+        self.problem = problem
+        self.mdim = problem.mdim
+
+    def update_interaction(self, problem: Problem | None = None):
+        check(lib().smgpu_ham_update_interaction(self._h, (problem or self.problem)._h))
+
+    def operate_H(self, v, precision="fp64"):
+        """w = H v (host float64 in/out, computed on the device in `precision`)."""
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        assert v.shape == (self.mdim,)
+        w = np.zeros(self.mdim, np.float64)
+        check(lib().smgpu_apply_H_host(self._h, _prec(precision), ptr(v), ptr(w)))
+        return w
+
+    def time_operate_H(self, precision="mixed", reps=20):
+        ms = ctypes.c_double()
+        check(lib().smgpu_time_apply_H(self._h, _prec(precision), int(reps), ctypes.byref(ms)))
+        return ms.value
+
+    def table_bytes(self, precision="mixed"):
+        b = np.zeros(64, np.int64)
+        n = ctypes.c_int32()
+        buf = ctypes.create_string_buffer(4096)
+        check(lib().smgpu_table_bytes(self._h, _prec(precision), 64, ptr(b), ctypes.byref(n), buf, 4096))
+        names = buf.value.decode().strip().split("\n")
+        return dict(zip(names, [int(x) for x in b[:n.value]]))
+
+    def work_terms(self):
+        t = np.zeros(5, np.int64)
+        check(lib().smgpu_work_terms(self._h, ptr(t)))
+        return dict(pn_padded=int(t[0]), pn=int(t[1]), pp=int(t[2]), nn=int(t[3]), diag=int(t[4]))
+
+    def trl(self, num_ev, lm=300, ls=20, tol=1e-8, num_history=3, precision="fp64", refine=True, seed=1234,
+            want_vectors=False, max_iter=0):
+        """Thick-restart Lanczos; returns (evals, evecs or None, stats dict)."""
+        p = SolveParams()
+        lib().smgpu_default_params(ctypes.byref(p))
+        p.num_ev = int(num_ev); p.lm = int(lm); p.ls = int(ls); p.tol = float(tol)
+        p.num_history = int(num_history); p.precision = _prec(precision); p.refine = int(bool(refine))
+        p.seed = int(seed); p.max_iter = int(max_iter)
+        nev = min(int(num_ev), self.mdim)
+        ev = np.zeros(nev, np.float64)
+        vec = np.zeros((nev, self.mdim), np.float64) if want_vectors else None
+        st = SolveStats()
+        check(lib().smgpu_trl_solve(self._h, ctypes.byref(p), ptr(ev), ptr(vec) if vec is not None else None,
+                                    ctypes.byref(st)))
+        stats = dict(iterations=st.iterations, n_hv=st.n_hv, n_hv64=st.n_hv64, n_restart=st.n_restart,
+                     wall_ms=st.wall_ms, ritz=np.array(st.ritz[:nev]))
+        return ev, vec, stats
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().smgpu_ham_destroy(h)
+            self._h = None
+
+
+def main_sm(sntf, target_nuc, num_ev, target_J=(), *, lm=300, ls=20, tol=1e-8, num_history=3,
+            precision="fp64", refine=True, device=0, seed=1234, return_stats=False):
+    """Lowest `num_ev` eigenvalues (MeV) of `target_nuc` with the interaction `sntf`.
+
+    Mirrors main_sm (shellmodel_main.jl:89-281) with target_J = [] (plain TRL):
+    2M = A % 2 (:110); mdim == 0 returns an empty array (:127); num_ev is
+    clamped to mdim (:128).  J projection is not part of this path.
+    """
+    if len(target_J) > 0:
+        raise NotImplementedError("J-projected Lanczos (target_J != []) is not built")
+    prob = Problem(sntf, target_nuc)
+    if prob.mdim == 0:
+        return (np.zeros(0), {}) if return_stats else np.zeros(0)
+    ham = Hamiltonian(prob, device)
+    nev = min(int(num_ev), prob.mdim)
+    ev, _, st = ham.trl(nev, lm=lm, ls=ls, tol=tol, num_history=num_history, precision=precision,
+                        refine=refine, seed=seed)
+    return (ev, st) if return_stats else ev

@@ -1,0 +1,115 @@
+"""GPU parity: H·v and TRL through the C ABI against the oracle.
+
+Tolerances: FP64 H·v relative max-norm error <= 1e-12; FP32 H·v <= 1e-5 of
+|H v|_inf; eigenvalues vs oracle / dense <= 1e-7 MeV (FP64 path) and
+<= 1e-5 MeV (mixed path, the target tolerance of the north-star metric).
+"""
+import numpy as np
+import pytest
+from scipy.sparse.linalg import LinearOperator, eigsh
+
+import brute_force as bf
+import sm_oracle as orc
+import smgpu
+from smgpu import synth
+
+pytestmark = pytest.mark.gpu
+
+CKPOT_BE8 = [-31.119, -27.300, -19.162, -18.249, -16.722, -14.925, -14.517, -14.017, -13.951, -13.478]
+
+
+def _cases(golden, snt_dir):
+    c1, _ = synth.config_snt(f"{snt_dir}/c1.snt", "sd", 4, 4, synth.BASE_SEED + 1, (-4, 0), 1.5)
+    pfs, _ = synth.config_snt(f"{snt_dir}/pf23.snt", "pf", 2, 3, 99, (-8, 0), 1.0)
+    return [(f"{golden}/ckpot.snt", "Be8"), (f"{golden}/ckpot.snt", "Li7"), (f"{golden}/usdb.snt", "O20"),
+            (f"{golden}/usdb.snt", "Na23"), (c1, "Mg24"), (pfs, "V48")]
+
+
+def test_hv_parity_fp64_fp32(golden, snt_dir):
+    rng = np.random.default_rng(11)
+    for path, nuc in _cases(golden, snt_dir):
+        print("case", nuc, flush=True)
+        p = smgpu.Problem(path, nuc)
+        h = smgpu.Hamiltonian(p)
+        o = orc.OracleHamiltonian(path, nuc)
+        for trial in range(2):
+            v = rng.standard_normal(p.mdim)
+            ref = o.operate_H(v)
+            w64 = h.operate_H(v, "fp64")
+            w32 = h.operate_H(v, "mixed")
+            scale = np.abs(ref).max()
+            assert np.abs(w64 - ref).max() <= 1e-12 * scale, (nuc, np.abs(w64 - ref).max() / scale)
+            assert np.abs(w32 - ref).max() <= 1e-5 * scale, (nuc, np.abs(w32 - ref).max() / scale)
+
+
+def test_hv_parity_cr48(snt_dir):
+    path, nuc = synth.config_snt(f"{snt_dir}/c2.snt", "pf", 4, 4, synth.BASE_SEED + 2, (-8, 0), 1.0)
+    p = smgpu.Problem(path, nuc)
+    h = smgpu.Hamiltonian(p)
+    o = orc.OracleHamiltonian(path, nuc)
+    v = np.random.default_rng(3).standard_normal(p.mdim)
+    ref = o.operate_H(v)
+    w64 = h.operate_H(v, "fp64")
+    w32 = h.operate_H(v, "mixed")
+    scale = np.abs(ref).max()
+    assert np.abs(w64 - ref).max() <= 1e-12 * scale
+    assert np.abs(w32 - ref).max() <= 1e-5 * scale
+    # linearity and symmetry at full size: <u, H v> == <H u, v>
+    u = np.random.default_rng(4).standard_normal(p.mdim)
+    hu = h.operate_H(u, "fp64")
+    assert abs(u @ w64 - hu @ v) <= 1e-11 * abs(u) @ np.abs(w64)
+    hsum = h.operate_H(u + 2.0 * v, "fp64")
+    assert np.abs(hsum - (hu + 2.0 * w64)).max() <= 1e-11 * scale
+
+
+def test_dense_equals_brute_force(golden):
+    H, _ = bf.brute_hamiltonian(f"{golden}/ckpot.snt", "Be8", 4, 4)
+    p = smgpu.Problem(f"{golden}/ckpot.snt", "Be8")
+    h = smgpu.Hamiltonian(p)
+    G = np.zeros_like(H)
+    e = np.zeros(p.mdim)
+    for k in range(p.mdim):
+        e[:] = 0
+        e[k] = 1
+        G[:, k] = h.operate_H(e, "fp64")
+    assert np.abs(G - H).max() < 1e-12
+
+
+def test_trl_ckpot_be8(golden):
+    for prec in ("fp64", "mixed"):
+        en = smgpu.main_sm(f"{golden}/ckpot.snt", "Be8", 10, precision=prec)
+        assert np.all((np.array(CKPOT_BE8) - en) ** 2 < 1e-6), (prec, en)
+        o = orc.OracleHamiltonian(f"{golden}/ckpot.snt", "Be8")
+        ev = np.linalg.eigvalsh(o.dense())[:10]
+        assert np.abs(en - ev).max() < (1e-8 if prec == "fp64" else 1e-5)
+
+
+def test_trl_restarts_match_dense(golden):
+    o = orc.OracleHamiltonian(f"{golden}/usdb.snt", "Ne22")
+    ev = np.linalg.eigvalsh(o.dense())[:6]
+    p = smgpu.Problem(f"{golden}/usdb.snt", "Ne22")
+    h = smgpu.Hamiltonian(p)
+    for prec in ("fp64", "mixed"):
+        en, vec, st = h.trl(6, lm=40, ls=10, precision=prec, want_vectors=True)
+        assert st["n_restart"] > 0
+        assert np.abs(en - ev).max() < (1e-7 if prec == "fp64" else 1e-5), (prec, en - ev)
+        # Ritz vectors are unit eigenvectors of the oracle's H
+        for j in range(6):
+            r = o.operate_H(vec[j]) - ev[j] * vec[j]
+            assert abs(np.linalg.norm(vec[j]) - 1) < 1e-10
+            assert np.linalg.norm(r) < (1e-4 if prec == "fp64" else 1e-2)
+
+
+def test_trl_mg24_vs_arpack(snt_dir):
+    path, nuc = synth.config_snt(f"{snt_dir}/c1.snt", "sd", 4, 4, synth.BASE_SEED + 1, (-4, 0), 1.5)
+    o = orc.OracleHamiltonian(path, nuc)
+    op = LinearOperator((o.mdim, o.mdim), matvec=lambda x: o.operate_H(np.ascontiguousarray(x).ravel()),
+                        dtype=np.float64)
+    ev = np.sort(eigsh(op, k=5, which="SA", tol=1e-13, v0=np.ones(o.mdim))[0])
+    p = smgpu.Problem(path, nuc)
+    h = smgpu.Hamiltonian(p)
+    en64, _, st64 = h.trl(5, precision="fp64")
+    enmx, _, stmx = h.trl(5, precision="mixed")
+    print("fp64", en64 - ev, st64["iterations"], "mixed", enmx - ev, stmx["iterations"], flush=True)
+    assert np.abs(en64 - ev).max() < 1e-7
+    assert np.abs(enmx - ev).max() < 1e-5
