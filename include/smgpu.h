@@ -133,6 +133,15 @@ void smgpu_default_params(smgpu_solve_params* p);
 int32_t smgpu_trl_solve(smgpu_ham h, const smgpu_solve_params* params, double* evals, double* evecs,
                         smgpu_solve_stats* stats);
 
+/* Same as smgpu_trl_solve with a caller-supplied start vector v0[mdim] (main_sm's
+ * in_wf start, shellmodel_main.jl:194-198); v0 == NULL uses params->seed. */
+int32_t smgpu_trl_solve_v0(smgpu_ham h, const smgpu_solve_params* params, const double* v0, double* evals,
+                           double* evecs, smgpu_solve_stats* stats);
+/* nsteps Lanczos steps of the TRL recurrence from v0 (no convergence test or
+ * restart): alpha[nsteps], beta[nsteps] of the projected tridiagonal matrix. */
+int32_t smgpu_lanczos_steps(smgpu_ham h, int32_t precision, int32_t nsteps, const double* v0, double* alpha,
+                            double* beta);
+
 /* Device memory helpers for callers that keep vectors resident. */
 int32_t smgpu_malloc(smgpu_ham h, int64_t bytes, void** dev);
 int32_t smgpu_free(smgpu_ham h, void* dev);
@@ -143,6 +152,13 @@ int32_t smgpu_synchronize(smgpu_ham h);
  * source the device kernels use): lowest k eigenpairs of T. */
 int32_t smgpu_selftest_tridiag(const double* alpha, const double* beta, int32_t n, int32_t k, double* vals,
                                double* vecs);
+
+/* Self-test of the thick-restart small problem (tri_restart) for the tridiagonal
+ * T(alpha[m], beta[m]; beta[m-1] = coupling to the residual vector): the rotation
+ * S[ls*m] (column j at S + j*m) and the new leading block alpha_out[ls],
+ * beta_out[ls].  device < 0 runs the host build, otherwise the device kernel. */
+int32_t smgpu_selftest_restart(int32_t device, const double* alpha, const double* beta, int32_t m, int32_t ls,
+                               double* S, double* alpha_out, double* beta_out);
 
 #ifdef __cplusplus
 }

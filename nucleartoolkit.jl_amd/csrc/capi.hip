@@ -375,6 +375,38 @@ int32_t smgpu_trl_solve(smgpu_ham h, const smgpu_solve_params* params, double* e
     });
 }
 
+int32_t smgpu_trl_solve_v0(smgpu_ham h, const smgpu_solve_params* params, const double* v0, double* evals,
+                           double* evecs, smgpu_solve_stats* stats) {
+    return guard([&]() -> int32_t {
+        if (!h || !params || !evals) throw std::invalid_argument("null argument");
+        smgpu_solve_params p = *params;
+        if (p.max_iter <= 0) p.max_iter = 100000;
+        if (p.num_history < 2) p.num_history = 3;
+        if (p.num_ev > 64) throw std::invalid_argument("num_ev above 64");
+        smgpu_solve_stats st;
+        std::memset(&st, 0, sizeof(st));
+        p.num_ev = (int32_t)std::min<int64_t>(p.num_ev, h->dev->mdim);
+        trl_solve(*h->dev, p, evals, evecs, st, v0);
+        if (stats) *stats = st;
+        return SMGPU_OK;
+    });
+}
+
+int32_t smgpu_lanczos_steps(smgpu_ham h, int32_t precision, int32_t nsteps, const double* v0, double* alpha,
+                            double* beta) {
+    return guard([&]() -> int32_t {
+        if (!h || !alpha || !beta) throw std::invalid_argument("null argument");
+        smgpu_solve_params p;
+        smgpu_default_params(&p);
+        p.precision = precision;
+        p.lm = nsteps + 2;
+        p.ls = 1;
+        p.num_ev = 1;
+        lanczos_steps(*h->dev, p, nsteps, v0, alpha, beta);
+        return SMGPU_OK;
+    });
+}
+
 int32_t smgpu_malloc(smgpu_ham h, int64_t bytes, void** dev) {
     return guard([&]() -> int32_t {
         if (!h || !dev || bytes < 0) throw std::invalid_argument("bad argument");
@@ -420,6 +452,23 @@ int32_t smgpu_selftest_tridiag(const double* alpha, const double* beta, int32_t 
         std::vector<double> b(std::max(1, n));
         for (int i = 0; i < n - 1; i++) b[i] = beta[i];
         tri_lowest(alpha, b.data(), n, k, vals, vecs, work.data());
+        return SMGPU_OK;
+    });
+}
+
+int32_t smgpu_selftest_restart(int32_t device, const double* alpha, const double* beta, int32_t m, int32_t ls,
+                               double* S, double* alpha_out, double* beta_out) {
+    return guard([&]() -> int32_t {
+        if (!alpha || !beta || !S || !alpha_out || !beta_out || m < 2 || ls < 1 || ls >= m || ls > 32)
+            throw std::invalid_argument("bad argument");
+        if (device < 0) {
+            std::vector<double> a(alpha, alpha + m), b(beta, beta + m);
+            std::vector<double> work(5 * (size_t)m + (size_t)ls * (m + 3 * ls + 8) + 64);
+            tri_restart(a.data(), b.data(), m, ls, S, work.data());
+            for (int i = 0; i < ls; i++) { alpha_out[i] = a[i]; beta_out[i] = b[i]; }
+        } else {
+            restart_small_device(device, alpha, beta, m, ls, S, alpha_out, beta_out);
+        }
         return SMGPU_OK;
     });
 }

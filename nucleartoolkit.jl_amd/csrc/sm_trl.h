@@ -8,6 +8,10 @@ namespace smgpu {
 using SolveParams = smgpu_solve_params;
 using SolveStats = smgpu_solve_stats;
 
-void trl_solve(DeviceHam& h, const SolveParams& p, double* evals, double* evecs, SolveStats& st);
+void trl_solve(DeviceHam& h, const SolveParams& p, double* evals, double* evecs, SolveStats& st,
+               const double* v0 = nullptr);
+void restart_small_device(int device, const double* alpha, const double* beta, int m, int ls, double* S,
+                          double* alpha_out, double* beta_out);
+void lanczos_steps(DeviceHam& h, const SolveParams& p, int nsteps, const double* v0, double* alpha, double* beta);
 
 }  // namespace smgpu

@@ -4,14 +4,21 @@
 // diagonalize_T! (:250-258), ReORTH (src/ShellModel/shellmodel_main.jl:1005-1012),
 // ThickRestart (:260-307) and the Ritz-vector assembly of main_sm (:208-224).
 //
-// Per iteration (it = index of the current basis vector v):
-//   w  = H v                                   (FP32 or FP64 H·v kernel)
-//   h  = V[0..it]^T w;   w -= V h              (classical Gram-Schmidt, FP64 sums)
-//   h2 = V[0..it]^T w;   w -= V h2             (second pass: "twice is enough")
-//   alpha[it] = h[it]  (= v.Hv, the reference's dot before ReORTH)
-//   beta[it]  = |w|,  V[it+1] = w / beta
-//   Ritz values of the tridiagonal T[0..it] by multisection bisection (FP64),
-//   convergence flag: all(en_prev - en_new < tol), the reference's rule.
+// Per iteration (it = index of the current basis vector v = V[it]):
+//   w  = H v                                  (FP32 or FP64 H·v kernel)
+//   classical Gram-Schmidt twice against V[0..it] (FP64 sums): h1 = V^T w,
+//   w1 = w - V h1, h2 = V^T w1, w = w1 - V h2, in three streaming passes over
+//   the basis (the middle pass computes w1 and the h2 partials from one read
+//   of an LDS-staged slab).  One pass is not enough: it multiplies existing
+//   non-orthogonality by ~|h|/beta every step;
+//   alpha[it] = h[it] (= v.Hv, the reference's dot before ReORTH),
+//   beta[it] = |w|,  V[it+1] = w / beta
+//   Ritz values of the tridiagonal T[0..it] (multisection bisection in FP64,
+//   bracketed by the previous step's values through Cauchy interlacing) and the
+//   reference's convergence rule all(en_prev - en_new < tol).
+// The host enqueues iteration it+1 before it reads the flag of iteration it,
+// so the device never waits for the host; a speculative iteration after
+// convergence only writes entries that are never read.
 // Restart keeps T tridiagonal: the kept Ritz vectors are rotated within their
 // span (tri_restart) so the projected matrix stays tridiagonal; its eigenvalues
 // equal those of the reference's arrowhead Tmat.
@@ -23,6 +30,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <random>
 #include <stdexcept>
 
@@ -33,6 +42,9 @@
 namespace smgpu {
 
 static constexpr int TB = 256;
+static constexpr int EPL = 8;                 // elements per lane in the streaming passes
+static constexpr int CHUNK = 64 * EPL;        // x per block in k_dotsw
+static constexpr int UCHUNK = TB * 4;         // x per block in k_updatew
 
 __device__ inline double wave_sum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -51,62 +63,155 @@ __device__ inline double block_sum(double v, double* red) {
     return s;   // valid in thread 0
 }
 
-// part[blk*nv + j] = sum over the block's chunk of B[j][x] * w[x]
+// part[blk*ldp + j] = sum over the block's CHUNK of B[j][x] w[x]  (j < nv);
+// part[blk*ldp + nv] = sum of w^2 when with_norm.  One wave per j, no barriers.
 template <typename TBv, typename TW>
-__global__ __launch_bounds__(TB) void k_dots(const TBv* __restrict__ B, int64_t ldb, int nv,
-                                             const TW* __restrict__ w, int64_t n, int64_t chunk,
-                                             double* __restrict__ part) {
-    __shared__ double red[TB / 64];
-    const int64_t x0 = (int64_t)blockIdx.x * chunk;
-    const int64_t x1 = min(n, x0 + chunk);
-    for (int j = 0; j < nv; j++) {
-        const TBv* b = B + (int64_t)j * ldb;
+__global__ __launch_bounds__(TB) void k_dotsw(const TBv* __restrict__ B, int64_t ldb, int nv,
+                                              const TW* __restrict__ w, int64_t n, double* __restrict__ part,
+                                              int ldp, int with_norm, const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t x0 = (int64_t)blockIdx.x * CHUNK + lane;
+    double wr[EPL];
+#pragma unroll
+    for (int e = 0; e < EPL; e++) {
+        const int64_t x = x0 + 64 * e;
+        wr[e] = x < n ? (double)w[x] : 0.0;
+    }
+    const int nj = nv + (with_norm ? 1 : 0);
+    for (int j = wv; j < nj; j += TB / 64) {
         double s = 0.0;
-        for (int64_t x = x0 + threadIdx.x; x < x1; x += TB) s += (double)b[x] * (double)w[x];
-        s = block_sum(s, red);
-        if (threadIdx.x == 0) part[(int64_t)blockIdx.x * nv + j] = s;
+        if (j < nv) {
+            const TBv* b = B + (int64_t)j * ldb;
+#pragma unroll
+            for (int e = 0; e < EPL; e++) {
+                const int64_t x = x0 + 64 * e;
+                if (x < n) s += (double)b[x] * wr[e];
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < EPL; e++) s += wr[e] * wr[e];
+        }
+        s = wave_sum(s);
+        if (lane == 0) part[(int64_t)blockIdx.x * ldp + j] = s;
     }
 }
 
-// out[j] = sum_g part[g*nv + j]  (+ optional sqrt into out2 for j == 0)
-__global__ __launch_bounds__(TB) void k_reduce(const double* __restrict__ part, int G, int nv,
-                                               double* __restrict__ out) {
+// out[j] = sum_g part[g*ldp + j], j < nj
+__global__ __launch_bounds__(TB) void k_reduce(const double* __restrict__ part, int G, int ldp,
+                                               double* __restrict__ out, const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;
     __shared__ double red[TB / 64];
     const int j = blockIdx.x;
     double s = 0.0;
-    for (int g = threadIdx.x; g < G; g += TB) s += part[(int64_t)g * nv + j];
+    for (int g = threadIdx.x; g < G; g += TB) s += part[(int64_t)g * ldp + j];
     s = block_sum(s, red);
     if (threadIdx.x == 0) out[j] = s;
 }
 
-// wout[x] = win[x] - sum_j h[j] B[j][x];  part[blk] = sum wout^2 over the chunk
+// wout[x] = win[x] - sum_j h[j] B[j][x];  part[blk] = sum wout^2 over the block
 template <typename TBv, typename TW>
-__global__ __launch_bounds__(TB) void k_update(const TBv* __restrict__ B, int64_t ldb, int nv,
-                                               const double* __restrict__ h, const TW* __restrict__ win,
-                                               double* __restrict__ wout, int64_t n, int64_t chunk,
-                                               double* __restrict__ part) {
+__global__ __launch_bounds__(TB) void k_updatew(const TBv* __restrict__ B, int64_t ldb, int nv,
+                                                const double* __restrict__ h, const TW* __restrict__ win,
+                                                double* __restrict__ wout, int64_t n, double* __restrict__ part,
+                                                const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;
     __shared__ double hs[1024];
     __shared__ double red[TB / 64];
     for (int j = threadIdx.x; j < nv; j += TB) hs[j] = h[j];
     __syncthreads();
-    const int64_t x0 = (int64_t)blockIdx.x * chunk;
-    const int64_t x1 = min(n, x0 + chunk);
+    const int64_t x0 = (int64_t)blockIdx.x * UCHUNK + threadIdx.x;
+    double a[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        const int64_t x = x0 + TB * e;
+        a[e] = x < n ? (double)win[x] : 0.0;
+    }
+    for (int j = 0; j < nv; j++) {
+        const TBv* b = B + (int64_t)j * ldb;
+        const double c = hs[j];
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int64_t x = x0 + TB * e;
+            if (x < n) a[e] -= c * (double)b[x];
+        }
+    }
     double nr = 0.0;
-    for (int64_t x = x0 + threadIdx.x; x < x1; x += TB) {
-        double a = (double)win[x];
-        for (int j = 0; j < nv; j++) a -= hs[j] * (double)B[(int64_t)j * ldb + x];
-        wout[x] = a;
-        nr += a * a;
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        const int64_t x = x0 + TB * e;
+        if (x < n) { wout[x] = a[e]; nr += a[e] * a[e]; }
     }
     nr = block_sum(nr, red);
     if (threadIdx.x == 0) part[blockIdx.x] = nr;
 }
 
-// alpha/beta bookkeeping after the two Gram-Schmidt passes
-__global__ void k_finish(const double* h1, int it, const double* part, int G, double* alpha, double* beta) {
+// Middle pass of CGS2, fused: w1 = win - V h1 (written to wout) and the partial
+// dot products V^T w1 of the second pass, from one read of V.  Each block
+// stages X-wide slabs V[0..nv-1][x0..x0+X) in LDS (row stride X+1, no bank
+// conflicts); thread t owns the second-pass sums of j = t + 256 q.
+template <typename TBv, typename TW, int X>
+__global__ __launch_bounds__(TB) void k_cgs_mid(const TBv* __restrict__ B, int64_t ldb, int nv,
+                                                const double* __restrict__ h1, const TW* __restrict__ win,
+                                                double* __restrict__ wout, int64_t n, double* __restrict__ part,
+                                                int ldp) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    double* hs = reinterpret_cast<double*>(lds_raw);             // nv
+    double* red = hs + 1024;                                     // TB
+    double* w1s = red + TB;                                      // X
+    TBv* Vs = reinterpret_cast<TBv*>(w1s + X);                   // nv * (X+1)
+    constexpr int S = TB / X;
+    const int t = threadIdx.x;
+    for (int j = t; j < nv; j += TB) hs[j] = h1[j];
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t c = blockIdx.x; c * X < n; c += gridDim.x) {
+        const int64_t x0 = c * X;
+        __syncthreads();
+        for (int idx = t; idx < nv * X; idx += TB) {
+            const int j = idx / X, xx = idx - j * X;
+            const int64_t x = x0 + xx;
+            Vs[j * (X + 1) + xx] = x < n ? B[(int64_t)j * ldb + x] : TBv(0);
+        }
+        __syncthreads();
+        {
+            const int xx = t % X, sl = t / X;
+            double sum = 0.0;
+            for (int j = sl; j < nv; j += S) sum += hs[j] * (double)Vs[j * (X + 1) + xx];
+            red[sl * X + xx] = sum;
+        }
+        __syncthreads();
+        if (t < X) {
+            double sum = 0.0;
+            for (int q = 0; q < S; q++) sum += red[q * X + t];
+            const int64_t x = x0 + t;
+            double w1 = 0.0;
+            if (x < n) { w1 = (double)win[x] - sum; wout[x] = w1; }
+            w1s[t] = w1;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int j = t + TB * q;
+            if (j < nv) {
+                double d = 0.0;
+#pragma unroll
+                for (int xx = 0; xx < X; xx++) d += (double)Vs[j * (X + 1) + xx] * w1s[xx];
+                acc[q] += d;
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int j = t + TB * q;
+        if (j < nv) part[(int64_t)blockIdx.x * ldp + j] = acc[q];
+    }
+}
+
+// alpha[it] = v.Hv (before any orthogonalisation), beta[it] = |w| after CGS2
+__global__ void k_finish(const double* h1, int it, const double* npart, int G, double* alpha, double* beta) {
     __shared__ double red[TB / 64];
     double s = 0.0;
-    for (int g = threadIdx.x; g < G; g += TB) s += part[g];
+    for (int g = threadIdx.x; g < G; g += TB) s += npart[g];
     s = block_sum(s, red);
     if (threadIdx.x == 0) {
         alpha[it] = h1[it];
@@ -114,10 +219,21 @@ __global__ void k_finish(const double* h1, int it, const double* part, int G, do
     }
 }
 
+__global__ void k_sqrt1(const double* a, double* b) { b[0] = sqrt(a[0]); }
+
+// norm of a vector into out[0] (as |x|)
+__global__ void k_norm_from_parts(const double* npart, int G, double* out) {
+    __shared__ double red[TB / 64];
+    double s = 0.0;
+    for (int g = threadIdx.x; g < G; g += TB) s += npart[g];
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) out[0] = sqrt(s);
+}
+
 template <typename TBv>
-__global__ __launch_bounds__(TB) void k_store(const double* __restrict__ w, const double* __restrict__ beta, int it,
+__global__ __launch_bounds__(TB) void k_store(const double* __restrict__ w, const double* __restrict__ den,
                                               TBv* __restrict__ dst, int64_t n) {
-    const double inv = 1.0 / beta[it];
+    const double inv = 1.0 / den[0];
     for (int64_t x = (int64_t)blockIdx.x * TB + threadIdx.x; x < n; x += (int64_t)gridDim.x * TB)
         dst[x] = (TBv)(w[x] * inv);
 }
@@ -129,23 +245,37 @@ __global__ __launch_bounds__(TB) void k_convert(const TA* __restrict__ a, TBv* _
 }
 
 // Ritz values: block j computes eigenvalue j of T[0..n-1] by 256-way multisection.
+// prev (nprev values) are the eigenvalues of the leading (n-1)x(n-1) block when
+// nprev > 0: lambda_j(T_n) lies in [prev[j-1], prev[j]] (Cauchy interlacing).
 __global__ __launch_bounds__(TB) void k_ritz(const double* __restrict__ alpha, const double* __restrict__ beta,
-                                             int n, double* __restrict__ out) {
+                                             int n, const double* __restrict__ prev, int nprev,
+                                             double* __restrict__ out) {
     __shared__ double sa[1024], sb[1024];
     __shared__ int firstbig;
     __shared__ double slo, shi;
     const int j = blockIdx.x;
     for (int i = threadIdx.x; i < n; i += TB) { sa[i] = alpha[i]; sb[i] = i < n - 1 ? beta[i] : 0.0; }
     __syncthreads();
+    const double pm = tri_pivmin(sb, n);
     if (threadIdx.x == 0) {
         double lo, hi;
         tri_bounds(sa, sb, n, &lo, &hi);
-        slo = lo; shi = hi;
+        double l2 = lo, h2 = hi;
+        if (nprev > 0) {
+            const double pad = 1e-9 * fmax(1.0, fmax(fabs(lo), fabs(hi)));
+            if (j >= 1 && j - 1 < nprev) l2 = fmax(lo, prev[j - 1] - pad);
+            if (j < nprev) h2 = fmin(hi, prev[j] + pad);
+            // keep the interlacing bracket only if the Sturm counts confirm it
+            if (!(l2 < h2 && tri_count(sa, sb, n, l2, pm) <= j && tri_count(sa, sb, n, h2, pm) > j)) {
+                l2 = lo; h2 = hi;
+            }
+        }
+        slo = l2; shi = h2;
     }
     __syncthreads();
-    const double pm = tri_pivmin(sb, n);
     for (int round = 0; round < 12; round++) {
         const double lo = slo, hi = shi;
+        if (hi - lo <= 1e-12 * fmax(1.0, fmax(fabs(lo), fabs(hi)))) break;
         if (threadIdx.x == 0) firstbig = TB;
         __syncthreads();
         const double x = lo + (hi - lo) * (double)(threadIdx.x + 1) / (double)(TB + 1);
@@ -159,19 +289,17 @@ __global__ __launch_bounds__(TB) void k_ritz(const double* __restrict__ alpha, c
             slo = nlo; shi = nhi;
         }
         __syncthreads();
-        if (shi - slo <= 8.0 * 2.220446049250313e-16 * fmax(1.0, fmax(fabs(slo), fabs(shi)))) break;
     }
     if (threadIdx.x == 0) out[j] = 0.5 * (slo + shi);
 }
 
-// history shift + convergence flag (diagonalize_T!): en[h][num_ev], h = 0 newest
-__global__ void k_conv(double* en, int nh, int num_ev, const double* newv, int n, double tol, int* flag) {
+// diagonalize_T! bookkeeping: entries n..num_ev-1 keep the previous values,
+// flag = all(prev - cur < tol)
+__global__ void k_conv(const double* prev, double* cur, int num_ev, int n, double tol, int* flag) {
     if (threadIdx.x != 0) return;
-    for (int h = nh - 1; h > 0; h--)
-        for (int k = 0; k < num_ev; k++) en[h * num_ev + k] = en[(h - 1) * num_ev + k];
-    for (int k = 0; k < n; k++) en[k] = newv[k];
+    for (int k = n; k < num_ev; k++) cur[k] = prev[k];
     int ok = 1;
-    for (int k = 0; k < num_ev; k++) ok &= (en[num_ev + k] - en[k] < tol) ? 1 : 0;
+    for (int k = 0; k < num_ev; k++) ok &= (prev[k] - cur[k] < tol) ? 1 : 0;
     *flag = ok;
 }
 
@@ -212,26 +340,8 @@ __global__ __launch_bounds__(TB) void k_rotate(const TBv* __restrict__ B, int64_
 namespace {
 
 struct Workspace {
-    DBuf basis, wH, w64, part, h1, h2, alpha, beta, en, newv, flag, S, swork, U, ritz;
+    DBuf basis, wH, w64, part, npart, h1, h2, alpha, beta, enring, flags, S, swork, U, scal;
 };
-
-int64_t grid_chunks(int64_t n, int64_t* chunk) {
-    int64_t G = std::max<int64_t>(1, std::min<int64_t>(1024, (n + 4 * TB - 1) / (4 * TB)));
-    *chunk = ((n + G - 1) / G + TB - 1) / TB * TB;
-    G = (n + *chunk - 1) / *chunk;
-    return std::max<int64_t>(G, 1);
-}
-
-template <typename TBv, typename TW>
-void cgs_pass(const TBv* B, int64_t ldb, int nv, const TW* w_in, double* w_out, int64_t n, double* part,
-              double* h, hipStream_t s, double* norm_part, int64_t* G_out) {
-    int64_t chunk;
-    int64_t G = grid_chunks(n, &chunk);
-    hipLaunchKernelGGL((k_dots<TBv, TW>), dim3(G), dim3(TB), 0, s, B, ldb, nv, w_in, n, chunk, part);
-    hipLaunchKernelGGL(k_reduce, dim3(nv), dim3(TB), 0, s, part, (int)G, nv, h);
-    hipLaunchKernelGGL((k_update<TBv, TW>), dim3(G), dim3(TB), 0, s, B, ldb, nv, h, w_in, w_out, n, chunk, norm_part);
-    *G_out = G;
-}
 
 template <typename TBv>
 struct Solver {
@@ -242,6 +352,9 @@ struct Solver {
     int64_t n, ld;
     int lm, ls, nev;
     int prec_hv;   // 0 fp64, 1 fp32
+    int ldp;       // partial-sum row stride
+    int64_t Gd, Gu;
+    int gstore;
     Workspace W;
 
     Solver(DeviceHam& h, const SolveParams& p, SolveStats& stats) : H(h), P(p), st(stats) {
@@ -252,112 +365,186 @@ struct Solver {
         ls = P.ls;
         nev = (int)std::min<int64_t>(P.num_ev, n);
         prec_hv = sizeof(TBv) == 4 ? 1 : 0;
+        ldp = lm + 2;
+        Gd = (n + CHUNK - 1) / CHUNK;
+        Gu = (n + UCHUNK - 1) / UCHUNK;
+        gstore = (int)std::min<int64_t>(4096, (n + TB - 1) / TB);
     }
 
     TBv* V(int j) { return W.basis.as<TBv>() + (int64_t)j * ld; }
+    double* en(int step) { return W.enring.as<double>() + (int64_t)((step + 1) & 3) * 64; }
+    int* flag(int step) { return W.flags.as<int>() + ((step + 1) & 3); }
 
-    void hv_into(int j) {   // W.wH = H V(j)
-        if (prec_hv == 0) launch_hv(H, 0, V(j), W.wH.p, s);
-        else launch_hv(H, 1, V(j), W.wH.p, s);
-    }
+    static constexpr int XM = sizeof(TBv) == 4 ? 32 : 16;   // slab width of k_cgs_mid (128-byte rows)
+    int64_t Gm = 1;
+    size_t mid_lds(int nv) const { return (size_t)(1024 + TB + XM) * 8 + (size_t)nv * (XM + 1) * sizeof(TBv); }
 
-    void orth(int nv, bool from_hv) {
-        int64_t G;
+    // CGS2 of `win` against V[0..nv-1] into w64 (classical Gram-Schmidt twice):
+    //   h1 = V^T win;  w1 = win - V h1;  h2 = V^T w1;  w64 = w1 - V h2
+    // W.h1 keeps the first-pass coefficients, W.npart the |w64|^2 partials.
+    template <typename TW>
+    void cgs2(int nv, const TW* win) {
+        const TBv* B = V(0);
         double* part = W.part.as<double>();
-        double* npart = part + (int64_t)1024 * (lm + 1);
-        if (from_hv) {
-            if (prec_hv == 0)
-                cgs_pass<TBv, double>((const TBv*)V(0), ld, nv, W.wH.as<double>(), W.w64.as<double>(), n, part,
-                                      W.h1.as<double>(), s, npart, &G);
-            else
-                cgs_pass<TBv, float>((const TBv*)V(0), ld, nv, W.wH.as<float>(), W.w64.as<double>(), n, part,
-                                     W.h1.as<double>(), s, npart, &G);
+        hipLaunchKernelGGL((k_dotsw<TBv, TW>), dim3(Gd), dim3(TB), 0, s, B, ld, nv, win, n, part, ldp, 0,
+                           (const int*)nullptr);
+        hipLaunchKernelGGL(k_reduce, dim3(nv), dim3(TB), 0, s, part, (int)Gd, ldp, W.h1.as<double>(),
+                           (const int*)nullptr);
+        if (mid_lds(nv) <= 64 * 1024) {
+            hipLaunchKernelGGL((k_cgs_mid<TBv, TW, XM>), dim3(Gm), dim3(TB), mid_lds(nv), s, B, ld, nv,
+                               W.h1.as<double>(), win, W.w64.as<double>(), n, part, ldp);
+            hipLaunchKernelGGL(k_reduce, dim3(nv), dim3(TB), 0, s, part, (int)Gm, ldp, W.h2.as<double>(),
+                               (const int*)nullptr);
         } else {
-            cgs_pass<TBv, double>((const TBv*)V(0), ld, nv, W.w64.as<double>(), W.w64.as<double>(), n, part,
-                                  W.h1.as<double>(), s, npart, &G);
+            hipLaunchKernelGGL((k_updatew<TBv, TW>), dim3(Gu), dim3(TB), 0, s, B, ld, nv, W.h1.as<double>(), win,
+                               W.w64.as<double>(), n, W.npart.as<double>(), (const int*)nullptr);
+            hipLaunchKernelGGL((k_dotsw<TBv, double>), dim3(Gd), dim3(TB), 0, s, B, ld, nv, W.w64.as<double>(), n,
+                               part, ldp, 0, (const int*)nullptr);
+            hipLaunchKernelGGL(k_reduce, dim3(nv), dim3(TB), 0, s, part, (int)Gd, ldp, W.h2.as<double>(),
+                               (const int*)nullptr);
         }
-        cgs_pass<TBv, double>((const TBv*)V(0), ld, nv, W.w64.as<double>(), W.w64.as<double>(), n, part,
-                              W.h2.as<double>(), s, npart, &G);
-        last_G = G;
+        hipLaunchKernelGGL((k_updatew<TBv, double>), dim3(Gu), dim3(TB), 0, s, B, ld, nv, W.h2.as<double>(),
+                           W.w64.as<double>(), W.w64.as<double>(), n, W.npart.as<double>(), (const int*)nullptr);
     }
-    int64_t last_G = 1;
 
-    void run(double* evals, double* evecs) {
+    void enqueue_iteration(int it, int step, int nprev) {
+        if (prec_hv == 0) launch_hv(H, 0, V(it), W.wH.p, s);
+        else launch_hv(H, 1, V(it), W.wH.p, s);
+        st.n_hv++;
+        const int nv = it + 1;
+        if (prec_hv == 0) cgs2<double>(nv, W.wH.as<double>());
+        else cgs2<float>(nv, W.wH.as<float>());
+        hipLaunchKernelGGL(k_finish, dim3(1), dim3(TB), 0, s, W.h1.as<double>(), it, W.npart.as<double>(), (int)Gu,
+                           W.alpha.as<double>(), W.beta.as<double>());
+        const int nn = std::min(nev, it + 1);
+        hipLaunchKernelGGL(k_ritz, dim3(nn), dim3(TB), 0, s, W.alpha.as<double>(), W.beta.as<double>(), it + 1,
+                           en(step - 1), nprev, en(step));
+        hipLaunchKernelGGL(k_conv, dim3(1), dim3(64), 0, s, en(step - 1), en(step), nev, nn, P.tol, flag(step));
+        if (it + 1 < lm)
+            hipLaunchKernelGGL((k_store<TBv>), dim3(gstore), dim3(TB), 0, s, W.w64.as<double>(),
+                               W.beta.as<double>() + it, V(it + 1), n);
+    }
+
+    void setup(const double* v0_in) {
+        W.basis.alloc((size_t)lm * ld * sizeof(TBv));
+        W.wH.alloc((size_t)ld * (prec_hv ? 4 : 8));
+        W.w64.alloc((size_t)ld * 8);
+        Gm = std::max<int64_t>(1, std::min<int64_t>(2048, (n + XM - 1) / XM));
+        W.part.alloc((size_t)std::max(Gd, Gm) * ldp * 8);
+        W.npart.alloc((size_t)Gu * 8);
+        W.h1.alloc((lm + 2) * 8); W.h2.alloc((lm + 2) * 8);
+        W.alpha.alloc((lm + 1) * 8); W.beta.alloc((lm + 1) * 8);
+        W.enring.alloc(4 * 64 * 8); W.flags.alloc(4 * sizeof(int));
+        W.S.alloc((size_t)lm * 32 * 8);
+        W.swork.alloc((size_t)(5 * lm + 32 * (lm + 3 * 32 + 8) + 64) * 8);
+        W.U.alloc((size_t)32 * ld * sizeof(TBv));
+        W.scal.alloc(64 * 8);
+        SM_HIP(hipMemsetAsync(W.basis.p, 0, W.basis.bytes, s));
+        SM_HIP(hipStreamSynchronize(s));
+        std::vector<double> e0(4 * 64, 1.0e4);
+        SM_HIP(hipMemcpy(W.enring.p, e0.data(), e0.size() * 8, hipMemcpyHostToDevice));
+        // start vector: caller's, or seeded normal (initialize_wf, :939-948); normalised in FP64
+        std::vector<double> v0(n);
+        if (v0_in) {
+            for (int64_t i = 0; i < n; i++) v0[i] = v0_in[i];
+        } else {
+            std::mt19937_64 rng(P.seed);
+            std::normal_distribution<double> nd(0.0, 1.0);
+            for (int64_t i = 0; i < n; i++) v0[i] = nd(rng);
+        }
+        double nr = 0.0;
+        for (int64_t i = 0; i < n; i++) nr += v0[i] * v0[i];
+        if (!(nr > 0.0)) throw std::invalid_argument("start vector is zero");
+        SM_HIP(hipMemcpy(W.w64.p, v0.data(), n * 8, hipMemcpyHostToDevice));
+        hipLaunchKernelGGL((k_convert<double, TBv>), dim3(gstore), dim3(TB), 0, s, W.w64.as<double>(), V(0), n,
+                           1.0 / std::sqrt(nr));
+        SM_HIP(hipStreamSynchronize(s));
+    }
+
+    // nsteps plain Lanczos steps (no convergence test, no restart); alpha/beta out
+    void run_steps(int nsteps, const double* v0_in, double* alpha_out, double* beta_out) {
+        if (nsteps < 1 || nsteps >= lm) throw std::invalid_argument("need 1 <= nsteps < lm");
+        setup(v0_in);
+        for (int it = 0; it < nsteps; it++) enqueue_iteration(it, it, it == 0 ? 0 : std::min(nev, it));
+        SM_HIP(hipStreamSynchronize(s));
+        SM_HIP(hipMemcpy(alpha_out, W.alpha.p, nsteps * 8, hipMemcpyDeviceToHost));
+        SM_HIP(hipMemcpy(beta_out, W.beta.p, nsteps * 8, hipMemcpyDeviceToHost));
+    }
+
+    void run(double* evals, double* evecs, const double* v0_in = nullptr) {
         auto t0 = std::chrono::steady_clock::now();
         if (lm < 3 || ls < 1 || ls > 32 || ls >= lm - 1 || nev < 1 || nev > ls)
             throw std::invalid_argument("need 1 <= num_ev <= ls <= 32 and ls < lm - 1");
         if (lm > 1000) throw std::invalid_argument("lm above 1000 is not supported");
-        int nh = std::max(2, P.num_history);
-        W.basis.alloc((size_t)lm * ld * sizeof(TBv));
-        W.wH.alloc((size_t)ld * (prec_hv ? 4 : 8));
-        W.w64.alloc((size_t)ld * 8);
-        W.part.alloc((size_t)1024 * (lm + 2) * 8);
-        W.h1.alloc((lm + 1) * 8); W.h2.alloc((lm + 1) * 8);
-        W.alpha.alloc((lm + 1) * 8); W.beta.alloc((lm + 1) * 8);
-        W.en.alloc((size_t)nh * nev * 8); W.newv.alloc((lm + 1) * 8); W.flag.alloc(64);
-        W.S.alloc((size_t)lm * 32 * 8);
-        W.swork.alloc((size_t)(5 * lm + 32 * (lm + 3 * 32 + 8) + 64) * 8);
-        W.U.alloc((size_t)32 * ld * sizeof(TBv));
-        SM_HIP(hipMemsetAsync(W.basis.p, 0, W.basis.bytes, s));
-        {
-            std::vector<double> en0((size_t)nh * nev, 1.0e4);
-            SM_HIP(hipMemcpyAsync(W.en.p, en0.data(), en0.size() * 8, hipMemcpyHostToDevice, s));
-        }
-        // start vector: seeded normal, normalised in FP64 (initialize_wf, :939-948)
-        {
-            std::mt19937_64 rng(P.seed);
-            std::normal_distribution<double> nd(0.0, 1.0);
-            std::vector<double> v0(n);
-            double nr = 0.0;
-            for (int64_t i = 0; i < n; i++) { v0[i] = nd(rng); nr += v0[i] * v0[i]; }
-            SM_HIP(hipMemcpyAsync(W.w64.p, v0.data(), n * 8, hipMemcpyHostToDevice, s));
-            int gb = (int)std::min<int64_t>(4096, (n + TB - 1) / TB);
-            hipLaunchKernelGGL((k_convert<double, TBv>), dim3(gb), dim3(TB), 0, s, W.w64.as<double>(), V(0), n,
-                               1.0 / std::sqrt(nr));
-            SM_HIP(hipStreamSynchronize(s));
-        }
+        setup(v0_in);
         int* hflag = nullptr;
-        SM_HIP(hipHostMalloc((void**)&hflag, 64, hipHostMallocDefault));
+        SM_HIP(hipHostMalloc((void**)&hflag, 4 * sizeof(int), hipHostMallocDefault));
+        hipEvent_t ev[4];
+        for (int q = 0; q < 4; q++) SM_HIP(hipEventCreateWithFlags(&ev[q], hipEventDisableTiming));
         int k = 0;
         bool conv = false;
-        int elit = 1;
+        int elit = 1, conv_step = 0;
+        int step = 0;
         int64_t total_steps = 0;
         st.n_hv = 0; st.n_restart = 0;
-        int gstore = (int)std::min<int64_t>(4096, (n + TB - 1) / TB);
+        bool after_restart = false;
+        auto check = [&](int stp) -> bool {
+            SM_HIP(hipEventSynchronize(ev[(stp + 1) & 3]));
+            return hflag[(stp + 1) & 3] != 0;
+        };
         while (!conv) {
-            for (int it = k; it < lm - 1; it++) {
-                hv_into(it);
-                st.n_hv++;
-                orth(it + 1, true);
-                hipLaunchKernelGGL(k_finish, dim3(1), dim3(TB), 0, s, W.h1.as<double>(), it,
-                                   W.part.as<double>() + (int64_t)1024 * (lm + 1), (int)last_G,
-                                   W.alpha.as<double>(), W.beta.as<double>());
-                int nn = std::min(nev, it + 1);
-                hipLaunchKernelGGL(k_ritz, dim3(nn), dim3(TB), 0, s, W.alpha.as<double>(), W.beta.as<double>(),
-                                   it + 1, W.newv.as<double>());
-                hipLaunchKernelGGL(k_conv, dim3(1), dim3(64), 0, s, W.en.as<double>(), nh, nev,
-                                   W.newv.as<double>(), nn, P.tol, W.flag.as<int>());
-                SM_HIP(hipMemcpyAsync(hflag, W.flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
-                SM_HIP(hipStreamSynchronize(s));
-                if (*hflag || it + 1 == n) { conv = true; elit = it + 1; break; }
-                if (++total_steps >= P.max_iter) throw std::runtime_error("TRL: iteration limit reached");
-                hipLaunchKernelGGL((k_store<TBv>), dim3(gstore), dim3(TB), 0, s, W.w64.as<double>(),
-                                   W.beta.as<double>(), it, V(it + 1), n);
+            int first_step = step;
+            int it = k;
+            for (; it < lm - 1; it++, step++) {
+                if (++total_steps > P.max_iter) throw std::runtime_error("TRL: iteration limit reached");
+                // interlacing brackets need T[0..it-1] to be the previous step's matrix
+                const int nprev = (it == k && after_restart) || it == 0 ? 0 : std::min(nev, it);
+                enqueue_iteration(it, step, nprev);
+                SM_HIP(hipMemcpyAsync(hflag + ((step + 1) & 3), flag(step), sizeof(int), hipMemcpyDeviceToHost, s));
+                SM_HIP(hipEventRecord(ev[(step + 1) & 3], s));
+                if (it + 1 == n) {                         // Krylov space exhausted (it == mdim)
+                    if (step > first_step && check(step - 1)) { conv_step = step - 1; elit = it; }
+                    else { conv_step = step; elit = it + 1; }
+                    conv = true;
+                    step++;
+                    break;
+                }
+                if (step > first_step && check(step - 1)) {
+                    conv = true; conv_step = step - 1; elit = it;
+                    step++;
+                    break;
+                }
             }
             if (conv) break;
+            // last iteration of the sweep: its flag decides between stop and restart
+            if (check(step - 1)) { conv = true; conv_step = step - 1; elit = lm - 1; break; }
             restart();
             st.n_restart++;
             k = ls;
+            after_restart = true;
         }
+        SM_HIP(hipStreamSynchronize(s));
         st.iterations = elit;
-        // Ritz values and vectors (main_sm :208-224)
+        std::vector<double> en_last(nev);
+        SM_HIP(hipMemcpy(en_last.data(), en(conv_step), nev * 8, hipMemcpyDeviceToHost));
+        for (int i = 0; i < nev; i++) { evals[i] = en_last[i]; st.ritz[i] = en_last[i]; }
         finish(elit, evals, evecs);
+        for (int q = 0; q < 4; q++) (void)hipEventDestroy(ev[q]);
         SM_HIP(hipHostFree(hflag));
         st.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
 
+    // V(j) = w64 / |w64|  (norm partials of w64 already in W.npart)
+    void store_normalised(int j) {
+        hipLaunchKernelGGL(k_norm_from_parts, dim3(1), dim3(TB), 0, s, W.npart.as<double>(), (int)Gu,
+                           W.scal.as<double>());
+        hipLaunchKernelGGL((k_store<TBv>), dim3(gstore), dim3(TB), 0, s, W.w64.as<double>(), W.scal.as<double>(),
+                           V(j), n);
+    }
+
     void restart() {
         int m = lm - 1;
+        if (std::getenv("SMGPU_DEBUG_RESTART")) debug_sweep();
         hipLaunchKernelGGL(k_restart_small, dim3(1), dim3(64), 0, s, W.alpha.as<double>(), W.beta.as<double>(),
                            m, ls, W.S.as<double>(), W.swork.as<double>());
         int gb = (int)std::min<int64_t>(2048, (n + TB - 1) / TB);
@@ -369,65 +556,108 @@ struct Solver {
         SM_HIP(hipMemcpyAsync(V(ls), V(lm - 1), n * sizeof(TBv), hipMemcpyDeviceToDevice, s));
         SM_HIP(hipMemcpyAsync(V(0), W.U.p, (size_t)ls * ld * sizeof(TBv), hipMemcpyDeviceToDevice, s));
         // re-orthonormalise the kept vectors (the reference's make_sure pass)
-        int gstore = (int)std::min<int64_t>(4096, (n + TB - 1) / TB);
         for (int j = 0; j < ls; j++) {
-            hipLaunchKernelGGL((k_convert<TBv, double>), dim3(gstore), dim3(TB), 0, s, (const TBv*)V(j),
-                               W.w64.as<double>(), n, 1.0);
-            if (j > 0) {
-                int64_t G;
-                double* part = W.part.as<double>();
-                double* npart = part + (int64_t)1024 * (lm + 1);
-                cgs_pass<TBv, double>((const TBv*)V(0), ld, j, W.w64.as<double>(), W.w64.as<double>(), n, part,
-                                      W.h1.as<double>(), s, npart, &G);
+            if (j == 0) {
+                hipLaunchKernelGGL((k_convert<TBv, double>), dim3(gstore), dim3(TB), 0, s, (const TBv*)V(0),
+                                   W.w64.as<double>(), n, 1.0);
+                hipLaunchKernelGGL((k_dotsw<double, double>), dim3(Gd), dim3(TB), 0, s, (const double*)nullptr,
+                                   ld, 0, W.w64.as<double>(), n, W.part.as<double>(), ldp, 1, (const int*)nullptr);
+                hipLaunchKernelGGL(k_reduce, dim3(1), dim3(TB), 0, s, W.part.as<double>(), (int)Gd, ldp,
+                                   W.scal.as<double>() + 1, (const int*)nullptr);
+                hipLaunchKernelGGL(k_sqrt1, dim3(1), dim3(1), 0, s, W.scal.as<double>() + 1, W.scal.as<double>());
+                hipLaunchKernelGGL((k_store<TBv>), dim3(gstore), dim3(TB), 0, s, W.w64.as<double>(),
+                                   W.scal.as<double>(), V(0), n);
+            } else {
+                cgs2<TBv>(j, (const TBv*)V(j));
+                store_normalised(j);
             }
-            norm_store(j);
         }
         // v_ls against the kept vectors (no renormalisation, as the reference)
-        {
-            hipLaunchKernelGGL((k_convert<TBv, double>), dim3(gstore), dim3(TB), 0, s, (const TBv*)V(ls),
-                               W.w64.as<double>(), n, 1.0);
-            int64_t G;
-            double* part = W.part.as<double>();
-            double* npart = part + (int64_t)1024 * (lm + 1);
-            cgs_pass<TBv, double>((const TBv*)V(0), ld, ls, W.w64.as<double>(), W.w64.as<double>(), n, part,
-                                  W.h1.as<double>(), s, npart, &G);
-            hipLaunchKernelGGL((k_convert<double, TBv>), dim3(gstore), dim3(TB), 0, s, W.w64.as<double>(), V(ls),
-                               n, 1.0);
-        }
-        SM_HIP(hipMemsetAsync(V(ls + 1), 0, (size_t)(lm - ls - 1) * ld * sizeof(TBv), s));
+        cgs2<TBv>(ls, (const TBv*)V(ls));
+        hipLaunchKernelGGL((k_convert<double, TBv>), dim3(gstore), dim3(TB), 0, s, W.w64.as<double>(), V(ls), n, 1.0);
+        if (std::getenv("SMGPU_DEBUG_RESTART")) debug_restart();
     }
 
-    // V(j) = w64 / |w64|
-    void norm_store(int j) {
-        int64_t chunk;
-        int64_t G = grid_chunks(n, &chunk);
-        double* part = W.part.as<double>();
-        hipLaunchKernelGGL((k_dots<double, double>), dim3(G), dim3(TB), 0, s, W.w64.as<double>(), ld, 1,
-                           W.w64.as<double>(), n, chunk, part);
-        hipLaunchKernelGGL(k_finish, dim3(1), dim3(TB), 0, s, W.h1.as<double>(), 0, part, (int)G,
-                           W.newv.as<double>(), W.newv.as<double>() + 1);
-        int gstore = (int)std::min<int64_t>(4096, (n + TB - 1) / TB);
-        hipLaunchKernelGGL((k_store<TBv>), dim3(gstore), dim3(TB), 0, s, W.w64.as<double>(),
-                           W.newv.as<double>() + 1, 0, V(j), n);
+    // opt-in check of the sweep before a restart: orthonormality of V[0..lm-1] and
+    // |H v_k - a_k v_k - b_{k-1} v_{k-1} - b_k v_{k+1}|
+    void debug_sweep() {
+        SM_HIP(hipStreamSynchronize(s));
+        std::vector<TBv> hv((size_t)lm * ld);
+        SM_HIP(hipMemcpy(hv.data(), V(0), hv.size() * sizeof(TBv), hipMemcpyDeviceToHost));
+        std::vector<double> al(lm), be(lm);
+        SM_HIP(hipMemcpy(al.data(), W.alpha.p, (lm - 1) * 8, hipMemcpyDeviceToHost));
+        SM_HIP(hipMemcpy(be.data(), W.beta.p, (lm - 1) * 8, hipMemcpyDeviceToHost));
+        for (int i = 0; i < lm; i++) {
+            double worst = 0.0;
+            for (int j = 0; j <= i; j++) {
+                double d = 0.0;
+                for (int64_t x = 0; x < n; x++) d += (double)hv[(size_t)i * ld + x] * (double)hv[(size_t)j * ld + x];
+                worst = std::max(worst, std::fabs(d - (i == j ? 1.0 : 0.0)));
+            }
+            if (i % 4 == 0 || i == lm - 1) fprintf(stderr, "[sweep] v%d max |<v_i,v_j>-d_ij| over j<=i: %.3e\n", i, worst);
+        }
+        std::vector<TBv> hw(n);
+        for (int k = 0; k < lm - 1; k += 4) {
+            if (prec_hv == 0) launch_hv(H, 0, V(k), W.wH.p, s);
+            else launch_hv(H, 1, V(k), W.wH.p, s);
+            SM_HIP(hipStreamSynchronize(s));
+            SM_HIP(hipMemcpy(hw.data(), W.wH.p, n * sizeof(TBv), hipMemcpyDeviceToHost));
+            double r2 = 0.0;
+            for (int64_t x = 0; x < n; x++) {
+                double e = (double)hw[x] - al[k] * (double)hv[(size_t)k * ld + x];
+                if (k > 0) e -= be[k - 1] * (double)hv[(size_t)(k - 1) * ld + x];
+                e -= be[k] * (double)hv[(size_t)(k + 1) * ld + x];
+                r2 += e * e;
+            }
+            fprintf(stderr, "[sweep] k=%d alpha=%.6f beta=%.6f |H v - T v|=%.3e\n", k, al[k], be[k], std::sqrt(r2));
+        }
+    }
+
+    // opt-in consistency check of the state after a restart: orthonormality of
+    // V[0..ls] and the tridiagonal relation H u_k = a_k u_k + b_{k-1} u_{k-1} + b_k u_{k+1}
+    void debug_restart() {
+        SM_HIP(hipStreamSynchronize(s));
+        std::vector<TBv> hv((size_t)(ls + 1) * ld);
+        SM_HIP(hipMemcpy(hv.data(), V(0), hv.size() * sizeof(TBv), hipMemcpyDeviceToHost));
+        std::vector<double> al(ls + 1), be(ls + 1);
+        SM_HIP(hipMemcpy(al.data(), W.alpha.p, ls * 8, hipMemcpyDeviceToHost));
+        SM_HIP(hipMemcpy(be.data(), W.beta.p, ls * 8, hipMemcpyDeviceToHost));
+        double worst = 0.0;
+        for (int i = 0; i <= ls; i++)
+            for (int j = 0; j <= i; j++) {
+                double d = 0.0;
+                for (int64_t x = 0; x < n; x++) d += (double)hv[(size_t)i * ld + x] * (double)hv[(size_t)j * ld + x];
+                worst = std::max(worst, std::fabs(d - (i == j ? 1.0 : 0.0)));
+            }
+        fprintf(stderr, "[restart] n=%lld orthonormality error %.3e\n", (long long)n, worst);
+        std::vector<TBv> hw(n);
+        for (int k = 0; k < ls; k++) {
+            if (prec_hv == 0) launch_hv(H, 0, V(k), W.wH.p, s);
+            else launch_hv(H, 1, V(k), W.wH.p, s);
+            SM_HIP(hipStreamSynchronize(s));
+            SM_HIP(hipMemcpy(hw.data(), W.wH.p, n * sizeof(TBv), hipMemcpyDeviceToHost));
+            double r2 = 0.0, rq = 0.0;
+            for (int64_t x = 0; x < n; x++) {
+                double e = (double)hw[x] - al[k] * (double)hv[(size_t)k * ld + x];
+                if (k > 0) e -= be[k - 1] * (double)hv[(size_t)(k - 1) * ld + x];
+                e -= be[k] * (double)hv[(size_t)(k + 1) * ld + x];
+                r2 += e * e;
+                rq += (double)hw[x] * (double)hv[(size_t)k * ld + x];
+            }
+            fprintf(stderr, "[restart] k=%d alpha=%.6f rq=%.6f beta=%.6f |H u - T u|=%.3e\n", k, al[k], rq, be[k],
+                    std::sqrt(r2));
+        }
     }
 
     void finish(int elit, double* evals, double* evecs) {
-        // lowest nev eigenpairs of T[0..elit-1]
+        bool need_vecs = evecs != nullptr || (prec_hv == 1 && P.refine);
+        if (!need_vecs) return;
         DBuf vals, Y, work;
         vals.alloc(nev * 8);
         Y.alloc((size_t)nev * elit * 8);
         work.alloc((size_t)(5 * elit + 8) * 8);
         hipLaunchKernelGGL(k_lowest_small, dim3(1), dim3(64), 0, s, W.alpha.as<double>(), W.beta.as<double>(),
                            elit, nev, vals.as<double>(), Y.as<double>(), work.as<double>());
-        std::vector<double> ritz(nev);
-        SM_HIP(hipMemcpyAsync(ritz.data(), vals.p, nev * 8, hipMemcpyDeviceToHost, s));
-        // the converged Ritz values of the last check (en[1])
-        std::vector<double> en_last(nev);
-        SM_HIP(hipMemcpyAsync(en_last.data(), W.en.p, nev * 8, hipMemcpyDeviceToHost, s));
-        SM_HIP(hipStreamSynchronize(s));
-        for (int i = 0; i < nev; i++) { evals[i] = en_last[i]; st.ritz[i] = en_last[i]; }
-        bool need_vecs = evecs != nullptr || (prec_hv == 1 && P.refine);
-        if (!need_vecs) return;
         DBuf R;
         R.alloc((size_t)nev * ld * 8);
         int gb = (int)std::min<int64_t>(2048, (n + TB - 1) / TB);
@@ -435,59 +665,86 @@ struct Solver {
         if (sh > 64 * 1024) throw std::invalid_argument("num_ev*iterations too large for the Ritz assembly");
         hipLaunchKernelGGL((k_rotate<TBv, double>), dim3(gb), dim3(TB), sh, s, (const TBv*)V(0), ld, elit,
                            Y.as<double>(), nev, R.as<double>(), ld, n);
-        // normalise (Rvec .*= 1/sqrt(dot))
-        int64_t chunk;
-        int64_t G = grid_chunks(n, &chunk);
-        double* part = W.part.as<double>();
-        std::vector<double> nrm(nev), rq(nev);
         DBuf hv64;
-        if (prec_hv == 1 && P.refine) hv64.alloc((size_t)ld * 8);
+        const bool refine = prec_hv == 1 && P.refine;
+        if (refine) hv64.alloc((size_t)ld * 8);
+        double* sc = W.scal.as<double>();
         for (int i = 0; i < nev; i++) {
             double* r = R.as<double>() + (int64_t)i * ld;
-            hipLaunchKernelGGL((k_dots<double, double>), dim3(G), dim3(TB), 0, s, r, ld, 1, r, n, chunk, part);
-            hipLaunchKernelGGL(k_reduce, dim3(1), dim3(TB), 0, s, part, (int)G, 1, W.newv.as<double>() + i);
-        }
-        SM_HIP(hipMemcpyAsync(nrm.data(), W.newv.p, nev * 8, hipMemcpyDeviceToHost, s));
-        SM_HIP(hipStreamSynchronize(s));
-        int gs = (int)std::min<int64_t>(4096, (n + TB - 1) / TB);
-        for (int i = 0; i < nev; i++) {
-            double* r = R.as<double>() + (int64_t)i * ld;
-            hipLaunchKernelGGL((k_convert<double, double>), dim3(gs), dim3(TB), 0, s, r, r, n, 1.0 / std::sqrt(nrm[i]));
-            if (prec_hv == 1 && P.refine) {
+            // |r|^2 -> sc[2i]; normalise (Rvec .*= 1/sqrt(dot))
+            hipLaunchKernelGGL((k_dotsw<double, double>), dim3(Gd), dim3(TB), 0, s, (const double*)nullptr, ld, 0,
+                               (const double*)r, n, W.part.as<double>(), ldp, 1, (const int*)nullptr);
+            hipLaunchKernelGGL(k_reduce, dim3(1), dim3(TB), 0, s, W.part.as<double>(), (int)Gd, ldp, sc + 2 * i,
+                               (const int*)nullptr);
+            hipLaunchKernelGGL(k_sqrt1, dim3(1), dim3(1), 0, s, sc + 2 * i, sc + 2 * i + 1);
+            hipLaunchKernelGGL((k_store<double>), dim3(gstore), dim3(TB), 0, s, (const double*)r, sc + 2 * i + 1,
+                               r, n);
+            if (refine) {
                 // FP64 Rayleigh quotient with the FP64 H·v
                 launch_hv(H, 0, r, hv64.p, s);
                 st.n_hv64++;
-                hipLaunchKernelGGL((k_dots<double, double>), dim3(G), dim3(TB), 0, s, r, ld, 1, hv64.as<double>(), n,
-                                   chunk, part);
-                hipLaunchKernelGGL(k_reduce, dim3(1), dim3(TB), 0, s, part, (int)G, 1, W.newv.as<double>() + i);
+                hipLaunchKernelGGL((k_dotsw<double, double>), dim3(Gd), dim3(TB), 0, s, (const double*)r, ld, 1,
+                                   hv64.as<double>(), n, W.part.as<double>(), ldp, 0, (const int*)nullptr);
+                hipLaunchKernelGGL(k_reduce, dim3(1), dim3(TB), 0, s, W.part.as<double>(), (int)Gd, ldp,
+                                   W.h1.as<double>() + i, (const int*)nullptr);
             }
         }
-        if (prec_hv == 1 && P.refine) {
-            SM_HIP(hipMemcpyAsync(rq.data(), W.newv.p, nev * 8, hipMemcpyDeviceToHost, s));
+        if (refine) {
+            std::vector<double> rq(nev);
+            SM_HIP(hipMemcpyAsync(rq.data(), W.h1.p, nev * 8, hipMemcpyDeviceToHost, s));
             SM_HIP(hipStreamSynchronize(s));
             for (int i = 0; i < nev; i++) evals[i] = rq[i];
-            // keep ascending order
             std::sort(evals, evals + nev);
         }
         if (evecs) {
             for (int i = 0; i < nev; i++)
                 SM_HIP(hipMemcpyAsync(evecs + (int64_t)i * n, R.as<double>() + (int64_t)i * ld, n * 8,
                                       hipMemcpyDeviceToHost, s));
-            SM_HIP(hipStreamSynchronize(s));
         }
+        SM_HIP(hipStreamSynchronize(s));
     }
 };
 
 }  // namespace
 
-void trl_solve(DeviceHam& h, const SolveParams& p, double* evals, double* evecs, SolveStats& st) {
+void trl_solve(DeviceHam& h, const SolveParams& p, double* evals, double* evecs, SolveStats& st,
+               const double* v0) {
     SM_HIP(hipSetDevice(h.device));
     if (p.precision == 0) {
         Solver<double> S(h, p, st);
-        S.run(evals, evecs);
+        S.run(evals, evecs, v0);
     } else {
         Solver<float> S(h, p, st);
-        S.run(evals, evecs);
+        S.run(evals, evecs, v0);
+    }
+}
+
+// device run of the restart small problem (tests compare it with the host build)
+void restart_small_device(int device, const double* alpha, const double* beta, int m, int ls, double* S,
+                          double* alpha_out, double* beta_out) {
+    SM_HIP(hipSetDevice(device));
+    DBuf a, b, Sd, w;
+    a.alloc((m + 1) * 8); b.alloc((m + 1) * 8); Sd.alloc((size_t)m * ls * 8);
+    w.alloc((size_t)(5 * m + ls * (m + 3 * ls + 8) + 64) * 8);
+    SM_HIP(hipMemcpy(a.p, alpha, m * 8, hipMemcpyHostToDevice));
+    SM_HIP(hipMemcpy(b.p, beta, m * 8, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_restart_small, dim3(1), dim3(64), 0, 0, a.as<double>(), b.as<double>(), m, ls,
+                       Sd.as<double>(), w.as<double>());
+    SM_HIP(hipDeviceSynchronize());
+    SM_HIP(hipMemcpy(S, Sd.p, (size_t)m * ls * 8, hipMemcpyDeviceToHost));
+    SM_HIP(hipMemcpy(alpha_out, a.p, ls * 8, hipMemcpyDeviceToHost));
+    SM_HIP(hipMemcpy(beta_out, b.p, ls * 8, hipMemcpyDeviceToHost));
+}
+
+void lanczos_steps(DeviceHam& h, const SolveParams& p, int nsteps, const double* v0, double* alpha, double* beta) {
+    SM_HIP(hipSetDevice(h.device));
+    SolveStats st{};
+    if (p.precision == 0) {
+        Solver<double> S(h, p, st);
+        S.run_steps(nsteps, v0, alpha, beta);
+    } else {
+        Solver<float> S(h, p, st);
+        S.run_steps(nsteps, v0, alpha, beta);
     }
 }
 

@@ -69,11 +69,14 @@ SIGNATURES = {
     "smgpu_time_apply_H": (I32, [P, I32, I32, P]),
     "smgpu_default_params": (None, [P]),
     "smgpu_trl_solve": (I32, [P, P, P, P, P]),
+    "smgpu_trl_solve_v0": (I32, [P, P, P, P, P, P]),
+    "smgpu_lanczos_steps": (I32, [P, I32, I32, P, P, P]),
     "smgpu_malloc": (I32, [P, I64, P]),
     "smgpu_free": (I32, [P, P]),
     "smgpu_memcpy": (I32, [P, P, P, I64, I32]),
     "smgpu_synchronize": (I32, [P]),
     "smgpu_selftest_tridiag": (I32, [P, P, I32, I32, P, P]),
+    "smgpu_selftest_restart": (I32, [I32, P, P, I32, I32, P, P, P]),
 }
 
 _LIB = None

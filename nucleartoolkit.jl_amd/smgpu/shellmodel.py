@@ -137,8 +137,16 @@ class Hamiltonian:
         check(lib().smgpu_work_terms(self._h, ptr(t)))
         return dict(pn_padded=int(t[0]), pn=int(t[1]), pp=int(t[2]), nn=int(t[3]), diag=int(t[4]))
 
+    def lanczos_steps(self, nsteps, v0, precision="fp64"):
+        """(alpha, beta) of `nsteps` steps of the device Lanczos recurrence from v0."""
+        v0 = np.ascontiguousarray(v0, dtype=np.float64)
+        a = np.zeros(nsteps)
+        b = np.zeros(nsteps)
+        check(lib().smgpu_lanczos_steps(self._h, _prec(precision), int(nsteps), ptr(v0), ptr(a), ptr(b)))
+        return a, b
+
     def trl(self, num_ev, lm=300, ls=20, tol=1e-8, num_history=3, precision="fp64", refine=True, seed=1234,
-            want_vectors=False, max_iter=0):
+            want_vectors=False, max_iter=0, v0=None):
         """Thick-restart Lanczos; returns (evals, evecs or None, stats dict)."""
         p = SolveParams()
         lib().smgpu_default_params(ctypes.byref(p))
@@ -149,8 +157,13 @@ class Hamiltonian:
         ev = np.zeros(nev, np.float64)
         vec = np.zeros((nev, self.mdim), np.float64) if want_vectors else None
         st = SolveStats()
-        check(lib().smgpu_trl_solve(self._h, ctypes.byref(p), ptr(ev), ptr(vec) if vec is not None else None,
-                                    ctypes.byref(st)))
+        v0p = None
+        if v0 is not None:
+            v0 = np.ascontiguousarray(v0, dtype=np.float64)
+            assert v0.shape == (self.mdim,)
+            v0p = ptr(v0)
+        check(lib().smgpu_trl_solve_v0(self._h, ctypes.byref(p), v0p, ptr(ev),
+                                       ptr(vec) if vec is not None else None, ctypes.byref(st)))
         stats = dict(iterations=st.iterations, n_hv=st.n_hv, n_hv64=st.n_hv64, n_restart=st.n_restart,
                      wall_ms=st.wall_ms, ritz=np.array(st.ritz[:nev]))
         return ev, vec, stats

@@ -93,11 +93,15 @@ def test_trl_restarts_match_dense(golden):
         en, vec, st = h.trl(6, lm=40, ls=10, precision=prec, want_vectors=True)
         assert st["n_restart"] > 0
         assert np.abs(en - ev).max() < (1e-7 if prec == "fp64" else 1e-5), (prec, en - ev)
-        # Ritz vectors are unit eigenvectors of the oracle's H
+        # Ritz vectors are unit eigenvectors of the oracle's H.  The stopping
+        # rule bounds the Ritz-value change (1e-8), so the residual is only
+        # bounded by sqrt(dE * gap) ~ 1e-3; the eigenvector overlap is checked too.
+        _, evec = np.linalg.eigh(o.dense())
         for j in range(6):
             r = o.operate_H(vec[j]) - ev[j] * vec[j]
             assert abs(np.linalg.norm(vec[j]) - 1) < 1e-10
-            assert np.linalg.norm(r) < (1e-4 if prec == "fp64" else 1e-2)
+            assert np.linalg.norm(r) < (3e-3 if prec == "fp64" else 1e-2)
+            assert abs(abs(vec[j] @ evec[:, j]) - 1) < 1e-5
 
 
 def test_trl_mg24_vs_arpack(snt_dir):
@@ -113,3 +117,61 @@ def test_trl_mg24_vs_arpack(snt_dir):
     print("fp64", en64 - ev, st64["iterations"], "mixed", enmx - ev, stmx["iterations"], flush=True)
     assert np.abs(en64 - ev).max() < 1e-7
     assert np.abs(enmx - ev).max() < 1e-5
+
+
+def _golden_cfg(golden, name):
+    import json
+    g = json.load(open(f"{golden}/eigs_{name}.json"))
+    return g
+
+
+@pytest.mark.parametrize("name,space,zv,nv,spe,sig", [("c1", "sd", 4, 4, (-4, 0), 1.5),
+                                                      ("c2", "pf", 4, 4, (-8, 0), 1.0)])
+def test_trl_vs_golden_arpack(golden, snt_dir, name, space, zv, nv, spe, sig):
+    """Eigenvalues of the C1/C2 workloads vs the committed oracle+ARPACK values
+    (tests/golden/make_golden.py): FP64 path <= 1e-7 MeV, mixed path <= 1e-5 MeV."""
+    g = _golden_cfg(golden, name)
+    path, nuc = synth.config_snt(f"{snt_dir}/{name}.snt", space, zv, nv, g["seed"], spe, sig)
+    assert nuc == g["nucleus"]
+    p = smgpu.Problem(path, nuc)
+    assert p.mdim == g["mdim"]
+    h = smgpu.Hamiltonian(p)
+    ref = np.array(g["eigenvalues_MeV"])
+    k = len(ref)
+    for prec, tol in (("fp64", 1e-7), ("mixed", 1e-5)):
+        en, _, st = h.trl(k, precision=prec)
+        print(name, prec, np.abs(en - ref).max(), st["iterations"], st["n_restart"], flush=True)
+        assert np.abs(en - ref).max() < tol, (prec, en - ref)
+    # a restarted solve (small lm) reaches the same values
+    en, _, st = h.trl(min(k, 5), lm=40, ls=max(10, min(k, 5)), precision="mixed")
+    assert st["n_restart"] > 0
+    assert np.abs(en - ref[:len(en)]).max() < 1e-5
+
+
+def test_lanczos_recurrence_matches_numpy(snt_dir):
+    """alpha/beta of the device recurrence vs numpy CGS2 Lanczos with the same
+    (device) H·v, at the Cr48 size: FP64 <= 1e-10, mixed <= 1e-5 (relative)."""
+    path, nuc = synth.config_snt(f"{snt_dir}/c2.snt", "pf", 4, 4, synth.BASE_SEED + 2, (-8, 0), 1.0)
+    p = smgpu.Problem(path, nuc)
+    h = smgpu.Hamiltonian(p)
+    n = p.mdim
+    v0 = np.random.default_rng(5).standard_normal(n)
+    K = 16
+    V = np.zeros((K + 1, n))
+    V[0] = v0 / np.linalg.norm(v0)
+    al = np.zeros(K)
+    be = np.zeros(K)
+    for it in range(K):
+        w = h.operate_H(V[it], "fp64")
+        hc = V[:it + 1] @ w
+        al[it] = hc[it]
+        w = w - hc @ V[:it + 1]
+        w = w - (V[:it + 1] @ w) @ V[:it + 1]
+        be[it] = np.linalg.norm(w)
+        V[it + 1] = w / be[it]
+    a64, b64 = h.lanczos_steps(K, v0, "fp64")
+    a32, b32 = h.lanczos_steps(K, v0, "mixed")
+    assert np.abs(a64 - al).max() < 1e-10 * np.abs(al).max()
+    assert np.abs(b64 - be).max() < 1e-10 * np.abs(be).max()
+    assert np.abs(a32 - al).max() < 1e-5 * np.abs(al).max()
+    assert np.abs(b32 - be).max() < 1e-5 * np.abs(be).max()

@@ -133,3 +133,22 @@ def test_tridiag_solver():
     vals = np.zeros(12); vecs = np.zeros((12, n))
     assert lib().smgpu_selftest_tridiag(ptr(a), ptr(b), n, 12, ptr(vals), ptr(vecs)) == 0
     assert np.abs(vecs @ vecs.T - np.eye(12)).max() < 1e-10
+
+
+def test_restart_small_problem_host():
+    """tri_restart: S orthonormal, S^T T S tridiagonal with the returned entries,
+    coupling only on the last kept vector, eigenvalues preserved."""
+    rng = np.random.default_rng(9)
+    for m, ls in [(10, 4), (60, 20), (299, 20)]:
+        a = rng.normal(size=m) * 10
+        b = rng.uniform(0.1, 3, size=m)
+        T = np.diag(a) + np.diag(b[:m - 1], 1) + np.diag(b[:m - 1], -1)
+        S = np.zeros((ls, m)); ao = np.zeros(ls); bo = np.zeros(ls)
+        assert lib().smgpu_selftest_restart(-1, ptr(a), ptr(b), m, ls, ptr(S), ptr(ao), ptr(bo)) == 0
+        St = S.T
+        Tn = np.diag(ao) + np.diag(bo[:ls - 1], 1) + np.diag(bo[:ls - 1], -1)
+        assert np.abs(St.T @ St - np.eye(ls)).max() < 1e-12
+        assert np.abs(St.T @ T @ St - Tn).max() < 1e-10
+        coup = b[m - 1] * St[m - 1, :]
+        assert np.abs(coup[:-1]).max() < 1e-10 and abs(abs(coup[-1]) - bo[-1]) < 1e-10
+        assert np.abs(np.linalg.eigvalsh(Tn) - np.linalg.eigvalsh(T)[:ls]).max() < 1e-10
