@@ -37,9 +37,11 @@ struct DBuf {
 
 // Kernel view of the Hamiltonian tables for one precision (all device pointers).
 struct HvTables {
-    int nb, R;
+    int nb, R, G;
     const int32_t* row_block;   // R
     const int32_t* block_row0;  // nb
+    const int32_t* grp_block;   // G: block of each row group
+    const int32_t* grp_fp0;     // G: first proton row of the group
     const int32_t* nP;          // nb
     const int32_t* nN;          // nb
     const int64_t* tdims;       // nb+1
@@ -70,7 +72,7 @@ struct HvTables {
     const int64_t* veff_off;          // per dM
     const int32_t* veff_cols2;        // per dM: 2*(cols+1)
     const void* veff;                 // (T) signed copies: [row][2*bd+neg]
-    int pch;        // p-hop rows staged per LDS round
+    int rg;         // proton rows per workgroup
     int nN_max;
     int cols2_max;
     int npt;        // ceil(nN_max / 256)
@@ -83,6 +85,7 @@ struct DeviceHam {
     hipStream_t stream = nullptr;
     int64_t mdim = 0;
     int nb = 0, R = 0;
+    DBuf grp_block[2], grp_fp0[2];   // row groups per precision (cost order)
     // index tables
     DBuf row_block, block_row0, nP, nN, tdims, en_off, pp_rowptr, pp_col, nn_slice_base, nn_slice_off,
         nn_slice_len, nn_col, pair_ptr, pair_bi, pair_dm, pair_ptab, pair_ntab, ptab_rowptr_base,

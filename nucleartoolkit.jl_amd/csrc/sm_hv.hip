@@ -1,4 +1,4 @@
-// H·v on gfx950: one workgroup per output proton row.
+// H·v on gfx950: one workgroup per group of 4 consecutive output proton rows.
 //
 // Output element (bf, fp, fn) (block, proton SD, neutron SD; global index
 // tdims[bf] + fp*nN + fn, the reference's order) is owned by one lane; all
@@ -24,6 +24,8 @@
 
 namespace smgpu {
 
+static constexpr size_t HV_LDS_TWO_ROWS = 80 * 1024;  // two-row groups if they fit (2 per CU)
+
 DBuf::~DBuf() {
     if (p) (void)hipFree(p);
 }
@@ -37,55 +39,86 @@ DeviceHam::~DeviceHam() {
     if (stream) (void)hipStreamDestroy(stream);
 }
 
-template <typename T, int NPT>
+template <typename T> struct Vec4;
+template <> struct Vec4<float> { using type = float4; };
+template <> struct Vec4<double> { using type = double4; };
+
+// One workgroup per group of RG consecutive proton rows of one block (lanes
+// along the neutron index, NPT columns per lane).  Every neutron-side table
+// entry a lane loads (nn and pn hop lists) is applied to all rows of the group.
+// pn: per coupled input block, the proton hops of the group's rows are staged
+// four per row per round ("slots"), slot-interleaved in LDS:
+//   X[in][slot] = s_p v[bi: ip_slot, in],   Vs[col][slot] = V_eff[dM][ac_slot][col]
+// so that a lane fetches the four hops of a row for its neutron hop (in, col)
+// with two 4-wide LDS loads at compile-time offsets.  Unused slots are zero.
+template <typename T, int NPT, int RG>
 __global__ __launch_bounds__(256) void k_hv(HvTables A, const T* __restrict__ v, T* __restrict__ w) {
+    constexpr int PCH = 4;            // hop slots per row and round
+    constexpr int SL = RG * PCH;      // slots per round
+    using T4 = typename Vec4<T>::type;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    T* xs_self = reinterpret_cast<T*>(smem_raw);
-    T* xrows = xs_self + A.nN_max;
-    T* vrows = xrows + (size_t)A.pch * A.nN_max;
-    const int r = blockIdx.x;
+    T* X = reinterpret_cast<T*>(smem_raw);                      // nN_max * SL
+    T* Vs = X + (size_t)A.nN_max * SL;                           // cols2_max * SL
+    T* xs = Vs + (size_t)A.cols2_max * SL;                       // nN_max * RG  ([col][row])
+    const int g = blockIdx.x;
     const int tid = threadIdx.x;
-    const int bf = A.row_block[r];
-    const int fp = r - A.block_row0[bf];
+    const int bf = A.grp_block[g];
+    const int fp0 = A.grp_fp0[g];
     const int nN = A.nN[bf];
+    const int nr = min(RG, A.nP[bf] - fp0);
+    const int row0 = A.block_row0[bf] + fp0;          // global row index of fp0
     const int64_t bstart = A.tdims[bf];
-    const int64_t base = bstart + (int64_t)fp * nN;
     const T* ep = reinterpret_cast<const T*>(A.ep);
     const T* en = reinterpret_cast<const T*>(A.en) + A.en_off[bf];
-    for (int i = tid; i < nN; i += 256) xs_self[i] = v[base + i];
+    for (int i = tid; i < nN; i += 256) {
+#pragma unroll
+        for (int r = 0; r < RG; r++)
+            xs[i * RG + r] = r < nr ? v[bstart + (int64_t)(fp0 + r) * nN + i] : T(0);
+    }
     __syncthreads();
 
-    T acc[NPT];
-    const T epr = ep[r];
+    T acc[NPT][RG];
+    T epr[RG];
+#pragma unroll
+    for (int r = 0; r < RG; r++) epr[r] = r < nr ? ep[row0 + r] : T(0);
     const T* nnv = reinterpret_cast<const T*>(A.nn_val);
     const int64_t nsb = A.nn_slice_base[bf];
 #pragma unroll
     for (int k = 0; k < NPT; k++) {
         const int fn = tid + k * 256;
-        acc[k] = T(0);
+#pragma unroll
+        for (int r = 0; r < RG; r++) acc[k][r] = T(0);
         if (fn < nN) {
-            T a = (epr + en[fn]) * xs_self[fn];
+            const T e_n = en[fn];
+#pragma unroll
+            for (int r = 0; r < RG; r++) acc[k][r] = (epr[r] + e_n) * xs[fn * RG + r];
             const int64_t s = nsb + (fn >> 6);
-            const int lane = fn & 63;
-            const int64_t off = A.nn_slice_off[s] + lane;
+            const int64_t off = A.nn_slice_off[s] + (fn & 63);
             const int len = A.nn_slice_len[s];
             for (int q = 0; q < len; q++) {
                 const int64_t e = off + (int64_t)q * 64;
-                a += nnv[e] * xs_self[A.nn_col[e]];
+                const T c = nnv[e];
+                const int col = A.nn_col[e];
+#pragma unroll
+                for (int r = 0; r < RG; r++) acc[k][r] += c * xs[col * RG + r];
             }
-            acc[k] = a;
         }
     }
     // pp: rows of the same block
     const T* ppv = reinterpret_cast<const T*>(A.pp_val);
-    const int64_t pe1 = A.pp_rowptr[r + 1];
-    for (int64_t e = A.pp_rowptr[r]; e < pe1; e++) {
-        const T c = ppv[e];
-        const T* src = v + bstart + (int64_t)A.pp_col[e] * nN;
 #pragma unroll
-        for (int k = 0; k < NPT; k++) {
-            const int fn = tid + k * 256;
-            if (fn < nN) acc[k] += c * src[fn];
+    for (int r = 0; r < RG; r++) {
+        if (r < nr) {
+            const int64_t pe1 = A.pp_rowptr[row0 + r + 1];
+            for (int64_t e = A.pp_rowptr[row0 + r]; e < pe1; e++) {
+                const T c = ppv[e];
+                const T* src = v + bstart + (int64_t)A.pp_col[e] * nN;
+#pragma unroll
+                for (int k = 0; k < NPT; k++) {
+                    const int fn = tid + k * 256;
+                    if (fn < nN) acc[k][r] += c * src[fn];
+                }
+            }
         }
     }
     // pn
@@ -93,9 +126,14 @@ __global__ __launch_bounds__(256) void k_hv(HvTables A, const T* __restrict__ v,
     const int q1 = A.pair_ptr[bf + 1];
     for (int q = A.pair_ptr[bf]; q < q1; q++) {
         const int pt = A.pair_ptab[q];
-        const int32_t* prp = A.ptab_rowptr + A.ptab_rowptr_base[pt];
-        const int p0 = prp[fp], p1 = prp[fp + 1];
-        if (p0 == p1) continue;
+        const int32_t* prp = A.ptab_rowptr + A.ptab_rowptr_base[pt] + fp0;
+        int rb[RG + 1];
+#pragma unroll
+        for (int r = 0; r <= RG; r++) rb[r] = prp[min(r, nr)];
+        int nch = 0;
+#pragma unroll
+        for (int r = 0; r < RG; r++) nch = max(nch, (rb[r + 1] - rb[r] + PCH - 1) / PCH);
+        if (nch == 0) continue;
         const int bi = A.pair_bi[q];
         const int dm = A.pair_dm[q];
         const int nt = A.pair_ntab[q];
@@ -105,61 +143,146 @@ __global__ __launch_bounds__(256) void k_hv(HvTables A, const T* __restrict__ v,
         const T* V2 = veff + A.veff_off[dm];
         const int c2 = A.veff_cols2[dm];
         const int64_t sb = A.ntab_slice_base[nt];
-        for (int pc = p0; pc < p1; pc += A.pch) {
-            const int np = min(A.pch, p1 - pc);
-            __syncthreads();
-            for (int j = 0; j < np; j++) {
-                const uint32_t e = pent[pc + j];
-                const int ip = (int)(e & 0xFFFFFu);
-                const int op = (int)((e >> 20) & 0x7FFu);
-                const T sg = (e >> 31) ? T(-1) : T(1);
-                const T* src = v + ibase + (int64_t)ip * nNi;
-                T* dst = xrows + (size_t)j * A.nN_max;
-                for (int i = tid; i < nNi; i += 256) dst[i] = sg * src[i];
-                const T* vr = V2 + (int64_t)op * c2;
-                T* vd = vrows + (size_t)j * A.cols2_max;
-                for (int i = tid; i < c2; i += 256) vd[i] = vr[i];
+        const uint32_t zent = (uint32_t)(c2 - 2) << 20;      // zero column of V_eff
+        int64_t off[NPT];
+        int len[NPT];
+        int maxlen = 0;
+#pragma unroll
+        for (int k = 0; k < NPT; k++) {
+            const int fn = tid + k * 256;
+            if (fn < nN) {
+                const int64_t s = sb + (fn >> 6);
+                off[k] = A.ntab_slice_off[s] + (fn & 63);
+                len[k] = A.ntab_slice_len[s];
+            } else {
+                off[k] = 0;
+                len[k] = 0;
+            }
+            maxlen = max(maxlen, len[k]);
+        }
+        for (int ch = 0; ch < nch; ch++) {
+            // slot metadata (uniform)
+            const T* src[SL];
+            const T* vsrc[SL];
+            T sg[SL];
+            bool act_row[RG];
+#pragma unroll
+            for (int r = 0; r < RG; r++) {
+                act_row[r] = rb[r] + ch * PCH < rb[r + 1];
+#pragma unroll
+                for (int j = 0; j < PCH; j++) {
+                    const int h = rb[r] + ch * PCH + j;
+                    const int sl = r * PCH + j;
+                    if (h < rb[r + 1]) {
+                        const uint32_t e = pent[h];
+                        src[sl] = v + ibase + (int64_t)(e & 0xFFFFFu) * nNi;
+                        vsrc[sl] = V2 + (int64_t)((e >> 20) & 0x7FFu) * c2;
+                        sg[sl] = (e >> 31) ? T(-1) : T(1);
+                    } else {
+                        src[sl] = nullptr;
+                        vsrc[sl] = nullptr;
+                        sg[sl] = T(0);
+                    }
+                }
             }
             __syncthreads();
+            // staged slots are written four at a time (16-byte stores, no bank conflicts
+            // between neighbouring elements)
+            for (int i = tid; i < nNi; i += 256) {
+#pragma unroll
+                for (int r = 0; r < RG; r++) {
+                    T4 x4;
+                    x4.x = src[r * PCH + 0] ? sg[r * PCH + 0] * src[r * PCH + 0][i] : T(0);
+                    x4.y = src[r * PCH + 1] ? sg[r * PCH + 1] * src[r * PCH + 1][i] : T(0);
+                    x4.z = src[r * PCH + 2] ? sg[r * PCH + 2] * src[r * PCH + 2][i] : T(0);
+                    x4.w = src[r * PCH + 3] ? sg[r * PCH + 3] * src[r * PCH + 3][i] : T(0);
+                    *reinterpret_cast<T4*>(X + (size_t)i * SL + r * PCH) = x4;
+                }
+            }
+            for (int c = tid; c < c2; c += 256) {
+#pragma unroll
+                for (int r = 0; r < RG; r++) {
+                    T4 v4;
+                    v4.x = vsrc[r * PCH + 0] ? vsrc[r * PCH + 0][c] : T(0);
+                    v4.y = vsrc[r * PCH + 1] ? vsrc[r * PCH + 1][c] : T(0);
+                    v4.z = vsrc[r * PCH + 2] ? vsrc[r * PCH + 2][c] : T(0);
+                    v4.w = vsrc[r * PCH + 3] ? vsrc[r * PCH + 3][c] : T(0);
+                    *reinterpret_cast<T4*>(Vs + (size_t)c * SL + r * PCH) = v4;
+                }
+            }
+            __syncthreads();
+            // neutron-hop entries are prefetched two steps ahead of their use
+            uint32_t e0[NPT], e1[NPT];
 #pragma unroll
             for (int k = 0; k < NPT; k++) {
-                const int fn = tid + k * 256;
-                if (fn < nN) {
-                    const int64_t s = sb + (fn >> 6);
-                    const int lane = fn & 63;
-                    const int64_t off = A.ntab_slice_off[s] + lane;
-                    const int len = A.ntab_slice_len[s];
-                    T a = acc[k];
-                    for (int qq = 0; qq < len; qq++) {
-                        const uint32_t e = A.ntab_ent[off + (int64_t)qq * 64];
-                        const int in = (int)(e & 0xFFFFFu);
-                        const int col = (int)(e >> 20);
-                        for (int j = 0; j < np; j++)
-                            a += vrows[(size_t)j * A.cols2_max + col] * xrows[(size_t)j * A.nN_max + in];
+                e0[k] = 0 < len[k] ? A.ntab_ent[off[k]] : zent;
+                e1[k] = 1 < len[k] ? A.ntab_ent[off[k] + 64] : zent;
+            }
+            for (int qq = 0; qq < maxlen; qq++) {
+                int in[NPT], col[NPT];
+#pragma unroll
+                for (int k = 0; k < NPT; k++) {
+                    const uint32_t e = e0[k];
+                    in[k] = (int)(e & 0xFFFFFu);
+                    col[k] = (int)(e >> 20);
+                    e0[k] = e1[k];
+                    e1[k] = qq + 2 < len[k] ? A.ntab_ent[off[k] + (int64_t)(qq + 2) * 64] : zent;
+                }
+#pragma unroll
+                for (int r = 0; r < RG; r++) {
+                    if (act_row[r]) {
+#pragma unroll
+                        for (int k = 0; k < NPT; k++) {
+                            const T4 xv = *reinterpret_cast<const T4*>(X + (size_t)in[k] * SL + r * PCH);
+                            const T4 vv = *reinterpret_cast<const T4*>(Vs + (size_t)col[k] * SL + r * PCH);
+                            T a = acc[k][r];
+                            a += vv.x * xv.x;
+                            a += vv.y * xv.y;
+                            a += vv.z * xv.z;
+                            a += vv.w * xv.w;
+                            acc[k][r] = a;
+                        }
                     }
-                    acc[k] = a;
                 }
             }
         }
     }
 #pragma unroll
-    for (int k = 0; k < NPT; k++) {
-        const int fn = tid + k * 256;
-        if (fn < nN) w[base + fn] = acc[k];
+    for (int r = 0; r < RG; r++) {
+        if (r < nr) {
+            T* dst = w + bstart + (int64_t)(fp0 + r) * nN;
+#pragma unroll
+            for (int k = 0; k < NPT; k++) {
+                const int fn = tid + k * 256;
+                if (fn < nN) dst[fn] = acc[k][r];
+            }
+        }
     }
+}
+
+#define HV_FOR_SHAPES(X_, T_)                                                                        \
+    X_(T_, 1, 1) X_(T_, 2, 1) X_(T_, 3, 1) X_(T_, 4, 1) X_(T_, 6, 1) X_(T_, 8, 1) X_(T_, 12, 1) X_(T_, 16, 1) \
+    X_(T_, 1, 2) X_(T_, 2, 2) X_(T_, 3, 2) X_(T_, 4, 2) X_(T_, 6, 2) X_(T_, 8, 2) X_(T_, 12, 2) X_(T_, 16, 2)
+
+template <typename T> static const void* hv_fn(int npt, int rg) {
+#define HV_CASE(T_, N_, R_) if (npt == N_ && rg == R_) return (const void*)k_hv<T_, N_, R_>;
+    HV_FOR_SHAPES(HV_CASE, T)
+#undef HV_CASE
+    return nullptr;
 }
 
 template <typename T>
 static void launch_t(const HvTables& A, const T* v, T* w, hipStream_t s) {
-    dim3 grid(A.R), block(256);
-    switch (A.npt) {
-        case 1: hipLaunchKernelGGL((k_hv<T, 1>), grid, block, A.lds_bytes, s, A, v, w); break;
-        case 2: hipLaunchKernelGGL((k_hv<T, 2>), grid, block, A.lds_bytes, s, A, v, w); break;
-        case 4: hipLaunchKernelGGL((k_hv<T, 4>), grid, block, A.lds_bytes, s, A, v, w); break;
-        case 8: hipLaunchKernelGGL((k_hv<T, 8>), grid, block, A.lds_bytes, s, A, v, w); break;
-        case 16: hipLaunchKernelGGL((k_hv<T, 16>), grid, block, A.lds_bytes, s, A, v, w); break;
-        default: throw std::runtime_error("neutron block wider than 4096 determinants");
+    dim3 grid(A.G), block(256);
+    bool done = false;
+#define HV_LAUNCH(T_, N_, R_)                                                                  \
+    if (!done && A.npt == N_ && A.rg == R_) {                                                  \
+        hipLaunchKernelGGL((k_hv<T_, N_, R_>), grid, block, A.lds_bytes, s, A, v, w);         \
+        done = true;                                                                           \
     }
+    HV_FOR_SHAPES(HV_LAUNCH, T)
+#undef HV_LAUNCH
+    if (!done) throw std::runtime_error("no H·v kernel for this neutron block width");
     SM_HIP(hipGetLastError());
 }
 
@@ -180,6 +303,7 @@ template <typename T> std::vector<T> cast_vec(const std::vector<double>& v) {
 
 struct HostLayout {
     std::vector<int32_t> row_block, block_row0, nP, nN;
+    std::vector<double> row_cost;    // estimated FMAs per proton row (workgroup ordering)
     std::vector<int64_t> tdims, en_off;
     std::vector<double> ep, en;
     std::vector<int64_t> pp_rowptr;
@@ -365,6 +489,26 @@ void index_layout(const Basis& b, const Tables& t, HostLayout& L) {
         }
         L.pair_ptr.push_back((int32_t)L.pair_bi.size());
     }
+    // per-row cost estimate: pn hops x neutron-list entries, pp entries x nN, nn entries
+    L.row_cost.assign(L.row_block.size(), 0.0);
+    for (int bf = 0; bf < b.nb; bf++) {
+        const int nN = L.nN[bf];
+        for (int q = L.pair_ptr[bf]; q < L.pair_ptr[bf + 1]; q++) {
+            int pt = L.pair_ptab[q], nt = L.pair_ntab[q];
+            int64_t ne = 0;
+            int nsl = (nN + 63) / 64;
+            for (int s0 = 0; s0 < nsl; s0++) ne += 64 * (int64_t)L.ntab_slice_len[L.ntab_slice_base[nt] + s0];
+            const int32_t* prp = L.ptab_rowptr.data() + L.ptab_rowptr_base[pt];
+            for (int fp = 0; fp < L.nP[bf]; fp++) {
+                int nh = prp[fp + 1] - prp[fp];
+                L.row_cost[L.block_row0[bf] + fp] += (double)((nh + 3) / 4 * 4) * (double)ne;
+            }
+        }
+        for (int fp = 0; fp < L.nP[bf]; fp++) {
+            int r = L.block_row0[bf] + fp;
+            L.row_cost[r] += (double)(L.pp_rowptr[r + 1] - L.pp_rowptr[r]) * nN + 64.0 * nN;
+        }
+    }
 }
 
 template <typename T> void set_view(DeviceHam& h, int prec, const HostLayout& L) {
@@ -385,16 +529,35 @@ template <typename T> void set_view(DeviceHam& h, int prec, const HostLayout& L)
     A.veff_off = h.veff_off.as<int64_t>(); A.veff_cols2 = h.veff_cols2.as<int32_t>(); A.veff = h.veff[prec].p;
     A.nN_max = std::max(1, L.nN_max);
     A.cols2_max = std::max(2, L.cols2_max);
-    const size_t budget = 64 * 1024 / sizeof(T);
-    if ((size_t)A.nN_max + A.nN_max + A.cols2_max > budget)
-        throw std::runtime_error("neutron block too wide for the LDS budget");
-    int pch = (int)((budget - A.nN_max) / (size_t)(A.nN_max + A.cols2_max));
-    A.pch = std::max(1, std::min(16, pch));
-    A.lds_bytes = ((size_t)A.nN_max + (size_t)A.pch * (A.nN_max + A.cols2_max)) * sizeof(T);
+    auto lds_for = [&](int rg) { return ((size_t)A.nN_max * 4 * rg + (size_t)A.cols2_max * 4 * rg +
+                                         (size_t)A.nN_max * rg) * sizeof(T); };
+    A.rg = lds_for(2) <= HV_LDS_TWO_ROWS ? 2 : 1;
+    A.lds_bytes = lds_for(A.rg);
+    if (A.lds_bytes > 160 * 1024) throw std::runtime_error("neutron block too wide for the LDS of one CU");
     int need = (A.nN_max + 255) / 256;
-    int npt = 1;
-    while (npt < need) npt *= 2;
-    A.npt = npt;
+    const int npts[] = {1, 2, 3, 4, 6, 8, 12, 16};
+    A.npt = 0;
+    for (int x : npts) if (x >= need) { A.npt = x; break; }
+    if (A.npt == 0) throw std::runtime_error("neutron block wider than 4096 determinants");
+    // row groups, most expensive first (longest-processing-time order)
+    std::vector<std::pair<double, std::pair<int, int>>> grp;
+    for (int k = 0; k < h.nb; k++)
+        for (int i = 0; i < L.nP[k]; i += A.rg) {
+            double c = 0.0;
+            for (int r = i; r < std::min(L.nP[k], i + A.rg); r++) c += L.row_cost[L.block_row0[k] + r];
+            grp.push_back({c, {k, i}});
+        }
+    std::stable_sort(grp.begin(), grp.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+    std::vector<int32_t> gb, gf;
+    for (auto& x : grp) { gb.push_back(x.second.first); gf.push_back(x.second.second); }
+    h.grp_block[prec].upload(gb);
+    h.grp_fp0[prec].upload(gf);
+    A.G = (int)gb.size();
+    A.grp_block = h.grp_block[prec].as<int32_t>();
+    A.grp_fp0 = h.grp_fp0[prec].as<int32_t>();
+    const void* fn = hv_fn<T>(A.npt, A.rg);
+    if (!fn) throw std::runtime_error("no H·v kernel instance for this shape");
+    SM_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)A.lds_bytes));
 }
 
 template <typename T> void upload_coef(DeviceHam& h, int prec, const HostLayout& L) {

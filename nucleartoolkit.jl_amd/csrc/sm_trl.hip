@@ -146,64 +146,154 @@ __global__ __launch_bounds__(TB) void k_updatew(const TBv* __restrict__ B, int64
     if (threadIdx.x == 0) part[blockIdx.x] = nr;
 }
 
-// Middle pass of CGS2, fused: w1 = win - V h1 (written to wout) and the partial
-// dot products V^T w1 of the second pass, from one read of V.  Each block
-// stages X-wide slabs V[0..nv-1][x0..x0+X) in LDS (row stride X+1, no bank
-// conflicts); thread t owns the second-pass sums of j = t + 256 q.
-template <typename TBv, typename TW, int X>
-__global__ __launch_bounds__(TB) void k_cgs_mid(const TBv* __restrict__ B, int64_t ldb, int nv,
-                                                const double* __restrict__ h1, const TW* __restrict__ win,
-                                                double* __restrict__ wout, int64_t n, double* __restrict__ part,
+// ---- CGS2 streaming passes.  Basis rows are read with 16-byte loads: a lane
+// owns 4 consecutive elements (FP32) or 2 (FP64) per load, a wave instruction
+// covers 1 KiB of one basis vector.  A block covers SEG elements of every basis
+// vector; its waves take different vectors (j = wave, wave + 4, ...).
+template <typename T> struct Ld16;
+template <> struct Ld16<float> {
+    static constexpr int W = 4;
+    __device__ static inline void get(const float* p, double* o) {
+        const float4 v = *reinterpret_cast<const float4*>(p);
+        o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+    }
+};
+template <> struct Ld16<double> {
+    static constexpr int W = 2;
+    __device__ static inline void get(const double* p, double* o) {
+        const double2 v = *reinterpret_cast<const double2*>(p);
+        o[0] = v.x; o[1] = v.y;
+    }
+};
+
+// elements per lane per basis vector and per-block segment length
+template <typename TBv> struct GsShape {
+    static constexpr int Q = 4;                              // 16-byte loads per lane per vector
+    static constexpr int E = Q * Ld16<TBv>::W;               // elements per lane
+    static constexpr int SEG = 64 * E;                       // elements per block (one wave width)
+};
+
+// element index of (lane, q, i) within the segment: lane-contiguous 16-byte pieces
+template <typename TBv> __device__ inline int64_t gs_x(int64_t seg0, int lane, int q, int i) {
+    constexpr int W = Ld16<TBv>::W;
+    return seg0 + (int64_t)q * 64 * W + (int64_t)lane * W + i;
+}
+
+// part[blk*ldp + j] = sum_{x in segment} B[j][x] w[x]   (ld of B is a multiple of 64)
+template <typename TBv, typename TW>
+__global__ __launch_bounds__(TB) void k_gs_dots(const TBv* __restrict__ B, int64_t ldb, int nv,
+                                                const TW* __restrict__ w, int64_t n, double* __restrict__ part,
                                                 int ldp) {
-    extern __shared__ __align__(16) unsigned char lds_raw[];
-    double* hs = reinterpret_cast<double*>(lds_raw);             // nv
-    double* red = hs + 1024;                                     // TB
-    double* w1s = red + TB;                                      // X
-    TBv* Vs = reinterpret_cast<TBv*>(w1s + X);                   // nv * (X+1)
-    constexpr int S = TB / X;
-    const int t = threadIdx.x;
-    for (int j = t; j < nv; j += TB) hs[j] = h1[j];
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int64_t c = blockIdx.x; c * X < n; c += gridDim.x) {
-        const int64_t x0 = c * X;
-        __syncthreads();
-        for (int idx = t; idx < nv * X; idx += TB) {
-            const int j = idx / X, xx = idx - j * X;
-            const int64_t x = x0 + xx;
-            Vs[j * (X + 1) + xx] = x < n ? B[(int64_t)j * ldb + x] : TBv(0);
-        }
-        __syncthreads();
-        {
-            const int xx = t % X, sl = t / X;
-            double sum = 0.0;
-            for (int j = sl; j < nv; j += S) sum += hs[j] * (double)Vs[j * (X + 1) + xx];
-            red[sl * X + xx] = sum;
-        }
-        __syncthreads();
-        if (t < X) {
-            double sum = 0.0;
-            for (int q = 0; q < S; q++) sum += red[q * X + t];
-            const int64_t x = x0 + t;
-            double w1 = 0.0;
-            if (x < n) { w1 = (double)win[x] - sum; wout[x] = w1; }
-            w1s[t] = w1;
-        }
-        __syncthreads();
+    using S = GsShape<TBv>;
+    constexpr int W = Ld16<TBv>::W;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t seg0 = (int64_t)blockIdx.x * S::SEG;
+    double a[S::E];
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int j = t + TB * q;
-            if (j < nv) {
-                double d = 0.0;
+    for (int q = 0; q < S::Q; q++)
 #pragma unroll
-                for (int xx = 0; xx < X; xx++) d += (double)Vs[j * (X + 1) + xx] * w1s[xx];
-                acc[q] += d;
+        for (int i = 0; i < W; i++) {
+            const int64_t x = gs_x<TBv>(seg0, lane, q, i);
+            a[q * W + i] = x < n ? (double)w[x] : 0.0;
+        }
+    const bool full = seg0 + S::SEG <= n;
+#pragma unroll 2
+    for (int j = wv; j < nv; j += TB / 64) {
+        const TBv* b = B + (int64_t)j * ldb;
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < S::Q; q++) {
+            const int64_t x = gs_x<TBv>(seg0, lane, q, 0);
+            double v[W];
+            if (full) {
+                Ld16<TBv>::get(b + x, v);
+            } else {
+                // tail segment: elements past n (padding of the basis rows) count as 0
+                if (x < ldb) Ld16<TBv>::get(b + x, v);
+                for (int i = 0; i < W; i++) if (!(x < ldb) || x + i >= n) v[i] = 0.0;
+            }
+#pragma unroll
+            for (int i = 0; i < W; i++) s += v[i] * a[q * W + i];
+        }
+        s = wave_sum(s);
+        if (lane == 0) part[(int64_t)blockIdx.x * ldp + j] = s;
+    }
+}
+
+// wout = win - sum_j h[j] B[j]  and, if dots, part = partial B^T wout; npart = |wout|^2 partial.
+// Each wave owns a quarter of the block's segment for the update and then takes
+// vectors j = wave, wave+4, ... for the dot products (its segment rows were
+// just streamed by the block, so this second read hits the caches).
+template <typename TBv, typename TW, bool DOTS>
+__global__ __launch_bounds__(TB) void k_gs_upd(const TBv* __restrict__ B, int64_t ldb, int nv,
+                                               const double* __restrict__ h, const TW* __restrict__ win,
+                                               double* __restrict__ wout, int64_t n, double* __restrict__ part,
+                                               int ldp, double* __restrict__ npart) {
+    using S = GsShape<TBv>;
+    constexpr int W = Ld16<TBv>::W;
+    constexpr int QW = S::Q / 4 > 0 ? S::Q / 4 : 1;            // 16-byte pieces per lane in the update
+    __shared__ double hs[1024];
+    __shared__ double ws[S::SEG];
+    __shared__ double red[TB / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int j = threadIdx.x; j < nv; j += TB) hs[j] = h[j];
+    __syncthreads();
+    const int64_t seg0 = (int64_t)blockIdx.x * S::SEG;
+    // update: thread t owns elements t*W .. t*W+W-1 of each of QW quarter-rows
+    double nr = 0.0;
+#pragma unroll
+    for (int qw = 0; qw < QW; qw++) {
+        const int loc = (qw * TB + threadIdx.x) * W;         // position inside the segment
+        const int64_t x = seg0 + loc;
+        double a[W];
+#pragma unroll
+        for (int i = 0; i < W; i++) a[i] = (x + i < n) ? (double)win[x + i] : 0.0;
+        if (x < ldb) {
+#pragma unroll 4
+            for (int j = 0; j < nv; j++) {
+                double v[W];
+                Ld16<TBv>::get(B + (int64_t)j * ldb + x, v);
+                const double c = hs[j];
+#pragma unroll
+                for (int i = 0; i < W; i++) a[i] -= c * v[i];
             }
         }
-    }
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const int j = t + TB * q;
-        if (j < nv) part[(int64_t)blockIdx.x * ldp + j] = acc[q];
+        for (int i = 0; i < W; i++) {
+            if (x + i < n) { wout[x + i] = a[i]; nr += a[i] * a[i]; }
+            else a[i] = 0.0;
+            ws[loc + i] = a[i];
+        }
+    }
+    nr = block_sum(nr, red);
+    if (threadIdx.x == 0) npart[blockIdx.x] = nr;
+    if (!DOTS) return;
+    __syncthreads();
+    double wr[S::E];
+#pragma unroll
+    for (int q = 0; q < S::Q; q++)
+#pragma unroll
+        for (int i = 0; i < W; i++) wr[q * W + i] = ws[q * 64 * W + lane * W + i];
+    const bool full = seg0 + S::SEG <= n;
+#pragma unroll 2
+    for (int j = wv; j < nv; j += TB / 64) {
+        const TBv* b = B + (int64_t)j * ldb;
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < S::Q; q++) {
+            const int64_t x = gs_x<TBv>(seg0, lane, q, 0);
+            double v[W];
+            if (full) {
+                Ld16<TBv>::get(b + x, v);
+            } else {
+                if (x < ldb) Ld16<TBv>::get(b + x, v);
+                for (int i = 0; i < W; i++) if (!(x < ldb) || x + i >= n) v[i] = 0.0;
+            }
+#pragma unroll
+            for (int i = 0; i < W; i++) s += v[i] * wr[q * W + i];
+        }
+        s = wave_sum(s);
+        if (lane == 0) part[(int64_t)blockIdx.x * ldp + j] = s;
     }
 }
 
@@ -353,7 +443,7 @@ struct Solver {
     int lm, ls, nev;
     int prec_hv;   // 0 fp64, 1 fp32
     int ldp;       // partial-sum row stride
-    int64_t Gd, Gu;
+    int64_t Gd, Gu, Gx;
     int gstore;
     Workspace W;
 
@@ -368,6 +458,7 @@ struct Solver {
         ldp = lm + 2;
         Gd = (n + CHUNK - 1) / CHUNK;
         Gu = (n + UCHUNK - 1) / UCHUNK;
+        Gx = (n + GsShape<TBv>::SEG - 1) / GsShape<TBv>::SEG;
         gstore = (int)std::min<int64_t>(4096, (n + TB - 1) / TB);
     }
 
@@ -375,36 +466,24 @@ struct Solver {
     double* en(int step) { return W.enring.as<double>() + (int64_t)((step + 1) & 3) * 64; }
     int* flag(int step) { return W.flags.as<int>() + ((step + 1) & 3); }
 
-    static constexpr int XM = sizeof(TBv) == 4 ? 32 : 16;   // slab width of k_cgs_mid (128-byte rows)
-    int64_t Gm = 1;
-    size_t mid_lds(int nv) const { return (size_t)(1024 + TB + XM) * 8 + (size_t)nv * (XM + 1) * sizeof(TBv); }
 
     // CGS2 of `win` against V[0..nv-1] into w64 (classical Gram-Schmidt twice):
     //   h1 = V^T win;  w1 = win - V h1;  h2 = V^T w1;  w64 = w1 - V h2
+    // in three streaming passes (the middle one computes w1 and the h2 partials).
     // W.h1 keeps the first-pass coefficients, W.npart the |w64|^2 partials.
     template <typename TW>
     void cgs2(int nv, const TW* win) {
         const TBv* B = V(0);
         double* part = W.part.as<double>();
-        hipLaunchKernelGGL((k_dotsw<TBv, TW>), dim3(Gd), dim3(TB), 0, s, B, ld, nv, win, n, part, ldp, 0,
+        hipLaunchKernelGGL((k_gs_dots<TBv, TW>), dim3(Gx), dim3(TB), 0, s, B, ld, nv, win, n, part, ldp);
+        hipLaunchKernelGGL(k_reduce, dim3(nv), dim3(TB), 0, s, part, (int)Gx, ldp, W.h1.as<double>(),
                            (const int*)nullptr);
-        hipLaunchKernelGGL(k_reduce, dim3(nv), dim3(TB), 0, s, part, (int)Gd, ldp, W.h1.as<double>(),
+        hipLaunchKernelGGL((k_gs_upd<TBv, TW, true>), dim3(Gx), dim3(TB), 0, s, B, ld, nv, W.h1.as<double>(), win,
+                           W.w64.as<double>(), n, part, ldp, W.npart.as<double>());
+        hipLaunchKernelGGL(k_reduce, dim3(nv), dim3(TB), 0, s, part, (int)Gx, ldp, W.h2.as<double>(),
                            (const int*)nullptr);
-        if (mid_lds(nv) <= 64 * 1024) {
-            hipLaunchKernelGGL((k_cgs_mid<TBv, TW, XM>), dim3(Gm), dim3(TB), mid_lds(nv), s, B, ld, nv,
-                               W.h1.as<double>(), win, W.w64.as<double>(), n, part, ldp);
-            hipLaunchKernelGGL(k_reduce, dim3(nv), dim3(TB), 0, s, part, (int)Gm, ldp, W.h2.as<double>(),
-                               (const int*)nullptr);
-        } else {
-            hipLaunchKernelGGL((k_updatew<TBv, TW>), dim3(Gu), dim3(TB), 0, s, B, ld, nv, W.h1.as<double>(), win,
-                               W.w64.as<double>(), n, W.npart.as<double>(), (const int*)nullptr);
-            hipLaunchKernelGGL((k_dotsw<TBv, double>), dim3(Gd), dim3(TB), 0, s, B, ld, nv, W.w64.as<double>(), n,
-                               part, ldp, 0, (const int*)nullptr);
-            hipLaunchKernelGGL(k_reduce, dim3(nv), dim3(TB), 0, s, part, (int)Gd, ldp, W.h2.as<double>(),
-                               (const int*)nullptr);
-        }
-        hipLaunchKernelGGL((k_updatew<TBv, double>), dim3(Gu), dim3(TB), 0, s, B, ld, nv, W.h2.as<double>(),
-                           W.w64.as<double>(), W.w64.as<double>(), n, W.npart.as<double>(), (const int*)nullptr);
+        hipLaunchKernelGGL((k_gs_upd<TBv, double, false>), dim3(Gx), dim3(TB), 0, s, B, ld, nv, W.h2.as<double>(),
+                           W.w64.as<double>(), W.w64.as<double>(), n, part, ldp, W.npart.as<double>());
     }
 
     void enqueue_iteration(int it, int step, int nprev) {
@@ -414,7 +493,7 @@ struct Solver {
         const int nv = it + 1;
         if (prec_hv == 0) cgs2<double>(nv, W.wH.as<double>());
         else cgs2<float>(nv, W.wH.as<float>());
-        hipLaunchKernelGGL(k_finish, dim3(1), dim3(TB), 0, s, W.h1.as<double>(), it, W.npart.as<double>(), (int)Gu,
+        hipLaunchKernelGGL(k_finish, dim3(1), dim3(TB), 0, s, W.h1.as<double>(), it, W.npart.as<double>(), (int)Gx,
                            W.alpha.as<double>(), W.beta.as<double>());
         const int nn = std::min(nev, it + 1);
         hipLaunchKernelGGL(k_ritz, dim3(nn), dim3(TB), 0, s, W.alpha.as<double>(), W.beta.as<double>(), it + 1,
@@ -429,15 +508,15 @@ struct Solver {
         W.basis.alloc((size_t)lm * ld * sizeof(TBv));
         W.wH.alloc((size_t)ld * (prec_hv ? 4 : 8));
         W.w64.alloc((size_t)ld * 8);
-        Gm = std::max<int64_t>(1, std::min<int64_t>(2048, (n + XM - 1) / XM));
-        W.part.alloc((size_t)std::max(Gd, Gm) * ldp * 8);
-        W.npart.alloc((size_t)Gu * 8);
+        W.part.alloc((size_t)std::max(Gd, Gx) * ldp * 8);
+        W.npart.alloc((size_t)std::max(Gu, Gx) * 8);
         W.h1.alloc((lm + 2) * 8); W.h2.alloc((lm + 2) * 8);
         W.alpha.alloc((lm + 1) * 8); W.beta.alloc((lm + 1) * 8);
         W.enring.alloc(4 * 64 * 8); W.flags.alloc(4 * sizeof(int));
         W.S.alloc((size_t)lm * 32 * 8);
         W.swork.alloc((size_t)(5 * lm + 32 * (lm + 3 * 32 + 8) + 64) * 8);
         W.U.alloc((size_t)32 * ld * sizeof(TBv));
+        SM_HIP(hipMemsetAsync(W.U.p, 0, W.U.bytes, s));
         W.scal.alloc(64 * 8);
         SM_HIP(hipMemsetAsync(W.basis.p, 0, W.basis.bytes, s));
         SM_HIP(hipStreamSynchronize(s));
@@ -536,7 +615,7 @@ struct Solver {
 
     // V(j) = w64 / |w64|  (norm partials of w64 already in W.npart)
     void store_normalised(int j) {
-        hipLaunchKernelGGL(k_norm_from_parts, dim3(1), dim3(TB), 0, s, W.npart.as<double>(), (int)Gu,
+        hipLaunchKernelGGL(k_norm_from_parts, dim3(1), dim3(TB), 0, s, W.npart.as<double>(), (int)Gx,
                            W.scal.as<double>());
         hipLaunchKernelGGL((k_store<TBv>), dim3(gstore), dim3(TB), 0, s, W.w64.as<double>(), W.scal.as<double>(),
                            V(j), n);
