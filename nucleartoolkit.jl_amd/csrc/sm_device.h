@@ -28,6 +28,7 @@ struct DBuf {
     DBuf& operator=(const DBuf&) = delete;
     ~DBuf();
     void alloc(size_t n);
+    void ensure(size_t n) { if (bytes < n || !p) alloc(n); }   // grow-only reuse
     template <typename T> void upload(const std::vector<T>& v) {
         alloc(v.size() * sizeof(T));
         if (!v.empty()) SM_HIP(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
@@ -98,6 +99,7 @@ struct DeviceHam {
     std::vector<std::pair<std::string, int64_t>> table_bytes(int prec) const;
     int64_t pn_terms = 0, pp_terms = 0, nn_terms = 0;   // FMAs per H·v (padded n-table entries included)
     int64_t pn_terms_exact = 0;
+    std::shared_ptr<void> trl_ws[2];   // Lanczos workspace per precision, reused across solves
     ~DeviceHam();
 };
 

@@ -393,13 +393,64 @@ __global__ void k_conv(const double* prev, double* cur, int num_ev, int n, doubl
     *flag = ok;
 }
 
-// restart / final small problems, one thread (rare, O(lm * ls) work)
-__global__ void k_restart_small(double* alpha, double* beta, int m, int ls, double* S, double* work) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) tri_restart(alpha, beta, m, ls, S, work);
+// Eigenvectors of T[0..n-1] for the k eigenvalues vals (ascending) by inverse
+// iteration: one thread per cluster of close eigenvalues (chained gaps below
+// 1e-3 |T|); members of a cluster are orthogonalised against the earlier ones
+// exactly as tri_lowest does.  work: k * 5n doubles.
+__global__ void k_tri_vecs(const double* __restrict__ alpha, const double* __restrict__ beta, int n,
+                           const double* __restrict__ vals, int k, double* __restrict__ Y, double* __restrict__ work) {
+    const int t = threadIdx.x;
+    if (t >= k) return;
+    const double tn = tri_norm(alpha, beta, n);
+    const double ortol = 1e-3 * tn;
+    if (t > 0 && fabs(vals[t] - vals[t - 1]) < ortol) return;       // not a cluster leader
+    int c1 = t + 1;
+    while (c1 < k && fabs(vals[c1] - vals[c1 - 1]) < ortol) c1++;
+    for (int j = t; j < c1; j++) {
+        const int first = tri_cluster_first(vals, j, ortol);
+        tri_eigvec(alpha, beta, n, vals[j], Y + (long)j * n, work + (long)t * 5 * n, Y + (long)first * n, j - first, tn);
+    }
 }
-__global__ void k_lowest_small(const double* alpha, const double* beta, int n, int k, double* vals, double* Y,
-                               double* work) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) tri_lowest(alpha, beta, n, k, vals, Y, work);
+
+// restart, step 3: coupling vector sigma and the small Lanczos on diag(theta);
+// overwrites alpha/beta[0..ls-1] with the new leading block
+__global__ void k_restart_q(double* alpha, double* beta, int m, int ls, const double* __restrict__ theta,
+                            const double* __restrict__ Y, double* __restrict__ Qrev, double* __restrict__ work) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double* sig = work;
+    const double bcoup = beta[m - 1];
+    for (int j = 0; j < ls; j++) sig[j] = bcoup * Y[(long)j * m + (m - 1)];
+    tri_diag_lanczos(theta, sig, ls, Qrev, alpha, beta, work + ls);
+}
+
+// restart, step 4: S = Y Qrev (m x ls), column j at S + j*m
+__global__ void k_restart_s(const double* __restrict__ Y, const double* __restrict__ Qrev, int m, int ls,
+                            double* __restrict__ S) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    for (int j = 0; j < ls; j++) {
+        double s = 0.0;
+        for (int t = 0; t < ls; t++) s += Y[(long)t * m + i] * Qrev[(long)j * ls + t];
+        S[(long)j * m + i] = s;
+    }
+}
+
+// start vector: counter-based normal deviates (SplitMix64 + Box-Muller), so the
+// seed alone fixes the vector for any launch shape
+__device__ inline uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+__global__ void k_randn(double* __restrict__ v, int64_t n, uint64_t seed) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t a = splitmix64(seed ^ splitmix64(2 * (uint64_t)i));
+        const uint64_t b = splitmix64(seed ^ splitmix64(2 * (uint64_t)i + 1));
+        const double u1 = ((double)(a >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+        const double u2 = (double)(b >> 11) * (1.0 / 9007199254740992.0);
+        v[i] = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+    }
 }
 
 // U[k][x] = sum_j S[k*m + j] * B[j][x]   (k < nk <= 32)
@@ -426,11 +477,24 @@ __global__ __launch_bounds__(TB) void k_rotate(const TBv* __restrict__ B, int64_
     }
 }
 
+// restart small problem on a stream: Ritz values, Ritz vectors, rotation to tridiagonal form
+static void restart_small_on_stream(double* alpha, double* beta, int m, int ls, double* S, double* vals,
+                                    double* Y, double* Qrev, double* work, hipStream_t st) {
+    hipLaunchKernelGGL(k_ritz, dim3(ls), dim3(TB), 0, st, (const double*)alpha, (const double*)beta, m,
+                       (const double*)nullptr, 0, vals);
+    hipLaunchKernelGGL(k_tri_vecs, dim3(1), dim3(64), 0, st, (const double*)alpha, (const double*)beta, m,
+                       (const double*)vals, ls, Y, work);
+    hipLaunchKernelGGL(k_restart_q, dim3(1), dim3(64), 0, st, alpha, beta, m, ls, (const double*)vals,
+                       (const double*)Y, Qrev, work);
+    hipLaunchKernelGGL(k_restart_s, dim3((m + 63) / 64), dim3(64), 0, st, (const double*)Y, (const double*)Qrev,
+                       m, ls, S);
+}
+
 // ------------------------------------------------------------------ driver
 namespace {
 
 struct Workspace {
-    DBuf basis, wH, w64, part, npart, h1, h2, alpha, beta, enring, flags, S, swork, U, scal;
+    DBuf basis, wH, w64, part, npart, h1, h2, alpha, beta, enring, flags, S, swork, U, scal, tvals, Y, Q, R, hv64;
 };
 
 template <typename TBv>
@@ -445,9 +509,15 @@ struct Solver {
     int ldp;       // partial-sum row stride
     int64_t Gd, Gu, Gx;
     int gstore;
-    Workspace W;
+    Workspace& W;
 
-    Solver(DeviceHam& h, const SolveParams& p, SolveStats& stats) : H(h), P(p), st(stats) {
+    static Workspace& cached(DeviceHam& h) {
+        const int k = sizeof(TBv) == 4 ? 1 : 0;
+        if (!h.trl_ws[k]) h.trl_ws[k] = std::make_shared<Workspace>();
+        return *static_cast<Workspace*>(h.trl_ws[k].get());
+    }
+
+    Solver(DeviceHam& h, const SolveParams& p, SolveStats& stats) : H(h), P(p), st(stats), W(cached(h)) {
         s = H.stream;
         n = H.mdim;
         ld = (n + 63) / 64 * 64;
@@ -505,39 +575,42 @@ struct Solver {
     }
 
     void setup(const double* v0_in) {
-        W.basis.alloc((size_t)lm * ld * sizeof(TBv));
-        W.wH.alloc((size_t)ld * (prec_hv ? 4 : 8));
-        W.w64.alloc((size_t)ld * 8);
-        W.part.alloc((size_t)std::max(Gd, Gx) * ldp * 8);
-        W.npart.alloc((size_t)std::max(Gu, Gx) * 8);
-        W.h1.alloc((lm + 2) * 8); W.h2.alloc((lm + 2) * 8);
-        W.alpha.alloc((lm + 1) * 8); W.beta.alloc((lm + 1) * 8);
-        W.enring.alloc(4 * 64 * 8); W.flags.alloc(4 * sizeof(int));
-        W.S.alloc((size_t)lm * 32 * 8);
-        W.swork.alloc((size_t)(5 * lm + 32 * (lm + 3 * 32 + 8) + 64) * 8);
-        W.U.alloc((size_t)32 * ld * sizeof(TBv));
-        SM_HIP(hipMemsetAsync(W.U.p, 0, W.U.bytes, s));
-        W.scal.alloc(64 * 8);
-        SM_HIP(hipMemsetAsync(W.basis.p, 0, W.basis.bytes, s));
-        SM_HIP(hipStreamSynchronize(s));
+        // grow-only buffers cached on the handle (no per-solve hipMalloc / memset)
+        W.basis.ensure((size_t)lm * ld * sizeof(TBv));
+        W.wH.ensure((size_t)ld * (prec_hv ? 4 : 8));
+        W.w64.ensure((size_t)ld * 8);
+        W.part.ensure((size_t)std::max(Gd, Gx) * ldp * 8);
+        W.npart.ensure((size_t)std::max(Gu, Gx) * 8);
+        W.h1.ensure((lm + 2) * 8); W.h2.ensure((lm + 2) * 8);
+        W.alpha.ensure((lm + 1) * 8); W.beta.ensure((lm + 1) * 8);
+        W.enring.ensure(4 * 64 * 8); W.flags.ensure(4 * sizeof(int));
+        W.S.ensure((size_t)lm * 32 * 8);
+        W.swork.ensure((size_t)(32 * 5 * lm + 4096) * 8);
+        W.U.ensure((size_t)32 * ld * sizeof(TBv));
+        W.scal.ensure(64 * 8);
+        W.tvals.ensure(64 * 8);
+        W.Y.ensure((size_t)32 * lm * 8);
+        W.Q.ensure(32 * 32 * 8);
         std::vector<double> e0(4 * 64, 1.0e4);
-        SM_HIP(hipMemcpy(W.enring.p, e0.data(), e0.size() * 8, hipMemcpyHostToDevice));
-        // start vector: caller's, or seeded normal (initialize_wf, :939-948); normalised in FP64
-        std::vector<double> v0(n);
+        SM_HIP(hipMemcpyAsync(W.enring.p, e0.data(), e0.size() * 8, hipMemcpyHostToDevice, s));
+        // start vector: caller's, or seeded normal deviates generated on the device
+        // (initialize_wf, :939-948); normalised in FP64
         if (v0_in) {
-            for (int64_t i = 0; i < n; i++) v0[i] = v0_in[i];
+            SM_HIP(hipMemcpyAsync(W.w64.p, v0_in, n * 8, hipMemcpyHostToDevice, s));
         } else {
-            std::mt19937_64 rng(P.seed);
-            std::normal_distribution<double> nd(0.0, 1.0);
-            for (int64_t i = 0; i < n; i++) v0[i] = nd(rng);
+            hipLaunchKernelGGL(k_randn, dim3(gstore), dim3(TB), 0, s, W.w64.as<double>(), n, (uint64_t)P.seed);
         }
-        double nr = 0.0;
-        for (int64_t i = 0; i < n; i++) nr += v0[i] * v0[i];
-        if (!(nr > 0.0)) throw std::invalid_argument("start vector is zero");
-        SM_HIP(hipMemcpy(W.w64.p, v0.data(), n * 8, hipMemcpyHostToDevice));
-        hipLaunchKernelGGL((k_convert<double, TBv>), dim3(gstore), dim3(TB), 0, s, W.w64.as<double>(), V(0), n,
-                           1.0 / std::sqrt(nr));
+        hipLaunchKernelGGL((k_dotsw<double, double>), dim3(Gd), dim3(TB), 0, s, (const double*)nullptr, ld, 0,
+                           (const double*)W.w64.as<double>(), n, W.part.as<double>(), ldp, 1, (const int*)nullptr);
+        hipLaunchKernelGGL(k_reduce, dim3(1), dim3(TB), 0, s, W.part.as<double>(), (int)Gd, ldp,
+                           W.scal.as<double>() + 1, (const int*)nullptr);
+        hipLaunchKernelGGL(k_sqrt1, dim3(1), dim3(1), 0, s, W.scal.as<double>() + 1, W.scal.as<double>());
+        hipLaunchKernelGGL((k_store<TBv>), dim3(gstore), dim3(TB), 0, s, W.w64.as<double>(), W.scal.as<double>(),
+                           V(0), n);
         SM_HIP(hipStreamSynchronize(s));
+        double nr = 0.0;
+        SM_HIP(hipMemcpy(&nr, W.scal.p, 8, hipMemcpyDeviceToHost));
+        if (!(nr > 0.0) || !std::isfinite(nr)) throw std::invalid_argument("start vector is zero or not finite");
     }
 
     // nsteps plain Lanczos steps (no convergence test, no restart); alpha/beta out
@@ -607,7 +680,7 @@ struct Solver {
         std::vector<double> en_last(nev);
         SM_HIP(hipMemcpy(en_last.data(), en(conv_step), nev * 8, hipMemcpyDeviceToHost));
         for (int i = 0; i < nev; i++) { evals[i] = en_last[i]; st.ritz[i] = en_last[i]; }
-        finish(elit, evals, evecs);
+        finish(elit, conv_step, evals, evecs);
         for (int q = 0; q < 4; q++) (void)hipEventDestroy(ev[q]);
         SM_HIP(hipHostFree(hflag));
         st.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -624,8 +697,8 @@ struct Solver {
     void restart() {
         int m = lm - 1;
         if (std::getenv("SMGPU_DEBUG_RESTART")) debug_sweep();
-        hipLaunchKernelGGL(k_restart_small, dim3(1), dim3(64), 0, s, W.alpha.as<double>(), W.beta.as<double>(),
-                           m, ls, W.S.as<double>(), W.swork.as<double>());
+        restart_small_on_stream(W.alpha.as<double>(), W.beta.as<double>(), m, ls, W.S.as<double>(),
+                                W.tvals.as<double>(), W.Y.as<double>(), W.Q.as<double>(), W.swork.as<double>(), s);
         int gb = (int)std::min<int64_t>(2048, (n + TB - 1) / TB);
         size_t sh = (size_t)ls * m * 8;
         if (sh > 64 * 1024) throw std::invalid_argument("ls*(lm-1) too large for the restart rotation");
@@ -728,25 +801,24 @@ struct Solver {
         }
     }
 
-    void finish(int elit, double* evals, double* evecs) {
+    void finish(int elit, int conv_step, double* evals, double* evecs) {
         bool need_vecs = evecs != nullptr || (prec_hv == 1 && P.refine);
         if (!need_vecs) return;
-        DBuf vals, Y, work;
-        vals.alloc(nev * 8);
-        Y.alloc((size_t)nev * elit * 8);
-        work.alloc((size_t)(5 * elit + 8) * 8);
-        hipLaunchKernelGGL(k_lowest_small, dim3(1), dim3(64), 0, s, W.alpha.as<double>(), W.beta.as<double>(),
-                           elit, nev, vals.as<double>(), Y.as<double>(), work.as<double>());
-        DBuf R;
-        R.alloc((size_t)nev * ld * 8);
+        // eigenvectors of T[0..elit-1] for the converged Ritz values
+        double* Y = W.Y.as<double>();
+        hipLaunchKernelGGL(k_tri_vecs, dim3(1), dim3(64), 0, s, (const double*)W.alpha.as<double>(),
+                           (const double*)W.beta.as<double>(), elit, (const double*)en(conv_step), nev, Y,
+                           W.swork.as<double>());
+        W.R.ensure((size_t)nev * ld * 8);
+        DBuf& R = W.R;
         int gb = (int)std::min<int64_t>(2048, (n + TB - 1) / TB);
         size_t sh = (size_t)nev * elit * 8;
         if (sh > 64 * 1024) throw std::invalid_argument("num_ev*iterations too large for the Ritz assembly");
         hipLaunchKernelGGL((k_rotate<TBv, double>), dim3(gb), dim3(TB), sh, s, (const TBv*)V(0), ld, elit,
-                           Y.as<double>(), nev, R.as<double>(), ld, n);
-        DBuf hv64;
+                           (const double*)Y, nev, R.as<double>(), ld, n);
         const bool refine = prec_hv == 1 && P.refine;
-        if (refine) hv64.alloc((size_t)ld * 8);
+        if (refine) W.hv64.ensure((size_t)ld * 8);
+        DBuf& hv64 = W.hv64;
         double* sc = W.scal.as<double>();
         for (int i = 0; i < nev; i++) {
             double* r = R.as<double>() + (int64_t)i * ld;
@@ -802,13 +874,14 @@ void trl_solve(DeviceHam& h, const SolveParams& p, double* evals, double* evecs,
 void restart_small_device(int device, const double* alpha, const double* beta, int m, int ls, double* S,
                           double* alpha_out, double* beta_out) {
     SM_HIP(hipSetDevice(device));
-    DBuf a, b, Sd, w;
+    DBuf a, b, Sd, vals, Y, Q, w;
     a.alloc((m + 1) * 8); b.alloc((m + 1) * 8); Sd.alloc((size_t)m * ls * 8);
-    w.alloc((size_t)(5 * m + ls * (m + 3 * ls + 8) + 64) * 8);
+    vals.alloc(ls * 8); Y.alloc((size_t)ls * m * 8); Q.alloc((size_t)ls * ls * 8);
+    w.alloc((size_t)(ls * 5 * m + 5 * ls + ls * ls + 64) * 8);
     SM_HIP(hipMemcpy(a.p, alpha, m * 8, hipMemcpyHostToDevice));
     SM_HIP(hipMemcpy(b.p, beta, m * 8, hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(k_restart_small, dim3(1), dim3(64), 0, 0, a.as<double>(), b.as<double>(), m, ls,
-                       Sd.as<double>(), w.as<double>());
+    restart_small_on_stream(a.as<double>(), b.as<double>(), m, ls, Sd.as<double>(), vals.as<double>(), Y.as<double>(),
+                            Q.as<double>(), w.as<double>(), 0);
     SM_HIP(hipDeviceSynchronize());
     SM_HIP(hipMemcpy(S, Sd.p, (size_t)m * ls * 8, hipMemcpyDeviceToHost));
     SM_HIP(hipMemcpy(alpha_out, a.p, ls * 8, hipMemcpyDeviceToHost));

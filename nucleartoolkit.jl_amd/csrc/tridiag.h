@@ -163,33 +163,39 @@ SMHD inline void tri_lowest(const double* alpha, const double* beta, int n, int 
     }
 }
 
-// Thick restart of a tridiagonal Lanczos matrix.
-//   in : alpha[0..m-1], beta[0..m-1]  (m = lm-1 basis vectors; beta[m-1] couples
-//        the last kept vector to the residual vector v_m)
-//   out: S[m x ls] (column j at S + j*m) — coefficients of the new basis vectors
-//        u_j = sum_i S[i,j] v_i, j < ls; new alpha[0..ls-1], beta[0..ls-1]
-//        (beta[ls-1] couples u_{ls-1} to v_m, which becomes basis vector ls).
-// work: 5m + ls*(m + 3*ls + 8) doubles.
-SMHD inline void tri_restart(double* alpha, double* beta, int m, int ls, double* S, double* work) {
-    double* theta = work;                  // ls
-    double* Y = theta + ls;                // ls * m : Ritz vectors of T
-    double* Q = Y + (long)ls * m;          // ls * ls
-    double* sig = Q + (long)ls * ls;       // ls
-    double* na = sig + ls;                 // ls
-    double* nb = na + ls;                  // ls
-    double* r = nb + ls;                   // ls
-    double* w5 = r + ls;                   // 5m
-    double bcoup = beta[m - 1];
-    tri_lowest(alpha, beta, m, ls, theta, Y, w5);
+// First index of the eigenvalue cluster of j (eigenvalues closer than ortol
+// are orthogonalised against each other during inverse iteration).
+SMHD inline int tri_cluster_first(const double* vals, int j, double ortol) {
+    int first = j;
+    while (first > 0 && fabs(vals[j] - vals[first - 1]) < ortol) first--;
+    return first;
+}
+
+// Eigenvectors of the cluster that starts at j0 (inclusive) and ends before j1,
+// in order; work: 5n doubles.
+SMHD inline void tri_cluster_vecs(const double* alpha, const double* beta, int n, const double* vals, int j0,
+                                  int j1, double* vecs, double* work, double tnorm) {
+    for (int j = j0; j < j1; j++)
+        tri_eigvec(alpha, beta, n, vals[j], vecs + (long)j * n, work, vecs + (long)j0 * n, j - j0, tnorm);
+}
+
+// Lanczos on diag(theta) from sigma/|sigma| with full reorthogonalisation,
+// order reversed so the coupling to the residual vector sits on the last kept
+// vector.  Outputs: Qrev (ls x ls, column j at Qrev + j*ls = Lanczos vector
+// ls-1-j), new alpha[0..ls-1], beta[0..ls-1] (beta[ls-1] = |sigma|).
+// work: 4*ls + ls*ls doubles.
+SMHD inline void tri_diag_lanczos(const double* theta, const double* sig, int ls, double* Qrev, double* alpha,
+                                  double* beta, double* work) {
+    double* Q = work;                      // ls * ls, column p = Lanczos vector p
+    double* na = Q + (long)ls * ls;
+    double* nb = na + ls;
+    double* r = nb + ls;
     double snorm = 0.0;
-    for (int j = 0; j < ls; j++) { sig[j] = bcoup * Y[(long)j * m + (m - 1)]; snorm += sig[j] * sig[j]; }
+    for (int j = 0; j < ls; j++) snorm += sig[j] * sig[j];
     snorm = sqrt(snorm);
-    // Lanczos on diag(theta) starting from sigma/|sigma| with full reorthogonalisation;
-    // Q column p at Q + p*ls
-    int p = 0;
     double bprev = 0.0;
     for (int i = 0; i < ls; i++) Q[i] = snorm > 0 ? sig[i] / snorm : (i == 0 ? 1.0 : 0.0);
-    for (p = 0; p < ls; p++) {
+    for (int p = 0; p < ls; p++) {
         double* q = Q + (long)p * ls;
         double a = 0.0;
         for (int i = 0; i < ls; i++) a += theta[i] * q[i] * q[i];
@@ -232,17 +238,35 @@ SMHD inline void tri_restart(double* alpha, double* beta, int m, int ls, double*
         nb[p] = b;
         bprev = b;
     }
-    // reversed order: new index j <-> Lanczos index ls-1-j, so that the coupling
-    // to v_m (through q_0 = sigma/|sigma|) sits on the last kept vector
     for (int j = 0; j < ls; j++) {
-        int pj = ls - 1 - j;
+        const int pj = ls - 1 - j;
         alpha[j] = na[pj];
         if (j < ls - 1) beta[j] = nb[pj - 1];
+        for (int i = 0; i < ls; i++) Qrev[(long)j * ls + i] = Q[(long)pj * ls + i];
     }
     beta[ls - 1] = snorm;
-    // S = Y * Q_rev  (m x ls)
+}
+
+// Thick restart of a tridiagonal Lanczos matrix (host composition of the
+// pieces above; the device runs the same pieces in parallel kernels).
+//   in : alpha[0..m-1], beta[0..m-1]  (m = lm-1 basis vectors; beta[m-1] couples
+//        the last kept vector to the residual vector v_m)
+//   out: S[m x ls] (column j at S + j*m) — coefficients of the new basis vectors
+//        u_j = sum_i S[i,j] v_i, j < ls; new alpha[0..ls-1], beta[0..ls-1]
+//        (beta[ls-1] couples u_{ls-1} to v_m, which becomes basis vector ls).
+// work: 5m + ls*(m + 3*ls + 8) doubles.
+SMHD inline void tri_restart(double* alpha, double* beta, int m, int ls, double* S, double* work) {
+    double* theta = work;                  // ls
+    double* Y = theta + ls;                // ls * m : Ritz vectors of T
+    double* Qrev = Y + (long)ls * m;       // ls * ls
+    double* sig = Qrev + (long)ls * ls;    // ls
+    double* w5 = sig + ls;                 // 5m, then reused for tri_diag_lanczos (4ls + ls^2)
+    const double bcoup = beta[m - 1];
+    tri_lowest(alpha, beta, m, ls, theta, Y, w5);
+    for (int j = 0; j < ls; j++) sig[j] = bcoup * Y[(long)j * m + (m - 1)];
+    tri_diag_lanczos(theta, sig, ls, Qrev, alpha, beta, w5);
     for (int j = 0; j < ls; j++) {
-        const double* qv = Q + (long)(ls - 1 - j) * ls;
+        const double* qv = Qrev + (long)j * ls;
         for (int i = 0; i < m; i++) {
             double s = 0.0;
             for (int t = 0; t < ls; t++) s += Y[(long)t * m + i] * qv[t];
