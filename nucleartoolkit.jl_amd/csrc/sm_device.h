@@ -80,20 +80,30 @@ struct HvTables {
     size_t lds_bytes;
 };
 
-// Device-resident Hamiltonian (index tables shared, coefficients per precision).
-struct DeviceHam {
+// Interaction-independent device tables of one (model space, particle numbers,
+// 2M): basis blocks, hop tables, row groups.  Shared by every interaction on
+// that basis (interaction variants, serving requests of the same nucleus).
+struct DeviceIndex {
     int device = 0;
-    hipStream_t stream = nullptr;
     int64_t mdim = 0;
     int nb = 0, R = 0;
     DBuf grp_block[2], grp_fp0[2];   // row groups per precision (cost order)
-    // index tables
     DBuf row_block, block_row0, nP, nN, tdims, en_off, pp_rowptr, pp_col, nn_slice_base, nn_slice_off,
         nn_slice_len, nn_col, pair_ptr, pair_bi, pair_dm, pair_ptab, pair_ntab, ptab_rowptr_base,
         ptab_rowptr, ptab_ent_base, ptab_ent, ntab_slice_base, ntab_slice_off, ntab_slice_len, ntab_ent,
         veff_off, veff_cols2;
-    // coefficients (index 0: FP64, 1: FP32)
-    DBuf ep[2], en[2], pp_val[2], nn_val[2], veff[2];
+    HvTables view[2];                // index part (coefficient pointers unset)
+    uint64_t pattern = 0;            // fingerprint of the pp / nn sparsity patterns
+    int64_t pn_terms = 0, pp_terms = 0, nn_terms = 0, pn_terms_exact = 0;
+};
+
+// Device-resident Hamiltonian: shared index tables + this interaction's coefficients.
+struct DeviceHam {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int64_t mdim = 0;
+    std::shared_ptr<DeviceIndex> idx;
+    DBuf ep[2], en[2], pp_val[2], nn_val[2], veff[2];   // index 0: FP64, 1: FP32
     HvTables view[2];
     // byte accounting of every array the H·v kernel streams, per precision
     std::vector<std::pair<std::string, int64_t>> table_bytes(int prec) const;
@@ -103,6 +113,11 @@ struct DeviceHam {
     ~DeviceHam();
 };
 
+// fingerprint of the interaction-dependent sparsity (pp / nn rows) on basis b
+uint64_t coefficient_pattern(const Basis& b, const Tables& t);
+std::shared_ptr<DeviceIndex> upload_index(const Basis& b, const Tables& t, int device);
+// new handle on an existing index; throws if t's pp / nn pattern differs from the index's
+std::unique_ptr<DeviceHam> attach_ham(std::shared_ptr<DeviceIndex> idx, const Basis& b, const Tables& t);
 std::unique_ptr<DeviceHam> upload_ham(const Basis& b, const Tables& t, int device);
 // w = H v for one vector; prec 0 = FP64 (double*), 1 = FP32 (float*)
 void launch_hv(const DeviceHam& h, int prec, const void* v, void* w, hipStream_t s);

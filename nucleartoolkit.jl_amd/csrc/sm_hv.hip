@@ -287,7 +287,7 @@ static void launch_t(const HvTables& A, const T* v, T* w, hipStream_t s) {
 }
 
 void launch_hv(const DeviceHam& h, int prec, const void* v, void* w, hipStream_t s) {
-    if (h.R == 0) return;
+    if (h.idx->R == 0) return;
     if (prec == 0) launch_t<double>(h.view[0], (const double*)v, (double*)w, s);
     else launch_t<float>(h.view[1], (const float*)v, (float*)w, s);
 }
@@ -511,22 +511,22 @@ void index_layout(const Basis& b, const Tables& t, HostLayout& L) {
     }
 }
 
-template <typename T> void set_view(DeviceHam& h, int prec, const HostLayout& L) {
+template <typename T> void set_view(DeviceIndex& h, int prec, const HostLayout& L) {
     HvTables& A = h.view[prec];
     A.nb = h.nb; A.R = h.R;
     A.row_block = h.row_block.as<int32_t>(); A.block_row0 = h.block_row0.as<int32_t>();
     A.nP = h.nP.as<int32_t>(); A.nN = h.nN.as<int32_t>(); A.tdims = h.tdims.as<int64_t>();
-    A.ep = h.ep[prec].p; A.en_off = h.en_off.as<int64_t>(); A.en = h.en[prec].p;
-    A.pp_rowptr = h.pp_rowptr.as<int64_t>(); A.pp_col = h.pp_col.as<int32_t>(); A.pp_val = h.pp_val[prec].p;
+    A.ep = nullptr; A.en_off = h.en_off.as<int64_t>(); A.en = nullptr;
+    A.pp_rowptr = h.pp_rowptr.as<int64_t>(); A.pp_col = h.pp_col.as<int32_t>(); A.pp_val = nullptr;
     A.nn_slice_base = h.nn_slice_base.as<int64_t>(); A.nn_slice_off = h.nn_slice_off.as<int64_t>();
-    A.nn_slice_len = h.nn_slice_len.as<int32_t>(); A.nn_col = h.nn_col.as<int32_t>(); A.nn_val = h.nn_val[prec].p;
+    A.nn_slice_len = h.nn_slice_len.as<int32_t>(); A.nn_col = h.nn_col.as<int32_t>(); A.nn_val = nullptr;
     A.pair_ptr = h.pair_ptr.as<int32_t>(); A.pair_bi = h.pair_bi.as<int32_t>(); A.pair_dm = h.pair_dm.as<int32_t>();
     A.pair_ptab = h.pair_ptab.as<int32_t>(); A.pair_ntab = h.pair_ntab.as<int32_t>();
     A.ptab_rowptr_base = h.ptab_rowptr_base.as<int64_t>(); A.ptab_rowptr = h.ptab_rowptr.as<int32_t>();
     A.ptab_ent_base = h.ptab_ent_base.as<int64_t>(); A.ptab_ent = h.ptab_ent.as<uint32_t>();
     A.ntab_slice_base = h.ntab_slice_base.as<int64_t>(); A.ntab_slice_off = h.ntab_slice_off.as<int64_t>();
     A.ntab_slice_len = h.ntab_slice_len.as<int32_t>(); A.ntab_ent = h.ntab_ent.as<uint32_t>();
-    A.veff_off = h.veff_off.as<int64_t>(); A.veff_cols2 = h.veff_cols2.as<int32_t>(); A.veff = h.veff[prec].p;
+    A.veff_off = h.veff_off.as<int64_t>(); A.veff_cols2 = h.veff_cols2.as<int32_t>(); A.veff = nullptr;
     A.nN_max = std::max(1, L.nN_max);
     A.cols2_max = std::max(2, L.cols2_max);
     auto lds_for = [&](int rg) { return ((size_t)A.nN_max * 4 * rg + (size_t)A.cols2_max * 4 * rg +
@@ -570,14 +570,25 @@ template <typename T> void upload_coef(DeviceHam& h, int prec, const HostLayout&
 
 }  // namespace
 
-std::unique_ptr<DeviceHam> upload_ham(const Basis& b, const Tables& t, int device) {
+uint64_t coefficient_pattern(const Basis& b, const Tables& t) {
+    uint64_t hsh = 1469598103934665603ull;
+    auto mix = [&](uint64_t x) { hsh ^= x + 0x9E3779B97F4A7C15ull + (hsh << 6) + (hsh >> 2); };
+    for (int k = 0; k < b.nb; k++) {
+        for (const PairSym* m : {&t.pp[b.pset[k]], &t.nn[b.nset[k]]}) {
+            mix(m->rowptr.size());
+            for (int64_t x : m->rowptr) mix((uint64_t)x);
+            for (int32_t c : m->col) mix((uint64_t)c);
+        }
+    }
+    return hsh;
+}
+
+std::shared_ptr<DeviceIndex> upload_index(const Basis& b, const Tables& t, int device) {
     SM_HIP(hipSetDevice(device));
-    auto h = std::make_unique<DeviceHam>();
+    auto h = std::make_shared<DeviceIndex>();
     h->device = device;
-    SM_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     HostLayout L;
     index_layout(b, t, L);
-    coef_layout(b, t, L);
     h->mdim = b.mdim;
     h->nb = b.nb;
     h->R = (int)L.row_block.size();
@@ -593,10 +604,9 @@ std::unique_ptr<DeviceHam> upload_ham(const Basis& b, const Tables& t, int devic
     h->ntab_slice_base.upload(L.ntab_slice_base); h->ntab_slice_off.upload(L.ntab_slice_off);
     h->ntab_slice_len.upload(L.ntab_slice_len); h->ntab_ent.upload(L.ntab_ent);
     h->veff_off.upload(L.veff_off); h->veff_cols2.upload(L.veff_cols2);
-    upload_coef<double>(*h, 0, L);
-    upload_coef<float>(*h, 1, L);
     set_view<double>(*h, 0, L);
     set_view<float>(*h, 1, L);
+    h->pattern = coefficient_pattern(b, t);
     h->pn_terms = L.pn_terms;
     h->pn_terms_exact = L.pn_terms_exact;
     h->pp_terms = L.pp_terms;
@@ -604,21 +614,63 @@ std::unique_ptr<DeviceHam> upload_ham(const Basis& b, const Tables& t, int devic
     return h;
 }
 
-void update_coefficients(DeviceHam& h, const Basis& b, const Tables& t) {
-    SM_HIP(hipSetDevice(h.device));
-    HostLayout L;
-    coef_layout(b, t, L);
-    upload_coef<double>(h, 0, L);
-    upload_coef<float>(h, 1, L);
+static void bind_coefficients(DeviceHam& h) {
     for (int prec = 0; prec < 2; prec++) {
         HvTables& A = h.view[prec];
+        A = h.idx->view[prec];
         A.ep = h.ep[prec].p; A.en = h.en[prec].p; A.pp_val = h.pp_val[prec].p;
         A.nn_val = h.nn_val[prec].p; A.veff = h.veff[prec].p;
     }
 }
 
+std::unique_ptr<DeviceHam> attach_ham(std::shared_ptr<DeviceIndex> idx, const Basis& b, const Tables& t) {
+    if (coefficient_pattern(b, t) != idx->pattern)
+        throw std::invalid_argument("interaction has a different pp/nn sparsity pattern than the shared tables");
+    SM_HIP(hipSetDevice(idx->device));
+    auto h = std::make_unique<DeviceHam>();
+    h->device = idx->device;
+    SM_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    h->mdim = idx->mdim;
+    h->idx = idx;
+    HostLayout L;
+    coef_layout(b, t, L);
+    upload_coef<double>(*h, 0, L);
+    upload_coef<float>(*h, 1, L);
+    bind_coefficients(*h);
+    h->pn_terms = idx->pn_terms;
+    h->pn_terms_exact = idx->pn_terms_exact;
+    h->pp_terms = idx->pp_terms;
+    h->nn_terms = idx->nn_terms;
+    return h;
+}
+
+std::unique_ptr<DeviceHam> upload_ham(const Basis& b, const Tables& t, int device) {
+    return attach_ham(upload_index(b, t, device), b, t);
+}
+
+void update_coefficients(DeviceHam& h, const Basis& b, const Tables& t) {
+    if (coefficient_pattern(b, t) != h.idx->pattern)
+        throw std::invalid_argument("interaction has a different pp/nn sparsity pattern than the shared tables");
+    SM_HIP(hipSetDevice(h.device));
+    HostLayout L;
+    coef_layout(b, t, L);
+    upload_coef<double>(h, 0, L);
+    upload_coef<float>(h, 1, L);
+    bind_coefficients(h);
+}
+
 std::vector<std::pair<std::string, int64_t>> DeviceHam::table_bytes(int prec) const {
     const size_t s = prec == 0 ? 8 : 4;
+    const DeviceIndex& I = *idx;
+    const DBuf &grp_block = I.grp_block[prec], &grp_fp0 = I.grp_fp0[prec], &row_block = I.row_block,
+               &block_row0 = I.block_row0, &nP = I.nP, &nN = I.nN, &tdims = I.tdims, &en_off = I.en_off,
+               &pp_rowptr = I.pp_rowptr, &pp_col = I.pp_col, &nn_slice_base = I.nn_slice_base,
+               &nn_slice_off = I.nn_slice_off, &nn_slice_len = I.nn_slice_len, &nn_col = I.nn_col,
+               &pair_ptr = I.pair_ptr, &pair_bi = I.pair_bi, &pair_dm = I.pair_dm, &pair_ptab = I.pair_ptab,
+               &pair_ntab = I.pair_ntab, &ptab_rowptr_base = I.ptab_rowptr_base, &ptab_rowptr = I.ptab_rowptr,
+               &ptab_ent_base = I.ptab_ent_base, &ptab_ent = I.ptab_ent, &ntab_slice_base = I.ntab_slice_base,
+               &ntab_slice_off = I.ntab_slice_off, &ntab_slice_len = I.ntab_slice_len, &ntab_ent = I.ntab_ent,
+               &veff_off = I.veff_off, &veff_cols2 = I.veff_cols2;
     std::vector<std::pair<std::string, int64_t>> v = {
         {"row_block", (int64_t)row_block.bytes}, {"block_row0", (int64_t)block_row0.bytes},
         {"nP", (int64_t)nP.bytes}, {"nN", (int64_t)nN.bytes}, {"tdims", (int64_t)tdims.bytes},

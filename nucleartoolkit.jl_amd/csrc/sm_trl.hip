@@ -511,14 +511,16 @@ struct Solver {
     int gstore;
     Workspace& W;
 
-    static Workspace& cached(DeviceHam& h) {
+    static Workspace& cached(std::shared_ptr<void>* slots) {
         const int k = sizeof(TBv) == 4 ? 1 : 0;
-        if (!h.trl_ws[k]) h.trl_ws[k] = std::make_shared<Workspace>();
-        return *static_cast<Workspace*>(h.trl_ws[k].get());
+        if (!slots[k]) slots[k] = std::make_shared<Workspace>();
+        return *static_cast<Workspace*>(slots[k].get());
     }
 
-    Solver(DeviceHam& h, const SolveParams& p, SolveStats& stats) : H(h), P(p), st(stats), W(cached(h)) {
-        s = H.stream;
+    // ctx: stream + workspace slots of the caller (a serving worker), or the handle's own
+    Solver(DeviceHam& h, const SolveParams& p, SolveStats& stats, const SolveCtx* ctx)
+        : H(h), P(p), st(stats), W(cached(ctx ? ctx->ws : h.trl_ws)) {
+        s = ctx ? ctx->stream : H.stream;
         n = H.mdim;
         ld = (n + 63) / 64 * 64;
         lm = P.lm;
@@ -859,13 +861,13 @@ struct Solver {
 }  // namespace
 
 void trl_solve(DeviceHam& h, const SolveParams& p, double* evals, double* evecs, SolveStats& st,
-               const double* v0) {
+               const double* v0, const SolveCtx* ctx) {
     SM_HIP(hipSetDevice(h.device));
     if (p.precision == 0) {
-        Solver<double> S(h, p, st);
+        Solver<double> S(h, p, st, ctx);
         S.run(evals, evecs, v0);
     } else {
-        Solver<float> S(h, p, st);
+        Solver<float> S(h, p, st, ctx);
         S.run(evals, evecs, v0);
     }
 }
@@ -892,10 +894,10 @@ void lanczos_steps(DeviceHam& h, const SolveParams& p, int nsteps, const double*
     SM_HIP(hipSetDevice(h.device));
     SolveStats st{};
     if (p.precision == 0) {
-        Solver<double> S(h, p, st);
+        Solver<double> S(h, p, st, nullptr);
         S.run_steps(nsteps, v0, alpha, beta);
     } else {
-        Solver<float> S(h, p, st);
+        Solver<float> S(h, p, st, nullptr);
         S.run_steps(nsteps, v0, alpha, beta);
     }
 }
