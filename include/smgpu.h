@@ -142,6 +142,80 @@ int32_t smgpu_trl_solve_v0(smgpu_ham h, const smgpu_solve_params* params, const 
 int32_t smgpu_lanczos_steps(smgpu_ham h, int32_t precision, int32_t nsteps, const double* v0, double* alpha,
                             double* beta);
 
+/* ------------------------------------------------------------------ serving
+ * Persistent request loop (no counterpart in the reference, whose main_sm
+ * solves one nucleus per call; each request is one main_sm eigenvalue solve,
+ * shellmodel_main.jl:89-281, with the observable en[1]).  Requests are prepared
+ * on host threads (readsmsnt through the table build), admitted in arrival
+ * order as device slots and HBM free up (a smaller later request may pass a
+ * waiting larger one a bounded number of times), and run concurrently, one
+ * HIP stream per slot.  Requests with equal orbits, valence numbers and 2M share
+ * one device copy of the index tables; each keeps its own coefficients and
+ * Lanczos basis.  All calls are thread-safe. */
+#define SMGPU_ETIMEDOUT (-7)
+
+#define SMGPU_REQ_QUEUED 0     /* waiting for a host thread */
+#define SMGPU_REQ_PREPARING 1  /* host tables being built */
+#define SMGPU_REQ_READY 2      /* waiting for a slot / HBM */
+#define SMGPU_REQ_RUNNING 3
+#define SMGPU_REQ_DONE 4
+#define SMGPU_REQ_FAILED 5
+#define SMGPU_REQ_CANCELED 6
+
+typedef struct {
+    int32_t device;
+    int32_t max_inflight;   /* concurrent solves (streams); <= 0: 4 */
+    int64_t hbm_budget;     /* HBM bytes the server may hold; <= 0: 90% of free memory at create */
+    int32_t host_threads;   /* requests prepared concurrently; <= 0: 2 */
+    int32_t table_threads;  /* threads per table build; <= 0: cores / host_threads */
+    int64_t keep_bytes;     /* an idle slot frees its buffers above this; <= 0: 16 GiB */
+} smgpu_server_config;
+
+typedef struct {
+    int32_t state;          /* SMGPU_REQ_* */
+    int32_t status;         /* SMGPU_OK or the failure code once finished */
+    int32_t num_ev;
+    int32_t index_shared;   /* 1: device index tables were already resident */
+    int32_t slot;           /* slot that ran it (-1 before admission) */
+    int32_t pad_;
+    int64_t mdim;
+    int64_t footprint;      /* HBM bytes the request needs (index + coefficients + workspace) */
+    int64_t index_bytes;    /* of which the shared index tables */
+    double submit_ms;       /* times on the server clock (ms since create) */
+    double prep_start_ms, prep_end_ms, admit_ms, done_ms;
+    double latency_ms;      /* done - submit */
+    smgpu_solve_stats solve;
+} smgpu_request_info;
+
+typedef struct {
+    int64_t submitted, completed, failed, canceled;
+    int32_t queued, preparing, ready, running;
+    int32_t peak_running, slots;
+    int64_t hbm_budget, hbm_in_use, hbm_peak;
+    int64_t index_hits, index_misses, index_uploads, index_evictions;
+    double clock_ms;        /* server clock now */
+} smgpu_server_stats;
+
+void smgpu_default_server_config(smgpu_server_config* cfg);
+int32_t smgpu_server_create(const smgpu_server_config* cfg, smgpu_server* out);
+/* Cancels requests not yet finished, waits for the slots, frees everything. */
+int32_t smgpu_server_destroy(smgpu_server s);
+/* One main_sm solve: interaction file, nucleus ("Cr48"), 2M (< 0: A % 2) and
+ * solver parameters (num_ev <= 64).  Returns at once with the request id. */
+int32_t smgpu_server_submit(smgpu_server s, const char* snt_path, const char* nucleus, int32_t twoM,
+                            const smgpu_solve_params* params, int64_t* request_id);
+int32_t smgpu_server_poll(smgpu_server s, int64_t request_id, int32_t* state);
+/* Block until the request has finished (done, failed or canceled) or timeout_ms
+ * (< 0: no limit) passes: SMGPU_OK or SMGPU_ETIMEDOUT. */
+int32_t smgpu_server_wait(smgpu_server s, int64_t request_id, double timeout_ms);
+/* Finished request: evals[num_ev] (may be NULL) and info (may be NULL); the
+ * record is dropped afterwards.  Returns the request's own status. */
+int32_t smgpu_server_result(smgpu_server s, int64_t request_id, double* evals, smgpu_request_info* info);
+/* Info of a request in any state (the record stays). */
+int32_t smgpu_server_info(smgpu_server s, int64_t request_id, smgpu_request_info* info);
+int32_t smgpu_server_cancel(smgpu_server s, int64_t request_id);
+int32_t smgpu_server_get_stats(smgpu_server s, smgpu_server_stats* st);
+
 /* Device memory helpers for callers that keep vectors resident. */
 int32_t smgpu_malloc(smgpu_ham h, int64_t bytes, void** dev);
 int32_t smgpu_free(smgpu_ham h, void* dev);

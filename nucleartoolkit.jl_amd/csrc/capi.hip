@@ -13,6 +13,7 @@
 #include "../../include/smgpu.h"
 #include "sm_device.h"
 #include "sm_host.h"
+#include "sm_status.h"
 #include "sm_trl.h"
 #include "tridiag.h"
 
@@ -38,30 +39,7 @@ __global__ void k_cast(const TA* a, TB* b, int64_t n) {
 
 static thread_local std::string g_err;
 
-static int32_t fail(int32_t code, const std::string& msg) {
-    g_err = msg;
-    return code;
-}
-
-template <typename F> static int32_t guard(F&& f) {
-    try {
-        return f();
-    } catch (const std::invalid_argument& e) {
-        return fail(SMGPU_EINVAL, e.what());
-    } catch (const std::bad_alloc&) {
-        return fail(SMGPU_ENOMEM, "out of host memory");
-    } catch (const std::runtime_error& e) {
-        std::string m = e.what();
-        if (m.rfind("HIP error out of memory", 0) == 0 || m.find("hipErrorOutOfMemory") != std::string::npos)
-            return fail(SMGPU_ENOMEM, m);
-        if (m.rfind("HIP error", 0) == 0) return fail(SMGPU_EDEVICE, m);
-        if (m.rfind("cannot open", 0) == 0) return fail(SMGPU_EIO, m);
-        if (m.rfind("TRL: iteration limit", 0) == 0) return fail(SMGPU_ENOTCONV, m);
-        return fail(SMGPU_EINVAL, m);
-    } catch (...) {
-        return fail(SMGPU_EINVAL, "unknown error");
-    }
-}
+void smgpu::set_last_error(const std::string& msg) { g_err = msg; }
 
 extern "C" {
 
@@ -85,17 +63,9 @@ int32_t smgpu_problem_create(const char* snt_path, const char* nucleus, int32_t 
         auto p = std::make_unique<smgpu_problem_s>();
         p->nucleus = nucleus;
         p->snt = read_snt(snt_path, nucleus, ignore_massop != 0);
-        std::string el;
-        for (char c : p->nucleus) if (!isdigit((unsigned char)c)) el += c;
-        int Z = element_z(el);
-        if (Z < 0) throw std::invalid_argument("unknown element in nucleus '" + p->nucleus + "'");
-        p->Z = Z;
-        p->N = p->snt.A - Z;
-        p->vp = Z - p->snt.cp;
-        p->vn = p->N - p->snt.cn;
-        if (p->vp < 0 || p->vn < 0) throw std::invalid_argument("nucleus below the core of the interaction");
-        int M = twoM < 0 ? p->snt.A % 2 : twoM;
-        if ((M - p->snt.A) % 2 != 0) throw std::invalid_argument("2M parity does not match the mass number");
+        NucleusSpec ns = resolve_nucleus(p->snt, p->nucleus, twoM);
+        p->Z = ns.Z; p->N = ns.N; p->vp = ns.vp; p->vn = ns.vn;
+        int M = ns.twoM;
         p->nthreads = nthreads > 0 ? nthreads : (int)std::max(1u, std::thread::hardware_concurrency());
         p->basis = build_basis(p->snt, p->vp, p->vn, M);
         p->tables = build_tables(p->snt, p->basis, p->nthreads);

@@ -19,18 +19,24 @@ namespace smgpu {
                                      " at " + __FILE__ + ":" + std::to_string(__LINE__)); \
     } while (0)
 
-// Device buffer owned by a handle.
+// Device buffer owned by a handle.  `bytes` is the size in use, `cap` the size
+// allocated; ensure() and upload() reuse the allocation when it is large enough.
 struct DBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    size_t cap = 0;
     DBuf() = default;
     DBuf(const DBuf&) = delete;
     DBuf& operator=(const DBuf&) = delete;
     ~DBuf();
-    void alloc(size_t n);
-    void ensure(size_t n) { if (bytes < n || !p) alloc(n); }   // grow-only reuse
+    void alloc(size_t n);      // fresh allocation of exactly n bytes
+    void release();
+    void ensure(size_t n) {    // grow-only reuse
+        if (cap < n || !p) alloc(n);
+        else bytes = n;
+    }
     template <typename T> void upload(const std::vector<T>& v) {
-        alloc(v.size() * sizeof(T));
+        ensure(v.size() * sizeof(T));
         if (!v.empty()) SM_HIP(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
     }
     template <typename T> T* as() const { return (T*)p; }
@@ -95,6 +101,7 @@ struct DeviceIndex {
     HvTables view[2];                // index part (coefficient pointers unset)
     uint64_t pattern = 0;            // fingerprint of the pp / nn sparsity patterns
     int64_t pn_terms = 0, pp_terms = 0, nn_terms = 0, pn_terms_exact = 0;
+    int64_t device_bytes() const;    // HBM held by the index tables
 };
 
 // Device-resident Hamiltonian: shared index tables + this interaction's coefficients.
@@ -110,8 +117,26 @@ struct DeviceHam {
     int64_t pn_terms = 0, pp_terms = 0, nn_terms = 0;   // FMAs per H·v (padded n-table entries included)
     int64_t pn_terms_exact = 0;
     std::shared_ptr<void> trl_ws[2];   // Lanczos workspace per precision, reused across solves
+    int64_t coefficient_capacity() const;   // HBM held by the coefficient buffers
+    void release_coefficients();
     ~DeviceHam();
 };
+
+// Host copies of the kernel-layout arrays, built on host threads ahead of an
+// upload (the serving loop prepares them while other solves use the device).
+struct HostIndex;          // interaction-independent index arrays
+struct HostCoefficients;   // one interaction's coefficients, FP64 and FP32
+std::shared_ptr<const HostIndex> pack_index(const Basis& b, const Tables& t);
+int64_t device_bytes(const HostIndex& x);            // HBM that upload_index allocates
+std::shared_ptr<DeviceIndex> upload_index(const HostIndex& x, int device);
+std::shared_ptr<const HostCoefficients> pack_coefficients(const Basis& b, const Tables& t);
+int64_t device_bytes(const HostCoefficients& c);     // HBM of the coefficients, both precisions
+int64_t coefficient_growth(const DeviceHam& h, const HostCoefficients& c);   // bind_ham's new allocation
+uint64_t pattern_of(const HostCoefficients& c);
+uint64_t pattern_of(const HostIndex& x);
+// Point h at idx with coefficients c, reusing h's coefficient buffers when large
+// enough (no allocation in steady state).  Throws if the sparsity patterns differ.
+void bind_ham(DeviceHam& h, std::shared_ptr<DeviceIndex> idx, const HostCoefficients& c);
 
 // fingerprint of the interaction-dependent sparsity (pp / nn rows) on basis b
 uint64_t coefficient_pattern(const Basis& b, const Tables& t);

@@ -64,6 +64,21 @@ int element_z(const std::string& symbol) {
     return -1;
 }
 
+NucleusSpec resolve_nucleus(const Snt& snt, const std::string& nucleus, int twoM) {
+    std::string el;
+    for (char c : nucleus) if (!isdigit((unsigned char)c)) el += c;
+    NucleusSpec r{};
+    r.Z = element_z(el);
+    if (r.Z < 0) throw std::invalid_argument("unknown element in nucleus '" + nucleus + "'");
+    r.N = snt.A - r.Z;
+    r.vp = r.Z - snt.cp;
+    r.vn = r.N - snt.cn;
+    if (r.vp < 0 || r.vn < 0) throw std::invalid_argument("nucleus below the core of the interaction");
+    r.twoM = twoM < 0 ? snt.A % 2 : twoM;
+    if ((r.twoM - snt.A) % 2 != 0) throw std::invalid_argument("2M parity does not match the mass number");
+    return r;
+}
+
 Snt read_snt(const std::string& path, const std::string& nucleus, bool ignore_massop) {
     Snt s;
     // mass number = first run of digits (util_func.jl:1 `reg`)

@@ -45,6 +45,11 @@ struct Snt {
 Snt read_snt(const std::string& path, const std::string& nucleus, bool ignore_massop);
 // Element symbol -> Z (getZNA, shellmodel_main.jl:908-916).
 int element_z(const std::string& symbol);
+
+// "Cr48" + interaction core -> Z, N, valence numbers and 2M (twoM < 0: A % 2,
+// the main_sm default, shellmodel_main.jl:110).  Throws std::invalid_argument.
+struct NucleusSpec { int Z, N, vp, vn, twoM; };
+NucleusSpec resolve_nucleus(const Snt& snt, const std::string& nucleus, int twoM);
 // <j1/2 m1/2; j2/2 m2/2 | J/2 M/2>, correctly rounded (quad-precision Racah formula).
 double clebsch_gordan(int j1, int m1, int j2, int m2, int J, int M);
 

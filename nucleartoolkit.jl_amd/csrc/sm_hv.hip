@@ -18,6 +18,8 @@
 // (src/ShellModel/lanczos_methods.jl:153-248).
 #include <algorithm>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <stdexcept>
 
 #include "sm_device.h"
@@ -29,10 +31,15 @@ static constexpr size_t HV_LDS_TWO_ROWS = 80 * 1024;  // two-row groups if they 
 DBuf::~DBuf() {
     if (p) (void)hipFree(p);
 }
+void DBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = cap = 0;
+}
 void DBuf::alloc(size_t n) {
-    if (p) { (void)hipFree(p); p = nullptr; }
-    bytes = n;
+    release();
     if (n) SM_HIP(hipMalloc(&p, n));
+    bytes = cap = n;
 }
 
 DeviceHam::~DeviceHam() {
@@ -511,6 +518,19 @@ void index_layout(const Basis& b, const Tables& t, HostLayout& L) {
     }
 }
 
+// The dynamic-LDS limit is a property of the kernel function, shared by every
+// index that launches it: only ever raise it (indices of different widths may
+// be resident and in use from several threads at once).
+void raise_lds_limit(const void* fn, size_t bytes) {
+    static std::mutex mu;
+    static std::map<const void*, size_t> set;
+    std::lock_guard<std::mutex> lk(mu);
+    size_t& cur = set[fn];
+    if (bytes <= cur) return;
+    SM_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    cur = bytes;
+}
+
 template <typename T> void set_view(DeviceIndex& h, int prec, const HostLayout& L) {
     HvTables& A = h.view[prec];
     A.nb = h.nb; A.R = h.R;
@@ -557,18 +577,21 @@ template <typename T> void set_view(DeviceIndex& h, int prec, const HostLayout& 
     A.grp_fp0 = h.grp_fp0[prec].as<int32_t>();
     const void* fn = hv_fn<T>(A.npt, A.rg);
     if (!fn) throw std::runtime_error("no H·v kernel instance for this shape");
-    SM_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)A.lds_bytes));
-}
-
-template <typename T> void upload_coef(DeviceHam& h, int prec, const HostLayout& L) {
-    h.ep[prec].upload(cast_vec<T>(L.ep));
-    h.en[prec].upload(cast_vec<T>(L.en));
-    h.pp_val[prec].upload(cast_vec<T>(L.pp_val));
-    h.nn_val[prec].upload(cast_vec<T>(L.nn_val));
-    h.veff[prec].upload(cast_vec<T>(L.veff));
+    raise_lds_limit(fn, A.lds_bytes);
 }
 
 }  // namespace
+
+struct HostIndex {
+    HostLayout L;          // index part
+    uint64_t pattern = 0;
+};
+
+struct HostCoefficients {
+    std::vector<double> ep, en, pp_val, nn_val, veff;
+    std::vector<float> ep32, en32, pp_val32, nn_val32, veff32;
+    uint64_t pattern = 0;
+};
 
 uint64_t coefficient_pattern(const Basis& b, const Tables& t) {
     uint64_t hsh = 1469598103934665603ull;
@@ -583,14 +606,65 @@ uint64_t coefficient_pattern(const Basis& b, const Tables& t) {
     return hsh;
 }
 
-std::shared_ptr<DeviceIndex> upload_index(const Basis& b, const Tables& t, int device) {
+uint64_t pattern_of(const HostCoefficients& c) { return c.pattern; }
+uint64_t pattern_of(const HostIndex& x) { return x.pattern; }
+
+std::shared_ptr<const HostIndex> pack_index(const Basis& b, const Tables& t) {
+    auto x = std::make_shared<HostIndex>();
+    index_layout(b, t, x->L);
+    x->pattern = coefficient_pattern(b, t);
+    return x;
+}
+
+template <typename V> static int64_t vbytes(const V& v) { return (int64_t)(v.size() * sizeof(v[0])); }
+
+int64_t device_bytes(const HostIndex& x) {
+    const HostLayout& L = x.L;
+    int64_t rows = (int64_t)L.row_block.size();
+    // row groups of both precisions: at most one group per row, block + first row
+    return vbytes(L.row_block) + vbytes(L.block_row0) + vbytes(L.nP) + vbytes(L.nN) + vbytes(L.tdims) +
+           vbytes(L.en_off) + vbytes(L.pp_rowptr) + vbytes(L.pp_col) + vbytes(L.nn_slice_base) +
+           vbytes(L.nn_slice_off) + vbytes(L.nn_slice_len) + vbytes(L.nn_col) + vbytes(L.pair_ptr) +
+           vbytes(L.pair_bi) + vbytes(L.pair_dm) + vbytes(L.pair_ptab) + vbytes(L.pair_ntab) +
+           vbytes(L.ptab_rowptr_base) + vbytes(L.ptab_rowptr) + vbytes(L.ptab_ent_base) + vbytes(L.ptab_ent) +
+           vbytes(L.ntab_slice_base) + vbytes(L.ntab_slice_off) + vbytes(L.ntab_slice_len) + vbytes(L.ntab_ent) +
+           vbytes(L.veff_off) + vbytes(L.veff_cols2) + 2 * 2 * rows * (int64_t)sizeof(int32_t);
+}
+
+std::shared_ptr<const HostCoefficients> pack_coefficients(const Basis& b, const Tables& t) {
+    HostLayout L;
+    coef_layout(b, t, L);
+    auto c = std::make_shared<HostCoefficients>();
+    c->ep32 = cast_vec<float>(L.ep); c->en32 = cast_vec<float>(L.en); c->pp_val32 = cast_vec<float>(L.pp_val);
+    c->nn_val32 = cast_vec<float>(L.nn_val); c->veff32 = cast_vec<float>(L.veff);
+    c->ep = std::move(L.ep); c->en = std::move(L.en); c->pp_val = std::move(L.pp_val);
+    c->nn_val = std::move(L.nn_val); c->veff = std::move(L.veff);
+    c->pattern = coefficient_pattern(b, t);
+    return c;
+}
+
+int64_t device_bytes(const HostCoefficients& c) {
+    return vbytes(c.ep) + vbytes(c.en) + vbytes(c.pp_val) + vbytes(c.nn_val) + vbytes(c.veff) + vbytes(c.ep32) +
+           vbytes(c.en32) + vbytes(c.pp_val32) + vbytes(c.nn_val32) + vbytes(c.veff32);
+}
+
+int64_t coefficient_growth(const DeviceHam& h, const HostCoefficients& c) {
+    int64_t g = 0;
+    auto f = [&](const DBuf& d, int64_t need) { if (need > (int64_t)d.cap) g += need - (int64_t)d.cap; };
+    f(h.ep[0], vbytes(c.ep)); f(h.en[0], vbytes(c.en)); f(h.pp_val[0], vbytes(c.pp_val));
+    f(h.nn_val[0], vbytes(c.nn_val)); f(h.veff[0], vbytes(c.veff));
+    f(h.ep[1], vbytes(c.ep32)); f(h.en[1], vbytes(c.en32)); f(h.pp_val[1], vbytes(c.pp_val32));
+    f(h.nn_val[1], vbytes(c.nn_val32)); f(h.veff[1], vbytes(c.veff32));
+    return g;
+}
+
+std::shared_ptr<DeviceIndex> upload_index(const HostIndex& x, int device) {
     SM_HIP(hipSetDevice(device));
+    const HostLayout& L = x.L;
     auto h = std::make_shared<DeviceIndex>();
     h->device = device;
-    HostLayout L;
-    index_layout(b, t, L);
-    h->mdim = b.mdim;
-    h->nb = b.nb;
+    h->mdim = L.tdims.empty() ? 0 : L.tdims.back();
+    h->nb = (int)L.nP.size();
     h->R = (int)L.row_block.size();
     h->row_block.upload(L.row_block); h->block_row0.upload(L.block_row0);
     h->nP.upload(L.nP); h->nN.upload(L.nN); h->tdims.upload(L.tdims); h->en_off.upload(L.en_off);
@@ -606,7 +680,7 @@ std::shared_ptr<DeviceIndex> upload_index(const Basis& b, const Tables& t, int d
     h->veff_off.upload(L.veff_off); h->veff_cols2.upload(L.veff_cols2);
     set_view<double>(*h, 0, L);
     set_view<float>(*h, 1, L);
-    h->pattern = coefficient_pattern(b, t);
+    h->pattern = x.pattern;
     h->pn_terms = L.pn_terms;
     h->pn_terms_exact = L.pn_terms_exact;
     h->pp_terms = L.pp_terms;
@@ -614,33 +688,60 @@ std::shared_ptr<DeviceIndex> upload_index(const Basis& b, const Tables& t, int d
     return h;
 }
 
-static void bind_coefficients(DeviceHam& h) {
+std::shared_ptr<DeviceIndex> upload_index(const Basis& b, const Tables& t, int device) {
+    return upload_index(*pack_index(b, t), device);
+}
+
+int64_t DeviceIndex::device_bytes() const {
+    int64_t s = 0;
+    for (const DBuf* d : {&grp_block[0], &grp_block[1], &grp_fp0[0], &grp_fp0[1], &row_block, &block_row0, &nP, &nN,
+                          &tdims, &en_off, &pp_rowptr, &pp_col, &nn_slice_base, &nn_slice_off, &nn_slice_len,
+                          &nn_col, &pair_ptr, &pair_bi, &pair_dm, &pair_ptab, &pair_ntab, &ptab_rowptr_base,
+                          &ptab_rowptr, &ptab_ent_base, &ptab_ent, &ntab_slice_base, &ntab_slice_off,
+                          &ntab_slice_len, &ntab_ent, &veff_off, &veff_cols2})
+        s += (int64_t)d->cap;
+    return s;
+}
+
+int64_t DeviceHam::coefficient_capacity() const {
+    int64_t s = 0;
+    for (int k = 0; k < 2; k++) s += (int64_t)(ep[k].cap + en[k].cap + pp_val[k].cap + nn_val[k].cap + veff[k].cap);
+    return s;
+}
+
+void DeviceHam::release_coefficients() {
+    for (int k = 0; k < 2; k++) { ep[k].release(); en[k].release(); pp_val[k].release(); nn_val[k].release(); veff[k].release(); }
+}
+
+void bind_ham(DeviceHam& h, std::shared_ptr<DeviceIndex> idx, const HostCoefficients& c) {
+    if (c.pattern != idx->pattern)
+        throw std::invalid_argument("interaction has a different pp/nn sparsity pattern than the shared tables");
+    SM_HIP(hipSetDevice(idx->device));
+    h.device = idx->device;
+    h.mdim = idx->mdim;
+    h.idx = std::move(idx);
+    h.ep[0].upload(c.ep); h.en[0].upload(c.en); h.pp_val[0].upload(c.pp_val); h.nn_val[0].upload(c.nn_val);
+    h.veff[0].upload(c.veff);
+    h.ep[1].upload(c.ep32); h.en[1].upload(c.en32); h.pp_val[1].upload(c.pp_val32); h.nn_val[1].upload(c.nn_val32);
+    h.veff[1].upload(c.veff32);
     for (int prec = 0; prec < 2; prec++) {
         HvTables& A = h.view[prec];
         A = h.idx->view[prec];
         A.ep = h.ep[prec].p; A.en = h.en[prec].p; A.pp_val = h.pp_val[prec].p;
         A.nn_val = h.nn_val[prec].p; A.veff = h.veff[prec].p;
     }
+    h.pn_terms = h.idx->pn_terms;
+    h.pn_terms_exact = h.idx->pn_terms_exact;
+    h.pp_terms = h.idx->pp_terms;
+    h.nn_terms = h.idx->nn_terms;
 }
 
 std::unique_ptr<DeviceHam> attach_ham(std::shared_ptr<DeviceIndex> idx, const Basis& b, const Tables& t) {
-    if (coefficient_pattern(b, t) != idx->pattern)
-        throw std::invalid_argument("interaction has a different pp/nn sparsity pattern than the shared tables");
-    SM_HIP(hipSetDevice(idx->device));
+    auto c = pack_coefficients(b, t);
     auto h = std::make_unique<DeviceHam>();
-    h->device = idx->device;
+    SM_HIP(hipSetDevice(idx->device));
     SM_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
-    h->mdim = idx->mdim;
-    h->idx = idx;
-    HostLayout L;
-    coef_layout(b, t, L);
-    upload_coef<double>(*h, 0, L);
-    upload_coef<float>(*h, 1, L);
-    bind_coefficients(*h);
-    h->pn_terms = idx->pn_terms;
-    h->pn_terms_exact = idx->pn_terms_exact;
-    h->pp_terms = idx->pp_terms;
-    h->nn_terms = idx->nn_terms;
+    bind_ham(*h, std::move(idx), *c);
     return h;
 }
 
@@ -649,14 +750,8 @@ std::unique_ptr<DeviceHam> upload_ham(const Basis& b, const Tables& t, int devic
 }
 
 void update_coefficients(DeviceHam& h, const Basis& b, const Tables& t) {
-    if (coefficient_pattern(b, t) != h.idx->pattern)
-        throw std::invalid_argument("interaction has a different pp/nn sparsity pattern than the shared tables");
-    SM_HIP(hipSetDevice(h.device));
-    HostLayout L;
-    coef_layout(b, t, L);
-    upload_coef<double>(h, 0, L);
-    upload_coef<float>(h, 1, L);
-    bind_coefficients(h);
+    auto c = pack_coefficients(b, t);
+    bind_ham(h, h.idx, *c);
 }
 
 std::vector<std::pair<std::string, int64_t>> DeviceHam::table_bytes(int prec) const {

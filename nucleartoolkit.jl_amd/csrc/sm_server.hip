@@ -1,0 +1,662 @@
+// Persistent serving loop for many main_sm eigenvalue solves on one device.
+//
+// Life of a request:
+//   submit  -> prep queue (host threads: readsmsnt, basis, tables; the
+//              interaction-independent index is built once per (orbits, Z_v,
+//              N_v, 2M) and cached, later requests only build coefficients)
+//           -> ready list, in arrival order
+//           -> admitted to an idle slot when the HBM it needs fits the budget
+//              (index tables if not resident + coefficient and workspace growth
+//              of that slot); a later request may pass a waiting head request a
+//              bounded number of times, so a large request is never starved
+//           -> the slot's thread uploads the index if needed, binds the
+//              coefficients into its own buffers and runs the TRL on its own
+//              stream; solves of different slots overlap on the device
+//           -> done / failed / canceled, latency stamped on the server clock.
+// Slots keep their buffers between requests (grow-only), so a steady stream
+// of similar requests allocates nothing; memory is reclaimed from idle slots
+// and unused index tables only when an admission needs it.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <list>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/smgpu.h"
+#include "sm_device.h"
+#include "sm_host.h"
+#include "sm_status.h"
+#include "sm_trl.h"
+
+namespace smgpu {
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+// Interaction-independent part of a request, shared by every request with the
+// same orbits, valence numbers and 2M.
+struct IndexEntry {
+    std::string key;
+    // host part (filled once by the first preparer; `ready` under the server mutex)
+    bool ready = false;
+    int32_t code = SMGPU_OK;
+    std::string error;
+    Basis basis;
+    OneBodyOps ob[2];
+    std::shared_ptr<const HostIndex> hidx;
+    int64_t index_bytes = 0;
+    // device part (server mutex)
+    std::shared_ptr<DeviceIndex> didx;
+    bool uploading = false;   // an admitted request is uploading it
+    bool upload_failed = false;
+    int users = 0;            // admitted requests holding it
+    uint64_t last_use = 0;
+};
+
+struct Request {
+    int64_t id = 0;
+    std::string snt_path, nucleus;
+    int twoM = -1;
+    SolveParams params{};
+    std::atomic<bool> cancel{false};
+    int state = SMGPU_REQ_QUEUED;
+    int32_t status = SMGPU_OK;
+    std::string error;
+    std::shared_ptr<IndexEntry> entry;
+    std::shared_ptr<const HostCoefficients> coef;
+    int64_t mdim = 0, ws_need = 0, coef_need = 0;
+    bool uploader = false, index_shared = false;
+    int bypassed = 0;
+    int slot = -1;
+    int64_t reserved = 0;     // HBM charged at admission
+    double t_submit = 0, t_prep0 = 0, t_prep1 = 0, t_admit = 0, t_done = 0;
+    std::vector<double> evals;
+    SolveStats st{};
+};
+
+struct Slot {
+    int id = 0;
+    hipStream_t stream = nullptr;
+    SolveCtx ctx;
+    DeviceHam ham;
+    std::shared_ptr<Request> job;
+    int64_t cap = 0;          // HBM held by this slot's buffers (server mutex)
+    bool release = false;     // free the buffers before waiting again
+    std::condition_variable cv;
+    std::thread th;
+};
+
+}  // namespace
+
+struct Server {
+    smgpu_server_config cfg{};
+    Clock::time_point t0 = Clock::now();
+    std::mutex m;
+    std::condition_variable done_cv, prep_cv;
+    bool stopping = false;
+    int64_t next_id = 1;
+    std::unordered_map<int64_t, std::shared_ptr<Request>> reqs;
+    std::deque<std::shared_ptr<Request>> prep_q;
+    std::list<std::shared_ptr<Request>> ready;
+    std::map<std::string, std::shared_ptr<IndexEntry>> cache;
+    std::condition_variable entry_cv;
+    std::vector<std::unique_ptr<Slot>> slots;
+    std::vector<std::thread> preparers;
+    uint64_t use_clock = 0;
+    // accounting
+    int64_t index_resident = 0;   // bytes of resident or uploading index tables
+    int64_t hbm_peak = 0;
+    smgpu_server_stats st{};
+
+    double now_ms() const { return std::chrono::duration<double, std::milli>(Clock::now() - t0).count(); }
+
+    int64_t in_use() const {
+        int64_t u = index_resident;
+        for (auto& s : slots) u += s->cap;
+        return u;
+    }
+
+    // ------------------------------------------------------------ preparation
+    static std::string key_of(const Snt& s, const NucleusSpec& ns) {
+        std::ostringstream k;
+        for (const auto& o : s.p_sps) k << o.n << ',' << o.l << ',' << o.j2 << ',' << o.tz << ';';
+        k << '|';
+        for (const auto& o : s.n_sps) k << o.n << ',' << o.l << ',' << o.j2 << ',' << o.tz << ';';
+        k << '|' << ns.vp << ',' << ns.vn << ',' << ns.twoM;
+        return k.str();
+    }
+
+    void prepare(Request& r) {
+        const int nthr = std::max(1, cfg.table_threads);
+        Snt snt = read_snt(r.snt_path, r.nucleus, false);
+        NucleusSpec ns = resolve_nucleus(snt, r.nucleus, r.twoM);
+        const std::string key = key_of(snt, ns);
+        std::shared_ptr<IndexEntry> e;
+        bool builder = false;
+        {
+            std::unique_lock<std::mutex> lk(m);
+            auto it = cache.find(key);
+            if (it == cache.end()) {
+                e = std::make_shared<IndexEntry>();
+                e->key = key;
+                cache[key] = e;
+                builder = true;
+                st.index_misses++;
+            } else {
+                e = it->second;
+                st.index_hits++;
+                entry_cv.wait(lk, [&] { return e->ready; });
+                if (e->code != SMGPU_OK) throw std::invalid_argument("index tables failed: " + e->error);
+            }
+        }
+        if (builder) {
+            try {
+                Basis b = build_basis(snt, ns.vp, ns.vn, ns.twoM);
+                if (b.mdim == 0) throw std::invalid_argument("empty M-scheme space (mdim = 0)");
+                Tables t = build_tables(snt, b, nthr);
+                auto hidx = pack_index(b, t);
+                auto coef = pack_coefficients(b, t);
+                std::lock_guard<std::mutex> lk(m);
+                e->ob[0] = t.ob[0];
+                e->ob[1] = t.ob[1];
+                e->basis = std::move(b);
+                e->hidx = hidx;
+                e->index_bytes = device_bytes(*hidx);
+                e->ready = true;
+                r.coef = coef;
+            } catch (...) {
+                std::string msg;
+                int32_t c = status_of_current_exception(msg);
+                std::lock_guard<std::mutex> lk(m);
+                e->code = c;
+                e->error = msg;
+                e->ready = true;
+                cache.erase(key);   // a later request retries
+                entry_cv.notify_all();
+                throw;
+            }
+            entry_cv.notify_all();
+        } else {
+            Tables t;
+            t.ob[0] = e->ob[0];
+            t.ob[1] = e->ob[1];
+            build_coefficients(snt, e->basis, t, nthr);
+            r.coef = pack_coefficients(e->basis, t);
+            if (pattern_of(*r.coef) != pattern_of(*e->hidx))
+                throw std::invalid_argument("interaction has a different pp/nn sparsity pattern than the shared tables");
+        }
+        r.entry = e;
+        r.mdim = e->basis.mdim;
+        r.ws_need = solve_workspace_bytes(r.mdim, r.params);
+        r.coef_need = device_bytes(*r.coef);
+    }
+
+    void preparer_loop() {
+        for (;;) {
+            std::shared_ptr<Request> r;
+            {
+                std::unique_lock<std::mutex> lk(m);
+                prep_cv.wait(lk, [&] { return stopping || !prep_q.empty(); });
+                if (stopping) return;
+                r = prep_q.front();
+                prep_q.pop_front();
+                if (r->cancel) { finish_locked(*r, SMGPU_REQ_CANCELED, SMGPU_ECANCELED, "canceled"); continue; }
+                r->state = SMGPU_REQ_PREPARING;
+                r->t_prep0 = now_ms();
+            }
+            int32_t code = SMGPU_OK;
+            std::string msg;
+            try {
+                prepare(*r);
+            } catch (...) {
+                code = status_of_current_exception(msg);
+            }
+            std::lock_guard<std::mutex> lk(m);
+            r->t_prep1 = now_ms();
+            if (code != SMGPU_OK) { finish_locked(*r, SMGPU_REQ_FAILED, code, msg); continue; }
+            if (r->cancel) { finish_locked(*r, SMGPU_REQ_CANCELED, SMGPU_ECANCELED, "canceled"); continue; }
+            r->state = SMGPU_REQ_READY;
+            ready.push_back(r);
+            schedule_locked();
+        }
+    }
+
+    // ------------------------------------------------------------- admission
+    void finish_locked(Request& r, int state, int32_t code, const std::string& msg) {
+        r.state = state;
+        r.status = code;
+        r.error = msg;
+        r.t_done = now_ms();
+        r.coef.reset();
+        if (state == SMGPU_REQ_DONE) st.completed++;
+        else if (state == SMGPU_REQ_CANCELED) st.canceled++;
+        else st.failed++;
+        done_cv.notify_all();
+    }
+
+    // free HBM not held by running work: unused index tables (least recently
+    // used first) and the buffers of idle slots; true if anything was freed
+    bool reclaim_locked(const IndexEntry* keep) {
+        std::vector<IndexEntry*> cand;
+        for (auto& kv : cache) {
+            IndexEntry* e = kv.second.get();
+            if (e != keep && e->didx && e->users == 0 && !e->uploading) cand.push_back(e);
+        }
+        std::sort(cand.begin(), cand.end(), [](IndexEntry* a, IndexEntry* b) { return a->last_use < b->last_use; });
+        if (!cand.empty()) {
+            IndexEntry* e = cand.front();
+            index_resident -= e->index_bytes;
+            e->didx.reset();
+            st.index_evictions++;
+            return true;
+        }
+        for (auto& s : slots)
+            if (!s->job && s->cap > 0) {
+                s->ham.idx.reset();
+                s->ham.release_coefficients();
+                release_workspace(s->ctx);
+                s->cap = 0;
+                return true;
+            }
+        return false;
+    }
+
+    bool try_admit_locked(const std::shared_ptr<Request>& r) {
+        IndexEntry& e = *r->entry;
+        const int64_t need_index = (e.didx || e.uploading) ? 0 : e.index_bytes;
+        if (need_index + r->ws_need + r->coef_need > cfg.hbm_budget) {
+            ready.remove(r);
+            std::ostringstream os;
+            os << "does not fit the HBM budget: needs " << (need_index + r->ws_need + r->coef_need) << " bytes, budget "
+               << cfg.hbm_budget;
+            finish_locked(*r, SMGPU_REQ_FAILED, SMGPU_ENOMEM, os.str());
+            return true;
+        }
+        for (;;) {
+            Slot* best = nullptr;
+            int64_t best_g = 0;
+            for (auto& s : slots) {
+                if (s->job) continue;
+                int64_t g = workspace_growth(s->ctx, r->mdim, r->params) + coefficient_growth(s->ham, *r->coef);
+                if (!best || g < best_g) { best = s.get(); best_g = g; }
+            }
+            if (!best) return false;
+            if (in_use() + need_index + best_g <= cfg.hbm_budget) {
+                ready.remove(r);
+                r->state = SMGPU_REQ_RUNNING;
+                r->slot = best->id;
+                r->t_admit = now_ms();
+                r->index_shared = e.didx != nullptr || e.uploading;
+                if (need_index) { e.uploading = true; r->uploader = true; index_resident += e.index_bytes; }
+                e.users++;
+                e.last_use = ++use_clock;
+                r->reserved = need_index + best_g;
+                best->cap += best_g;
+                best->job = r;
+                hbm_peak = std::max(hbm_peak, in_use());
+                int running = 0;
+                for (auto& s : slots) running += s->job ? 1 : 0;
+                st.peak_running = std::max(st.peak_running, running);
+                best->cv.notify_one();
+                return true;
+            }
+            if (!reclaim_locked(&e)) return false;
+        }
+    }
+
+    void schedule_locked() {
+        if (stopping) return;
+        const int max_bypass = 4 * (int)slots.size();
+        bool head_blocked = false;
+        for (auto it = ready.begin(); it != ready.end();) {
+            auto r = *it;
+            ++it;   // try_admit may remove r
+            if (head_blocked && ready.front()->bypassed >= max_bypass) break;
+            if (try_admit_locked(r)) {
+                if (head_blocked && r != ready.front()) ready.front()->bypassed++;
+                continue;
+            }
+            if (!head_blocked) head_blocked = true;
+        }
+    }
+
+    // ---------------------------------------------------------------- slots
+    void run_job(Slot& s, Request& r) {
+        IndexEntry& e = *r.entry;
+        std::shared_ptr<DeviceIndex> idx;
+        if (r.uploader) {
+            std::shared_ptr<DeviceIndex> up;
+            try {
+                up = upload_index(*e.hidx, cfg.device);
+            } catch (...) {
+                std::lock_guard<std::mutex> lk(m);
+                e.uploading = false;
+                e.upload_failed = true;
+                index_resident -= e.index_bytes;
+                entry_cv.notify_all();
+                throw;
+            }
+            std::lock_guard<std::mutex> lk(m);
+            e.didx = up;
+            e.uploading = false;
+            e.upload_failed = false;
+            st.index_uploads++;
+            entry_cv.notify_all();
+            idx = up;
+        } else {
+            std::unique_lock<std::mutex> lk(m);
+            entry_cv.wait(lk, [&] { return !e.uploading; });
+            if (!e.didx) throw std::runtime_error("index upload failed in another request");
+            idx = e.didx;
+        }
+        bind_ham(s.ham, idx, *r.coef);
+        r.evals.assign(std::max(1, r.params.num_ev), 0.0);
+        s.ctx.cancel = &r.cancel;
+        struct Unhook {
+            SolveCtx& c;
+            ~Unhook() { c.cancel = nullptr; }
+        } unhook{s.ctx};
+        trl_solve(s.ham, r.params, r.evals.data(), nullptr, r.st, nullptr, &s.ctx);
+        r.evals.resize((size_t)std::min<int64_t>(r.params.num_ev, r.mdim));
+    }
+
+    void slot_loop(Slot& s) {
+        (void)hipSetDevice(cfg.device);
+        for (;;) {
+            std::shared_ptr<Request> r;
+            {
+                std::unique_lock<std::mutex> lk(m);
+                s.cv.wait(lk, [&] { return stopping || s.job; });
+                if (!s.job) return;   // stopping
+                r = s.job;
+            }
+            int32_t code = SMGPU_OK;
+            std::string msg;
+            try {
+                if (r->cancel) throw Canceled();
+                run_job(s, *r);
+            } catch (...) {
+                code = status_of_current_exception(msg);
+            }
+            std::lock_guard<std::mutex> lk(m);
+            s.ham.idx.reset();
+            r->entry->users--;
+            r->entry->last_use = ++use_clock;
+            // charge what the slot really holds now
+            s.cap = workspace_capacity(s.ctx) + s.ham.coefficient_capacity();
+            if (s.cap > cfg.keep_bytes) {
+                s.ham.release_coefficients();
+                release_workspace(s.ctx);
+                s.cap = 0;
+            }
+            s.job.reset();
+            if (code == SMGPU_OK) finish_locked(*r, SMGPU_REQ_DONE, SMGPU_OK, "");
+            else if (code == SMGPU_ECANCELED) finish_locked(*r, SMGPU_REQ_CANCELED, code, msg);
+            else finish_locked(*r, SMGPU_REQ_FAILED, code, msg);
+            schedule_locked();
+        }
+    }
+
+    // ------------------------------------------------------------------ API
+    explicit Server(const smgpu_server_config& c) : cfg(c) {
+        SM_HIP(hipSetDevice(cfg.device));
+        if (cfg.max_inflight <= 0) cfg.max_inflight = 4;
+        if (cfg.max_inflight > 64) throw std::invalid_argument("max_inflight above 64");
+        if (cfg.host_threads <= 0) cfg.host_threads = 2;
+        if (cfg.table_threads <= 0)
+            cfg.table_threads = std::max(1, (int)std::thread::hardware_concurrency() / cfg.host_threads);
+        if (cfg.keep_bytes <= 0) cfg.keep_bytes = (int64_t)16 << 30;
+        if (cfg.hbm_budget <= 0) {
+            size_t fr = 0, tot = 0;
+            SM_HIP(hipMemGetInfo(&fr, &tot));
+            cfg.hbm_budget = (int64_t)(0.9 * (double)fr);
+        }
+        for (int i = 0; i < cfg.max_inflight; i++) {
+            auto s = std::make_unique<Slot>();
+            s->id = i;
+            SM_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+            s->ctx.stream = s->stream;
+            slots.push_back(std::move(s));
+        }
+        for (auto& s : slots) {
+            Slot* sp = s.get();
+            sp->th = std::thread([this, sp] { slot_loop(*sp); });
+        }
+        for (int i = 0; i < cfg.host_threads; i++) preparers.emplace_back([this] { preparer_loop(); });
+    }
+
+    ~Server() {
+        (void)hipSetDevice(cfg.device);
+        {
+            std::lock_guard<std::mutex> lk(m);
+            stopping = true;
+            for (auto& kv : reqs) kv.second->cancel = true;
+            prep_cv.notify_all();
+            for (auto& s : slots) s->cv.notify_all();
+        }
+        for (auto& t : preparers) t.join();
+        for (auto& s : slots) s->th.join();
+        std::lock_guard<std::mutex> lk(m);
+        for (auto& r : prep_q) finish_locked(*r, SMGPU_REQ_CANCELED, SMGPU_ECANCELED, "server destroyed");
+        for (auto& r : ready) finish_locked(*r, SMGPU_REQ_CANCELED, SMGPU_ECANCELED, "server destroyed");
+        prep_q.clear();
+        ready.clear();
+        for (auto& s : slots) {
+            s->ham.idx.reset();
+            release_workspace(s->ctx);
+            if (s->stream) (void)hipStreamDestroy(s->stream);
+        }
+        cache.clear();
+    }
+
+    int64_t submit(const char* path, const char* nuc, int twoM, const SolveParams& p) {
+        if (p.num_ev < 1 || p.num_ev > 64) throw std::invalid_argument("num_ev must be in 1..64");
+        auto r = std::make_shared<Request>();
+        r->snt_path = path;
+        r->nucleus = nuc;
+        r->twoM = twoM;
+        r->params = p;
+        if (r->params.max_iter <= 0) r->params.max_iter = 100000;
+        std::lock_guard<std::mutex> lk(m);
+        if (stopping) throw std::invalid_argument("server is shutting down");
+        r->id = next_id++;
+        r->t_submit = now_ms();
+        reqs[r->id] = r;
+        prep_q.push_back(r);
+        st.submitted++;
+        prep_cv.notify_one();
+        return r->id;
+    }
+
+    std::shared_ptr<Request> find_locked(int64_t id) {
+        auto it = reqs.find(id);
+        if (it == reqs.end()) throw std::invalid_argument("unknown request id");
+        return it->second;
+    }
+
+    static bool finished(const Request& r) { return r.state >= SMGPU_REQ_DONE; }
+
+    void fill_info(const Request& r, smgpu_request_info* info) {
+        std::memset(info, 0, sizeof(*info));
+        info->state = r.state;
+        info->status = r.status;
+        info->num_ev = (int32_t)r.evals.size();
+        info->index_shared = r.index_shared ? 1 : 0;
+        info->slot = r.slot;
+        info->mdim = r.mdim;
+        info->index_bytes = r.entry ? r.entry->index_bytes : 0;
+        info->footprint = info->index_bytes + r.ws_need + r.coef_need;
+        info->submit_ms = r.t_submit;
+        info->prep_start_ms = r.t_prep0;
+        info->prep_end_ms = r.t_prep1;
+        info->admit_ms = r.t_admit;
+        info->done_ms = r.t_done;
+        info->latency_ms = finished(r) ? r.t_done - r.t_submit : 0.0;
+        info->solve = r.st;
+    }
+
+    void cancel(int64_t id) {
+        std::lock_guard<std::mutex> lk(m);
+        auto r = find_locked(id);
+        if (finished(*r)) return;
+        r->cancel = true;
+        if (r->state == SMGPU_REQ_QUEUED) {
+            prep_q.erase(std::remove(prep_q.begin(), prep_q.end(), r), prep_q.end());
+            finish_locked(*r, SMGPU_REQ_CANCELED, SMGPU_ECANCELED, "canceled");
+        } else if (r->state == SMGPU_REQ_READY) {
+            ready.remove(r);
+            finish_locked(*r, SMGPU_REQ_CANCELED, SMGPU_ECANCELED, "canceled");
+            schedule_locked();
+        }
+        // preparing: dropped when the preparation ends; running: the solve
+        // polls the flag once per Lanczos iteration
+    }
+
+    void stats(smgpu_server_stats* out) {
+        std::lock_guard<std::mutex> lk(m);
+        smgpu_server_stats s = st;
+        s.queued = (int32_t)prep_q.size();
+        s.preparing = 0;
+        s.ready = (int32_t)ready.size();
+        s.running = 0;
+        for (auto& kv : reqs) if (kv.second->state == SMGPU_REQ_PREPARING) s.preparing++;
+        for (auto& sl : slots) if (sl->job) s.running++;
+        s.slots = (int32_t)slots.size();
+        s.hbm_budget = cfg.hbm_budget;
+        s.hbm_in_use = in_use();
+        s.hbm_peak = hbm_peak;
+        s.clock_ms = now_ms();
+        *out = s;
+    }
+};
+
+}  // namespace smgpu
+
+using namespace smgpu;
+
+struct smgpu_server_s {
+    std::unique_ptr<Server> impl;
+};
+
+extern "C" {
+
+void smgpu_default_server_config(smgpu_server_config* cfg) {
+    if (!cfg) return;
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->device = 0;
+    cfg->max_inflight = 4;
+    cfg->hbm_budget = 0;
+    cfg->host_threads = 2;
+    cfg->table_threads = 0;
+    cfg->keep_bytes = 0;
+}
+
+int32_t smgpu_server_create(const smgpu_server_config* cfg, smgpu_server* out) {
+    return guard([&]() -> int32_t {
+        if (!out) throw std::invalid_argument("null argument");
+        *out = nullptr;
+        smgpu_server_config c;
+        smgpu_default_server_config(&c);
+        if (cfg) c = *cfg;
+        auto s = std::make_unique<smgpu_server_s>();
+        s->impl = std::make_unique<Server>(c);
+        *out = s.release();
+        return SMGPU_OK;
+    });
+}
+
+int32_t smgpu_server_destroy(smgpu_server s) {
+    return guard([&]() -> int32_t {
+        delete s;
+        return SMGPU_OK;
+    });
+}
+
+int32_t smgpu_server_submit(smgpu_server s, const char* snt_path, const char* nucleus, int32_t twoM,
+                            const smgpu_solve_params* params, int64_t* request_id) {
+    return guard([&]() -> int32_t {
+        if (!s || !snt_path || !nucleus || !params || !request_id) throw std::invalid_argument("null argument");
+        *request_id = s->impl->submit(snt_path, nucleus, twoM, *params);
+        return SMGPU_OK;
+    });
+}
+
+int32_t smgpu_server_poll(smgpu_server s, int64_t request_id, int32_t* state) {
+    return guard([&]() -> int32_t {
+        if (!s || !state) throw std::invalid_argument("null argument");
+        std::lock_guard<std::mutex> lk(s->impl->m);
+        *state = s->impl->find_locked(request_id)->state;
+        return SMGPU_OK;
+    });
+}
+
+int32_t smgpu_server_wait(smgpu_server s, int64_t request_id, double timeout_ms) {
+    return guard([&]() -> int32_t {
+        if (!s) throw std::invalid_argument("null argument");
+        Server& S = *s->impl;
+        std::unique_lock<std::mutex> lk(S.m);
+        auto r = S.find_locked(request_id);
+        auto pred = [&] { return Server::finished(*r); };
+        if (timeout_ms < 0) {
+            S.done_cv.wait(lk, pred);
+            return SMGPU_OK;
+        }
+        bool ok = S.done_cv.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), pred);
+        if (!ok) set_last_error("timed out");
+        return ok ? SMGPU_OK : SMGPU_ETIMEDOUT;
+    });
+}
+
+int32_t smgpu_server_result(smgpu_server s, int64_t request_id, double* evals, smgpu_request_info* info) {
+    return guard([&]() -> int32_t {
+        if (!s) throw std::invalid_argument("null argument");
+        Server& S = *s->impl;
+        std::lock_guard<std::mutex> lk(S.m);
+        auto r = S.find_locked(request_id);
+        if (!Server::finished(*r)) throw std::invalid_argument("request has not finished");
+        if (info) S.fill_info(*r, info);
+        if (evals) std::copy(r->evals.begin(), r->evals.end(), evals);
+        S.reqs.erase(request_id);
+        if (r->status != SMGPU_OK) set_last_error(r->error);
+        return r->status;
+    });
+}
+
+int32_t smgpu_server_info(smgpu_server s, int64_t request_id, smgpu_request_info* info) {
+    return guard([&]() -> int32_t {
+        if (!s || !info) throw std::invalid_argument("null argument");
+        std::lock_guard<std::mutex> lk(s->impl->m);
+        s->impl->fill_info(*s->impl->find_locked(request_id), info);
+        return SMGPU_OK;
+    });
+}
+
+int32_t smgpu_server_cancel(smgpu_server s, int64_t request_id) {
+    return guard([&]() -> int32_t {
+        if (!s) throw std::invalid_argument("null argument");
+        s->impl->cancel(request_id);
+        return SMGPU_OK;
+    });
+}
+
+int32_t smgpu_server_get_stats(smgpu_server s, smgpu_server_stats* st) {
+    return guard([&]() -> int32_t {
+        if (!s || !st) throw std::invalid_argument("null argument");
+        s->impl->stats(st);
+        return SMGPU_OK;
+    });
+}
+
+}  // extern "C"

@@ -3,6 +3,9 @@
 #include "../../include/smgpu.h"
 #include "sm_device.h"
 
+#include <atomic>
+#include <stdexcept>
+
 namespace smgpu {
 
 using SolveParams = smgpu_solve_params;
@@ -13,7 +16,20 @@ using SolveStats = smgpu_solve_stats;
 struct SolveCtx {
     hipStream_t stream = nullptr;
     mutable std::shared_ptr<void> ws[2];   // filled on first use
+    const std::atomic<bool>* cancel = nullptr;   // polled once per Lanczos iteration
 };
+
+// thrown by a solve whose cancel flag was raised
+struct Canceled : std::runtime_error {
+    Canceled() : std::runtime_error("canceled") {}
+};
+
+// HBM a solve allocates in its workspace (Krylov basis, work vectors, small problems)
+int64_t solve_workspace_bytes(int64_t mdim, const SolveParams& p);
+int64_t workspace_capacity(const SolveCtx& ctx);
+// bytes by which ctx's workspace grows to run such a solve (buffers are grow-only)
+int64_t workspace_growth(const SolveCtx& ctx, int64_t mdim, const SolveParams& p);
+void release_workspace(SolveCtx& ctx);
 
 void trl_solve(DeviceHam& h, const SolveParams& p, double* evals, double* evecs, SolveStats& st,
                const double* v0 = nullptr, const SolveCtx* ctx = nullptr);

@@ -495,7 +495,57 @@ namespace {
 
 struct Workspace {
     DBuf basis, wH, w64, part, npart, h1, h2, alpha, beta, enring, flags, S, swork, U, scal, tvals, Y, Q, R, hv64;
+    int* hflag = nullptr;          // pinned mirror of the convergence-flag ring
+    hipEvent_t ev[4] = {};
+    void host_sync_objects() {
+        if (hflag) return;
+        SM_HIP(hipHostMalloc((void**)&hflag, 4 * sizeof(int), hipHostMallocDefault));
+        for (int q = 0; q < 4; q++) SM_HIP(hipEventCreateWithFlags(&ev[q], hipEventDisableTiming));
+    }
+    int64_t capacity() const {
+        int64_t c = 0;
+        for (const DBuf* d : {&basis, &wH, &w64, &part, &npart, &h1, &h2, &alpha, &beta, &enring, &flags, &S, &swork,
+                              &U, &scal, &tvals, &Y, &Q, &R, &hv64})
+            c += (int64_t)d->cap;
+        return c;
+    }
+    ~Workspace() {
+        for (int q = 0; q < 4; q++) if (ev[q]) (void)hipEventDestroy(ev[q]);
+        if (hflag) (void)hipHostFree(hflag);
+    }
 };
+
+// Every device buffer of a solve over mdim amplitudes with parameters p, in the
+// working precision of TBv: f(member, bytes).  setup() allocates exactly these.
+template <typename TBv, typename F>
+static void workspace_plan(int64_t n, const SolveParams& p, F&& f) {
+    const int64_t ld = (n + 63) / 64 * 64;
+    const int64_t lm = p.lm, ls = p.ls, nev = std::max<int64_t>(1, std::min<int64_t>(p.num_ev, n));
+    const int64_t ldp = lm + 2;
+    const int64_t Gd = (n + CHUNK - 1) / CHUNK, Gu = (n + UCHUNK - 1) / UCHUNK;
+    const int64_t Gx = (n + GsShape<TBv>::SEG - 1) / GsShape<TBv>::SEG;
+    const bool mixed = sizeof(TBv) == 4;
+    f(&Workspace::basis, lm * ld * (int64_t)sizeof(TBv));
+    f(&Workspace::wH, ld * (int64_t)sizeof(TBv));
+    f(&Workspace::w64, ld * 8);
+    f(&Workspace::part, std::max(Gd, Gx) * ldp * 8);
+    f(&Workspace::npart, std::max(Gu, Gx) * 8);
+    f(&Workspace::h1, (lm + 2) * 8);
+    f(&Workspace::h2, (lm + 2) * 8);
+    f(&Workspace::alpha, (lm + 1) * 8);
+    f(&Workspace::beta, (lm + 1) * 8);
+    f(&Workspace::enring, 4 * 64 * 8);
+    f(&Workspace::flags, 4 * (int64_t)sizeof(int));
+    f(&Workspace::S, lm * 32 * 8);
+    f(&Workspace::swork, (32 * 5 * lm + 4096) * 8);
+    f(&Workspace::U, ls * ld * (int64_t)sizeof(TBv));
+    f(&Workspace::scal, 64 * 8);
+    f(&Workspace::tvals, 64 * 8);
+    f(&Workspace::Y, 32 * lm * 8);
+    f(&Workspace::Q, 32 * 32 * 8);
+    f(&Workspace::R, nev * ld * 8);
+    f(&Workspace::hv64, mixed && p.refine ? ld * 8 : 0);
+}
 
 template <typename TBv>
 struct Solver {
@@ -510,6 +560,7 @@ struct Solver {
     int64_t Gd, Gu, Gx;
     int gstore;
     Workspace& W;
+    const std::atomic<bool>* cancel = nullptr;
 
     static Workspace& cached(std::shared_ptr<void>* slots) {
         const int k = sizeof(TBv) == 4 ? 1 : 0;
@@ -521,6 +572,7 @@ struct Solver {
     Solver(DeviceHam& h, const SolveParams& p, SolveStats& stats, const SolveCtx* ctx)
         : H(h), P(p), st(stats), W(cached(ctx ? ctx->ws : h.trl_ws)) {
         s = ctx ? ctx->stream : H.stream;
+        cancel = ctx ? ctx->cancel : nullptr;
         n = H.mdim;
         ld = (n + 63) / 64 * 64;
         lm = P.lm;
@@ -578,21 +630,8 @@ struct Solver {
 
     void setup(const double* v0_in) {
         // grow-only buffers cached on the handle (no per-solve hipMalloc / memset)
-        W.basis.ensure((size_t)lm * ld * sizeof(TBv));
-        W.wH.ensure((size_t)ld * (prec_hv ? 4 : 8));
-        W.w64.ensure((size_t)ld * 8);
-        W.part.ensure((size_t)std::max(Gd, Gx) * ldp * 8);
-        W.npart.ensure((size_t)std::max(Gu, Gx) * 8);
-        W.h1.ensure((lm + 2) * 8); W.h2.ensure((lm + 2) * 8);
-        W.alpha.ensure((lm + 1) * 8); W.beta.ensure((lm + 1) * 8);
-        W.enring.ensure(4 * 64 * 8); W.flags.ensure(4 * sizeof(int));
-        W.S.ensure((size_t)lm * 32 * 8);
-        W.swork.ensure((size_t)(32 * 5 * lm + 4096) * 8);
-        W.U.ensure((size_t)32 * ld * sizeof(TBv));
-        W.scal.ensure(64 * 8);
-        W.tvals.ensure(64 * 8);
-        W.Y.ensure((size_t)32 * lm * 8);
-        W.Q.ensure(32 * 32 * 8);
+        workspace_plan<TBv>(n, P, [&](DBuf Workspace::*m, int64_t b) { (W.*m).ensure((size_t)b); });
+        W.host_sync_objects();
         std::vector<double> e0(4 * 64, 1.0e4);
         SM_HIP(hipMemcpyAsync(W.enring.p, e0.data(), e0.size() * 8, hipMemcpyHostToDevice, s));
         // start vector: caller's, or seeded normal deviates generated on the device
@@ -631,10 +670,8 @@ struct Solver {
             throw std::invalid_argument("need 1 <= num_ev <= ls <= 32 and ls < lm - 1");
         if (lm > 1000) throw std::invalid_argument("lm above 1000 is not supported");
         setup(v0_in);
-        int* hflag = nullptr;
-        SM_HIP(hipHostMalloc((void**)&hflag, 4 * sizeof(int), hipHostMallocDefault));
-        hipEvent_t ev[4];
-        for (int q = 0; q < 4; q++) SM_HIP(hipEventCreateWithFlags(&ev[q], hipEventDisableTiming));
+        int* hflag = W.hflag;
+        hipEvent_t* ev = W.ev;
         int k = 0;
         bool conv = false;
         int elit = 1, conv_step = 0;
@@ -651,6 +688,7 @@ struct Solver {
             int it = k;
             for (; it < lm - 1; it++, step++) {
                 if (++total_steps > P.max_iter) throw std::runtime_error("TRL: iteration limit reached");
+                if (cancel && cancel->load(std::memory_order_relaxed)) throw Canceled();
                 // interlacing brackets need T[0..it-1] to be the previous step's matrix
                 const int nprev = (it == k && after_restart) || it == 0 ? 0 : std::min(nev, it);
                 enqueue_iteration(it, step, nprev);
@@ -683,8 +721,6 @@ struct Solver {
         SM_HIP(hipMemcpy(en_last.data(), en(conv_step), nev * 8, hipMemcpyDeviceToHost));
         for (int i = 0; i < nev; i++) { evals[i] = en_last[i]; st.ritz[i] = en_last[i]; }
         finish(elit, conv_step, evals, evecs);
-        for (int q = 0; q < 4; q++) (void)hipEventDestroy(ev[q]);
-        SM_HIP(hipHostFree(hflag));
         st.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
 
@@ -811,7 +847,6 @@ struct Solver {
         hipLaunchKernelGGL(k_tri_vecs, dim3(1), dim3(64), 0, s, (const double*)W.alpha.as<double>(),
                            (const double*)W.beta.as<double>(), elit, (const double*)en(conv_step), nev, Y,
                            W.swork.as<double>());
-        W.R.ensure((size_t)nev * ld * 8);
         DBuf& R = W.R;
         int gb = (int)std::min<int64_t>(2048, (n + TB - 1) / TB);
         size_t sh = (size_t)nev * elit * 8;
@@ -819,7 +854,6 @@ struct Solver {
         hipLaunchKernelGGL((k_rotate<TBv, double>), dim3(gb), dim3(TB), sh, s, (const TBv*)V(0), ld, elit,
                            (const double*)Y, nev, R.as<double>(), ld, n);
         const bool refine = prec_hv == 1 && P.refine;
-        if (refine) W.hv64.ensure((size_t)ld * 8);
         DBuf& hv64 = W.hv64;
         double* sc = W.scal.as<double>();
         for (int i = 0; i < nev; i++) {
@@ -859,6 +893,39 @@ struct Solver {
 };
 
 }  // namespace
+
+int64_t solve_workspace_bytes(int64_t mdim, const SolveParams& p) {
+    int64_t total = 0;
+    auto add = [&](DBuf Workspace::*, int64_t b) { total += b; };
+    if (p.precision == 0) workspace_plan<double>(mdim, p, add);
+    else workspace_plan<float>(mdim, p, add);
+    return total;
+}
+
+int64_t workspace_capacity(const SolveCtx& ctx) {
+    int64_t c = 0;
+    for (int k = 0; k < 2; k++)
+        if (ctx.ws[k]) c += static_cast<const Workspace*>(ctx.ws[k].get())->capacity();
+    return c;
+}
+
+int64_t workspace_growth(const SolveCtx& ctx, int64_t mdim, const SolveParams& p) {
+    int64_t g = 0;
+    const int k = p.precision == 0 ? 0 : 1;
+    const Workspace* W = static_cast<const Workspace*>(ctx.ws[k].get());
+    auto f = [&](DBuf Workspace::*m, int64_t b) {
+        int64_t cap = W ? (int64_t)(W->*m).cap : 0;
+        if (b > cap) g += b - cap;
+    };
+    if (k == 0) workspace_plan<double>(mdim, p, f);
+    else workspace_plan<float>(mdim, p, f);
+    return g;
+}
+
+void release_workspace(SolveCtx& ctx) {
+    for (int k = 0; k < 2; k++) ctx.ws[k].reset();
+}
+
 
 void trl_solve(DeviceHam& h, const SolveParams& p, double* evals, double* evecs, SolveStats& st,
                const double* v0, const SolveCtx* ctx) {

@@ -1,0 +1,131 @@
+# Julia binding of libsmgpu (include/smgpu.h) for NuclearToolkit.jl's main_sm path.
+#
+# The reference's seam is the body of `main_sm` (src/ShellModel/shellmodel_main.jl:89-281):
+# readsmsnt -> occ -> prep_Hamil_T1 / prep_2bjumps / prep_Hamil_pn / prep_pn -> TRL
+# -> Ritz assembly.  `SMGPU.main_sm` keeps that signature (target_J = [] only) and
+# returns main_sm's observable en[1]; `SMGPU.operate_H!` keeps operate_H!'s
+# accumulate-into-twf contract (src/ShellModel/lanczos_methods.jl:153-248).
+# Errors come back as Julia exceptions with the library's message.
+module SMGPU
+
+const libsmgpu = get(ENV, "SMGPU_LIB", joinpath(@__DIR__, "..", "build", "libsmgpu.so"))
+
+const FP64 = Int32(0)
+const MIXED = Int32(1)
+
+struct SolveParams              # smgpu_solve_params
+    num_ev::Int32
+    lm::Int32
+    ls::Int32
+    num_history::Int32
+    tol::Float64
+    precision::Int32
+    refine::Int32
+    max_iter::Int32
+    seed::UInt64
+end
+
+struct SolveStats               # smgpu_solve_stats
+    iterations::Int32
+    n_hv::Int32
+    n_hv64::Int32
+    n_restart::Int32
+    wall_ms::Float64
+    ritz::NTuple{64,Float64}
+end
+
+struct ProblemInfo              # smgpu_problem_info
+    Z::Int32; N::Int32; A::Int32; vp::Int32; vn::Int32; twoM::Int32
+    lp::Int32; ln::Int32; cp::Int32; cn::Int32
+    nblocks::Int32; n_msps_p::Int32; n_msps_n::Int32
+    mdim::Int64; n_psd::Int64; n_nsd::Int64
+    n_pp_ops::Int64; n_nn_ops::Int64; n_pn_ops::Int64; n_hop_entries::Int64
+end
+
+function check(code::Integer)
+    code == 0 && return nothing
+    msg = unsafe_string(ccall((:smgpu_last_error, libsmgpu), Cstring, ()))
+    error("smgpu status $code: $msg")
+end
+
+mutable struct Problem
+    h::Ptr{Cvoid}
+end
+
+function Problem(sntf::AbstractString, target_nuc::AbstractString; twoM::Integer=-1,
+                 ignore_massop::Bool=false, nthreads::Integer=Threads.nthreads())
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:smgpu_problem_create, libsmgpu), Int32,
+                (Cstring, Cstring, Int32, Int32, Int32, Ref{Ptr{Cvoid}}),
+                sntf, target_nuc, twoM, ignore_massop, nthreads, out))
+    p = Problem(out[])
+    finalizer(x -> ccall((:smgpu_problem_destroy, libsmgpu), Int32, (Ptr{Cvoid},), x.h), p)
+    return p
+end
+
+function info(p::Problem)
+    r = Ref{ProblemInfo}()
+    check(ccall((:smgpu_problem_get_info, libsmgpu), Int32, (Ptr{Cvoid}, Ref{ProblemInfo}), p.h, r))
+    return r[]
+end
+
+mutable struct Ham
+    h::Ptr{Cvoid}
+    mdim::Int
+end
+
+function Ham(p::Problem; device::Integer=0)
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:smgpu_ham_create, libsmgpu), Int32, (Ptr{Cvoid}, Int32, Ref{Ptr{Cvoid}}), p.h, device, out))
+    h = Ham(out[], info(p).mdim)
+    finalizer(x -> ccall((:smgpu_ham_destroy, libsmgpu), Int32, (Ptr{Cvoid},), x.h), h)
+    return h
+end
+
+# twf += H wf  (operate_H!'s contract; the device computes H wf in `precision`)
+function operate_H!(wf::Vector{Float64}, twf::Vector{Float64}, H::Ham; precision::Int32=FP64)
+    @assert length(wf) == H.mdim && length(twf) == H.mdim
+    w = similar(wf)
+    check(ccall((:smgpu_apply_H_host, libsmgpu), Int32, (Ptr{Cvoid}, Int32, Ptr{Float64}, Ptr{Float64}),
+                H.h, precision, wf, w))
+    twf .+= w
+    return nothing
+end
+
+function default_params()
+    r = Ref{SolveParams}()
+    ccall((:smgpu_default_params, libsmgpu), Cvoid, (Ref{SolveParams},), r)
+    return r[]
+end
+
+# TRL + Ritz assembly; returns (en[1], Ritz vectors as mdim x num_ev or nothing, stats)
+function trl(H::Ham, num_ev::Integer; lm=300, ls=20, tol=1e-8, num_history=3, precision::Int32=MIXED,
+             refine::Bool=true, seed::Integer=1234, vectors::Bool=false, in_wf=nothing)
+    d = default_params()
+    p = SolveParams(num_ev, lm, ls, num_history, tol, precision, refine, d.max_iter, UInt64(seed))
+    nev = min(num_ev, H.mdim)
+    evals = zeros(Float64, nev)
+    evecs = vectors ? zeros(Float64, H.mdim, nev) : nothing   # column k = C row k of num_ev x mdim
+    st = Ref{SolveStats}()
+    v0 = in_wf === nothing ? C_NULL : pointer(in_wf)
+    GC.@preserve in_wf evecs begin
+        check(ccall((:smgpu_trl_solve_v0, libsmgpu), Int32,
+                    (Ptr{Cvoid}, Ref{SolveParams}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ref{SolveStats}),
+                    H.h, p, v0, evals, evecs === nothing ? C_NULL : pointer(evecs), st))
+    end
+    return evals, evecs, st[]
+end
+
+# main_sm(sntf, target_nuc, num_ev, target_J; ...) for target_J = []: en[1]
+function main_sm(sntf, target_nuc, num_ev, target_J; num_history=3, lm=300, ls=20, tol=1e-8,
+                 precision::Int32=MIXED, device::Integer=0)
+    isempty(target_J) || error("target_J != [] (J projection) is not served by the GPU path")
+    prob = Problem(sntf, target_nuc)
+    mdim = info(prob).mdim
+    mdim == 0 && return Float64[]
+    H = Ham(prob; device=device)
+    en, _, _ = trl(H, min(num_ev, mdim); lm=lm, ls=ls, tol=tol, num_history=num_history, precision=precision)
+    return en
+end
+
+end # module

@@ -15,7 +15,12 @@ SMGPU_OK = 0
 SMGPU_FP64 = 0
 SMGPU_MIXED = 1
 
-ERRORS = {-1: "EINVAL", -2: "EIO", -3: "EDEVICE", -4: "ENOMEM", -5: "ENOTCONV", -6: "ECANCELED"}
+SMGPU_ECANCELED = -6
+SMGPU_ETIMEDOUT = -7
+ERRORS = {-1: "EINVAL", -2: "EIO", -3: "EDEVICE", -4: "ENOMEM", -5: "ENOTCONV", -6: "ECANCELED", -7: "ETIMEDOUT"}
+
+# request states of the serving loop
+REQ_STATES = {0: "queued", 1: "preparing", 2: "ready", 3: "running", 4: "done", 5: "failed", 6: "canceled"}
 
 
 class SmgpuError(RuntimeError):
@@ -42,6 +47,29 @@ class SolveParams(ctypes.Structure):
 class SolveStats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int32), ("n_hv", ctypes.c_int32), ("n_hv64", ctypes.c_int32),
                 ("n_restart", ctypes.c_int32), ("wall_ms", ctypes.c_double), ("ritz", ctypes.c_double * 64)]
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int32), ("max_inflight", ctypes.c_int32), ("hbm_budget", ctypes.c_int64),
+                ("host_threads", ctypes.c_int32), ("table_threads", ctypes.c_int32), ("keep_bytes", ctypes.c_int64)]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_int32), ("status", ctypes.c_int32), ("num_ev", ctypes.c_int32),
+                ("index_shared", ctypes.c_int32), ("slot", ctypes.c_int32), ("pad_", ctypes.c_int32),
+                ("mdim", ctypes.c_int64), ("footprint", ctypes.c_int64), ("index_bytes", ctypes.c_int64),
+                ("submit_ms", ctypes.c_double),
+                ("prep_start_ms", ctypes.c_double), ("prep_end_ms", ctypes.c_double), ("admit_ms", ctypes.c_double),
+                ("done_ms", ctypes.c_double), ("latency_ms", ctypes.c_double), ("solve", SolveStats)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_int64), ("completed", ctypes.c_int64), ("failed", ctypes.c_int64),
+                ("canceled", ctypes.c_int64), ("queued", ctypes.c_int32), ("preparing", ctypes.c_int32),
+                ("ready", ctypes.c_int32), ("running", ctypes.c_int32), ("peak_running", ctypes.c_int32),
+                ("slots", ctypes.c_int32), ("hbm_budget", ctypes.c_int64), ("hbm_in_use", ctypes.c_int64),
+                ("hbm_peak", ctypes.c_int64), ("index_hits", ctypes.c_int64), ("index_misses", ctypes.c_int64),
+                ("index_uploads", ctypes.c_int64), ("index_evictions", ctypes.c_int64), ("clock_ms", ctypes.c_double)]
 
 
 # exported symbol -> (restype, argtypes)
@@ -77,6 +105,16 @@ SIGNATURES = {
     "smgpu_synchronize": (I32, [P]),
     "smgpu_selftest_tridiag": (I32, [P, P, I32, I32, P, P]),
     "smgpu_selftest_restart": (I32, [I32, P, P, I32, I32, P, P, P]),
+    "smgpu_default_server_config": (None, [P]),
+    "smgpu_server_create": (I32, [P, P]),
+    "smgpu_server_destroy": (I32, [P]),
+    "smgpu_server_submit": (I32, [P, ctypes.c_char_p, ctypes.c_char_p, I32, P, P]),
+    "smgpu_server_poll": (I32, [P, I64, P]),
+    "smgpu_server_wait": (I32, [P, I64, ctypes.c_double]),
+    "smgpu_server_result": (I32, [P, I64, P, P]),
+    "smgpu_server_info": (I32, [P, I64, P]),
+    "smgpu_server_cancel": (I32, [P, I64]),
+    "smgpu_server_get_stats": (I32, [P, P]),
 }
 
 _LIB = None

@@ -196,3 +196,58 @@ def config_snt(path: str, space: str, zv: int, nv: int, seed: int, spe_range, si
     inter = make_interaction(space, seed, spe_range, sigma)
     write_snt(path, inter, mass)
     return path, nuc
+
+
+def variant_snts(directory: str, n_variants: int, space="pf", zv=4, nv=4, seed=BASE_SEED + 5,
+                 spe_range=(-8.0, 0.0), sigma=1.0, perturb_sigma=0.05) -> tuple:
+    """Interaction variants sharing one basis: a base isospin interaction (seed)
+    plus, per variant v, N(0, perturb_sigma) on every TBME drawn from seed
+    `seed * 1000 + v`; SPEs from the base.  Returns (paths, nucleus)."""
+    nuc = nucleus_name(space, zv, nv)
+    mass = CORES[space][0] + CORES[space][1] + zv + nv
+    base = make_interaction(space, seed, spe_range, sigma)
+    paths = []
+    for v in range(n_variants):
+        inter = make_interaction(space, seed * 1000 + v, base=base, perturb_sigma=perturb_sigma)
+        paths.append(write_snt(os.path.join(directory, f"variant_{v:03d}.snt"), inter, mass,
+                               comment=f"variant={v} base_seed={seed}"))
+    return paths, nuc
+
+
+@dataclass
+class ServingRequest:
+    index: int
+    space: str
+    zv: int
+    nv: int
+    num_ev: int
+    arrival_s: float     # offset from the start of the stream
+    snt: str
+    nucleus: str
+    seed: int
+
+
+def serving_stream(directory: str, n_requests: int, seed=BASE_SEED * 10 + 4, rate_per_s=50.0,
+                   spaces=("sd", "pf"), particles=(2, 6), num_ev=(1, 10), spe_range=(-6.0, 0.0), sigma=1.2,
+                   write=True) -> list:
+    """A seeded stream of eigensolve requests: space uniform over `spaces`,
+    proton and neutron valence numbers uniform in `particles` (inclusive), k
+    uniform in `num_ev`, Poisson arrivals (exponential gaps, mean 1/rate), and a
+    per-request interaction drawn from seed `seed * 100000 + i`.  2M is left to
+    the main_sm default (A % 2)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    t = 0.0
+    for i in range(n_requests):
+        space = spaces[int(rng.integers(len(spaces)))]
+        zv = int(rng.integers(particles[0], particles[1] + 1))
+        nv = int(rng.integers(particles[0], particles[1] + 1))
+        k = int(rng.integers(num_ev[0], num_ev[1] + 1))
+        t += float(rng.exponential(1.0 / rate_per_s))
+        rseed = seed * 100000 + i
+        path = os.path.join(directory, f"request_{i:04d}.snt")
+        nuc = nucleus_name(space, zv, nv)
+        if write:
+            config_snt(path, space, zv, nv, rseed, spe_range, sigma)
+        out.append(ServingRequest(i, space, zv, nv, k, t, path, nuc, rseed))
+    return out

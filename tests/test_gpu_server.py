@@ -1,0 +1,130 @@
+"""Serving loop on the GPU (smgpu_server_* through the C ABI).
+
+* alone vs in the mix: every request of a mixed stream (sd and pf spaces,
+  different particle numbers, k and interactions, several sharing a basis) gives
+  the same eigenvalues when served concurrently as when solved alone through
+  smgpu_trl_solve with the same parameters and seed (tolerance 1e-9 MeV; the
+  kernels are deterministic, so in practice the values are identical), and the
+  mixed-precision values agree with a plain FP64 solve to 1e-5 MeV;
+* admission by HBM footprint: with a budget that holds one request's
+  workspace only, the server never runs two at once and never exceeds the
+  budget; a request larger than the budget fails with ENOMEM;
+* cancel of a queued and of a running request; errors (missing file) are
+  reported per request; the server keeps serving afterwards.
+"""
+import time
+
+import numpy as np
+import pytest
+
+import smgpu
+from smgpu import synth
+from smgpu._lib import SMGPU_ECANCELED
+from smgpu.server import Server
+
+pytestmark = pytest.mark.gpu
+
+TOL_MIX = 1e-9       # alone vs in the mix, same precision and seed
+TOL_FP64 = 1e-5      # mixed precision vs FP64 (MeV)
+
+
+def _stream(snt_dir):
+    """12 requests: sd and pf, 2..4 valence particles per species, k 1..10;
+    two pairs share a basis (same orbits, Z_v, N_v) with different interactions."""
+    spec = [("sd", 2, 2, 3), ("pf", 2, 2, 5), ("sd", 3, 2, 4), ("pf", 2, 3, 2), ("sd", 4, 4, 5),
+            ("sd", 2, 4, 10), ("pf", 3, 3, 6), ("sd", 4, 4, 1), ("sd", 3, 3, 7), ("pf", 2, 2, 9),
+            ("sd", 2, 3, 8), ("pf", 3, 2, 3)]
+    out = []
+    for i, (space, zv, nv, k) in enumerate(spec):
+        path, nuc = synth.config_snt(f"{snt_dir}/srv_{i}.snt", space, zv, nv, 7000 + i, (-6, 0), 1.2)
+        out.append((path, nuc, k, 100 + i))
+    return out
+
+
+def _alone(path, nuc, k, seed, precision):
+    p = smgpu.Problem(path, nuc)
+    h = smgpu.Hamiltonian(p)
+    ev, _, _ = h.trl(min(k, p.mdim), precision=precision, seed=seed)
+    return ev
+
+
+def test_server_alone_vs_mix(snt_dir):
+    reqs = _stream(snt_dir)
+    alone = [_alone(p, n, k, s, "mixed") for p, n, k, s in reqs]
+    fp64 = [_alone(p, n, k, s, "fp64") for p, n, k, s in reqs]
+    with Server(max_inflight=4) as srv:
+        ids = [srv.submit(p, n, k, precision="mixed", seed=s) for p, n, k, s in reqs]
+        for rid in ids:
+            assert srv.wait(rid, 120000)
+        res = [srv.result(rid) for rid in ids]
+        st = srv.stats()
+    assert st["completed"] == len(reqs) and st["failed"] == 0
+    assert st["peak_running"] >= 2, st            # the mix really overlapped
+    assert st["hbm_peak"] <= st["hbm_budget"]
+    for (path, nuc, k, s), a, f, (ev, info) in zip(reqs, alone, fp64, res):
+        print(nuc, info["mdim"], k, info["latency_ms"], flush=True)
+        assert info["state"] == "done" and len(ev) == len(a)
+        assert np.abs(ev - a).max() <= TOL_MIX, (nuc, ev - a)
+        assert np.abs(ev - f).max() <= TOL_FP64, (nuc, ev - f)
+        assert info["latency_ms"] >= info["solve_ms"] > 0
+
+
+def test_server_shared_basis_and_budget(snt_dir):
+    """Same basis, different interactions: the index is uploaded once; a budget
+    of one request's footprint plus half its private part admits one at a time."""
+    paths = [synth.config_snt(f"{snt_dir}/bud_{i}.snt", "sd", 3, 3, 8000 + i, (-6, 0), 1.2)[0] for i in range(4)]
+    nuc = synth.nucleus_name("sd", 3, 3)
+    with Server(max_inflight=1) as srv:
+        rid = srv.submit(paths[0], nuc, 4)
+        assert srv.wait(rid, 60000)
+        ev0, info0 = srv.result(rid)
+    F, idx = info0["footprint"], info0["index_bytes"]
+    assert 0 < idx < F
+    budget = F + (F - idx) // 2
+    with Server(max_inflight=3, hbm_budget=budget) as srv:
+        ids = [srv.submit(p, nuc, 4) for p in paths]
+        for rid in ids:
+            assert srv.wait(rid, 60000)
+        infos = [srv.result(rid)[1] for rid in ids]
+        st = srv.stats()
+    assert st["peak_running"] == 1, st
+    assert st["hbm_peak"] <= budget
+    assert st["index_uploads"] == 1 and st["index_misses"] == 1 and st["index_hits"] == 3, st
+    assert sum(i["index_shared"] for i in infos) == 3
+    # a budget below one request's footprint: ENOMEM for that request only
+    with Server(max_inflight=2, hbm_budget=F // 2) as srv:
+        rid = srv.submit(paths[0], nuc, 4)
+        assert srv.wait(rid, 60000)
+        _, info = srv.result(rid, raise_on_error=False)
+        assert info["state"] == "failed" and info["status"] == -4
+
+
+def test_server_cancel_and_errors(snt_dir, golden):
+    path, nuc = synth.config_snt(f"{snt_dir}/cancel.snt", "sd", 4, 4, 9001, (-6, 0), 1.2)
+    with Server(max_inflight=1) as srv:
+        # tol = -1 never converges (the Ritz values never rise by 1 MeV): it
+        # restarts until canceled (the flag is polled once per iteration)
+        long_id = srv.submit(path, nuc, 5, tol=-1.0, ls=10, lm=60, max_iter=20000)
+        queued_id = srv.submit(path, nuc, 5)
+        t0 = time.time()
+        while srv.poll(long_id) != "running":
+            assert time.time() - t0 < 60
+            time.sleep(0.005)
+        srv.cancel(queued_id)
+        srv.cancel(long_id)
+        assert srv.wait(long_id, 60000) and srv.wait(queued_id, 60000)
+        for rid in (long_id, queued_id):
+            _, info = srv.result(rid, raise_on_error=False)
+            print(rid, info, flush=True)
+            assert info["state"] == "canceled" and info["status"] == SMGPU_ECANCELED, info
+        bad = srv.submit(f"{golden}/does_not_exist.snt", "Be8", 2)
+        assert srv.wait(bad, 60000)
+        _, info = srv.result(bad, raise_on_error=False)
+        assert info["state"] == "failed" and info["status"] == -2
+        # still serving
+        ok = srv.submit(f"{golden}/ckpot.snt", "Be8", 5, precision="fp64")
+        assert srv.wait(ok, 60000)
+        ev, info = srv.result(ok)
+        np.testing.assert_allclose(ev, [-31.119, -27.300, -19.162, -18.249, -16.722], atol=1e-3)
+        with pytest.raises(smgpu.SmgpuError):
+            srv.submit(path, nuc, 0)
