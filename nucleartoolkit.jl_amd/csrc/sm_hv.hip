@@ -162,16 +162,18 @@ __global__ __launch_bounds__(256) void k_hv(HvTables A, const T* __restrict__ v,
                 off[k] = A.ntab_slice_off[s] + (fn & 63);
                 len[k] = A.ntab_slice_len[s];
             } else {
-                off[k] = 0;
+                off[k] = A.ntab_slice_off[sb] + (fn & 63);   // a valid row: every slice has one
                 len[k] = 0;
             }
             maxlen = max(maxlen, len[k]);
         }
         for (int ch = 0; ch < nch; ch++) {
-            // slot metadata (uniform)
-            const T* src[SL];
-            const T* vsrc[SL];
-            T sg[SL];
+            // slot metadata (uniform): offsets into v and V_eff, not pointers, so
+            // that every gather below is an unconditional global load from a
+            // kernel argument (inactive slots read a valid element times 0)
+            int64_t xo[SL];
+            int vo[SL];
+            T sg[SL], vm[SL];
             bool act_row[RG];
 #pragma unroll
             for (int r = 0; r < RG; r++) {
@@ -182,72 +184,88 @@ __global__ __launch_bounds__(256) void k_hv(HvTables A, const T* __restrict__ v,
                     const int sl = r * PCH + j;
                     if (h < rb[r + 1]) {
                         const uint32_t e = pent[h];
-                        src[sl] = v + ibase + (int64_t)(e & 0xFFFFFu) * nNi;
-                        vsrc[sl] = V2 + (int64_t)((e >> 20) & 0x7FFu) * c2;
+                        xo[sl] = ibase + (int64_t)(e & 0xFFFFFu) * nNi;
+                        vo[sl] = (int)((e >> 20) & 0x7FFu) * c2;
                         sg[sl] = (e >> 31) ? T(-1) : T(1);
+                        vm[sl] = T(1);
                     } else {
-                        src[sl] = nullptr;
-                        vsrc[sl] = nullptr;
+                        xo[sl] = ibase;
+                        vo[sl] = 0;
                         sg[sl] = T(0);
+                        vm[sl] = T(0);
                     }
                 }
             }
             __syncthreads();
-            // staged slots are written four at a time (16-byte stores, no bank conflicts
-            // between neighbouring elements)
+            // staged slots are written four at a time (16-byte stores)
             for (int i = tid; i < nNi; i += 256) {
+                T xv[SL];
+#pragma unroll
+                for (int sl = 0; sl < SL; sl++) xv[sl] = v[xo[sl] + i];
 #pragma unroll
                 for (int r = 0; r < RG; r++) {
                     T4 x4;
-                    x4.x = src[r * PCH + 0] ? sg[r * PCH + 0] * src[r * PCH + 0][i] : T(0);
-                    x4.y = src[r * PCH + 1] ? sg[r * PCH + 1] * src[r * PCH + 1][i] : T(0);
-                    x4.z = src[r * PCH + 2] ? sg[r * PCH + 2] * src[r * PCH + 2][i] : T(0);
-                    x4.w = src[r * PCH + 3] ? sg[r * PCH + 3] * src[r * PCH + 3][i] : T(0);
+                    x4.x = sg[r * PCH + 0] * xv[r * PCH + 0];
+                    x4.y = sg[r * PCH + 1] * xv[r * PCH + 1];
+                    x4.z = sg[r * PCH + 2] * xv[r * PCH + 2];
+                    x4.w = sg[r * PCH + 3] * xv[r * PCH + 3];
                     *reinterpret_cast<T4*>(X + (size_t)i * SL + r * PCH) = x4;
                 }
             }
             for (int c = tid; c < c2; c += 256) {
+                T cv[SL];
+#pragma unroll
+                for (int sl = 0; sl < SL; sl++) cv[sl] = V2[vo[sl] + c];
 #pragma unroll
                 for (int r = 0; r < RG; r++) {
                     T4 v4;
-                    v4.x = vsrc[r * PCH + 0] ? vsrc[r * PCH + 0][c] : T(0);
-                    v4.y = vsrc[r * PCH + 1] ? vsrc[r * PCH + 1][c] : T(0);
-                    v4.z = vsrc[r * PCH + 2] ? vsrc[r * PCH + 2][c] : T(0);
-                    v4.w = vsrc[r * PCH + 3] ? vsrc[r * PCH + 3][c] : T(0);
+                    v4.x = vm[r * PCH + 0] * cv[r * PCH + 0];
+                    v4.y = vm[r * PCH + 1] * cv[r * PCH + 1];
+                    v4.z = vm[r * PCH + 2] * cv[r * PCH + 2];
+                    v4.w = vm[r * PCH + 3] * cv[r * PCH + 3];
                     *reinterpret_cast<T4*>(Vs + (size_t)c * SL + r * PCH) = v4;
                 }
             }
             __syncthreads();
-            // neutron-hop entries are prefetched two steps ahead of their use
-            uint32_t e0[NPT], e1[NPT];
+            // neutron-hop entries, prefetched PF steps ahead with unconditional loads:
+            // a lane past its slice length re-reads its last entry (every slice has
+            // at least one) and uses the zero column instead.  The step loop is
+            // unrolled by PF so each ring slot keeps its register (no moves that
+            // would wait for loads in flight); the length test is applied when an
+            // entry is consumed.  Steps past maxlen (< PF per round) add zeros.
+            constexpr int PF = 4;
+            uint32_t pf[PF][NPT];
 #pragma unroll
-            for (int k = 0; k < NPT; k++) {
-                e0[k] = 0 < len[k] ? A.ntab_ent[off[k]] : zent;
-                e1[k] = 1 < len[k] ? A.ntab_ent[off[k] + 64] : zent;
-            }
-            for (int qq = 0; qq < maxlen; qq++) {
-                int in[NPT], col[NPT];
+            for (int d = 0; d < PF; d++)
 #pragma unroll
-                for (int k = 0; k < NPT; k++) {
-                    const uint32_t e = e0[k];
-                    in[k] = (int)(e & 0xFFFFFu);
-                    col[k] = (int)(e >> 20);
-                    e0[k] = e1[k];
-                    e1[k] = qq + 2 < len[k] ? A.ntab_ent[off[k] + (int64_t)(qq + 2) * 64] : zent;
-                }
+                for (int k = 0; k < NPT; k++)
+                    pf[d][k] = A.ntab_ent[off[k] + (int64_t)min(d, max(len[k] - 1, 0)) * 64];
+            for (int q0 = 0; q0 < maxlen; q0 += PF) {
 #pragma unroll
-                for (int r = 0; r < RG; r++) {
-                    if (act_row[r]) {
+                for (int d = 0; d < PF; d++) {
+                    const int qq = q0 + d;
+                    int in[NPT], col[NPT];
 #pragma unroll
-                        for (int k = 0; k < NPT; k++) {
-                            const T4 xv = *reinterpret_cast<const T4*>(X + (size_t)in[k] * SL + r * PCH);
-                            const T4 vv = *reinterpret_cast<const T4*>(Vs + (size_t)col[k] * SL + r * PCH);
-                            T a = acc[k][r];
-                            a += vv.x * xv.x;
-                            a += vv.y * xv.y;
-                            a += vv.z * xv.z;
-                            a += vv.w * xv.w;
-                            acc[k][r] = a;
+                    for (int k = 0; k < NPT; k++) {
+                        const uint32_t e = qq < len[k] ? pf[d][k] : zent;
+                        in[k] = (int)(e & 0xFFFFFu);
+                        col[k] = (int)(e >> 20);
+                        pf[d][k] = A.ntab_ent[off[k] + (int64_t)min(qq + PF, max(len[k] - 1, 0)) * 64];
+                    }
+#pragma unroll
+                    for (int r = 0; r < RG; r++) {
+                        if (act_row[r]) {
+#pragma unroll
+                            for (int k = 0; k < NPT; k++) {
+                                const T4 xv = *reinterpret_cast<const T4*>(X + (size_t)in[k] * SL + r * PCH);
+                                const T4 vv = *reinterpret_cast<const T4*>(Vs + (size_t)col[k] * SL + r * PCH);
+                                T a = acc[k][r];
+                                a += vv.x * xv.x;
+                                a += vv.y * xv.y;
+                                a += vv.z * xv.z;
+                                a += vv.w * xv.w;
+                                acc[k][r] = a;
+                            }
                         }
                     }
                 }
@@ -437,7 +455,9 @@ void index_layout(const Basis& b, const Tables& t, HostLayout& L) {
             int cols = kp >= 0 ? t.veff_cols[kp] : 0;
             int nf = (int)H.rowptr.size() - 1;
             for (int s0 = 0; s0 < nf; s0 += 64) {
-                int len = 0;
+                // at least one row per slice (zero-column entries): the kernel reads a
+                // slice's last row unconditionally when prefetching past its length
+                int len = 1;
                 for (int f = s0; f < std::min(nf, s0 + 64); f++) len = std::max<int>(len, (int)(H.rowptr[f + 1] - H.rowptr[f]));
                 L.ntab_slice_off.push_back((int64_t)L.ntab_ent.size());
                 L.ntab_slice_len.push_back(len);
@@ -486,7 +506,7 @@ void index_layout(const Basis& b, const Tables& t, HostLayout& L) {
             int64_t npad = 0;
             int nf = (int)HN.rowptr.size() - 1;
             for (int s0 = 0; s0 < nf; s0 += 64) {
-                int len = 0;
+                int len = 1;
                 for (int f = s0; f < std::min(nf, s0 + 64); f++) len = std::max<int>(len, (int)(HN.rowptr[f + 1] - HN.rowptr[f]));
                 npad += (int64_t)len * std::min(64, nf - s0);
             }

@@ -34,6 +34,7 @@
 #include <cstdlib>
 #include <random>
 #include <stdexcept>
+#include <string>
 
 #include "sm_device.h"
 #include "sm_trl.h"
@@ -179,11 +180,20 @@ template <typename TBv> __device__ inline int64_t gs_x(int64_t seg0, int lane, i
     return seg0 + (int64_t)q * 64 * W + (int64_t)lane * W + i;
 }
 
-// part[blk*ldp + j] = sum_{x in segment} B[j][x] w[x]   (ld of B is a multiple of 64)
-template <typename TBv, typename TW>
+// part[blk*ldp + j] = sum_{x in segment} B[j][x] w'[x]   (ld of B is a multiple of 64)
+// LOCAL = false: w' = w.
+// LOCAL = true (the Lanczos local step): w' = w - alpha v_c - beta v_p with
+// alpha = *alpha, beta = *bprev (v_p and bprev may be null), formed on the fly;
+// w' is also written to wout (FP64) and |w'|^2 partials to npart[blk].
+// gate: skip the launch when *gate == 0 (conditional second pass).
+template <typename TBv, typename TW, bool LOCAL>
 __global__ __launch_bounds__(TB) void k_gs_dots(const TBv* __restrict__ B, int64_t ldb, int nv,
                                                 const TW* __restrict__ w, int64_t n, double* __restrict__ part,
-                                                int ldp) {
+                                                int ldp, const int* __restrict__ gate,
+                                                const TBv* __restrict__ vc, const TBv* __restrict__ vp,
+                                                const double* __restrict__ alpha, const double* __restrict__ bprev,
+                                                double* __restrict__ wout, double* __restrict__ npart) {
+    if (gate && *gate == 0) return;
     using S = GsShape<TBv>;
     constexpr int W = Ld16<TBv>::W;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -196,6 +206,28 @@ __global__ __launch_bounds__(TB) void k_gs_dots(const TBv* __restrict__ B, int64
             const int64_t x = gs_x<TBv>(seg0, lane, q, i);
             a[q * W + i] = x < n ? (double)w[x] : 0.0;
         }
+    if (LOCAL) {
+        const double al = *alpha;
+        const double be = (vp && bprev) ? *bprev : 0.0;
+        double nr = 0.0;
+#pragma unroll
+        for (int q = 0; q < S::Q; q++)
+#pragma unroll
+            for (int i = 0; i < W; i++) {
+                const int64_t x = gs_x<TBv>(seg0, lane, q, i);
+                if (x < n) {
+                    double t = a[q * W + i] - al * (double)vc[x];
+                    if (vp) t -= be * (double)vp[x];
+                    a[q * W + i] = t;
+                    nr += t * t;
+                    if (wv == 0) wout[x] = t;
+                }
+            }
+        if (wv == 0) {
+            nr = wave_sum(nr);
+            if (lane == 0) npart[blockIdx.x] = nr;
+        }
+    }
     const bool full = seg0 + S::SEG <= n;
 #pragma unroll 2
     for (int j = wv; j < nv; j += TB / 64) {
@@ -228,7 +260,8 @@ template <typename TBv, typename TW, bool DOTS>
 __global__ __launch_bounds__(TB) void k_gs_upd(const TBv* __restrict__ B, int64_t ldb, int nv,
                                                const double* __restrict__ h, const TW* __restrict__ win,
                                                double* __restrict__ wout, int64_t n, double* __restrict__ part,
-                                               int ldp, double* __restrict__ npart) {
+                                               int ldp, double* __restrict__ npart, const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;
     using S = GsShape<TBv>;
     constexpr int W = Ld16<TBv>::W;
     constexpr int QW = S::Q / 4 > 0 ? S::Q / 4 : 1;            // 16-byte pieces per lane in the update
@@ -305,6 +338,35 @@ __global__ void k_finish(const double* h1, int it, const double* npart, int G, d
     s = block_sum(s, red);
     if (threadIdx.x == 0) {
         alpha[it] = h1[it];
+        beta[it] = sqrt(s);
+    }
+}
+
+// Daniel-Gragg-Kaufman-Stewart test after one CGS pass of the local-step vector:
+// loc[1] = |w'|^2, loc[2] = |w''|^2; a second pass is needed (gate = 1) when the
+// pass removed more than half of |w'|^2 (|w''| < |w'| / sqrt(2)).
+__global__ void k_dgks(const double* npart_after, const double* npart_before, int G, double* loc, int* gate) {
+    __shared__ double red[TB / 64];
+    double a = 0.0, b = 0.0;
+    for (int g = threadIdx.x; g < G; g += TB) { a += npart_after[g]; b += npart_before[g]; }
+    a = block_sum(a, red);
+    __syncthreads();
+    b = block_sum(b, red);
+    if (threadIdx.x == 0) {
+        loc[1] = b;
+        loc[2] = a;
+        gate[0] = (a < 0.5 * b) ? 1 : 0;
+    }
+}
+
+// alpha[it] = v_it . H v_it (the local-step coefficient), beta[it] = |w| at the end
+__global__ void k_finish_local(const double* loc, int it, const double* npart, int G, double* alpha, double* beta) {
+    __shared__ double red[TB / 64];
+    double s = 0.0;
+    for (int g = threadIdx.x; g < G; g += TB) s += npart[g];
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) {
+        alpha[it] = loc[0];
         beta[it] = sqrt(s);
     }
 }
@@ -494,7 +556,8 @@ static void restart_small_on_stream(double* alpha, double* beta, int m, int ls, 
 namespace {
 
 struct Workspace {
-    DBuf basis, wH, w64, part, npart, h1, h2, alpha, beta, enring, flags, S, swork, U, scal, tvals, Y, Q, R, hv64;
+    DBuf basis, wH, w64, part, npart, npart2, h1, h2, alpha, beta, enring, flags, S, swork, U, scal, tvals, Y, Q,
+        R, hv64, loc, gate;
     int* hflag = nullptr;          // pinned mirror of the convergence-flag ring
     hipEvent_t ev[4] = {};
     void host_sync_objects() {
@@ -504,8 +567,8 @@ struct Workspace {
     }
     int64_t capacity() const {
         int64_t c = 0;
-        for (const DBuf* d : {&basis, &wH, &w64, &part, &npart, &h1, &h2, &alpha, &beta, &enring, &flags, &S, &swork,
-                              &U, &scal, &tvals, &Y, &Q, &R, &hv64})
+        for (const DBuf* d : {&basis, &wH, &w64, &part, &npart, &npart2, &h1, &h2, &alpha, &beta, &enring, &flags, &S,
+                              &swork, &U, &scal, &tvals, &Y, &Q, &R, &hv64, &loc, &gate})
             c += (int64_t)d->cap;
         return c;
     }
@@ -530,6 +593,9 @@ static void workspace_plan(int64_t n, const SolveParams& p, F&& f) {
     f(&Workspace::w64, ld * 8);
     f(&Workspace::part, std::max(Gd, Gx) * ldp * 8);
     f(&Workspace::npart, std::max(Gu, Gx) * 8);
+    f(&Workspace::npart2, Gx * 8);
+    f(&Workspace::loc, 16 * 8);
+    f(&Workspace::gate, 4 * (int64_t)sizeof(int));
     f(&Workspace::h1, (lm + 2) * 8);
     f(&Workspace::h2, (lm + 2) * 8);
     f(&Workspace::alpha, (lm + 1) * 8);
@@ -561,6 +627,7 @@ struct Solver {
     int gstore;
     Workspace& W;
     const std::atomic<bool>* cancel = nullptr;
+    bool reorth_cgs2 = false;   // SMGPU_REORTH=cgs2: unconditional CGS2 instead of the local step
 
     static Workspace& cached(std::shared_ptr<void>* slots) {
         const int k = sizeof(TBv) == 4 ? 1 : 0;
@@ -573,6 +640,7 @@ struct Solver {
         : H(h), P(p), st(stats), W(cached(ctx ? ctx->ws : h.trl_ws)) {
         s = ctx ? ctx->stream : H.stream;
         cancel = ctx ? ctx->cancel : nullptr;
+        if (const char* e = std::getenv("SMGPU_REORTH")) reorth_cgs2 = std::string(e) == "cgs2";
         n = H.mdim;
         ld = (n + 63) / 64 * 64;
         lm = P.lm;
@@ -599,26 +667,80 @@ struct Solver {
     void cgs2(int nv, const TW* win) {
         const TBv* B = V(0);
         double* part = W.part.as<double>();
-        hipLaunchKernelGGL((k_gs_dots<TBv, TW>), dim3(Gx), dim3(TB), 0, s, B, ld, nv, win, n, part, ldp);
+        hipLaunchKernelGGL((k_gs_dots<TBv, TW, false>), dim3(Gx), dim3(TB), 0, s, B, ld, nv, win, n, part, ldp,
+                           (const int*)nullptr, (const TBv*)nullptr, (const TBv*)nullptr, (const double*)nullptr,
+                           (const double*)nullptr, (double*)nullptr, (double*)nullptr);
         hipLaunchKernelGGL(k_reduce, dim3(nv), dim3(TB), 0, s, part, (int)Gx, ldp, W.h1.as<double>(),
                            (const int*)nullptr);
         hipLaunchKernelGGL((k_gs_upd<TBv, TW, true>), dim3(Gx), dim3(TB), 0, s, B, ld, nv, W.h1.as<double>(), win,
-                           W.w64.as<double>(), n, part, ldp, W.npart.as<double>());
+                           W.w64.as<double>(), n, part, ldp, W.npart.as<double>(), (const int*)nullptr);
         hipLaunchKernelGGL(k_reduce, dim3(nv), dim3(TB), 0, s, part, (int)Gx, ldp, W.h2.as<double>(),
                            (const int*)nullptr);
         hipLaunchKernelGGL((k_gs_upd<TBv, double, false>), dim3(Gx), dim3(TB), 0, s, B, ld, nv, W.h2.as<double>(),
-                           W.w64.as<double>(), W.w64.as<double>(), n, part, ldp, W.npart.as<double>());
+                           W.w64.as<double>(), W.w64.as<double>(), n, part, ldp, W.npart.as<double>(),
+                           (const int*)nullptr);
+    }
+
+    // Reorthogonalisation of w = H v_it against V[0..it] (full, as ReORTH,
+    // shellmodel_main.jl:1005-1012), in the form of a Lanczos step:
+    //   alpha = v_it . w;  w' = w - alpha v_it - beta_{it-1} v_{it-1}   (local step, FP64)
+    //   one CGS pass  w'' = w' - V (V^T w')
+    //   a second pass only if |w''| < |w'|/sqrt(2) (decided on the device).
+    // After the local step the projections are of rounding size, so one pass
+    // leaves w'' orthogonal to the stored basis to second order; the test
+    // catches the iterations where it is not (the first after a restart).
+    // Two streaming reads of V in the common case (CGS2 needs three).
+    // W.h1[0..it] = projections of the pass, alpha/beta written by k_finish_local.
+    template <typename TW>
+    void reorth_local(int it, const TW* win) {
+        const int nv = it + 1;
+        const TBv* B = V(0);
+        double* part = W.part.as<double>();
+        double* loc = W.loc.as<double>();
+        int* gate = W.gate.as<int>();
+        const TBv* vp = it > 0 ? (const TBv*)V(it - 1) : nullptr;
+        const double* bp = it > 0 ? W.beta.as<double>() + (it - 1) : nullptr;
+        hipLaunchKernelGGL((k_gs_dots<TBv, TW, false>), dim3(Gx), dim3(TB), 0, s, (const TBv*)V(it), ld, 1, win, n,
+                           part, ldp, (const int*)nullptr, (const TBv*)nullptr, (const TBv*)nullptr,
+                           (const double*)nullptr, (const double*)nullptr, (double*)nullptr, (double*)nullptr);
+        hipLaunchKernelGGL(k_reduce, dim3(1), dim3(TB), 0, s, part, (int)Gx, ldp, loc, (const int*)nullptr);
+        hipLaunchKernelGGL((k_gs_dots<TBv, TW, true>), dim3(Gx), dim3(TB), 0, s, B, ld, nv, win, n, part, ldp,
+                           (const int*)nullptr, (const TBv*)V(it), vp, (const double*)loc, bp, W.w64.as<double>(),
+                           W.npart2.as<double>());
+        hipLaunchKernelGGL(k_reduce, dim3(nv), dim3(TB), 0, s, part, (int)Gx, ldp, W.h1.as<double>(),
+                           (const int*)nullptr);
+        hipLaunchKernelGGL((k_gs_upd<TBv, double, false>), dim3(Gx), dim3(TB), 0, s, B, ld, nv, W.h1.as<double>(),
+                           W.w64.as<double>(), W.w64.as<double>(), n, part, ldp, W.npart.as<double>(),
+                           (const int*)nullptr);
+        hipLaunchKernelGGL(k_dgks, dim3(1), dim3(TB), 0, s, (const double*)W.npart.as<double>(),
+                           (const double*)W.npart2.as<double>(), (int)Gx, loc, gate);
+        hipLaunchKernelGGL((k_gs_dots<TBv, double, false>), dim3(Gx), dim3(TB), 0, s, B, ld, nv,
+                           (const double*)W.w64.as<double>(), n, part, ldp, (const int*)gate, (const TBv*)nullptr,
+                           (const TBv*)nullptr, (const double*)nullptr, (const double*)nullptr, (double*)nullptr,
+                           (double*)nullptr);
+        hipLaunchKernelGGL(k_reduce, dim3(nv), dim3(TB), 0, s, part, (int)Gx, ldp, W.h2.as<double>(),
+                           (const int*)gate);
+        hipLaunchKernelGGL((k_gs_upd<TBv, double, false>), dim3(Gx), dim3(TB), 0, s, B, ld, nv, W.h2.as<double>(),
+                           W.w64.as<double>(), W.w64.as<double>(), n, part, ldp, W.npart.as<double>(),
+                           (const int*)gate);
+        hipLaunchKernelGGL(k_finish_local, dim3(1), dim3(TB), 0, s, (const double*)loc, it,
+                           (const double*)W.npart.as<double>(), (int)Gx, W.alpha.as<double>(), W.beta.as<double>());
     }
 
     void enqueue_iteration(int it, int step, int nprev) {
         if (prec_hv == 0) launch_hv(H, 0, V(it), W.wH.p, s);
         else launch_hv(H, 1, V(it), W.wH.p, s);
         st.n_hv++;
-        const int nv = it + 1;
-        if (prec_hv == 0) cgs2<double>(nv, W.wH.as<double>());
-        else cgs2<float>(nv, W.wH.as<float>());
-        hipLaunchKernelGGL(k_finish, dim3(1), dim3(TB), 0, s, W.h1.as<double>(), it, W.npart.as<double>(), (int)Gx,
-                           W.alpha.as<double>(), W.beta.as<double>());
+        if (reorth_cgs2) {
+            const int nv = it + 1;
+            if (prec_hv == 0) cgs2<double>(nv, W.wH.as<double>());
+            else cgs2<float>(nv, W.wH.as<float>());
+            hipLaunchKernelGGL(k_finish, dim3(1), dim3(TB), 0, s, W.h1.as<double>(), it, W.npart.as<double>(),
+                               (int)Gx, W.alpha.as<double>(), W.beta.as<double>());
+        } else {
+            if (prec_hv == 0) reorth_local<double>(it, W.wH.as<double>());
+            else reorth_local<float>(it, W.wH.as<float>());
+        }
         const int nn = std::min(nev, it + 1);
         hipLaunchKernelGGL(k_ritz, dim3(nn), dim3(TB), 0, s, W.alpha.as<double>(), W.beta.as<double>(), it + 1,
                            en(step - 1), nprev, en(step));
