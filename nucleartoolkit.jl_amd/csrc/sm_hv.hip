@@ -150,23 +150,29 @@ __global__ __launch_bounds__(256) void k_hv(HvTables A, const T* __restrict__ v,
         const T* V2 = veff + A.veff_off[dm];
         const int c2 = A.veff_cols2[dm];
         const int64_t sb = A.ntab_slice_base[nt];
-        const uint32_t zent = (uint32_t)(c2 - 2) << 20;      // zero column of V_eff
+        // column k of a lane is fn = tid + 256 k: slice (tid >> 6) + 4 k, the same
+        // for the whole wave, so slice lengths (and every test on them) are
+        // wave-uniform; lanes past nN inside the last slice read its zero-column
+        // padding, waves past the last slice get length 0 and skip all work
+        const int nsl = (nN + 63) >> 6;
+        const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
         int64_t off[NPT];
         int len[NPT];
         int maxlen = 0;
 #pragma unroll
         for (int k = 0; k < NPT; k++) {
-            const int fn = tid + k * 256;
-            if (fn < nN) {
-                const int64_t s = sb + (fn >> 6);
-                off[k] = A.ntab_slice_off[s] + (fn & 63);
-                len[k] = A.ntab_slice_len[s];
+            const int sk = wave + 4 * k;
+            if (sk < nsl) {
+                const int64_t s = sb + sk;
+                off[k] = A.ntab_slice_off[s] + (tid & 63);
+                len[k] = __builtin_amdgcn_readfirstlane(A.ntab_slice_len[s]);
             } else {
-                off[k] = A.ntab_slice_off[sb] + (fn & 63);   // a valid row: every slice has one
+                off[k] = A.ntab_slice_off[sb] + (tid & 63);   // a valid row: every slice has one
                 len[k] = 0;
             }
             maxlen = max(maxlen, len[k]);
         }
+        maxlen = __builtin_amdgcn_readfirstlane(maxlen);
         for (int ch = 0; ch < nch; ch++) {
             // slot metadata (uniform): offsets into v and V_eff, not pointers, so
             // that every gather below is an unconditional global load from a
@@ -227,38 +233,46 @@ __global__ __launch_bounds__(256) void k_hv(HvTables A, const T* __restrict__ v,
                 }
             }
             __syncthreads();
-            // neutron-hop entries, prefetched PF steps ahead with unconditional loads:
-            // a lane past its slice length re-reads its last entry (every slice has
-            // at least one) and uses the zero column instead.  The step loop is
-            // unrolled by PF so each ring slot keeps its register (no moves that
-            // would wait for loads in flight); the length test is applied when an
-            // entry is consumed.  Steps past maxlen (< PF per round) add zeros.
-            constexpr int PF = 4;
-            uint32_t pf[PF][NPT];
+            // neutron-hop entries in chunks of PF steps, double-buffered: a chunk is
+            // decoded into registers, the next chunk's loads are issued into the
+            // raw buffer, and the decoded chunk is computed while they are in
+            // flight.  Loads are unconditional (a column past its slice length
+            // re-reads its last entry; every slice has at least one); the LDS reads
+            // and FMAs of a column stop at its slice length (wave-uniform).
+            constexpr int PF = NPT <= 4 ? 4 : (NPT <= 8 ? 2 : 1);
+            uint32_t raw[PF][NPT];
 #pragma unroll
             for (int d = 0; d < PF; d++)
 #pragma unroll
                 for (int k = 0; k < NPT; k++)
-                    pf[d][k] = A.ntab_ent[off[k] + (int64_t)min(d, max(len[k] - 1, 0)) * 64];
+                    raw[d][k] = A.ntab_ent[off[k] + (int64_t)min(d, max(len[k] - 1, 0)) * 64];
             for (int q0 = 0; q0 < maxlen; q0 += PF) {
+                int in[PF][NPT], col[PF][NPT];
+#pragma unroll
+                for (int d = 0; d < PF; d++)
+#pragma unroll
+                    for (int k = 0; k < NPT; k++) {
+                        in[d][k] = (int)(raw[d][k] & 0xFFFFFu);
+                        col[d][k] = (int)(raw[d][k] >> 20);
+                    }
+                if (q0 + PF < maxlen) {
+#pragma unroll
+                    for (int d = 0; d < PF; d++)
+#pragma unroll
+                        for (int k = 0; k < NPT; k++)
+                            raw[d][k] = A.ntab_ent[off[k] + (int64_t)min(q0 + PF + d, max(len[k] - 1, 0)) * 64];
+                }
 #pragma unroll
                 for (int d = 0; d < PF; d++) {
                     const int qq = q0 + d;
-                    int in[NPT], col[NPT];
-#pragma unroll
-                    for (int k = 0; k < NPT; k++) {
-                        const uint32_t e = qq < len[k] ? pf[d][k] : zent;
-                        in[k] = (int)(e & 0xFFFFFu);
-                        col[k] = (int)(e >> 20);
-                        pf[d][k] = A.ntab_ent[off[k] + (int64_t)min(qq + PF, max(len[k] - 1, 0)) * 64];
-                    }
 #pragma unroll
                     for (int r = 0; r < RG; r++) {
                         if (act_row[r]) {
 #pragma unroll
                             for (int k = 0; k < NPT; k++) {
-                                const T4 xv = *reinterpret_cast<const T4*>(X + (size_t)in[k] * SL + r * PCH);
-                                const T4 vv = *reinterpret_cast<const T4*>(Vs + (size_t)col[k] * SL + r * PCH);
+                                if (qq >= len[k]) continue;          // wave-uniform
+                                const T4 xv = *reinterpret_cast<const T4*>(X + (size_t)in[d][k] * SL + r * PCH);
+                                const T4 vv = *reinterpret_cast<const T4*>(Vs + (size_t)col[d][k] * SL + r * PCH);
                                 T a = acc[k][r];
                                 a += vv.x * xv.x;
                                 a += vv.y * xv.y;
