@@ -65,6 +65,8 @@ typedef struct {
     int32_t n_restart;
     double wall_ms;
     double ritz[64];     /* en[1]: Ritz values of the last convergence check */
+    int32_t n_reorth2;   /* iterations that needed a second Gram-Schmidt pass */
+    int32_t pad_;
 } smgpu_solve_stats;
 
 /* library */

@@ -98,6 +98,18 @@ __global__ __launch_bounds__(TB) void k_dotsw(const TBv* __restrict__ B, int64_t
     }
 }
 
+// part[blk * ldp] = sum over the block's grid-stride range of a[x] b[x]
+template <typename TA, typename TB2>
+__global__ __launch_bounds__(TB) void k_dot2(const TA* __restrict__ a, const TB2* __restrict__ b, int64_t n,
+                                             double* __restrict__ part, int ldp) {
+    __shared__ double red[TB / 64];
+    double s = 0.0;
+    for (int64_t x = (int64_t)blockIdx.x * TB + threadIdx.x; x < n; x += (int64_t)gridDim.x * TB)
+        s += (double)a[x] * (double)b[x];
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) part[(int64_t)blockIdx.x * ldp] = s;
+}
+
 // out[j] = sum_g part[g*ldp + j], j < nj
 __global__ __launch_bounds__(TB) void k_reduce(const double* __restrict__ part, int G, int ldp,
                                                double* __restrict__ out, const int* __restrict__ gate) {
@@ -356,11 +368,12 @@ __global__ void k_dgks(const double* npart_after, const double* npart_before, in
         loc[1] = b;
         loc[2] = a;
         gate[0] = (a < 0.5 * b) ? 1 : 0;
+        loc[4] = gate[0];
     }
 }
 
 // alpha[it] = v_it . H v_it (the local-step coefficient), beta[it] = |w| at the end
-__global__ void k_finish_local(const double* loc, int it, const double* npart, int G, double* alpha, double* beta) {
+__global__ void k_finish_local(double* loc, int it, const double* npart, int G, double* alpha, double* beta) {
     __shared__ double red[TB / 64];
     double s = 0.0;
     for (int g = threadIdx.x; g < G; g += TB) s += npart[g];
@@ -368,6 +381,7 @@ __global__ void k_finish_local(const double* loc, int it, const double* npart, i
     if (threadIdx.x == 0) {
         alpha[it] = loc[0];
         beta[it] = sqrt(s);
+        loc[3] += loc[4];          // second CGS passes so far (loc[4] = this step's gate)
     }
 }
 
@@ -700,10 +714,9 @@ struct Solver {
         int* gate = W.gate.as<int>();
         const TBv* vp = it > 0 ? (const TBv*)V(it - 1) : nullptr;
         const double* bp = it > 0 ? W.beta.as<double>() + (it - 1) : nullptr;
-        hipLaunchKernelGGL((k_gs_dots<TBv, TW, false>), dim3(Gx), dim3(TB), 0, s, (const TBv*)V(it), ld, 1, win, n,
-                           part, ldp, (const int*)nullptr, (const TBv*)nullptr, (const TBv*)nullptr,
-                           (const double*)nullptr, (const double*)nullptr, (double*)nullptr, (double*)nullptr);
-        hipLaunchKernelGGL(k_reduce, dim3(1), dim3(TB), 0, s, part, (int)Gx, ldp, loc, (const int*)nullptr);
+        const int G2 = (int)std::min<int64_t>(1024, Gd);
+        hipLaunchKernelGGL((k_dot2<TBv, TW>), dim3(G2), dim3(TB), 0, s, (const TBv*)V(it), win, n, part, ldp);
+        hipLaunchKernelGGL(k_reduce, dim3(1), dim3(TB), 0, s, part, G2, ldp, loc, (const int*)nullptr);
         hipLaunchKernelGGL((k_gs_dots<TBv, TW, true>), dim3(Gx), dim3(TB), 0, s, B, ld, nv, win, n, part, ldp,
                            (const int*)nullptr, (const TBv*)V(it), vp, (const double*)loc, bp, W.w64.as<double>(),
                            W.npart2.as<double>());
@@ -723,7 +736,7 @@ struct Solver {
         hipLaunchKernelGGL((k_gs_upd<TBv, double, false>), dim3(Gx), dim3(TB), 0, s, B, ld, nv, W.h2.as<double>(),
                            W.w64.as<double>(), W.w64.as<double>(), n, part, ldp, W.npart.as<double>(),
                            (const int*)gate);
-        hipLaunchKernelGGL(k_finish_local, dim3(1), dim3(TB), 0, s, (const double*)loc, it,
+        hipLaunchKernelGGL(k_finish_local, dim3(1), dim3(TB), 0, s, loc, it,
                            (const double*)W.npart.as<double>(), (int)Gx, W.alpha.as<double>(), W.beta.as<double>());
     }
 
@@ -756,6 +769,7 @@ struct Solver {
         W.host_sync_objects();
         std::vector<double> e0(4 * 64, 1.0e4);
         SM_HIP(hipMemcpyAsync(W.enring.p, e0.data(), e0.size() * 8, hipMemcpyHostToDevice, s));
+        SM_HIP(hipMemsetAsync(W.loc.p, 0, W.loc.bytes, s));
         // start vector: caller's, or seeded normal deviates generated on the device
         // (initialize_wf, :939-948); normalised in FP64
         if (v0_in) {
@@ -841,6 +855,9 @@ struct Solver {
         st.iterations = elit;
         std::vector<double> en_last(nev);
         SM_HIP(hipMemcpy(en_last.data(), en(conv_step), nev * 8, hipMemcpyDeviceToHost));
+        double nre2 = 0.0;
+        SM_HIP(hipMemcpy(&nre2, W.loc.as<double>() + 3, 8, hipMemcpyDeviceToHost));
+        st.n_reorth2 = (int32_t)nre2;
         for (int i = 0; i < nev; i++) { evals[i] = en_last[i]; st.ritz[i] = en_last[i]; }
         finish(elit, conv_step, evals, evecs);
         st.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

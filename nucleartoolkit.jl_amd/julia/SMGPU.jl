@@ -32,6 +32,8 @@ struct SolveStats               # smgpu_solve_stats
     n_restart::Int32
     wall_ms::Float64
     ritz::NTuple{64,Float64}
+    n_reorth2::Int32
+    pad_::Int32
 end
 
 struct ProblemInfo              # smgpu_problem_info

@@ -46,7 +46,8 @@ class SolveParams(ctypes.Structure):
 
 class SolveStats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int32), ("n_hv", ctypes.c_int32), ("n_hv64", ctypes.c_int32),
-                ("n_restart", ctypes.c_int32), ("wall_ms", ctypes.c_double), ("ritz", ctypes.c_double * 64)]
+                ("n_restart", ctypes.c_int32), ("wall_ms", ctypes.c_double), ("ritz", ctypes.c_double * 64),
+                ("n_reorth2", ctypes.c_int32), ("pad_", ctypes.c_int32)]
 
 
 class ServerConfig(ctypes.Structure):

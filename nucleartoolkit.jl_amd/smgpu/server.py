@@ -36,7 +36,8 @@ def _info_dict(info: RequestInfo):
                 index_bytes=info.index_bytes,
                 submit_ms=info.submit_ms, prep_start_ms=info.prep_start_ms, prep_end_ms=info.prep_end_ms,
                 admit_ms=info.admit_ms, done_ms=info.done_ms, latency_ms=info.latency_ms,
-                iterations=s.iterations, n_hv=s.n_hv, n_hv64=s.n_hv64, n_restart=s.n_restart, solve_ms=s.wall_ms)
+                iterations=s.iterations, n_hv=s.n_hv, n_hv64=s.n_hv64, n_restart=s.n_restart, solve_ms=s.wall_ms,
+                n_reorth2=s.n_reorth2)
 
 
 class Server:

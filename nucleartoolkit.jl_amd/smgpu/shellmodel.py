@@ -165,7 +165,7 @@ class Hamiltonian:
         check(lib().smgpu_trl_solve_v0(self._h, ctypes.byref(p), v0p, ptr(ev),
                                        ptr(vec) if vec is not None else None, ctypes.byref(st)))
         stats = dict(iterations=st.iterations, n_hv=st.n_hv, n_hv64=st.n_hv64, n_restart=st.n_restart,
-                     wall_ms=st.wall_ms, ritz=np.array(st.ritz[:nev]))
+                     wall_ms=st.wall_ms, ritz=np.array(st.ritz[:nev]), n_reorth2=st.n_reorth2)
         return ev, vec, stats
 
     def __del__(self):
