@@ -14,7 +14,7 @@ tolerance count.  Beside it: the plain FP64 GPU path one solve at a time
 (tables resident, solve only), H·v timing and roofline, and the oracle's CPU
 restatement timed on a bounded sample.
 
-usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c1|c3] [--inflight M]
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config cr48|mg24|mn50] [--inflight M]
 """
 import argparse
 import json
@@ -33,9 +33,9 @@ HBM_PEAK_GBS = 8000.0
 
 CONFIGS = {
     # name: (space, zv, nv, seed offset, spe range, sigma, num_ev, lm, ls)
-    "c1": ("sd", 4, 4, 1, (-4.0, 0.0), 1.5, 5, 300, 20),
-    "c2": ("pf", 4, 4, 2, (-8.0, 0.0), 1.0, 10, 300, 20),
-    "c3": ("pf", 5, 5, 3, (-8.0, 0.0), 1.0, 5, 100, 20),
+    "mg24": ("sd", 4, 4, 1, (-4.0, 0.0), 1.5, 5, 300, 20),
+    "cr48": ("pf", 4, 4, 2, (-8.0, 0.0), 1.0, 10, 300, 20),
+    "mn50": ("pf", 5, 5, 3, (-8.0, 0.0), 1.0, 5, 100, 20),
 }
 
 
@@ -106,7 +106,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=16)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="cr48", choices=sorted(CONFIGS))
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--hv-reps", type=int, default=20)

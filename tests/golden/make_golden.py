@@ -1,7 +1,7 @@
 """Writes tests/golden/eigs_<cfg>.json: lowest eigenvalues of the synthetic
 configurations, computed with the oracle's operate_H! restatement (C) and
 ARPACK (scipy.sparse.linalg.eigsh, tol 1e-13) — an eigen-solver independent of
-both the oracle's TRL and the GPU path.  Run: python tests/golden/make_golden.py [c1 c2 ...]
+both the oracle's TRL and the GPU path.  Run: python tests/golden/make_golden.py [mg24 cr48 ...]
 """
 import json
 import os
@@ -20,8 +20,8 @@ import sm_oracle as orc  # noqa: E402
 from smgpu import synth  # noqa: E402
 
 CFG = {
-    "c1": ("sd", 4, 4, 1, (-4.0, 0.0), 1.5, 5),
-    "c2": ("pf", 4, 4, 2, (-8.0, 0.0), 1.0, 10),
+    "mg24": ("sd", 4, 4, 1, (-4.0, 0.0), 1.5, 5),
+    "cr48": ("pf", 4, 4, 2, (-8.0, 0.0), 1.0, 10),
 }
 
 

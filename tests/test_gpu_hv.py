@@ -19,7 +19,7 @@ CKPOT_BE8 = [-31.119, -27.300, -19.162, -18.249, -16.722, -14.925, -14.517, -14.
 
 
 def _cases(golden, snt_dir):
-    c1, _ = synth.config_snt(f"{snt_dir}/c1.snt", "sd", 4, 4, synth.BASE_SEED + 1, (-4, 0), 1.5)
+    c1, _ = synth.config_snt(f"{snt_dir}/mg24.snt", "sd", 4, 4, synth.BASE_SEED + 1, (-4, 0), 1.5)
     pfs, _ = synth.config_snt(f"{snt_dir}/pf23.snt", "pf", 2, 3, 99, (-8, 0), 1.0)
     return [(f"{golden}/ckpot.snt", "Be8"), (f"{golden}/ckpot.snt", "Li7"), (f"{golden}/usdb.snt", "O20"),
             (f"{golden}/usdb.snt", "Na23"), (c1, "Mg24"), (pfs, "V48")]
@@ -43,7 +43,7 @@ def test_hv_parity_fp64_fp32(golden, snt_dir):
 
 
 def test_hv_parity_cr48(snt_dir):
-    path, nuc = synth.config_snt(f"{snt_dir}/c2.snt", "pf", 4, 4, synth.BASE_SEED + 2, (-8, 0), 1.0)
+    path, nuc = synth.config_snt(f"{snt_dir}/cr48.snt", "pf", 4, 4, synth.BASE_SEED + 2, (-8, 0), 1.0)
     p = smgpu.Problem(path, nuc)
     h = smgpu.Hamiltonian(p)
     o = orc.OracleHamiltonian(path, nuc)
@@ -105,7 +105,7 @@ def test_trl_restarts_match_dense(golden):
 
 
 def test_trl_mg24_vs_arpack(snt_dir):
-    path, nuc = synth.config_snt(f"{snt_dir}/c1.snt", "sd", 4, 4, synth.BASE_SEED + 1, (-4, 0), 1.5)
+    path, nuc = synth.config_snt(f"{snt_dir}/mg24.snt", "sd", 4, 4, synth.BASE_SEED + 1, (-4, 0), 1.5)
     o = orc.OracleHamiltonian(path, nuc)
     op = LinearOperator((o.mdim, o.mdim), matvec=lambda x: o.operate_H(np.ascontiguousarray(x).ravel()),
                         dtype=np.float64)
@@ -125,8 +125,8 @@ def _golden_cfg(golden, name):
     return g
 
 
-@pytest.mark.parametrize("name,space,zv,nv,spe,sig", [("c1", "sd", 4, 4, (-4, 0), 1.5),
-                                                      ("c2", "pf", 4, 4, (-8, 0), 1.0)])
+@pytest.mark.parametrize("name,space,zv,nv,spe,sig", [("mg24", "sd", 4, 4, (-4, 0), 1.5),
+                                                      ("cr48", "pf", 4, 4, (-8, 0), 1.0)])
 def test_trl_vs_golden_arpack(golden, snt_dir, name, space, zv, nv, spe, sig):
     """Eigenvalues of the C1/C2 workloads vs the committed oracle+ARPACK values
     (tests/golden/make_golden.py): FP64 path <= 1e-7 MeV, mixed path <= 1e-5 MeV."""
@@ -151,7 +151,7 @@ def test_trl_vs_golden_arpack(golden, snt_dir, name, space, zv, nv, spe, sig):
 def test_lanczos_recurrence_matches_numpy(snt_dir):
     """alpha/beta of the device recurrence vs numpy CGS2 Lanczos with the same
     (device) H·v, at the Cr48 size: FP64 <= 1e-10, mixed <= 1e-5 (relative)."""
-    path, nuc = synth.config_snt(f"{snt_dir}/c2.snt", "pf", 4, 4, synth.BASE_SEED + 2, (-8, 0), 1.0)
+    path, nuc = synth.config_snt(f"{snt_dir}/cr48.snt", "pf", 4, 4, synth.BASE_SEED + 2, (-8, 0), 1.0)
     p = smgpu.Problem(path, nuc)
     h = smgpu.Hamiltonian(p)
     n = p.mdim

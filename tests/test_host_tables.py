@@ -30,7 +30,7 @@ def test_header_symbols_exported():
 
 
 def _cases(golden, snt_dir):
-    c1, _ = synth.config_snt(f"{snt_dir}/c1.snt", "sd", 4, 4, synth.BASE_SEED + 1, (-4, 0), 1.5)
+    c1, _ = synth.config_snt(f"{snt_dir}/mg24.snt", "sd", 4, 4, synth.BASE_SEED + 1, (-4, 0), 1.5)
     pfs, _ = synth.config_snt(f"{snt_dir}/pf22.snt", "pf", 2, 3, 99, (-8, 0), 1.0)
     return [(f"{golden}/ckpot.snt", "Be8"), (f"{golden}/ckpot.snt", "Li7"), (f"{golden}/usdb.snt", "O20"),
             (f"{golden}/usdb.snt", "Na23"), (f"{golden}/random_snts/tmp_0.snt", "Ne20"), (c1, "Mg24"),
