@@ -108,6 +108,11 @@ int32_t smgpu_pair_export(smgpu_problem p, int32_t species, int64_t* n, int32_t*
 /* Upload tables to a device (the "uploaded once" step). */
 int32_t smgpu_ham_create(smgpu_problem p, int32_t device, smgpu_ham* out);
 int32_t smgpu_ham_destroy(smgpu_ham h);
+/* Device tables of J^2 on the problem's basis (the operator of prep_J /
+ * operate_J!, shellmodel_main.jl:149-157, 226-230, built as the interaction
+ * sum_a j_a(j_a+1) n_a + sum_{ab,J} [J(J+1) - ja(ja+1) - jb(jb+1)] A+_J(ab) A_J(ab)):
+ * smgpu_apply_H on this handle applies J^2, so <v|J^2|v> = J(J+1) of main_sm's Js. */
+int32_t smgpu_ham_create_j2(smgpu_problem p, int32_t device, smgpu_ham* out);
 /* Re-upload the coefficients after smgpu_problem_set_interaction (index tables stay). */
 int32_t smgpu_ham_update_interaction(smgpu_ham h, smgpu_problem p);
 

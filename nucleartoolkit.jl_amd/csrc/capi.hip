@@ -201,6 +201,20 @@ int32_t smgpu_ham_create(smgpu_problem p, int32_t device, smgpu_ham* out) {
     });
 }
 
+int32_t smgpu_ham_create_j2(smgpu_problem p, int32_t device, smgpu_ham* out) {
+    return guard([&]() -> int32_t {
+        if (!p || !out) throw std::invalid_argument("null argument");
+        *out = nullptr;
+        if (p->basis.mdim == 0) throw std::invalid_argument("empty M-scheme space (mdim = 0)");
+        Snt j2 = j2_operator(p->snt);
+        Tables t = build_tables(j2, p->basis, p->nthreads);
+        auto h = std::make_unique<smgpu_ham_s>();
+        h->dev = upload_ham(p->basis, t, device);
+        *out = h.release();
+        return SMGPU_OK;
+    });
+}
+
 int32_t smgpu_ham_destroy(smgpu_ham h) {
     return guard([&]() -> int32_t {
         if (h && h->dev) SM_HIP(hipSetDevice(h->dev->device));

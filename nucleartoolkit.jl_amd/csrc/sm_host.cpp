@@ -172,6 +172,42 @@ Snt read_snt(const std::string& path, const std::string& nucleus, bool ignore_ma
     return s;
 }
 
+Snt j2_operator(const Snt& model) {
+    Snt s;
+    s.lp = model.lp; s.ln = model.ln; s.cp = model.cp; s.cn = model.cn;
+    s.A = model.A; s.massop = 0; s.Aref = model.A; s.pw = 0.0;
+    s.p_sps = model.p_sps; s.n_sps = model.n_sps;
+    auto jj = [](int j2) { return 0.5 * j2 * (0.5 * j2 + 1.0); };
+    s.spe_p.resize(s.lp); s.spe_n.resize(s.ln);
+    for (int a = 0; a < s.lp; a++) s.spe_p[a] = jj(s.p_sps[a].j2);
+    for (int a = 0; a < s.ln; a++) s.spe_n[a] = jj(s.n_sps[a].j2);
+    int line = 0;
+    // like particles: a <= b, J in |ja-jb|..ja+jb, J even when a == b (antisymmetry)
+    for (int ch = 0; ch < 2; ch++) {
+        const auto& sps = ch == 0 ? s.p_sps : s.n_sps;
+        const int off = ch == 0 ? 0 : s.lp;
+        for (int a = 0; a < (int)sps.size(); a++)
+            for (int b = a; b < (int)sps.size(); b++) {
+                const int ja = sps[a].j2, jb = sps[b].j2;
+                for (int J = std::abs(ja - jb) / 2; J <= (ja + jb) / 2; J++) {
+                    if (a == b && J % 2 == 1) continue;
+                    s.labels[ch].push_back({a + 1 + off, b + 1 + off, a + 1 + off, b + 1 + off, J, ++line});
+                    s.tbmes[ch].push_back(J * (J + 1.0) - jj(ja) - jj(jb));
+                }
+            }
+    }
+    // proton-neutron: every (a_p, b_n, J)
+    for (int a = 0; a < s.lp; a++)
+        for (int b = 0; b < s.ln; b++) {
+            const int ja = s.p_sps[a].j2, jb = s.n_sps[b].j2;
+            for (int J = std::abs(ja - jb) / 2; J <= (ja + jb) / 2; J++) {
+                s.labels[2].push_back({a + 1, b + 1 + s.lp, a + 1, b + 1 + s.lp, J, ++line});
+                s.tbmes[2].push_back(J * (J + 1.0) - jj(ja) - jj(jb));
+            }
+        }
+    return s;
+}
+
 // ------------------------------------------------------------------ CG
 static __float128 qfact(int n) {
     __float128 r = 1;

@@ -110,6 +110,14 @@ struct Tables {
     int64_t n_pp_ops = 0, n_nn_ops = 0, n_pn_ops = 0;
 };
 
+// The total angular momentum squared J^2 written as an interaction on the orbits
+// of `model`: one-body j(j+1) per orbit and, for every two-particle state |ab;J>
+// (pp, nn, pn), the diagonal matrix element <ab;J| 2 j1.j2 |ab;J> =
+// J(J+1) - ja(ja+1) - jb(jb+1), since J^2 = sum_i j_i^2 + 2 sum_{i<j} j_i.j_j.
+// Its tables on a basis give <v|J^2|v> with the H.v machinery (the quantity
+// main_sm gets from prep_J / operate_J!, shellmodel_main.jl:149-157, 226-230).
+Snt j2_operator(const Snt& model);
+
 // Interaction-dependent parts only (V_eff, pp, nn, e_p, e_n) can be rebuilt
 // for a new interaction on the same basis with `build_coefficients`.
 Tables build_tables(const Snt& snt, const Basis& b, int nthreads);

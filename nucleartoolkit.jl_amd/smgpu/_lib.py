@@ -90,6 +90,7 @@ SIGNATURES = {
     "smgpu_pair_export": (I32, [P, I32, P, P, P, P, P]),
     "smgpu_ham_create": (I32, [P, I32, P]),
     "smgpu_ham_destroy": (I32, [P]),
+    "smgpu_ham_create_j2": (I32, [P, I32, P]),
     "smgpu_ham_update_interaction": (I32, [P, P]),
     "smgpu_table_bytes": (I32, [P, I32, I32, P, P, ctypes.c_char_p, I32]),
     "smgpu_work_terms": (I32, [P, P]),

@@ -91,6 +91,11 @@ class Problem:
         check(L.smgpu_pair_export(self._h, species, ctypes.byref(n), ptr(b), ptr(f), ptr(i), ptr(v)))
         return b, f, i, v
 
+    def j2_hamiltonian(self, device=0):
+        """Device tables of J^2 on this basis (prep_J / operate_J!,
+        shellmodel_main.jl:149-157, 226-230): its operate_H applies J^2."""
+        return Hamiltonian(self, device, _j2=True)
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
@@ -101,9 +106,10 @@ class Problem:
 class Hamiltonian:
     """Device-resident H for a Problem on one GPU."""
 
-    def __init__(self, problem: Problem, device=0):
+    def __init__(self, problem: Problem, device=0, _j2=False):
         h = ctypes.c_void_p()
-        check(lib().smgpu_ham_create(problem._h, int(device), ctypes.byref(h)))
+        create = lib().smgpu_ham_create_j2 if _j2 else lib().smgpu_ham_create
+        check(create(problem._h, int(device), ctypes.byref(h)))
         self._h = h
         self.problem = problem
         self.mdim = problem.mdim
@@ -175,13 +181,35 @@ class Hamiltonian:
             self._h = None
 
 
+def j_from_jj1(jj, tol=1e-6):
+    """J_from_JJ1 (shellmodel_main.jl:1105-1113): the J in 0, 1/2, ..., 50 with
+    |J(J+1) - jj| < tol, else -1.0."""
+    for J2 in range(0, 101):
+        hJ = 0.5 * J2
+        if abs(hJ * (hJ + 1.0) - jj) < tol:
+            return hJ
+    return -1.0
+
+
+def expect_j2(problem: Problem, vecs, device=0):
+    """<v|J^2|v> of each row of `vecs` (main_sm's Js, shellmodel_main.jl:226-230)."""
+    hj = problem.j2_hamiltonian(device)
+    return np.array([float(v @ hj.operate_H(v, "fp64")) for v in np.atleast_2d(vecs)])
+
+
 def main_sm(sntf, target_nuc, num_ev, target_J=(), *, lm=300, ls=20, tol=1e-8, num_history=3,
-            precision="fp64", refine=True, device=0, seed=1234, return_stats=False):
+            precision="fp64", refine=True, device=0, seed=1234, return_stats=False, return_J=False,
+            save_wav=False, wav_path=None, in_wf=""):
     """Lowest `num_ev` eigenvalues (MeV) of `target_nuc` with the interaction `sntf`.
 
     Mirrors main_sm (shellmodel_main.jl:89-281) with target_J = [] (plain TRL):
     2M = A % 2 (:110); mdim == 0 returns an empty array (:127); num_ev is
-    clamped to mdim (:128).  J projection is not part of this path.
+    clamped to mdim (:128).  return_J adds the J of every state from <J^2> of
+    its Ritz vector (totJs, :226-232); save_wav writes the Ritz vectors in the
+    reference's .wav layout (writeRitzvecs, :1215-1229), by default to
+    ./<nucleus>_<interaction>.wav (:241-246).  in_wf starts the Lanczos from the
+    first vector of a .wav file (read_appwav, :194-198).  J projection is not
+    part of this path.
     """
     if len(target_J) > 0:
         raise NotImplementedError("J-projected Lanczos (target_J != []) is not built")
@@ -190,6 +218,24 @@ def main_sm(sntf, target_nuc, num_ev, target_J=(), *, lm=300, ls=20, tol=1e-8, n
         return (np.zeros(0), {}) if return_stats else np.zeros(0)
     ham = Hamiltonian(prob, device)
     nev = min(int(num_ev), prob.mdim)
-    ev, _, st = ham.trl(nev, lm=lm, ls=ls, tol=tol, num_history=num_history, precision=precision,
-                        refine=refine, seed=seed)
-    return (ev, st) if return_stats else ev
+    need_vecs = return_J or save_wav
+    v0 = None
+    if in_wf:
+        from .wav import read_appwav
+        v0 = read_appwav(in_wf, prob.mdim)
+    ev, vecs, st = ham.trl(nev, lm=lm, ls=ls, tol=tol, num_history=num_history, precision=precision,
+                           refine=refine, seed=seed, want_vectors=need_vecs, v0=v0)
+    out = [ev]
+    if need_vecs:
+        totJs = np.array([j_from_jj1(x) for x in expect_j2(prob, vecs, device)])
+        if save_wav:
+            from .wav import write_ritzvecs
+            if wav_path is None:
+                csnt = os.path.basename(sntf).split(".")[0]
+                wav_path = f"./{target_nuc}_{csnt}.wav"
+            write_ritzvecs(wav_path, prob.info.twoM, ev, totJs, vecs)
+        if return_J:
+            out.append(totJs)
+    if return_stats:
+        out.append(st)
+    return out[0] if len(out) == 1 else tuple(out)

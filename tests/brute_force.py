@@ -162,3 +162,89 @@ def brute_hamiltonian(path, nuc, Z, N, twoM=None):
         if sorted({a, b}) != sorted({c, d}):
             add_term(c, d, a, b, J, V)
     return H, [(s[4], s[5]) for s in states]
+
+
+def _sector(orbs, lp, vp, vn, twoM):
+    """Modes and reference-ordered states of one M sector (as brute_hamiltonian)."""
+    modes = []
+    for (idx, n, l, j, tz) in orbs:
+        for m in range(-j, j + 1, 2):
+            modes.append((idx, m))
+    Lp = sum(orbs[i][3] + 1 for i in range(lp))
+    Ln = len(modes) - Lp
+    states = []
+    for pc in itertools.combinations(range(Lp), vp):
+        for nc in itertools.combinations(range(Lp, Lp + Ln), vn):
+            M = sum(modes[k][1] for k in pc) + sum(modes[k][1] for k in nc)
+            if M != twoM:
+                continue
+            occ = 0
+            for k in pc + nc:
+                occ |= 1 << k
+            pint = sum(1 << (Lp - 1 - k) for k in pc)
+            nint = sum(1 << (Ln - 1 - (k - Lp)) for k in nc)
+            Mp = sum(modes[k][1] for k in pc)
+            states.append((Mp, -pint, -nint, occ))
+    states.sort()
+    return modes, [s[3] for s in states]
+
+
+def brute_j2(path, nuc, Z, N, twoM=None):
+    """Dense J^2 = J- J+ + Jz^2 + Jz in the reference basis order, from the ladder
+    operator J+ = sum_{orbit, m} sqrt(j(j+1) - m(m+1)) a+_{m+1} a_m (Jordan-Wigner
+    signs as in brute_hamiltonian); independent of any two-body bookkeeping."""
+    lp, ln, cp, cn, orbs, spe, lines, A = _parse(path, nuc)
+    vp, vn = Z - cp, N - cn
+    if twoM is None:
+        twoM = A % 2
+    modes, lo = _sector(orbs, lp, vp, vn, twoM)
+    _, hi = _sector(orbs, lp, vp, vn, twoM + 2)
+    mode_of = {mk: k for k, mk in enumerate(modes)}
+    jof = {o[0]: o[3] for o in orbs}
+    hidx = {occ: i for i, occ in enumerate(hi)}
+    Jp = np.zeros((len(hi), len(lo)))
+    for i, occ in enumerate(lo):
+        for k, (idx, m) in enumerate(modes):
+            if not (occ >> k) & 1 or m + 2 > jof[idx]:
+                continue
+            k2 = mode_of[(idx, m + 2)]
+            if (occ >> k2) & 1:
+                continue
+            # a+_{k2} a_k |occ>
+            s1 = -1 if bin(occ & ((1 << k) - 1)).count("1") % 2 else 1
+            o1 = occ & ~(1 << k)
+            s2 = -1 if bin(o1 & ((1 << k2) - 1)).count("1") % 2 else 1
+            f = o1 | (1 << k2)
+            j, mm = jof[idx] / 2.0, m / 2.0
+            Jp[hidx[f], i] += s1 * s2 * np.sqrt(j * (j + 1) - mm * (mm + 1))
+    Mz = twoM / 2.0
+    return Jp.T @ Jp + (Mz * Mz + Mz) * np.eye(len(lo))
+
+
+def write_j2_snt(model_path, out_path, nuc):
+    """The J^2 operator as an interaction file on the orbits of model_path:
+    SPE j(j+1); diagonal TBMEs J(J+1) - ja(ja+1) - jb(jb+1) for pp, nn, pn."""
+    lp, ln, cp, cn, orbs, spe, lines, A = _parse(model_path, nuc)
+    jof = {o[0]: o[3] for o in orbs}
+    jj = lambda j2: 0.5 * j2 * (0.5 * j2 + 1)
+    tb = []
+    for lo_, hi_ in ((1, lp), (lp + 1, lp + ln)):
+        for a in range(lo_, hi_ + 1):
+            for b in range(a, hi_ + 1):
+                for J in range(abs(jof[a] - jof[b]) // 2, (jof[a] + jof[b]) // 2 + 1):
+                    if a == b and J % 2:
+                        continue
+                    tb.append((a, b, a, b, J, J * (J + 1) - jj(jof[a]) - jj(jof[b])))
+    for a in range(1, lp + 1):
+        for b in range(lp + 1, lp + ln + 1):
+            for J in range(abs(jof[a] - jof[b]) // 2, (jof[a] + jof[b]) // 2 + 1):
+                tb.append((a, b, a, b, J, J * (J + 1) - jj(jof[a]) - jj(jof[b])))
+    out = [f"{lp} {ln} {cp} {cn}"]
+    out += [" ".join(map(str, o)) for o in orbs]
+    out.append(f"{lp + ln} 0")
+    out += [f"{o[0]} {o[0]} {jj(o[3]):.12e}" for o in orbs]
+    out.append(f"{len(tb)} 0 {A}")
+    out += [f"{a} {b} {c} {d} {J} {v:.12e}" for a, b, c, d, J, v in tb]
+    with open(out_path, "w") as f:
+        f.write("\n".join(out) + "\n")
+    return out_path
