@@ -16,6 +16,7 @@
 #include <cmath>
 #include <fstream>
 #include <functional>
+#include <mutex>
 #include <sstream>
 #include <stdexcept>
 #include <thread>
@@ -215,7 +216,43 @@ static __float128 qfact(int n) {
     return r;
 }
 
+static double clebsch_gordan_eval(int j1, int m1, int j2, int m2, int J, int M);
+
+// Every coefficient with 2j1, 2j2 <= CG_JMAX is computed once per process (about
+// 4e4 non-zero values, tens of ms); the hot loops of the table build then read it.
+static constexpr int CG_JMAX = 11;
+static constexpr int CG_NJ = CG_JMAX + 1, CG_NJJ = 2 * CG_JMAX + 1;
+static const double* cg_table() {
+    static std::vector<double> tab;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        tab.assign((size_t)CG_NJ * CG_NJ * CG_NJJ * CG_NJ * CG_NJ, 0.0);
+        for (int j1 = 0; j1 <= CG_JMAX; j1++)
+            for (int j2 = 0; j2 <= CG_JMAX; j2++)
+                for (int J = std::abs(j1 - j2); J <= j1 + j2; J += 2)
+                    for (int m1 = -j1; m1 <= j1; m1 += 2)
+                        for (int m2 = -j2; m2 <= j2; m2 += 2) {
+                            if (std::abs(m1 + m2) > J) continue;
+                            size_t i = ((((size_t)j1 * CG_NJ + j2) * CG_NJJ + J) * CG_NJ + (m1 + j1) / 2) * CG_NJ +
+                                       (m2 + j2) / 2;
+                            tab[i] = clebsch_gordan_eval(j1, m1, j2, m2, J, m1 + m2);
+                        }
+    });
+    return tab.data();
+}
+
 double clebsch_gordan(int j1, int m1, int j2, int m2, int J, int M) {
+    if (m1 + m2 != M || std::abs(m1) > j1 || std::abs(m2) > j2 || std::abs(M) > J) return 0.0;
+    if (J > j1 + j2 || J < std::abs(j1 - j2)) return 0.0;
+    if (((j1 + m1) & 1) || ((j2 + m2) & 1) || ((J + M) & 1) || ((j1 + j2 + J) & 1)) return 0.0;
+    if (j1 <= CG_JMAX && j2 <= CG_JMAX)
+        return cg_table()[((((size_t)j1 * CG_NJ + j2) * CG_NJJ + J) * CG_NJ + (m1 + j1) / 2) * CG_NJ + (m2 + j2) / 2];
+    return clebsch_gordan_eval(j1, m1, j2, m2, J, M);
+}
+
+// correctly rounded <j1 m1; j2 m2 | J M> (doubled arguments): Racah's formula in
+// quad precision, rounded once to double
+static double clebsch_gordan_eval(int j1, int m1, int j2, int m2, int J, int M) {
     if (m1 + m2 != M || std::abs(m1) > j1 || std::abs(m2) > j2 || std::abs(M) > J) return 0.0;
     if (J > j1 + j2 || J < std::abs(j1 - j2)) return 0.0;
     if (((j1 + m1) & 1) || ((j2 + m2) & 1) || ((J + M) & 1) || ((j1 + j2 + J) & 1)) return 0.0;
@@ -486,26 +523,53 @@ static HopTable build_hops(const Species& s, const OneBodyOps& ob, int sp, int i
     return h;
 }
 
-static PairSym build_pair_matrix(const Species& s, int set, const OpList& L) {
+// operators of a list grouped by their annihilated pair (c | d): an SD can only be
+// acted on by the operators of the pairs it occupies
+using OpsByPair = std::unordered_map<uint64_t, std::vector<int32_t>>;
+static OpsByPair group_by_pair(const OpList& L) {
+    OpsByPair g;
+    for (size_t q = 0; q < L.ops.size(); q++)
+        if (L.V[q] != 0.0) g[L.ops[q].c | L.ops[q].d].push_back((int32_t)q);
+    return g;
+}
+
+static PairSym build_pair_matrix(const Species& s, int set, const OpList& L, const OpsByPair& byp) {
     const auto& S = s.sets[set];
     size_t n = S.size();
-    std::vector<std::map<int32_t, double>> rows(n);
-    for (size_t i = 0; i < n; i++)
-        for (size_t q = 0; q < L.ops.size(); q++) {
-            if (L.V[q] == 0.0) continue;
-            uint64_t fsd; int neg;
-            if (!apply2(S[i], L.ops[q], &fsd, &neg)) continue;
-            int64_t f = s.find(set, fsd);
-            if (f < 0 || (int64_t)i > f) continue;          // entries kept only for i <= f
-            double coef = L.V[q] * (neg ? -1.0 : 1.0) * ((int64_t)i == f ? 0.5 : 1.0);
-            // the reference applies every entry in both directions
-            rows[f][(int32_t)i] += coef;
-            rows[i][(int32_t)f] += coef;
+    std::vector<std::vector<std::pair<int32_t, double>>> rows(n);
+    for (size_t i = 0; i < n; i++) {
+        const uint64_t sd = S[i];
+        for (uint64_t r1 = sd; r1; r1 &= r1 - 1) {
+            const uint64_t b1 = r1 & (~r1 + 1);
+            for (uint64_t r2 = r1 & (r1 - 1); r2; r2 &= r2 - 1) {
+                const uint64_t b2 = r2 & (~r2 + 1);
+                auto it = byp.find(b1 | b2);
+                if (it == byp.end()) continue;
+                for (int32_t q : it->second) {
+                    uint64_t fsd; int neg;
+                    if (!apply2(sd, L.ops[q], &fsd, &neg)) continue;
+                    int64_t f = s.find(set, fsd);
+                    if (f < 0 || (int64_t)i > f) continue;          // entries kept only for i <= f
+                    double coef = L.V[q] * (neg ? -1.0 : 1.0) * ((int64_t)i == f ? 0.5 : 1.0);
+                    // the reference applies every entry in both directions
+                    rows[f].push_back({(int32_t)i, coef});
+                    rows[i].push_back({(int32_t)f, coef});
+                }
+            }
         }
+    }
     PairSym m;
     m.rowptr.assign(n + 1, 0);
     for (size_t r = 0; r < n; r++) {
-        for (auto& kv : rows[r]) { m.col.push_back(kv.first); m.val.push_back(kv.second); }
+        auto& R = rows[r];
+        std::stable_sort(R.begin(), R.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+        for (size_t k = 0; k < R.size();) {
+            int32_t c = R[k].first;
+            double v = 0.0;
+            for (; k < R.size() && R[k].first == c; k++) v += R[k].second;
+            m.col.push_back(c);
+            m.val.push_back(v);
+        }
         m.rowptr[r + 1] = (int64_t)m.col.size();
     }
     return m;
@@ -542,9 +606,10 @@ void build_coefficients(const Snt& snt, const Basis& b, Tables& t, int nthreads)
     std::vector<char> pneed(P.sets.size(), 0), nneed(N.sets.size(), 0);
     for (int k = 0; k < b.nb; k++) { pneed[b.pset[k]] = 1; nneed[b.nset[k]] = 1; }
     int64_t np = (int64_t)P.sets.size();
+    const OpsByPair ppg = group_by_pair(pp), nng = group_by_pair(nn);
     parallel_for(np + (int64_t)N.sets.size(), nthreads, [&](int64_t q) {
-        if (q < np) { if (pneed[q]) t.pp[q] = build_pair_matrix(P, (int)q, pp); }
-        else { int64_t r = q - np; if (nneed[r]) t.nn[r] = build_pair_matrix(N, (int)r, nn); }
+        if (q < np) { if (pneed[q]) t.pp[q] = build_pair_matrix(P, (int)q, pp, ppg); }
+        else { int64_t r = q - np; if (nneed[r]) t.nn[r] = build_pair_matrix(N, (int)r, nn, nng); }
     });
     // V_eff
     auto pn = pn_ops(snt, P, N);
