@@ -26,7 +26,7 @@
 
 namespace smgpu {
 
-static constexpr size_t HV_LDS_TWO_ROWS = 80 * 1024;  // two-row groups if they fit (2 per CU)
+static constexpr size_t HV_LDS_TWO_ROWS = 40 * 1024;  // two-row groups if four fit one CU's LDS
 
 DBuf::~DBuf() {
     if (p) (void)hipFree(p);
@@ -90,40 +90,78 @@ __global__ __launch_bounds__(256) void k_hv(HvTables A, const T* __restrict__ v,
     for (int r = 0; r < RG; r++) epr[r] = r < nr ? ep[row0 + r] : T(0);
     const T* nnv = reinterpret_cast<const T*>(A.nn_val);
     const int64_t nsb = A.nn_slice_base[bf];
+    // Column k of a lane is fn = tid + 256 k, in slice (tid >> 6) + 4 k: the same for
+    // the whole wave, so slice tests are wave-uniform.  Lanes past nN inside the
+    // last slice work on the clamped column nN - 1 (valid addresses; their results
+    // are never stored) and read the zero padding of the ELL tables.
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int nsl = (nN + 63) >> 6;
+    // diagonal (e_p + e_n) and nn (sliced ELL, entries loaded four at a time)
 #pragma unroll
     for (int k = 0; k < NPT; k++) {
-        const int fn = tid + k * 256;
 #pragma unroll
         for (int r = 0; r < RG; r++) acc[k][r] = T(0);
-        if (fn < nN) {
-            const T e_n = en[fn];
+        const int sk = wave + 4 * k;
+        if (sk < nsl) {
+            const int fnc = min(tid + k * 256, nN - 1);
+            const T e_n = en[fnc];
 #pragma unroll
-            for (int r = 0; r < RG; r++) acc[k][r] = (epr[r] + e_n) * xs[fn * RG + r];
-            const int64_t s = nsb + (fn >> 6);
-            const int64_t off = A.nn_slice_off[s] + (fn & 63);
-            const int len = A.nn_slice_len[s];
-            for (int q = 0; q < len; q++) {
-                const int64_t e = off + (int64_t)q * 64;
-                const T c = nnv[e];
-                const int col = A.nn_col[e];
+            for (int r = 0; r < RG; r++) acc[k][r] = (epr[r] + e_n) * xs[fnc * RG + r];
+            const int64_t s = nsb + sk;
+            const int64_t off = A.nn_slice_off[s] + (tid & 63);
+            const int len = __builtin_amdgcn_readfirstlane(A.nn_slice_len[s]);
+            int q = 0;
+            for (; q + 4 <= len; q += 4) {
+                T c[4];
+                int col[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    c[u] = nnv[off + (int64_t)(q + u) * 64];
+                    col[u] = A.nn_col[off + (int64_t)(q + u) * 64];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+#pragma unroll
+                    for (int r = 0; r < RG; r++) acc[k][r] += c[u] * xs[col[u] * RG + r];
+            }
+            for (; q < len; q++) {
+                const T c = nnv[off + (int64_t)q * 64];
+                const int col = A.nn_col[off + (int64_t)q * 64];
 #pragma unroll
                 for (int r = 0; r < RG; r++) acc[k][r] += c * xs[col * RG + r];
             }
         }
     }
-    // pp: rows of the same block
-    const T* ppv = reinterpret_cast<const T*>(A.pp_val);
+    // pp: rows of the same block; UP entries (uniform) per step, their row gathers
+    // issued together as unconditional loads
+    {
+        constexpr int UP = NPT <= 2 ? 8 : (NPT <= 4 ? 4 : 2);
+        const T* ppv = reinterpret_cast<const T*>(A.pp_val);
 #pragma unroll
-    for (int r = 0; r < RG; r++) {
-        if (r < nr) {
-            const int64_t pe1 = A.pp_rowptr[row0 + r + 1];
-            for (int64_t e = A.pp_rowptr[row0 + r]; e < pe1; e++) {
-                const T c = ppv[e];
-                const T* src = v + bstart + (int64_t)A.pp_col[e] * nN;
+        for (int r = 0; r < RG; r++) {
+            if (r < nr) {
+                const int64_t pe0 = A.pp_rowptr[row0 + r], pe1 = A.pp_rowptr[row0 + r + 1];
+                for (int64_t e = pe0; e < pe1; e += UP) {
+                    T c[UP];
+                    int64_t so[UP];
 #pragma unroll
-                for (int k = 0; k < NPT; k++) {
-                    const int fn = tid + k * 256;
-                    if (fn < nN) acc[k][r] += c * src[fn];
+                    for (int u = 0; u < UP; u++) {
+                        const int64_t eu = min(e + u, pe1 - 1);
+                        c[u] = e + u < pe1 ? ppv[eu] : T(0);
+                        so[u] = bstart + (int64_t)A.pp_col[eu] * nN;
+                    }
+#pragma unroll
+                    for (int k = 0; k < NPT; k++) {
+                        if (wave + 4 * k >= nsl) continue;          // wave-uniform
+                        const int fnc = min(tid + k * 256, nN - 1);
+                        T x[UP];
+#pragma unroll
+                        for (int u = 0; u < UP; u++) x[u] = v[so[u] + fnc];
+                        T a = acc[k][r];
+#pragma unroll
+                        for (int u = 0; u < UP; u++) a += c[u] * x[u];
+                        acc[k][r] = a;
+                    }
                 }
             }
         }
@@ -154,8 +192,6 @@ __global__ __launch_bounds__(256) void k_hv(HvTables A, const T* __restrict__ v,
         // for the whole wave, so slice lengths (and every test on them) are
         // wave-uniform; lanes past nN inside the last slice read its zero-column
         // padding, waves past the last slice get length 0 and skip all work
-        const int nsl = (nN + 63) >> 6;
-        const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
         int64_t off[NPT];
         int len[NPT];
         int maxlen = 0;
@@ -585,7 +621,7 @@ template <typename T> void set_view(DeviceIndex& h, int prec, const HostLayout& 
     A.cols2_max = std::max(2, L.cols2_max);
     auto lds_for = [&](int rg) { return ((size_t)A.nN_max * 4 * rg + (size_t)A.cols2_max * 4 * rg +
                                          (size_t)A.nN_max * rg) * sizeof(T); };
-    A.rg = lds_for(2) <= HV_LDS_TWO_ROWS ? 2 : 1;
+    A.rg = lds_for(2) <= HV_LDS_TWO_ROWS ? 2 : 1;   // two-row groups only while 4 fit a CU
     A.lds_bytes = lds_for(A.rg);
     if (A.lds_bytes > 160 * 1024) throw std::runtime_error("neutron block too wide for the LDS of one CU");
     int need = (A.nN_max + 255) / 256;
