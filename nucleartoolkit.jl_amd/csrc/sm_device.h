@@ -80,6 +80,7 @@ struct HvTables {
     const int32_t* veff_cols2;        // per dM: 2*(cols+1)
     const void* veff;                 // (T) signed copies: [row][2*bd+neg]
     int rg;         // proton rows per workgroup
+    int pch;        // hop slots per row and staging round (multiple of 4)
     int nN_max;
     int cols2_max;
     int npt;        // ceil(nN_max / 256)

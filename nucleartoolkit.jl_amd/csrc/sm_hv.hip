@@ -26,7 +26,7 @@
 
 namespace smgpu {
 
-static constexpr size_t HV_LDS_TWO_ROWS = 40 * 1024;  // two-row groups if four fit one CU's LDS
+static constexpr size_t HV_LDS_TWO_ROWS = 16 * 1024;  // two-row groups for narrow blocks only
 
 DBuf::~DBuf() {
     if (p) (void)hipFree(p);
@@ -58,9 +58,10 @@ template <> struct Vec4<double> { using type = double4; };
 //   X[in][slot] = s_p v[bi: ip_slot, in],   Vs[col][slot] = V_eff[dM][ac_slot][col]
 // so that a lane fetches the four hops of a row for its neutron hop (in, col)
 // with two 4-wide LDS loads at compile-time offsets.  Unused slots are zero.
-template <typename T, int NPT, int RG>
+template <typename T, int NPT, int RG, int PCH>
 __global__ __launch_bounds__(256) void k_hv(HvTables A, const T* __restrict__ v, T* __restrict__ w) {
-    constexpr int PCH = 4;            // hop slots per row and round
+    static_assert(PCH % 4 == 0, "slots come in groups of four");
+    constexpr int NG = PCH / 4;       // slot groups per row and round
     constexpr int SL = RG * PCH;      // slots per round
     using T4 = typename Vec4<T>::type;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -210,64 +211,51 @@ __global__ __launch_bounds__(256) void k_hv(HvTables A, const T* __restrict__ v,
         }
         maxlen = __builtin_amdgcn_readfirstlane(maxlen);
         for (int ch = 0; ch < nch; ch++) {
-            // slot metadata (uniform): offsets into v and V_eff, not pointers, so
-            // that every gather below is an unconditional global load from a
-            // kernel argument (inactive slots read a valid element times 0)
-            int64_t xo[SL];
-            int vo[SL];
-            T sg[SL], vm[SL];
-            bool act_row[RG];
+            __syncthreads();
+            // Stage the round's active slot groups (four hops of one row each):
+            // X[in][slot] = s v[bi: ip, in] and Vs[col][slot] = V_eff[op][col].
+            // Offsets, not pointers, so every gather is an unconditional global
+            // load from a kernel argument (a group's unused slots read a valid
+            // element times 0).  Groups past a row's hops are neither staged nor read.
 #pragma unroll
-            for (int r = 0; r < RG; r++) {
-                act_row[r] = rb[r] + ch * PCH < rb[r + 1];
+            for (int r = 0; r < RG; r++)
 #pragma unroll
-                for (int j = 0; j < PCH; j++) {
-                    const int h = rb[r] + ch * PCH + j;
-                    const int sl = r * PCH + j;
-                    if (h < rb[r + 1]) {
-                        const uint32_t e = pent[h];
-                        xo[sl] = ibase + (int64_t)(e & 0xFFFFFu) * nNi;
-                        vo[sl] = (int)((e >> 20) & 0x7FFu) * c2;
-                        sg[sl] = (e >> 31) ? T(-1) : T(1);
-                        vm[sl] = T(1);
-                    } else {
-                        xo[sl] = ibase;
-                        vo[sl] = 0;
-                        sg[sl] = T(0);
-                        vm[sl] = T(0);
+                for (int g = 0; g < NG; g++) {
+                    const int h0 = rb[r] + ch * PCH + g * 4;
+                    if (h0 >= rb[r + 1]) continue;                 // uniform
+                    int64_t xo[4];
+                    int vo[4];
+                    T sg[4], vm[4];
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        if (h0 + j < rb[r + 1]) {
+                            const uint32_t e = pent[h0 + j];
+                            xo[j] = ibase + (int64_t)(e & 0xFFFFFu) * nNi;
+                            vo[j] = (int)((e >> 20) & 0x7FFu) * c2;
+                            sg[j] = (e >> 31) ? T(-1) : T(1);
+                            vm[j] = T(1);
+                        } else {
+                            xo[j] = ibase; vo[j] = 0; sg[j] = T(0); vm[j] = T(0);
+                        }
+                    }
+                    const int so = r * PCH + g * 4;
+                    for (int i = tid; i < nNi; i += 256) {
+                        T4 x4;
+                        x4.x = sg[0] * v[xo[0] + i];
+                        x4.y = sg[1] * v[xo[1] + i];
+                        x4.z = sg[2] * v[xo[2] + i];
+                        x4.w = sg[3] * v[xo[3] + i];
+                        *reinterpret_cast<T4*>(X + (size_t)i * SL + so) = x4;
+                    }
+                    for (int c = tid; c < c2; c += 256) {
+                        T4 v4;
+                        v4.x = vm[0] * V2[vo[0] + c];
+                        v4.y = vm[1] * V2[vo[1] + c];
+                        v4.z = vm[2] * V2[vo[2] + c];
+                        v4.w = vm[3] * V2[vo[3] + c];
+                        *reinterpret_cast<T4*>(Vs + (size_t)c * SL + so) = v4;
                     }
                 }
-            }
-            __syncthreads();
-            // staged slots are written four at a time (16-byte stores)
-            for (int i = tid; i < nNi; i += 256) {
-                T xv[SL];
-#pragma unroll
-                for (int sl = 0; sl < SL; sl++) xv[sl] = v[xo[sl] + i];
-#pragma unroll
-                for (int r = 0; r < RG; r++) {
-                    T4 x4;
-                    x4.x = sg[r * PCH + 0] * xv[r * PCH + 0];
-                    x4.y = sg[r * PCH + 1] * xv[r * PCH + 1];
-                    x4.z = sg[r * PCH + 2] * xv[r * PCH + 2];
-                    x4.w = sg[r * PCH + 3] * xv[r * PCH + 3];
-                    *reinterpret_cast<T4*>(X + (size_t)i * SL + r * PCH) = x4;
-                }
-            }
-            for (int c = tid; c < c2; c += 256) {
-                T cv[SL];
-#pragma unroll
-                for (int sl = 0; sl < SL; sl++) cv[sl] = V2[vo[sl] + c];
-#pragma unroll
-                for (int r = 0; r < RG; r++) {
-                    T4 v4;
-                    v4.x = vm[r * PCH + 0] * cv[r * PCH + 0];
-                    v4.y = vm[r * PCH + 1] * cv[r * PCH + 1];
-                    v4.z = vm[r * PCH + 2] * cv[r * PCH + 2];
-                    v4.w = vm[r * PCH + 3] * cv[r * PCH + 3];
-                    *reinterpret_cast<T4*>(Vs + (size_t)c * SL + r * PCH) = v4;
-                }
-            }
             __syncthreads();
             // neutron-hop entries in chunks of PF steps, double-buffered: a chunk is
             // decoded into registers, the next chunk's loads are issued into the
@@ -302,13 +290,16 @@ __global__ __launch_bounds__(256) void k_hv(HvTables A, const T* __restrict__ v,
                 for (int d = 0; d < PF; d++) {
                     const int qq = q0 + d;
 #pragma unroll
-                    for (int r = 0; r < RG; r++) {
-                        if (act_row[r]) {
+                    for (int r = 0; r < RG; r++)
+#pragma unroll
+                        for (int g = 0; g < NG; g++) {
+                            if (rb[r] + ch * PCH + g * 4 >= rb[r + 1]) continue;   // uniform: group not staged
+                            const int so = r * PCH + g * 4;
 #pragma unroll
                             for (int k = 0; k < NPT; k++) {
                                 if (qq >= len[k]) continue;          // wave-uniform
-                                const T4 xv = *reinterpret_cast<const T4*>(X + (size_t)in[d][k] * SL + r * PCH);
-                                const T4 vv = *reinterpret_cast<const T4*>(Vs + (size_t)col[d][k] * SL + r * PCH);
+                                const T4 xv = *reinterpret_cast<const T4*>(X + (size_t)in[d][k] * SL + so);
+                                const T4 vv = *reinterpret_cast<const T4*>(Vs + (size_t)col[d][k] * SL + so);
                                 T a = acc[k][r];
                                 a += vv.x * xv.x;
                                 a += vv.y * xv.y;
@@ -317,7 +308,6 @@ __global__ __launch_bounds__(256) void k_hv(HvTables A, const T* __restrict__ v,
                                 acc[k][r] = a;
                             }
                         }
-                    }
                 }
             }
         }
@@ -335,12 +325,13 @@ __global__ __launch_bounds__(256) void k_hv(HvTables A, const T* __restrict__ v,
     }
 }
 
-#define HV_FOR_SHAPES(X_, T_)                                                                        \
-    X_(T_, 1, 1) X_(T_, 2, 1) X_(T_, 3, 1) X_(T_, 4, 1) X_(T_, 6, 1) X_(T_, 8, 1) X_(T_, 12, 1) X_(T_, 16, 1) \
-    X_(T_, 1, 2) X_(T_, 2, 2) X_(T_, 3, 2) X_(T_, 4, 2) X_(T_, 6, 2) X_(T_, 8, 2) X_(T_, 12, 2) X_(T_, 16, 2)
+// instantiated shapes (NPT, RG, PCH)
+#define HV_NPTS(X_, T_, R_, P_) X_(T_, 1, R_, P_) X_(T_, 2, R_, P_) X_(T_, 3, R_, P_) X_(T_, 4, R_, P_) \
+    X_(T_, 6, R_, P_) X_(T_, 8, R_, P_) X_(T_, 12, R_, P_) X_(T_, 16, R_, P_)
+#define HV_FOR_SHAPES(X_, T_) HV_NPTS(X_, T_, 2, 4) HV_NPTS(X_, T_, 1, 4)
 
-template <typename T> static const void* hv_fn(int npt, int rg) {
-#define HV_CASE(T_, N_, R_) if (npt == N_ && rg == R_) return (const void*)k_hv<T_, N_, R_>;
+template <typename T> static const void* hv_fn(int npt, int rg, int pch) {
+#define HV_CASE(T_, N_, R_, P_) if (npt == N_ && rg == R_ && pch == P_) return (const void*)k_hv<T_, N_, R_, P_>;
     HV_FOR_SHAPES(HV_CASE, T)
 #undef HV_CASE
     return nullptr;
@@ -350,9 +341,9 @@ template <typename T>
 static void launch_t(const HvTables& A, const T* v, T* w, hipStream_t s) {
     dim3 grid(A.G), block(256);
     bool done = false;
-#define HV_LAUNCH(T_, N_, R_)                                                                  \
-    if (!done && A.npt == N_ && A.rg == R_) {                                                  \
-        hipLaunchKernelGGL((k_hv<T_, N_, R_>), grid, block, A.lds_bytes, s, A, v, w);         \
+#define HV_LAUNCH(T_, N_, R_, P_)                                                              \
+    if (!done && A.npt == N_ && A.rg == R_ && A.pch == P_) {                                   \
+        hipLaunchKernelGGL((k_hv<T_, N_, R_, P_>), grid, block, A.lds_bytes, s, A, v, w);     \
         done = true;                                                                           \
     }
     HV_FOR_SHAPES(HV_LAUNCH, T)
@@ -619,10 +610,16 @@ template <typename T> void set_view(DeviceIndex& h, int prec, const HostLayout& 
     A.veff_off = h.veff_off.as<int64_t>(); A.veff_cols2 = h.veff_cols2.as<int32_t>(); A.veff = nullptr;
     A.nN_max = std::max(1, L.nN_max);
     A.cols2_max = std::max(2, L.cols2_max);
-    auto lds_for = [&](int rg) { return ((size_t)A.nN_max * 4 * rg + (size_t)A.cols2_max * 4 * rg +
-                                         (size_t)A.nN_max * rg) * sizeof(T); };
-    A.rg = lds_for(2) <= HV_LDS_TWO_ROWS ? 2 : 1;   // two-row groups only while 4 fit a CU
-    A.lds_bytes = lds_for(A.rg);
+    // LDS per workgroup: X (nN_max x slots), Vs (cols2_max x slots), own rows (nN_max x rg)
+    auto lds_for = [&](int rg, int pch) {
+        return ((size_t)A.nN_max * pch * rg + (size_t)A.cols2_max * pch * rg + (size_t)A.nN_max * rg) * sizeof(T);
+    };
+    // two-row groups for narrow blocks, one row otherwise; four slots per row and
+    // round (measured on Cr48 and Mn50: 8 or 16 slots per round, i.e. fewer
+    // staging rounds, are no faster and cost LDS occupancy)
+    A.rg = lds_for(2, 4) <= HV_LDS_TWO_ROWS ? 2 : 1;
+    A.pch = 4;
+    A.lds_bytes = lds_for(A.rg, A.pch);
     if (A.lds_bytes > 160 * 1024) throw std::runtime_error("neutron block too wide for the LDS of one CU");
     int need = (A.nN_max + 255) / 256;
     const int npts[] = {1, 2, 3, 4, 6, 8, 12, 16};
@@ -645,7 +642,7 @@ template <typename T> void set_view(DeviceIndex& h, int prec, const HostLayout& 
     A.G = (int)gb.size();
     A.grp_block = h.grp_block[prec].as<int32_t>();
     A.grp_fp0 = h.grp_fp0[prec].as<int32_t>();
-    const void* fn = hv_fn<T>(A.npt, A.rg);
+    const void* fn = hv_fn<T>(A.npt, A.rg, A.pch);
     if (!fn) throw std::runtime_error("no H·v kernel instance for this shape");
     raise_lds_limit(fn, A.lds_bytes);
 }
