@@ -66,7 +66,7 @@ typedef struct {
     double wall_ms;
     double ritz[64];     /* en[1]: Ritz values of the last convergence check */
     int32_t n_reorth2;   /* iterations that needed a second Gram-Schmidt pass */
-    int32_t pad_;
+    int32_t n_jproj;     /* J-projected solves: vectors that were projected */
 } smgpu_solve_stats;
 
 /* library */
@@ -144,6 +144,13 @@ int32_t smgpu_trl_solve(smgpu_ham h, const smgpu_solve_params* params, double* e
  * in_wf start, shellmodel_main.jl:194-198); v0 == NULL uses params->seed. */
 int32_t smgpu_trl_solve_v0(smgpu_ham h, const smgpu_solve_params* params, const double* v0, double* evals,
                            double* evecs, smgpu_solve_stats* stats);
+/* J-projected TRL: main_sm with target_J = [twoJ] (shellmodel_main.jl:111-114, TRL's
+ * doubleLanczos with Jlanczos, lanczos_methods.jl:1-91, 114-151).  h and j2 (from
+ * smgpu_ham_create_j2) must be built on a problem with 2M = twoJ; the lowest num_ev
+ * states with J = twoJ/2.  Every new Lanczos vector whose <J^2> is off J(J+1) by more
+ * than 1e-7 (Jlanczos' Jtol) is replaced by its exact projection on that J. */
+int32_t smgpu_trl_solve_j(smgpu_ham h, smgpu_ham j2, int32_t twoJ, const smgpu_solve_params* params,
+                          const double* v0, double* evals, double* evecs, smgpu_solve_stats* stats);
 /* nsteps Lanczos steps of the TRL recurrence from v0 (no convergence test or
  * restart): alpha[nsteps], beta[nsteps] of the projected tridiagonal matrix. */
 int32_t smgpu_lanczos_steps(smgpu_ham h, int32_t precision, int32_t nsteps, const double* v0, double* alpha,

@@ -210,6 +210,7 @@ int32_t smgpu_ham_create_j2(smgpu_problem p, int32_t device, smgpu_ham* out) {
         Tables t = build_tables(j2, p->basis, p->nthreads);
         auto h = std::make_unique<smgpu_ham_s>();
         h->dev = upload_ham(p->basis, t, device);
+        h->dev->twoJmax = p->basis.sp[0].Ms.back() + p->basis.sp[1].Ms.back();
         *out = h.release();
         return SMGPU_OK;
     });
@@ -371,6 +372,29 @@ int32_t smgpu_trl_solve_v0(smgpu_ham h, const smgpu_solve_params* params, const 
         std::memset(&st, 0, sizeof(st));
         p.num_ev = (int32_t)std::min<int64_t>(p.num_ev, h->dev->mdim);
         trl_solve(*h->dev, p, evals, evecs, st, v0);
+        if (stats) *stats = st;
+        return SMGPU_OK;
+    });
+}
+
+int32_t smgpu_trl_solve_j(smgpu_ham h, smgpu_ham j2, int32_t twoJ, const smgpu_solve_params* params,
+                          const double* v0, double* evals, double* evecs, smgpu_solve_stats* stats) {
+    return guard([&]() -> int32_t {
+        if (!h || !j2 || !params || !evals) throw std::invalid_argument("null argument");
+        if (j2->dev->twoJmax < 0) throw std::invalid_argument("j2 is not a J^2 handle (smgpu_ham_create_j2)");
+        if (twoJ < 0 || twoJ > j2->dev->twoJmax) throw std::invalid_argument("target 2J outside the space");
+        smgpu_solve_params p = *params;
+        if (p.max_iter <= 0) p.max_iter = 100000;
+        if (p.num_history < 2) p.num_history = 3;
+        if (p.num_ev > 64) throw std::invalid_argument("num_ev above 64");
+        smgpu_solve_stats st;
+        std::memset(&st, 0, sizeof(st));
+        p.num_ev = (int32_t)std::min<int64_t>(p.num_ev, h->dev->mdim);
+        JProj jp;
+        jp.j2 = j2->dev.get();
+        jp.twoJ = twoJ;
+        jp.twoJmax = j2->dev->twoJmax;
+        trl_solve(*h->dev, p, evals, evecs, st, v0, nullptr, &jp);
         if (stats) *stats = st;
         return SMGPU_OK;
     });

@@ -118,6 +118,7 @@ struct DeviceHam {
     int64_t pn_terms = 0, pp_terms = 0, nn_terms = 0;   // FMAs per H·v (padded n-table entries included)
     int64_t pn_terms_exact = 0;
     std::shared_ptr<void> trl_ws[2];   // Lanczos workspace per precision, reused across solves
+    int twoJmax = -1;                  // J^2 handles: largest 2J of the basis
     int64_t coefficient_capacity() const;   // HBM held by the coefficient buffers
     void release_coefficients();
     ~DeviceHam();

@@ -31,8 +31,19 @@ int64_t workspace_capacity(const SolveCtx& ctx);
 int64_t workspace_growth(const SolveCtx& ctx, int64_t mdim, const SolveParams& p);
 void release_workspace(SolveCtx& ctx);
 
+// J projection of the Lanczos vectors (TRL's doubleLanczos, lanczos_methods.jl:1-91, 114-151):
+// j2 applies J^2 on the same basis, which has 2M = twoJ; twoJmax is the largest 2J of
+// the space.  A vector whose <J^2> differs from J(J+1) by more than tol is replaced
+// by its projection prod_{J'} (J^2 - J'(J'+1)) / (J(J+1) - J'(J'+1)), J' = J+1..Jmax,
+// re-orthogonalised against the basis and normalised.
+struct JProj {
+    const DeviceHam* j2 = nullptr;
+    int twoJ = 0, twoJmax = 0;
+    double tol = 1e-7;
+};
+
 void trl_solve(DeviceHam& h, const SolveParams& p, double* evals, double* evecs, SolveStats& st,
-               const double* v0 = nullptr, const SolveCtx* ctx = nullptr);
+               const double* v0 = nullptr, const SolveCtx* ctx = nullptr, const JProj* jp = nullptr);
 void restart_small_device(int device, const double* alpha, const double* beta, int m, int ls, double* S,
                           double* alpha_out, double* beta_out);
 void lanczos_steps(DeviceHam& h, const SolveParams& p, int nsteps, const double* v0, double* alpha, double* beta);

@@ -46,6 +46,9 @@ static constexpr int TB = 256;
 static constexpr int EPL = 8;                 // elements per lane in the streaming passes
 static constexpr int CHUNK = 64 * EPL;        // x per block in k_dotsw
 static constexpr int UCHUNK = TB * 4;         // x per block in k_updatew
+// a J-filtered Lanczos vector that keeps less than this of its norm was noise
+// outside the J space: the J-space Krylov subspace is exhausted
+static constexpr double J_EXHAUSTED = 1e-3;
 
 __device__ inline double wave_sum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -387,6 +390,13 @@ __global__ void k_finish_local(double* loc, int it, const double* npart, int G, 
 
 __global__ void k_sqrt1(const double* a, double* b) { b[0] = sqrt(a[0]); }
 
+// one factor of the J projector: y = (z - c y) * inv, with z = J^2 y
+__global__ __launch_bounds__(TB) void k_jfactor(double* __restrict__ y, const double* __restrict__ z, int64_t n,
+                                                double c, double inv) {
+    for (int64_t x = (int64_t)blockIdx.x * TB + threadIdx.x; x < n; x += (int64_t)gridDim.x * TB)
+        y[x] = (z[x] - c * y[x]) * inv;
+}
+
 // norm of a vector into out[0] (as |x|)
 __global__ void k_norm_from_parts(const double* npart, int G, double* out) {
     __shared__ double red[TB / 64];
@@ -571,7 +581,7 @@ namespace {
 
 struct Workspace {
     DBuf basis, wH, w64, part, npart, npart2, h1, h2, alpha, beta, enring, flags, S, swork, U, scal, tvals, Y, Q,
-        R, hv64, loc, gate;
+        R, hv64, loc, gate, jy, jz;
     int* hflag = nullptr;          // pinned mirror of the convergence-flag ring
     hipEvent_t ev[4] = {};
     void host_sync_objects() {
@@ -582,7 +592,7 @@ struct Workspace {
     int64_t capacity() const {
         int64_t c = 0;
         for (const DBuf* d : {&basis, &wH, &w64, &part, &npart, &npart2, &h1, &h2, &alpha, &beta, &enring, &flags, &S,
-                              &swork, &U, &scal, &tvals, &Y, &Q, &R, &hv64, &loc, &gate})
+                              &swork, &U, &scal, &tvals, &Y, &Q, &R, &hv64, &loc, &gate, &jy, &jz})
             c += (int64_t)d->cap;
         return c;
     }
@@ -625,6 +635,8 @@ static void workspace_plan(int64_t n, const SolveParams& p, F&& f) {
     f(&Workspace::Q, 32 * 32 * 8);
     f(&Workspace::R, nev * ld * 8);
     f(&Workspace::hv64, mixed && p.refine ? ld * 8 : 0);
+    f(&Workspace::jy, ld * 8);     // J projection scratch (FP64), small next to the basis
+    f(&Workspace::jz, ld * 8);
 }
 
 template <typename TBv>
@@ -642,6 +654,7 @@ struct Solver {
     Workspace& W;
     const std::atomic<bool>* cancel = nullptr;
     bool reorth_cgs2 = false;   // SMGPU_REORTH=cgs2: unconditional CGS2 instead of the local step
+    const JProj* jp = nullptr;  // J projection (doubleLanczos), or none
 
     static Workspace& cached(std::shared_ptr<void>* slots) {
         const int k = sizeof(TBv) == 4 ? 1 : 0;
@@ -650,8 +663,8 @@ struct Solver {
     }
 
     // ctx: stream + workspace slots of the caller (a serving worker), or the handle's own
-    Solver(DeviceHam& h, const SolveParams& p, SolveStats& stats, const SolveCtx* ctx)
-        : H(h), P(p), st(stats), W(cached(ctx ? ctx->ws : h.trl_ws)) {
+    Solver(DeviceHam& h, const SolveParams& p, SolveStats& stats, const SolveCtx* ctx, const JProj* jproj = nullptr)
+        : H(h), P(p), st(stats), W(cached(ctx ? ctx->ws : h.trl_ws)), jp(jproj) {
         s = ctx ? ctx->stream : H.stream;
         cancel = ctx ? ctx->cancel : nullptr;
         if (const char* e = std::getenv("SMGPU_REORTH")) reorth_cgs2 = std::string(e) == "cgs2";
@@ -806,6 +819,7 @@ struct Solver {
             throw std::invalid_argument("need 1 <= num_ev <= ls <= 32 and ls < lm - 1");
         if (lm > 1000) throw std::invalid_argument("lm above 1000 is not supported");
         setup(v0_in);
+        if (jp) jfilter(0, 0, true);
         int* hflag = W.hflag;
         hipEvent_t* ev = W.ev;
         int k = 0;
@@ -830,6 +844,19 @@ struct Solver {
                 enqueue_iteration(it, step, nprev);
                 SM_HIP(hipMemcpyAsync(hflag + ((step + 1) & 3), flag(step), sizeof(int), hipMemcpyDeviceToHost, s));
                 SM_HIP(hipEventRecord(ev[(step + 1) & 3], s));
+                if (jp && it + 1 < lm && it + 1 < n) {
+                    // the J subspace is exhausted when beta vanishes (TRL: tbeta < Jtol stops)
+                    double b = 0.0;
+                    SM_HIP(hipMemcpyAsync(&b, W.beta.as<double>() + it, 8, hipMemcpyDeviceToHost, s));
+                    SM_HIP(hipStreamSynchronize(s));
+                    // ... or when the projection of the next vector keeps almost nothing of
+                    // it: the residual was rounding noise outside the (invariant) J space
+                    if (b < jp->tol || jfilter(it + 1, it + 1, false) < J_EXHAUSTED) {
+                        conv = true; conv_step = step; elit = it + 1;
+                        step++;
+                        break;
+                    }
+                }
                 if (it + 1 == n) {                         // Krylov space exhausted (it == mdim)
                     if (step > first_step && check(step - 1)) { conv_step = step - 1; elit = it; }
                     else { conv_step = step; elit = it + 1; }
@@ -852,6 +879,7 @@ struct Solver {
             after_restart = true;
         }
         SM_HIP(hipStreamSynchronize(s));
+        if (elit < nev) throw std::invalid_argument("the Krylov space holds fewer states than num_ev (J space too small)");
         st.iterations = elit;
         std::vector<double> en_last(nev);
         SM_HIP(hipMemcpy(en_last.data(), en(conv_step), nev * 8, hipMemcpyDeviceToHost));
@@ -861,6 +889,69 @@ struct Solver {
         for (int i = 0; i < nev; i++) { evals[i] = en_last[i]; st.ritz[i] = en_last[i]; }
         finish(elit, conv_step, evals, evecs);
         st.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+
+    // <y|J^2|y> / <y|y> with z = J^2 y (synchronous)
+    double j_expect(const double* y, double* z) {
+        launch_hv(*jp->j2, 0, y, z, s);
+        const int G2 = (int)std::min<int64_t>(1024, Gd);
+        double* part = W.part.as<double>();
+        double* sc = W.scal.as<double>();
+        hipLaunchKernelGGL((k_dot2<double, double>), dim3(G2), dim3(TB), 0, s, y, (const double*)z, n, part, ldp);
+        hipLaunchKernelGGL(k_reduce, dim3(1), dim3(TB), 0, s, part, G2, ldp, sc + 60, (const int*)nullptr);
+        hipLaunchKernelGGL((k_dot2<double, double>), dim3(G2), dim3(TB), 0, s, y, y, n, part, ldp);
+        hipLaunchKernelGGL(k_reduce, dim3(1), dim3(TB), 0, s, part, G2, ldp, sc + 61, (const int*)nullptr);
+        double r[2];
+        SM_HIP(hipMemcpyAsync(r, sc + 60, 16, hipMemcpyDeviceToHost, s));
+        SM_HIP(hipStreamSynchronize(s));
+        return r[0] / r[1];
+    }
+
+    // J filter of V(j) (the doubleLanczos step of TRL, lanczos_methods.jl:62-75): if
+    // <J^2> is off J(J+1) by more than the tolerance (or `force`), V(j) becomes the
+    // normalised, re-orthogonalised (against V[0..nv-1]) exact projection on J.
+    // Returns the norm that projection and re-orthogonalisation kept of the unit
+    // vector V(j) (1 when V(j) already has the J).
+    double jfilter(int j, int nv, bool force) {
+        double* y = W.jy.as<double>();
+        double* z = W.jz.as<double>();
+        hipLaunchKernelGGL((k_convert<TBv, double>), dim3(gstore), dim3(TB), 0, s, (const TBv*)V(j), y, n, 1.0);
+        const double hJ = 0.5 * jp->twoJ, cJ = hJ * (hJ + 1.0);
+        if (!force && std::fabs(j_expect(y, z) - cJ) <= jp->tol) return 1.0;
+        // factors J' = J+1 .. Jmax, alternately from the top and the bottom (bounded norms)
+        std::vector<int> two;
+        for (int t = jp->twoJ + 2; t <= jp->twoJmax; t += 2) two.push_back(t);
+        std::vector<int> order;
+        for (size_t lo = 0, hi = two.size(); lo < hi;) {
+            order.push_back(two[--hi]);
+            if (lo < hi) order.push_back(two[lo++]);
+        }
+        for (int t : order) {
+            const double hp = 0.5 * t, c = hp * (hp + 1.0);
+            launch_hv(*jp->j2, 0, y, z, s);
+            hipLaunchKernelGGL(k_jfactor, dim3(gstore), dim3(TB), 0, s, y, (const double*)z, n, c, 1.0 / (cJ - c));
+        }
+        st.n_jproj++;
+        if (nv > 0) {
+            cgs2<double>(nv, (const double*)y);          // -> w64, |w64|^2 partials in npart
+            store_normalised(j);
+        } else {
+            SM_HIP(hipMemcpyAsync(W.w64.p, y, n * 8, hipMemcpyDeviceToDevice, s));
+            hipLaunchKernelGGL((k_dotsw<double, double>), dim3(Gd), dim3(TB), 0, s, (const double*)nullptr, ld, 0,
+                               (const double*)W.w64.as<double>(), n, W.part.as<double>(), ldp, 1,
+                               (const int*)nullptr);
+            hipLaunchKernelGGL(k_reduce, dim3(1), dim3(TB), 0, s, W.part.as<double>(), (int)Gd, ldp,
+                               W.scal.as<double>() + 1, (const int*)nullptr);
+            hipLaunchKernelGGL(k_sqrt1, dim3(1), dim3(1), 0, s, W.scal.as<double>() + 1, W.scal.as<double>());
+            hipLaunchKernelGGL((k_store<TBv>), dim3(gstore), dim3(TB), 0, s, W.w64.as<double>(),
+                               W.scal.as<double>(), V(j), n);
+        }
+        double kept = 0.0;
+        SM_HIP(hipMemcpyAsync(&kept, W.scal.p, 8, hipMemcpyDeviceToHost, s));
+        SM_HIP(hipStreamSynchronize(s));
+        if (nv == 0 && !(kept > 1e-10))
+            throw std::invalid_argument("start vector has no component with the requested J");
+        return kept;
     }
 
     // V(j) = w64 / |w64|  (norm partials of w64 already in W.npart)
@@ -1067,13 +1158,15 @@ void release_workspace(SolveCtx& ctx) {
 
 
 void trl_solve(DeviceHam& h, const SolveParams& p, double* evals, double* evecs, SolveStats& st,
-               const double* v0, const SolveCtx* ctx) {
+               const double* v0, const SolveCtx* ctx, const JProj* jp) {
     SM_HIP(hipSetDevice(h.device));
+    if (jp && (jp->j2 == nullptr || jp->j2->mdim != h.mdim))
+        throw std::invalid_argument("J^2 handle does not match the Hamiltonian's basis");
     if (p.precision == 0) {
-        Solver<double> S(h, p, st, ctx);
+        Solver<double> S(h, p, st, ctx, jp);
         S.run(evals, evecs, v0);
     } else {
-        Solver<float> S(h, p, st, ctx);
+        Solver<float> S(h, p, st, ctx, jp);
         S.run(evals, evecs, v0);
     }
 }

@@ -33,7 +33,7 @@ struct SolveStats               # smgpu_solve_stats
     wall_ms::Float64
     ritz::NTuple{64,Float64}
     n_reorth2::Int32
-    pad_::Int32
+    n_jproj::Int32
 end
 
 struct ProblemInfo              # smgpu_problem_info

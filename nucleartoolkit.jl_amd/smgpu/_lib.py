@@ -47,7 +47,7 @@ class SolveParams(ctypes.Structure):
 class SolveStats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int32), ("n_hv", ctypes.c_int32), ("n_hv64", ctypes.c_int32),
                 ("n_restart", ctypes.c_int32), ("wall_ms", ctypes.c_double), ("ritz", ctypes.c_double * 64),
-                ("n_reorth2", ctypes.c_int32), ("pad_", ctypes.c_int32)]
+                ("n_reorth2", ctypes.c_int32), ("n_jproj", ctypes.c_int32)]
 
 
 class ServerConfig(ctypes.Structure):
@@ -100,6 +100,7 @@ SIGNATURES = {
     "smgpu_default_params": (None, [P]),
     "smgpu_trl_solve": (I32, [P, P, P, P, P]),
     "smgpu_trl_solve_v0": (I32, [P, P, P, P, P, P]),
+    "smgpu_trl_solve_j": (I32, [P, P, I32, P, P, P, P, P]),
     "smgpu_lanczos_steps": (I32, [P, I32, I32, P, P, P]),
     "smgpu_malloc": (I32, [P, I64, P]),
     "smgpu_free": (I32, [P, P]),

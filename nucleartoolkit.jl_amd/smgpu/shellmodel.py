@@ -152,8 +152,10 @@ class Hamiltonian:
         return a, b
 
     def trl(self, num_ev, lm=300, ls=20, tol=1e-8, num_history=3, precision="fp64", refine=True, seed=1234,
-            want_vectors=False, max_iter=0, v0=None):
-        """Thick-restart Lanczos; returns (evals, evecs or None, stats dict)."""
+            want_vectors=False, max_iter=0, v0=None, j2=None, twoJ=None):
+        """Thick-restart Lanczos; returns (evals, evecs or None, stats dict).
+        With j2 (a J^2 Hamiltonian on this basis) and twoJ: J-projected TRL
+        (doubleLanczos), the lowest states with J = twoJ/2."""
         p = SolveParams()
         lib().smgpu_default_params(ctypes.byref(p))
         p.num_ev = int(num_ev); p.lm = int(lm); p.ls = int(ls); p.tol = float(tol)
@@ -168,10 +170,15 @@ class Hamiltonian:
             v0 = np.ascontiguousarray(v0, dtype=np.float64)
             assert v0.shape == (self.mdim,)
             v0p = ptr(v0)
-        check(lib().smgpu_trl_solve_v0(self._h, ctypes.byref(p), v0p, ptr(ev),
-                                       ptr(vec) if vec is not None else None, ctypes.byref(st)))
+        if j2 is not None:
+            check(lib().smgpu_trl_solve_j(self._h, j2._h, int(twoJ), ctypes.byref(p), v0p, ptr(ev),
+                                          ptr(vec) if vec is not None else None, ctypes.byref(st)))
+        else:
+            check(lib().smgpu_trl_solve_v0(self._h, ctypes.byref(p), v0p, ptr(ev),
+                                           ptr(vec) if vec is not None else None, ctypes.byref(st)))
         stats = dict(iterations=st.iterations, n_hv=st.n_hv, n_hv64=st.n_hv64, n_restart=st.n_restart,
-                     wall_ms=st.wall_ms, ritz=np.array(st.ritz[:nev]), n_reorth2=st.n_reorth2)
+                     wall_ms=st.wall_ms, ritz=np.array(st.ritz[:nev]), n_reorth2=st.n_reorth2,
+                     n_jproj=st.n_jproj)
         return ev, vec, stats
 
     def __del__(self):
@@ -202,18 +209,19 @@ def main_sm(sntf, target_nuc, num_ev, target_J=(), *, lm=300, ls=20, tol=1e-8, n
             save_wav=False, wav_path=None, in_wf=""):
     """Lowest `num_ev` eigenvalues (MeV) of `target_nuc` with the interaction `sntf`.
 
-    Mirrors main_sm (shellmodel_main.jl:89-281) with target_J = [] (plain TRL):
-    2M = A % 2 (:110); mdim == 0 returns an empty array (:127); num_ev is
-    clamped to mdim (:128).  return_J adds the J of every state from <J^2> of
+    Mirrors main_sm (shellmodel_main.jl:89-281): with target_J = [] the plain
+    TRL at 2M = A % 2 (:110); with target_J = [tJ] (2J, one value, :109-116) the
+    J-projected TRL at 2M = tJ (doubleLanczos).  mdim == 0 returns an empty
+    array (:127); num_ev is clamped to mdim (:128).  return_J adds the J of every state from <J^2> of
     its Ritz vector (totJs, :226-232); save_wav writes the Ritz vectors in the
     reference's .wav layout (writeRitzvecs, :1215-1229), by default to
-    ./<nucleus>_<interaction>.wav (:241-246).  in_wf starts the Lanczos from the
-    first vector of a .wav file (read_appwav, :194-198).  J projection is not
-    part of this path.
+    ./<nucleus>_<interaction>.wav (:241-246; ..._j<tJ>.wav with target_J).  in_wf
+    starts the Lanczos from the first vector of a .wav file (read_appwav, :194-198).
     """
-    if len(target_J) > 0:
-        raise NotImplementedError("J-projected Lanczos (target_J != []) is not built")
-    prob = Problem(sntf, target_nuc)
+    if len(target_J) > 1:
+        raise ValueError("Multiple J is not supported (main_sm asserts length(target_J) <= 1)")
+    tJ = int(target_J[0]) if len(target_J) else None
+    prob = Problem(sntf, target_nuc, twoM=-1 if tJ is None else tJ)
     if prob.mdim == 0:
         return (np.zeros(0), {}) if return_stats else np.zeros(0)
     ham = Hamiltonian(prob, device)
@@ -223,8 +231,9 @@ def main_sm(sntf, target_nuc, num_ev, target_J=(), *, lm=300, ls=20, tol=1e-8, n
     if in_wf:
         from .wav import read_appwav
         v0 = read_appwav(in_wf, prob.mdim)
+    j2 = prob.j2_hamiltonian(device) if tJ is not None else None
     ev, vecs, st = ham.trl(nev, lm=lm, ls=ls, tol=tol, num_history=num_history, precision=precision,
-                           refine=refine, seed=seed, want_vectors=need_vecs, v0=v0)
+                           refine=refine, seed=seed, want_vectors=need_vecs, v0=v0, j2=j2, twoJ=tJ)
     out = [ev]
     if need_vecs:
         totJs = np.array([j_from_jj1(x) for x in expect_j2(prob, vecs, device)])
@@ -232,7 +241,7 @@ def main_sm(sntf, target_nuc, num_ev, target_J=(), *, lm=300, ls=20, tol=1e-8, n
             from .wav import write_ritzvecs
             if wav_path is None:
                 csnt = os.path.basename(sntf).split(".")[0]
-                wav_path = f"./{target_nuc}_{csnt}.wav"
+                wav_path = f"./{target_nuc}_{csnt}.wav" if tJ is None else f"./{target_nuc}_{csnt}_j{tJ}.wav"
             write_ritzvecs(wav_path, prob.info.twoM, ev, totJs, vecs)
         if return_J:
             out.append(totJs)

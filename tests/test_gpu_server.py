@@ -105,11 +105,11 @@ def test_server_cancel_and_errors(snt_dir, golden):
         # tol = -1 never converges (the Ritz values never rise by 1 MeV): it
         # restarts until canceled (the flag is polled once per iteration)
         long_id = srv.submit(path, nuc, 5, tol=-1.0, ls=10, lm=60, max_iter=20000)
-        queued_id = srv.submit(path, nuc, 5)
         t0 = time.time()
         while srv.poll(long_id) != "running":
             assert time.time() - t0 < 60
             time.sleep(0.005)
+        queued_id = srv.submit(path, nuc, 5)      # the only slot is busy: it waits
         srv.cancel(queued_id)
         srv.cancel(long_id)
         assert srv.wait(long_id, 60000) and srv.wait(queued_id, 60000)
