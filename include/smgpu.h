@@ -151,6 +151,18 @@ int32_t smgpu_trl_solve_v0(smgpu_ham h, const smgpu_solve_params* params, const 
  * than 1e-7 (Jlanczos' Jtol) is replaced by its exact projection on that J. */
 int32_t smgpu_trl_solve_j(smgpu_ham h, smgpu_ham j2, int32_t twoJ, const smgpu_solve_params* params,
                           const double* v0, double* evals, double* evecs, smgpu_solve_stats* stats);
+/* Thick-restart block Lanczos: main_sm with is_block=true and block size q
+ * (TRBL, lanczos_methods.jl:312-420, with bl_operate_H!, bl_ReORTH, bl_QR! and
+ * bl_ThickRestart :422-617; shellmodel_main.jl:167-191, 211-224).  params->lm is the
+ * basis cap (lm/q blocks), params->ls the kept Ritz vectors (q*(ls/q) <= 32).
+ * v0: q start vectors (q*mdim, read_appwav's block start) or NULL for seeded
+ * random ones.  j2 (from smgpu_ham_create_j2) may be NULL; with it, every new
+ * block vector off J(J+1) by more than 1e-7 is projected on J = twoJ/2 (the
+ * doubleLanczos filter of TRBL) and the lowest num_ev states of that J result.
+ * If the block Krylov space is exhausted (it spans the basis or an invariant
+ * subspace) the solve stops with T the exact projection on it. */
+int32_t smgpu_trbl_solve(smgpu_ham h, smgpu_ham j2, int32_t twoJ, int32_t q, const smgpu_solve_params* params,
+                         const double* v0, double* evals, double* evecs, smgpu_solve_stats* stats);
 /* nsteps Lanczos steps of the TRL recurrence from v0 (no convergence test or
  * restart): alpha[nsteps], beta[nsteps] of the projected tridiagonal matrix. */
 int32_t smgpu_lanczos_steps(smgpu_ham h, int32_t precision, int32_t nsteps, const double* v0, double* alpha,
@@ -240,6 +252,12 @@ int32_t smgpu_synchronize(smgpu_ham h);
  * source the device kernels use): lowest k eigenpairs of T. */
 int32_t smgpu_selftest_tridiag(const double* alpha, const double* beta, int32_t n, int32_t k, double* vals,
                                double* vecs);
+
+/* Self-test of the host dense symmetric eigen-solver of the block Lanczos matrix
+ * (Householder reduction + tridiagonal bisection / inverse iteration): lowest k
+ * eigenvalues of the n x n row-major symmetric A, and their unit eigenvectors
+ * (vector j at vecs + j*n) unless vecs is NULL. */
+int32_t smgpu_selftest_symeig(const double* A, int32_t n, int32_t k, double* vals, double* vecs);
 
 /* Self-test of the thick-restart small problem (tri_restart) for the tridiagonal
  * T(alpha[m], beta[m]; beta[m-1] = coupling to the residual vector): the rotation

@@ -11,6 +11,7 @@
 #include <thread>
 
 #include "../../include/smgpu.h"
+#include "dense_eig.h"
 #include "sm_device.h"
 #include "sm_host.h"
 #include "sm_status.h"
@@ -400,6 +401,31 @@ int32_t smgpu_trl_solve_j(smgpu_ham h, smgpu_ham j2, int32_t twoJ, const smgpu_s
     });
 }
 
+int32_t smgpu_trbl_solve(smgpu_ham h, smgpu_ham j2, int32_t twoJ, int32_t q, const smgpu_solve_params* params,
+                         const double* v0, double* evals, double* evecs, smgpu_solve_stats* stats) {
+    return guard([&]() -> int32_t {
+        if (!h || !params || !evals) throw std::invalid_argument("null argument");
+        smgpu_solve_params p = *params;
+        if (p.max_iter <= 0) p.max_iter = 100000;
+        if (p.num_history < 2) p.num_history = 3;
+        if (p.num_ev > 32) throw std::invalid_argument("num_ev above 32");
+        smgpu_solve_stats st;
+        std::memset(&st, 0, sizeof(st));
+        p.num_ev = (int32_t)std::min<int64_t>(p.num_ev, h->dev->mdim);
+        JProj jp;
+        if (j2) {
+            if (j2->dev->twoJmax < 0) throw std::invalid_argument("j2 is not a J^2 handle (smgpu_ham_create_j2)");
+            if (twoJ < 0 || twoJ > j2->dev->twoJmax) throw std::invalid_argument("target 2J outside the space");
+            jp.j2 = j2->dev.get();
+            jp.twoJ = twoJ;
+            jp.twoJmax = j2->dev->twoJmax;
+        }
+        trbl_solve(*h->dev, q, p, evals, evecs, st, v0, j2 ? &jp : nullptr);
+        if (stats) *stats = st;
+        return SMGPU_OK;
+    });
+}
+
 int32_t smgpu_lanczos_steps(smgpu_ham h, int32_t precision, int32_t nsteps, const double* v0, double* alpha,
                             double* beta) {
     return guard([&]() -> int32_t {
@@ -460,6 +486,17 @@ int32_t smgpu_selftest_tridiag(const double* alpha, const double* beta, int32_t 
         std::vector<double> b(std::max(1, n));
         for (int i = 0; i < n - 1; i++) b[i] = beta[i];
         tri_lowest(alpha, b.data(), n, k, vals, vecs, work.data());
+        return SMGPU_OK;
+    });
+}
+
+int32_t smgpu_selftest_symeig(const double* A, int32_t n, int32_t k, double* vals, double* vecs) {
+    return guard([&]() -> int32_t {
+        if (!A || !vals || n < 1 || k < 1 || k > n) throw std::invalid_argument("bad argument");
+        DenseEig e;
+        e.reduce(A, n, n);
+        if (vecs) e.lowest_pairs(k, vals, vecs);
+        else e.lowest(k, vals);
         return SMGPU_OK;
     });
 }

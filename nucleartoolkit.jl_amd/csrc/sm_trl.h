@@ -44,6 +44,9 @@ struct JProj {
 
 void trl_solve(DeviceHam& h, const SolveParams& p, double* evals, double* evecs, SolveStats& st,
                const double* v0 = nullptr, const SolveCtx* ctx = nullptr, const JProj* jp = nullptr);
+// thick-restart block Lanczos with block size q (TRBL); v0: q start vectors (q*mdim) or null
+void trbl_solve(DeviceHam& h, int q, const SolveParams& p, double* evals, double* evecs, SolveStats& st,
+                const double* v0 = nullptr, const JProj* jp = nullptr);
 void restart_small_device(int device, const double* alpha, const double* beta, int m, int ls, double* S,
                           double* alpha_out, double* beta_out);
 void lanczos_steps(DeviceHam& h, const SolveParams& p, int nsteps, const double* v0, double* alpha, double* beta);

@@ -36,6 +36,7 @@
 #include <stdexcept>
 #include <string>
 
+#include "dense_eig.h"
 #include "sm_device.h"
 #include "sm_trl.h"
 #include "tridiag.h"
@@ -1122,6 +1123,376 @@ struct Solver {
     }
 };
 
+// ------------------------------------------------------------------ block TRL
+__global__ __launch_bounds__(TB) void k_scale_h(double* __restrict__ x, int64_t n, double a) {
+    for (int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TB) x[i] *= a;
+}
+// y += a x
+__global__ __launch_bounds__(TB) void k_axpy_h(double* __restrict__ y, const double* __restrict__ x, int64_t n,
+                                               double a) {
+    for (int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TB) y[i] += a * x[i];
+}
+
+// Thick-restart block Lanczos with block size q (TRBL, lanczos_methods.jl:312-420;
+// bl_ReORTH :422-433; bl_QR! shellmodel_main.jl:1049-1066; add_bl_T! :1201-1205;
+// bl_ThickRestart lanczos_methods.jl:563-617).  One block step:
+//   HV = H V_it  (q H·v);  R = V_it^T HV, lower triangle mirrored (diagonal block of T)
+//   HV <- HV - V R, re-orthogonalised against the whole basis (CGS2 per vector;
+//         the reference sweeps MGS over the earlier blocks)
+//   HV = V_{it+1} B  (MGS QR), B and B^T the next off-diagonal blocks of T.
+// T lives on the host in FP64; its lowest eigenvalues are checked after every
+// block with diagonalize_T!'s rule.  After lm/q - 1 blocks without convergence,
+// the basis restarts from the ls = q*(ls/q) lowest Ritz vectors and the last
+// block (arrowhead coupling B Y_last).
+// Rank loss: a QR column whose norm after re-orthogonalisation is at rounding
+// level (the block Krylov space is exhausted: the basis spans the space, or an
+// invariant subspace) is dropped; the remaining columns form a last, partial
+// block, after which the solve stops with T the exact projection on the span.
+// The reference normalises such columns as they are.
+template <typename TBv>
+struct BlockSolver {
+    DeviceHam& H;
+    const SolveParams& P;
+    SolveStats& st;
+    const JProj* jp;
+    const std::atomic<bool>* cancel = nullptr;
+    hipStream_t s;
+    int q, nbl, lsub, lsv, nev, lmT, ldp, gstore, prec_hv;
+    int64_t n, ld, Gd, Gx;
+    DBuf basis, hv, w, part, npart, h1, h2, rd, U, Sd, R, hv64, jz;
+    std::vector<double> T;      // lmT x lmT, row-major
+    std::vector<double> Bq;     // q x q coupling of the last full QR
+    DenseEig eig;
+
+    BlockSolver(DeviceHam& h, const SolveParams& p, SolveStats& stats, int qq, const JProj* j)
+        : H(h), P(p), st(stats), jp(j), q(qq) {
+        s = H.stream;
+        n = H.mdim;
+        ld = (n + 63) / 64 * 64;
+        nbl = P.lm / q;
+        lsub = P.ls / q;
+        lsv = q * lsub;
+        nev = (int)std::min<int64_t>(P.num_ev, n);
+        lmT = q * nbl;
+        ldp = lmT + 2;
+        prec_hv = sizeof(TBv) == 4 ? 1 : 0;
+        Gd = (n + CHUNK - 1) / CHUNK;
+        Gx = (n + GsShape<TBv>::SEG - 1) / GsShape<TBv>::SEG;
+        gstore = (int)std::min<int64_t>(4096, (n + TB - 1) / TB);
+    }
+    TBv* V(int j) { return basis.as<TBv>() + (int64_t)j * ld; }
+    TBv* HVb(int b) { return hv.as<TBv>() + (int64_t)b * ld; }
+    double* Wb(int b) { return w.as<double>() + (int64_t)b * ld; }
+    double& t(int i, int j) { return T[(size_t)i * lmT + j]; }
+
+    void check_args() {
+        if (q < 1 || q > 16) throw std::invalid_argument("block size q must be in 1..16");
+        if (lsub < 1 || lsv > 32) throw std::invalid_argument("need q <= ls and q*(ls/q) <= 32");
+        if (nev < 1 || nev > lsv) throw std::invalid_argument("need 1 <= num_ev <= q*(ls/q)");
+        if (nbl < lsub + 3) throw std::invalid_argument("need lm/q >= ls/q + 3");
+        if (lmT > 1000) throw std::invalid_argument("lm above 1000 is not supported");
+        if ((size_t)lsv * lmT * 8 > 64 * 1024) throw std::invalid_argument("ls*lm too large for the restart rotation");
+        if (n < q) throw std::invalid_argument("block size larger than the basis");
+    }
+
+    void alloc() {
+        basis.alloc((size_t)lmT * ld * sizeof(TBv));
+        hv.alloc((size_t)q * ld * sizeof(TBv));
+        w.alloc((size_t)q * ld * 8);
+        part.alloc((size_t)std::max(Gd, Gx) * ldp * 8);
+        npart.alloc((size_t)Gx * 8);
+        h1.alloc((size_t)q * ldp * 8);
+        h2.alloc((size_t)q * ldp * 8);
+        rd.alloc(64 * 8);
+        U.alloc((size_t)lsv * ld * sizeof(TBv));
+        Sd.alloc((size_t)lmT * 32 * 8);
+        R.alloc((size_t)nev * ld * 8);
+        if (prec_hv == 1 && P.refine) hv64.alloc((size_t)ld * 8);
+        if (jp) jz.alloc((size_t)ld * 8);
+    }
+
+    // out[0] = a . b  (device scalar)
+    void dot_to(const double* a, const double* b, double* out) {
+        const int G2 = (int)std::min<int64_t>(1024, Gd);
+        hipLaunchKernelGGL((k_dot2<double, double>), dim3(G2), dim3(TB), 0, s, a, b, n, part.as<double>(), ldp);
+        hipLaunchKernelGGL(k_reduce, dim3(1), dim3(TB), 0, s, part.as<double>(), G2, ldp, out, (const int*)nullptr);
+    }
+
+    // wout = win - V c1 - V c2 with c1 = V^T win, c2 = V^T (win - V c1) over V[0..nv-1]
+    // (CGS2, three streaming passes); c1 -> hh1, c2 -> hh2
+    template <typename TW>
+    void cgs2_into(int nv, const TW* win, double* wout, double* hh1, double* hh2) {
+        double* pt = part.as<double>();
+        hipLaunchKernelGGL((k_gs_dots<TBv, TW, false>), dim3(Gx), dim3(TB), 0, s, (const TBv*)V(0), ld, nv, win, n,
+                           pt, ldp, (const int*)nullptr, (const TBv*)nullptr, (const TBv*)nullptr,
+                           (const double*)nullptr, (const double*)nullptr, (double*)nullptr, (double*)nullptr);
+        hipLaunchKernelGGL(k_reduce, dim3(nv), dim3(TB), 0, s, pt, (int)Gx, ldp, hh1, (const int*)nullptr);
+        hipLaunchKernelGGL((k_gs_upd<TBv, TW, true>), dim3(Gx), dim3(TB), 0, s, (const TBv*)V(0), ld, nv,
+                           (const double*)hh1, win, wout, n, pt, ldp, npart.as<double>(), (const int*)nullptr);
+        hipLaunchKernelGGL(k_reduce, dim3(nv), dim3(TB), 0, s, pt, (int)Gx, ldp, hh2, (const int*)nullptr);
+        hipLaunchKernelGGL((k_gs_upd<TBv, double, false>), dim3(Gx), dim3(TB), 0, s, (const TBv*)V(0), ld, nv,
+                           (const double*)hh2, (const double*)wout, wout, n, pt, ldp, npart.as<double>(),
+                           (const int*)nullptr);
+    }
+
+    // MGS QR of Wb(0..qe-1) in place (bl_QR!): Rq (q x q, row-major) gets the
+    // coefficients; a column whose norm is <= thr is dropped.  Returns the kept columns.
+    std::vector<int> qr_block(int qe, double thr, std::vector<double>& Rq) {
+        Rq.assign((size_t)q * q, 0.0);
+        std::vector<int> ok;
+        double* r = rd.as<double>();
+        std::vector<double> d(q);
+        for (int i = 0; i < qe; i++) {
+            for (int k = i; k < qe; k++) dot_to(Wb(i), Wb(k), r + k);
+            SM_HIP(hipMemcpyAsync(d.data() + i, r + i, (qe - i) * 8, hipMemcpyDeviceToHost, s));
+            SM_HIP(hipStreamSynchronize(s));
+            const double nr = std::sqrt(std::max(d[i], 0.0));
+            if (!(nr > thr)) continue;          // dropped (also NaN)
+            Rq[(size_t)i * q + i] = nr;
+            hipLaunchKernelGGL(k_scale_h, dim3(gstore), dim3(TB), 0, s, Wb(i), n, 1.0 / nr);
+            for (int k = i + 1; k < qe; k++) {
+                const double c = d[k] / nr;
+                Rq[(size_t)i * q + k] = c;
+                hipLaunchKernelGGL(k_axpy_h, dim3(gstore), dim3(TB), 0, s, Wb(k), (const double*)Wb(i), n, -c);
+            }
+            ok.push_back(i);
+        }
+        return ok;
+    }
+
+    // J projection of the FP64 vector y in place (the exact form of bl_JJ_Lanczos'
+    // filter, lanczos_methods.jl:704-760): skipped when <J^2> is within tol of J(J+1)
+    // unless forced.  Returns whether it projected.
+    bool jproject(double* y, bool force) {
+        const double hJ = 0.5 * jp->twoJ, cJ = hJ * (hJ + 1.0);
+        double* z = jz.as<double>();
+        if (!force) {
+            launch_hv(*jp->j2, 0, y, z, s);
+            double* r = rd.as<double>();
+            dot_to(y, z, r + 40);
+            dot_to(y, y, r + 41);
+            double e[2];
+            SM_HIP(hipMemcpyAsync(e, r + 40, 16, hipMemcpyDeviceToHost, s));
+            SM_HIP(hipStreamSynchronize(s));
+            if (std::fabs(e[0] / e[1] - cJ) <= jp->tol) return false;
+        }
+        std::vector<int> two;
+        for (int t2 = jp->twoJ + 2; t2 <= jp->twoJmax; t2 += 2) two.push_back(t2);
+        std::vector<int> order;
+        for (size_t lo = 0, hi = two.size(); lo < hi;) {
+            order.push_back(two[--hi]);
+            if (lo < hi) order.push_back(two[lo++]);
+        }
+        for (int t2 : order) {
+            const double hp = 0.5 * t2, c = hp * (hp + 1.0);
+            launch_hv(*jp->j2, 0, y, z, s);
+            hipLaunchKernelGGL(k_jfactor, dim3(gstore), dim3(TB), 0, s, y, (const double*)z, n, c, 1.0 / (cJ - c));
+        }
+        st.n_jproj++;
+        return true;
+    }
+
+    // lowest min(nev, k) eigenvalues of T[0..k-1] into cur (entries past keep prev);
+    // returns diagonalize_T!'s convergence flag all(prev - cur < tol)
+    bool converged(int k, std::vector<double>& prev, std::vector<double>& cur) {
+        eig.reduce(T.data(), lmT, k);
+        const int m = std::min(nev, k);
+        cur = prev;
+        eig.lowest(m, cur.data());
+        bool ok = true;
+        for (int i = 0; i < nev; i++) ok = ok && (prev[i] - cur[i] < P.tol);
+        prev = cur;
+        return ok;
+    }
+
+    void store_block(int j0, const std::vector<int>& cols) {
+        for (size_t i = 0; i < cols.size(); i++)
+            hipLaunchKernelGGL((k_convert<double, TBv>), dim3(gstore), dim3(TB), 0, s, (const double*)Wb(cols[i]),
+                               V(j0 + (int)i), n, 1.0);
+    }
+
+    void start_block(const double* v0) {
+        for (int b = 0; b < q; b++) {
+            if (v0) SM_HIP(hipMemcpyAsync(Wb(b), v0 + (int64_t)b * n, n * 8, hipMemcpyHostToDevice, s));
+            else hipLaunchKernelGGL(k_randn, dim3(gstore), dim3(TB), 0, s, Wb(b), n,
+                                    (uint64_t)P.seed + 0x9E3779B97F4A7C15ull * (uint64_t)b);
+            if (jp) jproject(Wb(b), true);
+        }
+        std::vector<double> Rq;
+        auto ok = qr_block(q, 1e-10, Rq);
+        if ((int)ok.size() < q)
+            throw std::invalid_argument(jp ? "start block has too few components with the requested J"
+                                           : "start vectors are linearly dependent");
+        store_block(0, ok);
+    }
+
+    void restart() {
+        const int m = q * (nbl - 1);
+        eig.reduce(T.data(), lmT, m);
+        std::vector<double> vals(lsv), Y((size_t)lsv * m);
+        eig.lowest_pairs(lsv, vals.data(), Y.data());
+        std::vector<double> r((size_t)q * lsv, 0.0);     // r = B Y[m-q..m-1, :]
+        for (int i = 0; i < q; i++)
+            for (int k = 0; k < lsv; k++) {
+                double a = 0.0;
+                for (int c = 0; c < q; c++) a += Bq[(size_t)i * q + c] * Y[(size_t)k * m + (m - q + c)];
+                r[(size_t)i * lsv + k] = a;
+            }
+        std::fill(T.begin(), T.end(), 0.0);
+        for (int k = 0; k < lsv; k++) t(k, k) = vals[k];
+        for (int i = 0; i < q; i++)
+            for (int k = 0; k < lsv; k++) t(lsv + i, k) = t(k, lsv + i) = r[(size_t)i * lsv + k];
+        // kept Ritz vectors U = V[0..m-1] Y, then the new block moves behind them
+        SM_HIP(hipMemcpyAsync(Sd.p, Y.data(), Y.size() * 8, hipMemcpyHostToDevice, s));
+        const int gb = (int)std::min<int64_t>(2048, (n + TB - 1) / TB);
+        hipLaunchKernelGGL((k_rotate<TBv, TBv>), dim3(gb), dim3(TB), (size_t)lsv * m * 8, s, (const TBv*)V(0), ld, m,
+                           (const double*)Sd.as<double>(), lsv, U.as<TBv>(), ld, n);
+        SM_HIP(hipMemcpyAsync(V(lsv), V(m), (size_t)q * ld * sizeof(TBv), hipMemcpyDeviceToDevice, s));
+        SM_HIP(hipMemcpyAsync(V(0), U.p, (size_t)lsv * ld * sizeof(TBv), hipMemcpyDeviceToDevice, s));
+        // blocks 1..ls/q re-orthonormalised against the earlier ones (bl_ThickRestart's final sweep)
+        std::vector<double> Rq;
+        for (int b = 1; b <= lsub; b++) {
+            for (int i = 0; i < q; i++)
+                cgs2_into<TBv>(b * q, (const TBv*)V(b * q + i), Wb(i), h1.as<double>(), h2.as<double>());
+            auto ok = qr_block(q, 0.0, Rq);
+            if ((int)ok.size() < q) throw std::runtime_error("TRBL restart: kept block lost rank");
+            store_block(b * q, ok);
+        }
+        st.n_restart++;
+    }
+
+    void run(double* evals, double* evecs, const double* v0) {
+        auto t0 = std::chrono::steady_clock::now();
+        check_args();
+        alloc();
+        T.assign((size_t)lmT * lmT, 0.0);
+        Bq.assign((size_t)q * q, 0.0);
+        st.n_hv = 0; st.n_restart = 0;
+        start_block(v0);
+        std::vector<double> prev(nev, 1.0e4), cur(nev, 1.0e4);
+        const double drop = prec_hv ? 1e-5 : 1e-10;   // rank-loss threshold relative to max|T|
+        int itmin = 0, qe = q, kfin = 0;
+        int64_t total = 0;
+        bool done = false;
+        while (!done) {
+            for (int it = itmin; it < nbl - 1 || (qe < q && it < nbl); it++) {
+                if (cancel && cancel->load(std::memory_order_relaxed)) throw Canceled();
+                total += qe;
+                if (total > P.max_iter) throw std::runtime_error("TRBL: iteration limit reached");
+                const int j0 = it * q;
+                for (int b = 0; b < qe; b++) {
+                    launch_hv(H, prec_hv, V(j0 + b), HVb(b), s);
+                    st.n_hv++;
+                }
+                const int k = j0 + qe;
+                for (int b = 0; b < qe; b++)
+                    cgs2_into<TBv>(k, (const TBv*)HVb(b), Wb(b), h1.as<double>() + (int64_t)b * ldp,
+                                   h2.as<double>() + (int64_t)b * ldp);
+                std::vector<double> c1((size_t)qe * qe), c2((size_t)qe * qe);
+                for (int b = 0; b < qe; b++) {
+                    SM_HIP(hipMemcpyAsync(c1.data() + (size_t)b * qe, h1.as<double>() + (int64_t)b * ldp + j0, qe * 8,
+                                          hipMemcpyDeviceToHost, s));
+                    SM_HIP(hipMemcpyAsync(c2.data() + (size_t)b * qe, h2.as<double>() + (int64_t)b * ldp + j0, qe * 8,
+                                          hipMemcpyDeviceToHost, s));
+                }
+                SM_HIP(hipStreamSynchronize(s));
+                // Rr[i][j] = v_i . H v_j ; the diagonal block takes the lower triangle
+                auto Rr = [&](int i, int j) { return c1[(size_t)j * qe + i] + c2[(size_t)j * qe + i]; };
+                for (int i = 0; i < qe; i++)
+                    for (int j = 0; j < qe; j++) t(j0 + i, j0 + j) = i >= j ? Rr(i, j) : Rr(j, i);
+                const bool conv = converged(k, prev, cur);
+                if (qe < q || k >= n) { kfin = k; done = true; break; }      // space exhausted
+                double scale = 1.0;
+                for (int i = 0; i < k; i++)
+                    for (int j = std::max(0, i - 2 * q); j <= i; j++) scale = std::max(scale, std::fabs(t(i, j)));
+                std::vector<double> Rq;
+                auto ok = qr_block(qe, drop * scale, Rq);
+                // src: columns of Wb to store (in order); rows: their rows of Rq
+                std::vector<int> src = ok, rows = ok;
+                if (jp && !ok.empty()) {
+                    // doubleLanczos: stop when the new block vanishes, else filter it on J
+                    double bn = 0.0;
+                    for (int i = 0; i < q; i++) bn = std::max(bn, std::fabs(Rq[(size_t)i * q + i]));
+                    if (bn < jp->tol) ok.clear();
+                    bool any = false;
+                    for (int i : ok) any = jproject(Wb(i), false) || any;
+                    src = ok; rows = ok;
+                    if (any) {
+                        // filtered columns go back into the complement of the span with unit
+                        // norm (compacted to Wb(0..)); one that keeps less than J_EXHAUSTED of
+                        // its norm was noise outside the (exhausted) J space and is dropped
+                        for (size_t a = 0; a < ok.size(); a++)
+                            cgs2_into<double>(k, (const double*)Wb(ok[a]), Wb((int)a), h1.as<double>(),
+                                              h2.as<double>());
+                        std::vector<double> Rj;
+                        src = qr_block((int)ok.size(), J_EXHAUSTED, Rj);
+                        rows.clear();
+                        for (int a : src) rows.push_back(ok[a]);
+                    }
+                }
+                if ((int64_t)src.size() > n - k) { src.resize(n - k); rows.resize(n - k); }
+                if (src.empty()) { kfin = k; done = true; break; }
+                // coupling blocks (add_bl_T!): T[k+i][j0+c] = T[j0+c][k+i] = B[row_i][c]
+                for (size_t i2 = 0; i2 < src.size(); i2++)
+                    for (int c = 0; c < q; c++)
+                        t(k + (int)i2, j0 + c) = t(j0 + c, k + (int)i2) = Rq[(size_t)rows[i2] * q + c];
+                store_block(k, src);
+                if (conv) { kfin = k; done = true; break; }
+                if ((int)src.size() < q) qe = (int)src.size();   // last, partial block
+                else Bq = Rq;
+            }
+            if (done) break;
+            restart();
+            itmin = lsub;
+        }
+        SM_HIP(hipStreamSynchronize(s));
+        st.iterations = kfin;
+        for (int i = 0; i < nev; i++) { evals[i] = cur[i]; st.ritz[i] = cur[i]; }
+        if (kfin < nev) throw std::invalid_argument("the Krylov space holds fewer states than num_ev");
+        finish(kfin, evals, evecs);
+        st.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+
+    // Ritz vectors of T[0..k-1] (main_sm's Rvecs for is_block, shellmodel_main.jl:211-224),
+    // FP64 Rayleigh quotients with the FP64 H·v when H·v ran in FP32
+    void finish(int k, double* evals, double* evecs) {
+        const bool refine = prec_hv == 1 && P.refine;
+        if (!evecs && !refine) return;
+        if ((size_t)nev * k * 8 > 64 * 1024) throw std::invalid_argument("num_ev*basis too large for the Ritz assembly");
+        eig.reduce(T.data(), lmT, k);
+        std::vector<double> vals(nev), Y((size_t)nev * k);
+        eig.lowest_pairs(nev, vals.data(), Y.data());
+        SM_HIP(hipMemcpyAsync(Sd.p, Y.data(), Y.size() * 8, hipMemcpyHostToDevice, s));
+        const int gb = (int)std::min<int64_t>(2048, (n + TB - 1) / TB);
+        hipLaunchKernelGGL((k_rotate<TBv, double>), dim3(gb), dim3(TB), (size_t)nev * k * 8, s, (const TBv*)V(0), ld,
+                           k, (const double*)Sd.as<double>(), nev, R.as<double>(), ld, n);
+        double* r = rd.as<double>();
+        for (int i = 0; i < nev; i++) {
+            double* x = R.as<double>() + (int64_t)i * ld;
+            dot_to(x, x, r + 48);
+            double nr2 = 0.0;
+            SM_HIP(hipMemcpyAsync(&nr2, r + 48, 8, hipMemcpyDeviceToHost, s));
+            SM_HIP(hipStreamSynchronize(s));
+            hipLaunchKernelGGL(k_scale_h, dim3(gstore), dim3(TB), 0, s, x, n, 1.0 / std::sqrt(nr2));
+            if (refine) {
+                launch_hv(H, 0, x, hv64.p, s);
+                st.n_hv64++;
+                dot_to(x, hv64.as<double>(), r + 49);
+                double rq = 0.0;
+                SM_HIP(hipMemcpyAsync(&rq, r + 49, 8, hipMemcpyDeviceToHost, s));
+                SM_HIP(hipStreamSynchronize(s));
+                evals[i] = rq;
+            }
+        }
+        if (refine) std::sort(evals, evals + nev);
+        if (evecs)
+            for (int i = 0; i < nev; i++)
+                SM_HIP(hipMemcpyAsync(evecs + (int64_t)i * n, R.as<double>() + (int64_t)i * ld, n * 8,
+                                      hipMemcpyDeviceToHost, s));
+        SM_HIP(hipStreamSynchronize(s));
+    }
+};
+
 }  // namespace
 
 int64_t solve_workspace_bytes(int64_t mdim, const SolveParams& p) {
@@ -1172,6 +1543,20 @@ void trl_solve(DeviceHam& h, const SolveParams& p, double* evals, double* evecs,
 }
 
 // device run of the restart small problem (tests compare it with the host build)
+void trbl_solve(DeviceHam& h, int q, const SolveParams& p, double* evals, double* evecs, SolveStats& st,
+                const double* v0, const JProj* jp) {
+    SM_HIP(hipSetDevice(h.device));
+    if (jp && (jp->j2 == nullptr || jp->j2->mdim != h.mdim))
+        throw std::invalid_argument("J^2 handle does not match the Hamiltonian's basis");
+    if (p.precision == 0) {
+        BlockSolver<double> S(h, p, st, q, jp);
+        S.run(evals, evecs, v0);
+    } else {
+        BlockSolver<float> S(h, p, st, q, jp);
+        S.run(evals, evecs, v0);
+    }
+}
+
 void restart_small_device(int device, const double* alpha, const double* beta, int m, int ls, double* S,
                           double* alpha_out, double* beta_out) {
     SM_HIP(hipSetDevice(device));

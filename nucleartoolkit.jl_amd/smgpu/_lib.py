@@ -101,12 +101,14 @@ SIGNATURES = {
     "smgpu_trl_solve": (I32, [P, P, P, P, P]),
     "smgpu_trl_solve_v0": (I32, [P, P, P, P, P, P]),
     "smgpu_trl_solve_j": (I32, [P, P, I32, P, P, P, P, P]),
+    "smgpu_trbl_solve": (I32, [P, P, I32, I32, P, P, P, P, P]),
     "smgpu_lanczos_steps": (I32, [P, I32, I32, P, P, P]),
     "smgpu_malloc": (I32, [P, I64, P]),
     "smgpu_free": (I32, [P, P]),
     "smgpu_memcpy": (I32, [P, P, P, I64, I32]),
     "smgpu_synchronize": (I32, [P]),
     "smgpu_selftest_tridiag": (I32, [P, P, I32, I32, P, P]),
+    "smgpu_selftest_symeig": (I32, [P, I32, I32, P, P]),
     "smgpu_selftest_restart": (I32, [I32, P, P, I32, I32, P, P, P]),
     "smgpu_default_server_config": (None, [P]),
     "smgpu_server_create": (I32, [P, P]),

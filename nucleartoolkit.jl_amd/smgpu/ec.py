@@ -7,10 +7,11 @@ Mirrors prepEC's "sample" mode and solveEC
   sample interaction on one M-scheme basis (2M = tJ, as prepEC's Mtot = tJ).
   The index tables are built and uploaded once, and only the coefficients are
   swapped per interaction (smgpu_problem_set_interaction /
-  smgpu_ham_update_interaction).  The Ritz vectors with J = tJ/2 (from <J^2>)
-  are kept as the EC basis.  The reference reaches J = tJ/2 by J-projected
-  Lanczos; the filter here keeps the same states among the lowest num_ev.
-  Optionally each sample is written as a .wav file like prepEC's save_wav.
+  smgpu_ham_update_interaction).  With tJ given, every solve is J-projected
+  Lanczos (smgpu_trl_solve_j, as prepEC's TRL with doubleLanczos), so the
+  num_ev lowest states of J = tJ/2 are the EC basis; without it, the lowest
+  num_ev states of 2M = A % 2 are.  Optionally each sample is written as a
+  .wav file like prepEC's save_wav.
 * `solve_ec(Hs, target_nuc, samples, num_ev_target)` builds, for every target
   interaction, H_ij = <psi_i|H|psi_j> and N_ij = <psi_i|psi_j> and solves
   H c = E N c through the Cholesky factor of N, with solveEC's epsilon shift
@@ -47,13 +48,15 @@ def prep_ec(Hs, target_nuc, num_ev, tJ=None, *, lm=300, ls=20, tol=1e-8, precisi
     prob = Problem(Hs[0], target_nuc, twoM=-1 if tJ is None else int(tJ))
     twoM = int(prob.info.twoM)
     ham = Hamiltonian(prob, device)
+    j2 = prob.j2_hamiltonian(device) if tJ is not None else None
     vecs, src, energies, Js = [], [], [], []
     for k, sntf in enumerate(Hs):
         if k > 0:
             prob.set_interaction(sntf)
             ham.update_interaction(prob)
         nev = min(int(num_ev), prob.mdim)
-        ev, R, _ = ham.trl(nev, lm=lm, ls=ls, tol=tol, precision=precision, seed=seed, want_vectors=True)
+        ev, R, _ = ham.trl(nev, lm=lm, ls=ls, tol=tol, precision=precision, seed=seed, want_vectors=True,
+                           j2=j2, twoJ=None if tJ is None else int(tJ))
         J = np.array([j_from_jj1(x) for x in expect_j2(prob, R, device)])
         energies.append(ev)
         Js.append(J)

@@ -181,6 +181,33 @@ class Hamiltonian:
                      n_jproj=st.n_jproj)
         return ev, vec, stats
 
+    def trbl(self, num_ev, q, lm=300, ls=20, tol=1e-8, num_history=3, precision="fp64", refine=True, seed=1234,
+             want_vectors=False, max_iter=0, v0=None, j2=None, twoJ=None):
+        """Thick-restart block Lanczos with block size q (TRBL); returns (evals,
+        evecs or None, stats dict).  v0: q start vectors [q, mdim].  With j2 and
+        twoJ: J-projected block Lanczos, the lowest states with J = twoJ/2."""
+        p = SolveParams()
+        lib().smgpu_default_params(ctypes.byref(p))
+        p.num_ev = int(num_ev); p.lm = int(lm); p.ls = int(ls); p.tol = float(tol)
+        p.num_history = int(num_history); p.precision = _prec(precision); p.refine = int(bool(refine))
+        p.seed = int(seed); p.max_iter = int(max_iter)
+        nev = min(int(num_ev), self.mdim)
+        ev = np.zeros(nev, np.float64)
+        vec = np.zeros((nev, self.mdim), np.float64) if want_vectors else None
+        st = SolveStats()
+        v0p = None
+        if v0 is not None:
+            v0 = np.ascontiguousarray(np.atleast_2d(v0), dtype=np.float64)
+            assert v0.shape == (int(q), self.mdim)
+            v0p = ptr(v0)
+        check(lib().smgpu_trbl_solve(self._h, j2._h if j2 is not None else None, int(twoJ or 0), int(q),
+                                     ctypes.byref(p), v0p, ptr(ev), ptr(vec) if vec is not None else None,
+                                     ctypes.byref(st)))
+        stats = dict(iterations=st.iterations, n_hv=st.n_hv, n_hv64=st.n_hv64, n_restart=st.n_restart,
+                     wall_ms=st.wall_ms, ritz=np.array(st.ritz[:nev]), n_reorth2=st.n_reorth2,
+                     n_jproj=st.n_jproj)
+        return ev, vec, stats
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
@@ -206,7 +233,7 @@ def expect_j2(problem: Problem, vecs, device=0):
 
 def main_sm(sntf, target_nuc, num_ev, target_J=(), *, lm=300, ls=20, tol=1e-8, num_history=3,
             precision="fp64", refine=True, device=0, seed=1234, return_stats=False, return_J=False,
-            save_wav=False, wav_path=None, in_wf=""):
+            save_wav=False, wav_path=None, in_wf="", q=1, is_block=False):
     """Lowest `num_ev` eigenvalues (MeV) of `target_nuc` with the interaction `sntf`.
 
     Mirrors main_sm (shellmodel_main.jl:89-281): with target_J = [] the plain
@@ -217,6 +244,8 @@ def main_sm(sntf, target_nuc, num_ev, target_J=(), *, lm=300, ls=20, tol=1e-8, n
     reference's .wav layout (writeRitzvecs, :1215-1229), by default to
     ./<nucleus>_<interaction>.wav (:241-246; ..._j<tJ>.wav with target_J).  in_wf
     starts the Lanczos from the first vector of a .wav file (read_appwav, :194-198).
+    is_block=True runs the thick-restart block Lanczos with block size q instead
+    (TRBL, :167-191; in_wf then supplies the first q vectors).
     """
     if len(target_J) > 1:
         raise ValueError("Multiple J is not supported (main_sm asserts length(target_J) <= 1)")
@@ -230,10 +259,14 @@ def main_sm(sntf, target_nuc, num_ev, target_J=(), *, lm=300, ls=20, tol=1e-8, n
     v0 = None
     if in_wf:
         from .wav import read_appwav
-        v0 = read_appwav(in_wf, prob.mdim)
+        v0 = read_appwav(in_wf, prob.mdim, int(q) if is_block else 1)
     j2 = prob.j2_hamiltonian(device) if tJ is not None else None
-    ev, vecs, st = ham.trl(nev, lm=lm, ls=ls, tol=tol, num_history=num_history, precision=precision,
-                           refine=refine, seed=seed, want_vectors=need_vecs, v0=v0, j2=j2, twoJ=tJ)
+    if is_block:
+        ev, vecs, st = ham.trbl(nev, int(q), lm=lm, ls=ls, tol=tol, num_history=num_history, precision=precision,
+                                refine=refine, seed=seed, want_vectors=need_vecs, v0=v0, j2=j2, twoJ=tJ)
+    else:
+        ev, vecs, st = ham.trl(nev, lm=lm, ls=ls, tol=tol, num_history=num_history, precision=precision,
+                               refine=refine, seed=seed, want_vectors=need_vecs, v0=v0, j2=j2, twoJ=tJ)
     out = [ev]
     if need_vecs:
         totJs = np.array([j_from_jj1(x) for x in expect_j2(prob, vecs, device)])

@@ -137,6 +137,55 @@ def test_tridiag_solver():
     assert np.abs(vecs @ vecs.T - np.eye(12)).max() < 1e-10
 
 
+def _block_lanczos_like(rng, q, nb, lsv=0):
+    """Block tridiagonal (q x q blocks, upper-triangular couplings) with an
+    optional thick-restart arrowhead: diag(theta[lsv]) coupled to the next block."""
+    n = q * nb
+    T = np.zeros((n, n))
+    for b in range(nb):
+        A = rng.normal(size=(q, q)) * 5
+        T[b * q:(b + 1) * q, b * q:(b + 1) * q] = 0.5 * (A + A.T)
+        if b + 1 < nb:
+            B = np.triu(rng.normal(size=(q, q)))
+            T[(b + 1) * q:(b + 2) * q, b * q:(b + 1) * q] = B
+            T[b * q:(b + 1) * q, (b + 1) * q:(b + 2) * q] = B.T
+    if lsv:
+        T[:lsv, :] = 0; T[:, :lsv] = 0
+        T[np.arange(lsv), np.arange(lsv)] = np.sort(rng.normal(size=lsv) * 10)
+        r = rng.normal(size=(q, lsv))
+        T[lsv:lsv + q, :lsv] = r; T[:lsv, lsv:lsv + q] = r.T
+    return T
+
+
+def test_dense_symeig_host():
+    """Host eigen-solver of the block Lanczos matrix (Householder + tridiagonal
+    bisection / inverse iteration) against LAPACK (numpy.linalg.eigh): block
+    tridiagonal and restarted (arrowhead) shapes, a dense random matrix, an
+    already-tridiagonal one, a 1x1, and a matrix with a repeated eigenvalue."""
+    rng = np.random.default_rng(17)
+    cases = [_block_lanczos_like(rng, 2, 60), _block_lanczos_like(rng, 3, 40, lsv=21),
+             _block_lanczos_like(rng, 4, 12, lsv=8), np.array([[3.5]])]
+    A = rng.normal(size=(70, 70)); cases.append(A + A.T)
+    cases.append(np.diag(rng.normal(size=50)) + np.diag(np.full(49, 0.7), 1) + np.diag(np.full(49, 0.7), -1))
+    Q, _ = np.linalg.qr(rng.normal(size=(30, 30)))
+    d = np.concatenate([[-5.0, -5.0, -5.0], np.linspace(-4, 4, 27)])
+    cases.append(Q @ np.diag(d) @ Q.T)
+    for T in cases:
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        n = T.shape[0]
+        k = min(n, 12)
+        ev = np.linalg.eigvalsh(T)
+        scale = max(1.0, np.abs(ev).max())
+        vals = np.zeros(k)
+        assert lib().smgpu_selftest_symeig(ptr(T), n, k, ptr(vals), None) == 0
+        assert np.abs(vals - ev[:k]).max() < 1e-12 * scale * max(1, n / 10)
+        vecs = np.zeros((k, n))
+        assert lib().smgpu_selftest_symeig(ptr(T), n, k, ptr(vals), ptr(vecs)) == 0
+        assert np.abs(vecs @ vecs.T - np.eye(k)).max() < 1e-10
+        res = T @ vecs.T - vecs.T * vals
+        assert np.abs(res).max() < 1e-10 * scale
+
+
 def test_restart_small_problem_host():
     """tri_restart: S orthonormal, S^T T S tridiagonal with the returned entries,
     coupling only on the last kept vector, eigenvalues preserved."""
