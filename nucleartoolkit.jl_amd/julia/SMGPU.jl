@@ -2,8 +2,8 @@
 #
 # The reference's seam is the body of `main_sm` (src/ShellModel/shellmodel_main.jl:89-281):
 # readsmsnt -> occ -> prep_Hamil_T1 / prep_2bjumps / prep_Hamil_pn / prep_pn -> TRL
-# -> Ritz assembly.  `SMGPU.main_sm` keeps that signature (target_J = [] only) and
-# returns main_sm's observable en[1]; `SMGPU.operate_H!` keeps operate_H!'s
+# -> Ritz assembly.  `SMGPU.main_sm` keeps that signature (target_J = [] or [2J],
+# is_block / q) and returns main_sm's observable en[1]; `SMGPU.operate_H!` keeps operate_H!'s
 # accumulate-into-twf contract (src/ShellModel/lanczos_methods.jl:153-248).
 # Errors come back as Julia exceptions with the library's message.
 module SMGPU
@@ -84,6 +84,15 @@ function Ham(p::Problem; device::Integer=0)
     return h
 end
 
+# J^2 on the problem's basis (prep_J / operate_J!, shellmodel_main.jl:149-157, 226-230)
+function HamJ2(p::Problem; device::Integer=0)
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:smgpu_ham_create_j2, libsmgpu), Int32, (Ptr{Cvoid}, Int32, Ref{Ptr{Cvoid}}), p.h, device, out))
+    h = Ham(out[], info(p).mdim)
+    finalizer(x -> ccall((:smgpu_ham_destroy, libsmgpu), Int32, (Ptr{Cvoid},), x.h), h)
+    return h
+end
+
 # twf += H wf  (operate_H!'s contract; the device computes H wf in `precision`)
 function operate_H!(wf::Vector{Float64}, twf::Vector{Float64}, H::Ham; precision::Int32=FP64)
     @assert length(wf) == H.mdim && length(twf) == H.mdim
@@ -100,9 +109,11 @@ function default_params()
     return r[]
 end
 
-# TRL + Ritz assembly; returns (en[1], Ritz vectors as mdim x num_ev or nothing, stats)
+# TRL + Ritz assembly; returns (en[1], Ritz vectors as mdim x num_ev or nothing, stats).
+# j2 (HamJ2 on the same basis, built with twoM = twoJ) and twoJ: J-projected TRL (doubleLanczos).
 function trl(H::Ham, num_ev::Integer; lm=300, ls=20, tol=1e-8, num_history=3, precision::Int32=MIXED,
-             refine::Bool=true, seed::Integer=1234, vectors::Bool=false, in_wf=nothing)
+             refine::Bool=true, seed::Integer=1234, vectors::Bool=false, in_wf=nothing,
+             j2::Union{Ham,Nothing}=nothing, twoJ::Integer=0)
     d = default_params()
     p = SolveParams(num_ev, lm, ls, num_history, tol, precision, refine, d.max_iter, UInt64(seed))
     nev = min(num_ev, H.mdim)
@@ -111,22 +122,61 @@ function trl(H::Ham, num_ev::Integer; lm=300, ls=20, tol=1e-8, num_history=3, pr
     st = Ref{SolveStats}()
     v0 = in_wf === nothing ? C_NULL : pointer(in_wf)
     GC.@preserve in_wf evecs begin
-        check(ccall((:smgpu_trl_solve_v0, libsmgpu), Int32,
-                    (Ptr{Cvoid}, Ref{SolveParams}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ref{SolveStats}),
-                    H.h, p, v0, evals, evecs === nothing ? C_NULL : pointer(evecs), st))
+        if j2 === nothing
+            check(ccall((:smgpu_trl_solve_v0, libsmgpu), Int32,
+                        (Ptr{Cvoid}, Ref{SolveParams}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ref{SolveStats}),
+                        H.h, p, v0, evals, evecs === nothing ? C_NULL : pointer(evecs), st))
+        else
+            check(ccall((:smgpu_trl_solve_j, libsmgpu), Int32,
+                        (Ptr{Cvoid}, Ptr{Cvoid}, Int32, Ref{SolveParams}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+                         Ref{SolveStats}),
+                        H.h, j2.h, twoJ, p, v0, evals, evecs === nothing ? C_NULL : pointer(evecs), st))
+        end
     end
     return evals, evecs, st[]
 end
 
-# main_sm(sntf, target_nuc, num_ev, target_J; ...) for target_J = []: en[1]
+# Thick-restart block Lanczos (TRBL) with block size q; in_wf: mdim x q start block
+function trbl(H::Ham, num_ev::Integer, q::Integer; lm=300, ls=20, tol=1e-8, num_history=3,
+              precision::Int32=MIXED, refine::Bool=true, seed::Integer=1234, vectors::Bool=false,
+              in_wf=nothing, j2::Union{Ham,Nothing}=nothing, twoJ::Integer=0)
+    d = default_params()
+    p = SolveParams(num_ev, lm, ls, num_history, tol, precision, refine, d.max_iter, UInt64(seed))
+    nev = min(num_ev, H.mdim)
+    evals = zeros(Float64, nev)
+    evecs = vectors ? zeros(Float64, H.mdim, nev) : nothing
+    st = Ref{SolveStats}()
+    in_wf === nothing || @assert size(in_wf) == (H.mdim, q)
+    v0 = in_wf === nothing ? C_NULL : pointer(in_wf)
+    GC.@preserve in_wf evecs begin
+        check(ccall((:smgpu_trbl_solve, libsmgpu), Int32,
+                    (Ptr{Cvoid}, Ptr{Cvoid}, Int32, Int32, Ref{SolveParams}, Ptr{Float64}, Ptr{Float64},
+                     Ptr{Float64}, Ref{SolveStats}),
+                    H.h, j2 === nothing ? C_NULL : j2.h, twoJ, q, p, v0, evals,
+                    evecs === nothing ? C_NULL : pointer(evecs), st))
+    end
+    return evals, evecs, st[]
+end
+
+# main_sm(sntf, target_nuc, num_ev, target_J; ...): en[1].  target_J = [2J] runs the
+# J-projected solve at 2M = 2J (:109-116); is_block=true the block Lanczos with block size q.
 function main_sm(sntf, target_nuc, num_ev, target_J; num_history=3, lm=300, ls=20, tol=1e-8,
-                 precision::Int32=MIXED, device::Integer=0)
-    isempty(target_J) || error("target_J != [] (J projection) is not served by the GPU path")
-    prob = Problem(sntf, target_nuc)
+                 precision::Int32=MIXED, device::Integer=0, q::Integer=1, is_block::Bool=false)
+    length(target_J) <= 1 || error("Multiple J is not supported")
+    tJ = isempty(target_J) ? -1 : Int(target_J[1])
+    prob = Problem(sntf, target_nuc; twoM=tJ)
     mdim = info(prob).mdim
     mdim == 0 && return Float64[]
     H = Ham(prob; device=device)
-    en, _, _ = trl(H, min(num_ev, mdim); lm=lm, ls=ls, tol=tol, num_history=num_history, precision=precision)
+    j2 = tJ < 0 ? nothing : HamJ2(prob; device=device)
+    nev = min(num_ev, mdim)
+    if is_block
+        en, _, _ = trbl(H, nev, q; lm=lm, ls=ls, tol=tol, num_history=num_history, precision=precision,
+                        j2=j2, twoJ=max(tJ, 0))
+    else
+        en, _, _ = trl(H, nev; lm=lm, ls=ls, tol=tol, num_history=num_history, precision=precision,
+                       j2=j2, twoJ=max(tJ, 0))
+    end
     return en
 end
 
