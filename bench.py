@@ -284,6 +284,10 @@ def main():
     t0 = time.perf_counter()
     ev1, _, st1 = ham.trl(nev, lm=lm, ls=ls, precision="mixed")
     mixed_single_ms = (time.perf_counter() - t0) * 1e3
+    # the same with a projection pass at every step (the reference's ReORTH schedule)
+    t0 = time.perf_counter()
+    evf, _, stf = ham.trl(nev, lm=lm, ls=ls, precision="mixed", reorth="full")
+    mixed_full_ms = (time.perf_counter() - t0) * 1e3
 
     # serving form: K requests through the persistent loop, M in flight
     from smgpu.server import Server
@@ -361,10 +365,16 @@ def main():
                        "solve_p50": percentile(solve_ms, 50), "prep_p50": percentile(prep_ms, 50),
                        "note": "submit to result; all K requests submitted at t=0"},
         "fp64": {"solves_per_s": fp64_rate, "latency_ms_p50": percentile(lat64, 50),
-                 "iterations": st64["iterations"], "restarts": st64["n_restart"], "n_hv": st64["n_hv"]},
+                 "iterations": st64["iterations"], "restarts": st64["n_restart"], "n_hv": st64["n_hv"],
+                 "n_reorth": st64["n_reorth"]},
         "speedup_vs_fp64": value / fp64_rate,
         "mixed_single": {"solves_per_s": 1e3 / mixed_single_ms, "iterations": st1["iterations"],
-                         "restarts": st1["n_restart"], "n_hv": st1["n_hv"], "n_hv64": st1["n_hv64"]},
+                         "restarts": st1["n_restart"], "n_hv": st1["n_hv"], "n_hv64": st1["n_hv64"],
+                         "n_reorth": st1["n_reorth"], "reorth": "partial (omega recurrence)",
+                         "max_abs_dE_vs_fp64_MeV": float(np.abs(np.array(ev1) - np.array(ev64)).max())},
+        "mixed_single_full_reorth": {"solves_per_s": 1e3 / mixed_full_ms, "iterations": stf["iterations"],
+                                     "n_reorth": stf["n_reorth"],
+                                     "max_abs_dE_vs_fp64_MeV": float(np.abs(np.array(evf) - np.array(ev64)).max())},
         "served": {"within_tol": n_ok, "requests": len(res), "max_abs_dE_vs_fp64_MeV": max(dE),
                    "peak_running": sstats["peak_running"], "hbm_peak_bytes": sstats["hbm_peak"],
                    "iterations": stats["iterations"]},

@@ -56,7 +56,19 @@ typedef struct {
     int32_t refine;      /* MIXED only: FP64 Rayleigh quotient of the Ritz vectors */
     int32_t max_iter;    /* safety limit on Lanczos iterations (0 = 100000) */
     uint64_t seed;       /* start vector seed */
+    int32_t reorth;      /* SMGPU_REORTH_*: re-orthogonalisation of the TRL basis */
+    int32_t reserved_;
 } smgpu_solve_params;
+
+/* Re-orthogonalisation of each new Lanczos vector against the stored basis:
+ * DEFAULT is FULL for FP64 (the reference's ReORTH at every step) and PARTIAL for
+ * MIXED.  PARTIAL follows the loss of orthogonality with Simon's recurrence for
+ * omega_jk = v_j . v_k and re-orthogonalises a step (and the next one) only when
+ * max |omega| exceeds sqrt(eps) of the basis precision, keeping the basis
+ * semi-orthogonal (H.D. Simon, Math. Comp. 42 (1984) 115; Lin. Alg. Appl. 61 (1984) 101). */
+#define SMGPU_REORTH_DEFAULT 0
+#define SMGPU_REORTH_FULL 1
+#define SMGPU_REORTH_PARTIAL 2
 
 typedef struct {
     int32_t iterations;  /* elit of the last Lanczos sweep */
@@ -67,6 +79,8 @@ typedef struct {
     double ritz[64];     /* en[1]: Ritz values of the last convergence check */
     int32_t n_reorth2;   /* iterations that needed a second Gram-Schmidt pass */
     int32_t n_jproj;     /* J-projected solves: vectors that were projected */
+    int32_t n_reorth;    /* iterations that ran a re-orthogonalisation pass */
+    int32_t reserved_;
 } smgpu_solve_stats;
 
 /* library */

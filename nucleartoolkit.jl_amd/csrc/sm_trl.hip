@@ -360,8 +360,14 @@ __global__ void k_finish(const double* h1, int it, const double* npart, int G, d
 
 // Daniel-Gragg-Kaufman-Stewart test after one CGS pass of the local-step vector:
 // loc[1] = |w'|^2, loc[2] = |w''|^2; a second pass is needed (gate = 1) when the
-// pass removed more than half of |w'|^2 (|w''| < |w'| / sqrt(2)).
-__global__ void k_dgks(const double* npart_after, const double* npart_before, int G, double* loc, int* gate) {
+// pass removed more than half of |w'|^2 (|w''| < |w'| / sqrt(2)).  first: the gate
+// of the first pass (null: it always ran); no first pass, no second.
+__global__ void k_dgks(const double* npart_after, const double* npart_before, int G, double* loc, int* gate,
+                       const int* first) {
+    if (first && *first == 0) {
+        if (threadIdx.x == 0) { gate[0] = 0; loc[4] = 0.0; }
+        return;
+    }
     __shared__ double red[TB / 64];
     double a = 0.0, b = 0.0;
     for (int g = threadIdx.x; g < G; g += TB) { a += npart_after[g]; b += npart_before[g]; }
@@ -390,6 +396,98 @@ __global__ void k_finish_local(double* loc, int it, const double* npart, int G, 
 }
 
 __global__ void k_sqrt1(const double* a, double* b) { b[0] = sqrt(a[0]); }
+
+// Local Lanczos step without a projection pass: w = win - alpha v_c - beta v_p (FP64)
+// with |w|^2 partials per segment of GsShape<TBv>::SEG elements into npart and npart2
+// (the same partition as the Gram-Schmidt kernels, so k_finish_local reads either).
+template <typename TBv, typename TW>
+__global__ __launch_bounds__(TB) void k_local_step(const TW* __restrict__ win, const TBv* __restrict__ vc,
+                                                   const TBv* __restrict__ vp, const double* __restrict__ alpha,
+                                                   const double* __restrict__ bprev, double* __restrict__ wout,
+                                                   int64_t n, double* __restrict__ npart, double* __restrict__ npart2) {
+    __shared__ double red[TB / 64];
+    const int64_t seg0 = (int64_t)blockIdx.x * GsShape<TBv>::SEG;
+    const int64_t seg1 = min(seg0 + (int64_t)GsShape<TBv>::SEG, n);
+    const double al = *alpha;
+    const double be = (vp && bprev) ? *bprev : 0.0;
+    double nr = 0.0;
+    for (int64_t x = seg0 + threadIdx.x; x < seg1; x += TB) {
+        double t = (double)win[x] - al * (double)vc[x];
+        if (vp) t -= be * (double)vp[x];
+        wout[x] = t;
+        nr += t * t;
+    }
+    nr = block_sum(nr, red);
+    if (threadIdx.x == 0) { npart[blockIdx.x] = nr; npart2[blockIdx.x] = nr; }
+}
+
+// Partial re-orthogonalisation (Simon's omega recurrence) for step j = it: with
+// beta' = |w'| from the partials, a_j = loc[0] and the stored alpha/beta of steps < j,
+//   beta' om_{j+1,k} = b_k om_{j,k+1} + (a_k - a_j) om_{j,k} + b_{k-1} om_{j,k-1}
+//                      - b_{j-1} om_{j-1,k} +- eps (b_k + beta') * 3        (k < j)
+// (rounding term taken with the sign that grows |om|), om_{j+1,j} = eps, om_{j+1,j+1} = 1.
+// Rows om_{j-1}, om_j, om_{j+1} rotate through three rows of om (stride ldo).
+// gate[1] = 1 runs the projection pass of this step: when max_k |om_{j+1,k}| >
+// sqrt(eps) (and then also for the next step, gate[2] counts forced steps), after
+// which om_{j+1,k} = eps.  loc[5] counts the steps that ran it.
+__global__ void k_omega(const double* __restrict__ alpha, const double* __restrict__ beta, int j,
+                        const double* __restrict__ npart, int G, double* __restrict__ om, int ldo,
+                        double* __restrict__ loc, int* __restrict__ gate, double eps) {
+    __shared__ double red[TB / 64];
+    __shared__ double mx[TB / 64];
+    const int forced = gate[2];        // read before the barriers below; thread 0 rewrites it last
+    double s = 0.0;
+    for (int g = threadIdx.x; g < G; g += TB) s += npart[g];
+    s = block_sum(s, red);
+    __shared__ double bsh;
+    if (threadIdx.x == 0) bsh = sqrt(s);
+    __syncthreads();
+    const double bj = bsh;              // |w'| in every thread
+    const double* cur = om + (int64_t)(j % 3) * ldo;
+    const double* prev = om + (int64_t)((j + 2) % 3) * ldo;
+    double* next = om + (int64_t)((j + 1) % 3) * ldo;
+    const double aj = loc[0];
+    const double bjm1 = j > 0 ? beta[j - 1] : 0.0;
+    double m = 0.0;
+    if (bj > 0.0) {
+        for (int k = threadIdx.x; k < j; k += TB) {
+            double t = beta[k] * cur[k + 1] + (alpha[k] - aj) * cur[k] - bjm1 * prev[k];
+            if (k > 0) t += beta[k - 1] * cur[k - 1];
+            const double th = 3.0 * eps * (beta[k] + bj);
+            t += t >= 0.0 ? th : -th;
+            t /= bj;
+            next[k] = t;
+            m = fmax(m, fabs(t));
+        }
+    }
+    // block max
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) mx[threadIdx.x >> 6] = m;
+    __syncthreads();
+    m = fmax(fmax(mx[0], mx[1]), fmax(mx[2], mx[3]));
+    const int run = (bj == 0.0 || forced > 0 || m > sqrt(eps)) ? 1 : 0;
+    if (run) for (int k = threadIdx.x; k < j; k += TB) next[k] = eps;
+    if (threadIdx.x == 0) {
+        next[j] = eps;
+        next[j + 1] = 1.0;
+        gate[1] = run;
+        gate[2] = forced > 0 ? forced - 1 : (run ? 1 : 0);
+        loc[5] += run;
+    }
+}
+
+// omega rows after a thick restart at step j: the kept vectors and v_j are
+// orthonormalised, so om_{j,k} = om_{j-1,k} = eps off the diagonal; the next two
+// steps run their projection pass.
+__global__ void k_omega_reset(double* __restrict__ om, int ldo, int j, double eps, int* __restrict__ gate) {
+    double* cur = om + (int64_t)(j % 3) * ldo;
+    double* prev = om + (int64_t)((j + 2) % 3) * ldo;
+    for (int k = threadIdx.x; k <= j; k += TB) {
+        cur[k] = k == j ? 1.0 : eps;
+        prev[k] = k == j - 1 ? 1.0 : (k < j - 1 ? eps : 0.0);
+    }
+    if (threadIdx.x == 0) gate[2] = 2;
+}
 
 // one factor of the J projector: y = (z - c y) * inv, with z = J^2 y
 __global__ __launch_bounds__(TB) void k_jfactor(double* __restrict__ y, const double* __restrict__ z, int64_t n,
@@ -582,7 +680,7 @@ namespace {
 
 struct Workspace {
     DBuf basis, wH, w64, part, npart, npart2, h1, h2, alpha, beta, enring, flags, S, swork, U, scal, tvals, Y, Q,
-        R, hv64, loc, gate, jy, jz;
+        R, hv64, loc, gate, jy, jz, om;
     int* hflag = nullptr;          // pinned mirror of the convergence-flag ring
     hipEvent_t ev[4] = {};
     void host_sync_objects() {
@@ -593,7 +691,7 @@ struct Workspace {
     int64_t capacity() const {
         int64_t c = 0;
         for (const DBuf* d : {&basis, &wH, &w64, &part, &npart, &npart2, &h1, &h2, &alpha, &beta, &enring, &flags, &S,
-                              &swork, &U, &scal, &tvals, &Y, &Q, &R, &hv64, &loc, &gate, &jy, &jz})
+                              &swork, &U, &scal, &tvals, &Y, &Q, &R, &hv64, &loc, &gate, &jy, &jz, &om})
             c += (int64_t)d->cap;
         return c;
     }
@@ -620,6 +718,7 @@ static void workspace_plan(int64_t n, const SolveParams& p, F&& f) {
     f(&Workspace::npart, std::max(Gu, Gx) * 8);
     f(&Workspace::npart2, Gx * 8);
     f(&Workspace::loc, 16 * 8);
+    f(&Workspace::om, 3 * (lm + 2) * 8);
     f(&Workspace::gate, 4 * (int64_t)sizeof(int));
     f(&Workspace::h1, (lm + 2) * 8);
     f(&Workspace::h2, (lm + 2) * 8);
@@ -655,6 +754,8 @@ struct Solver {
     Workspace& W;
     const std::atomic<bool>* cancel = nullptr;
     bool reorth_cgs2 = false;   // SMGPU_REORTH=cgs2: unconditional CGS2 instead of the local step
+    bool partial = false;       // partial re-orthogonalisation (omega recurrence)
+    double eps_basis = 0.0;     // unit of the basis precision for the omega recurrence
     const JProj* jp = nullptr;  // J projection (doubleLanczos), or none
 
     static Workspace& cached(std::shared_ptr<void>* slots) {
@@ -669,6 +770,9 @@ struct Solver {
         s = ctx ? ctx->stream : H.stream;
         cancel = ctx ? ctx->cancel : nullptr;
         if (const char* e = std::getenv("SMGPU_REORTH")) reorth_cgs2 = std::string(e) == "cgs2";
+        partial = !reorth_cgs2 && !jp &&
+                  (P.reorth == SMGPU_REORTH_PARTIAL || (P.reorth == SMGPU_REORTH_DEFAULT && sizeof(TBv) == 4));
+        eps_basis = sizeof(TBv) == 4 ? 1.1920928955078125e-07 : 2.220446049250313e-16;
         n = H.mdim;
         ld = (n + 63) / 64 * 64;
         lm = P.lm;
@@ -740,7 +844,50 @@ struct Solver {
                            W.w64.as<double>(), W.w64.as<double>(), n, part, ldp, W.npart.as<double>(),
                            (const int*)nullptr);
         hipLaunchKernelGGL(k_dgks, dim3(1), dim3(TB), 0, s, (const double*)W.npart.as<double>(),
-                           (const double*)W.npart2.as<double>(), (int)Gx, loc, gate);
+                           (const double*)W.npart2.as<double>(), (int)Gx, loc, gate, (const int*)nullptr);
+        hipLaunchKernelGGL((k_gs_dots<TBv, double, false>), dim3(Gx), dim3(TB), 0, s, B, ld, nv,
+                           (const double*)W.w64.as<double>(), n, part, ldp, (const int*)gate, (const TBv*)nullptr,
+                           (const TBv*)nullptr, (const double*)nullptr, (const double*)nullptr, (double*)nullptr,
+                           (double*)nullptr);
+        hipLaunchKernelGGL(k_reduce, dim3(nv), dim3(TB), 0, s, part, (int)Gx, ldp, W.h2.as<double>(),
+                           (const int*)gate);
+        hipLaunchKernelGGL((k_gs_upd<TBv, double, false>), dim3(Gx), dim3(TB), 0, s, B, ld, nv, W.h2.as<double>(),
+                           W.w64.as<double>(), W.w64.as<double>(), n, part, ldp, W.npart.as<double>(),
+                           (const int*)gate);
+        hipLaunchKernelGGL(k_finish_local, dim3(1), dim3(TB), 0, s, loc, it,
+                           (const double*)W.npart.as<double>(), (int)Gx, W.alpha.as<double>(), W.beta.as<double>());
+    }
+
+    // Lanczos step with partial re-orthogonalisation: the local step, then the omega
+    // recurrence decides on the device whether this step runs one CGS pass against
+    // V[0..it] (plus the DGKS second pass when that removes more than half of |w|^2).
+    template <typename TW>
+    void reorth_partial(int it, const TW* win) {
+        const int nv = it + 1;
+        const TBv* B = V(0);
+        double* part = W.part.as<double>();
+        double* loc = W.loc.as<double>();
+        int* gate = W.gate.as<int>();
+        const TBv* vp = it > 0 ? (const TBv*)V(it - 1) : nullptr;
+        const double* bp = it > 0 ? W.beta.as<double>() + (it - 1) : nullptr;
+        const int G2 = (int)std::min<int64_t>(1024, Gd);
+        hipLaunchKernelGGL((k_dot2<TBv, TW>), dim3(G2), dim3(TB), 0, s, (const TBv*)V(it), win, n, part, ldp);
+        hipLaunchKernelGGL(k_reduce, dim3(1), dim3(TB), 0, s, part, G2, ldp, loc, (const int*)nullptr);
+        hipLaunchKernelGGL((k_local_step<TBv, TW>), dim3(Gx), dim3(TB), 0, s, win, (const TBv*)V(it), vp,
+                           (const double*)loc, bp, W.w64.as<double>(), n, W.npart.as<double>(), W.npart2.as<double>());
+        hipLaunchKernelGGL(k_omega, dim3(1), dim3(TB), 0, s, (const double*)W.alpha.as<double>(),
+                           (const double*)W.beta.as<double>(), it, (const double*)W.npart.as<double>(), (int)Gx,
+                           W.om.as<double>(), lm + 2, loc, gate, eps_basis);
+        const int* g1 = gate + 1;
+        hipLaunchKernelGGL((k_gs_dots<TBv, double, false>), dim3(Gx), dim3(TB), 0, s, B, ld, nv,
+                           (const double*)W.w64.as<double>(), n, part, ldp, g1, (const TBv*)nullptr,
+                           (const TBv*)nullptr, (const double*)nullptr, (const double*)nullptr, (double*)nullptr,
+                           (double*)nullptr);
+        hipLaunchKernelGGL(k_reduce, dim3(nv), dim3(TB), 0, s, part, (int)Gx, ldp, W.h1.as<double>(), g1);
+        hipLaunchKernelGGL((k_gs_upd<TBv, double, false>), dim3(Gx), dim3(TB), 0, s, B, ld, nv, W.h1.as<double>(),
+                           W.w64.as<double>(), W.w64.as<double>(), n, part, ldp, W.npart.as<double>(), g1);
+        hipLaunchKernelGGL(k_dgks, dim3(1), dim3(TB), 0, s, (const double*)W.npart.as<double>(),
+                           (const double*)W.npart2.as<double>(), (int)Gx, loc, gate, g1);
         hipLaunchKernelGGL((k_gs_dots<TBv, double, false>), dim3(Gx), dim3(TB), 0, s, B, ld, nv,
                            (const double*)W.w64.as<double>(), n, part, ldp, (const int*)gate, (const TBv*)nullptr,
                            (const TBv*)nullptr, (const double*)nullptr, (const double*)nullptr, (double*)nullptr,
@@ -764,6 +911,9 @@ struct Solver {
             else cgs2<float>(nv, W.wH.as<float>());
             hipLaunchKernelGGL(k_finish, dim3(1), dim3(TB), 0, s, W.h1.as<double>(), it, W.npart.as<double>(),
                                (int)Gx, W.alpha.as<double>(), W.beta.as<double>());
+        } else if (partial) {
+            if (prec_hv == 0) reorth_partial<double>(it, W.wH.as<double>());
+            else reorth_partial<float>(it, W.wH.as<float>());
         } else {
             if (prec_hv == 0) reorth_local<double>(it, W.wH.as<double>());
             else reorth_local<float>(it, W.wH.as<float>());
@@ -784,6 +934,12 @@ struct Solver {
         std::vector<double> e0(4 * 64, 1.0e4);
         SM_HIP(hipMemcpyAsync(W.enring.p, e0.data(), e0.size() * 8, hipMemcpyHostToDevice, s));
         SM_HIP(hipMemsetAsync(W.loc.p, 0, W.loc.bytes, s));
+        SM_HIP(hipMemsetAsync(W.gate.p, 0, W.gate.bytes, s));
+        SM_HIP(hipMemsetAsync(W.om.p, 0, W.om.bytes, s));
+        {
+            const double one = 1.0;          // om_{0,0}
+            SM_HIP(hipMemcpyAsync(W.om.p, &one, 8, hipMemcpyHostToDevice, s));
+        }
         // start vector: caller's, or seeded normal deviates generated on the device
         // (initialize_wf, :939-948); normalised in FP64
         if (v0_in) {
@@ -884,9 +1040,10 @@ struct Solver {
         st.iterations = elit;
         std::vector<double> en_last(nev);
         SM_HIP(hipMemcpy(en_last.data(), en(conv_step), nev * 8, hipMemcpyDeviceToHost));
-        double nre2 = 0.0;
-        SM_HIP(hipMemcpy(&nre2, W.loc.as<double>() + 3, 8, hipMemcpyDeviceToHost));
-        st.n_reorth2 = (int32_t)nre2;
+        double lc[6] = {};
+        SM_HIP(hipMemcpy(lc, W.loc.p, 6 * 8, hipMemcpyDeviceToHost));
+        st.n_reorth2 = (int32_t)lc[3];
+        st.n_reorth = partial ? (int32_t)lc[5] : st.n_hv;
         for (int i = 0; i < nev; i++) { evals[i] = en_last[i]; st.ritz[i] = en_last[i]; }
         finish(elit, conv_step, evals, evecs);
         st.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -996,6 +1153,9 @@ struct Solver {
         // v_ls against the kept vectors (no renormalisation, as the reference)
         cgs2<TBv>(ls, (const TBv*)V(ls));
         hipLaunchKernelGGL((k_convert<double, TBv>), dim3(gstore), dim3(TB), 0, s, W.w64.as<double>(), V(ls), n, 1.0);
+        if (partial)
+            hipLaunchKernelGGL(k_omega_reset, dim3(1), dim3(TB), 0, s, W.om.as<double>(), lm + 2, ls, eps_basis,
+                               W.gate.as<int>());
         if (std::getenv("SMGPU_DEBUG_RESTART")) debug_restart();
     }
 

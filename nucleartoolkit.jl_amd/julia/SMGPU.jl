@@ -23,7 +23,15 @@ struct SolveParams              # smgpu_solve_params
     refine::Int32
     max_iter::Int32
     seed::UInt64
+    reorth::Int32                # REORTH_DEFAULT / _FULL / _PARTIAL
+    reserved_::Int32
 end
+SolveParams(num_ev, lm, ls, num_history, tol, precision, refine, max_iter, seed) =
+    SolveParams(num_ev, lm, ls, num_history, tol, precision, refine, max_iter, seed, Int32(0), Int32(0))
+
+const REORTH_DEFAULT = Int32(0)
+const REORTH_FULL = Int32(1)
+const REORTH_PARTIAL = Int32(2)
 
 struct SolveStats               # smgpu_solve_stats
     iterations::Int32
@@ -34,6 +42,8 @@ struct SolveStats               # smgpu_solve_stats
     ritz::NTuple{64,Float64}
     n_reorth2::Int32
     n_jproj::Int32
+    n_reorth::Int32
+    reserved_::Int32
 end
 
 struct ProblemInfo              # smgpu_problem_info

@@ -41,13 +41,15 @@ class ProblemInfo(ctypes.Structure):
 class SolveParams(ctypes.Structure):
     _fields_ = [("num_ev", ctypes.c_int32), ("lm", ctypes.c_int32), ("ls", ctypes.c_int32),
                 ("num_history", ctypes.c_int32), ("tol", ctypes.c_double), ("precision", ctypes.c_int32),
-                ("refine", ctypes.c_int32), ("max_iter", ctypes.c_int32), ("seed", ctypes.c_uint64)]
+                ("refine", ctypes.c_int32), ("max_iter", ctypes.c_int32), ("seed", ctypes.c_uint64),
+                ("reorth", ctypes.c_int32), ("reserved_", ctypes.c_int32)]
 
 
 class SolveStats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int32), ("n_hv", ctypes.c_int32), ("n_hv64", ctypes.c_int32),
                 ("n_restart", ctypes.c_int32), ("wall_ms", ctypes.c_double), ("ritz", ctypes.c_double * 64),
-                ("n_reorth2", ctypes.c_int32), ("n_jproj", ctypes.c_int32)]
+                ("n_reorth2", ctypes.c_int32), ("n_jproj", ctypes.c_int32), ("n_reorth", ctypes.c_int32),
+                ("reserved_", ctypes.c_int32)]
 
 
 class ServerConfig(ctypes.Structure):

@@ -152,15 +152,17 @@ class Hamiltonian:
         return a, b
 
     def trl(self, num_ev, lm=300, ls=20, tol=1e-8, num_history=3, precision="fp64", refine=True, seed=1234,
-            want_vectors=False, max_iter=0, v0=None, j2=None, twoJ=None):
+            want_vectors=False, max_iter=0, v0=None, j2=None, twoJ=None, reorth="default"):
         """Thick-restart Lanczos; returns (evals, evecs or None, stats dict).
         With j2 (a J^2 Hamiltonian on this basis) and twoJ: J-projected TRL
-        (doubleLanczos), the lowest states with J = twoJ/2."""
+        (doubleLanczos), the lowest states with J = twoJ/2.  reorth: "default"
+        (full for fp64, partial for mixed), "full" or "partial"."""
         p = SolveParams()
         lib().smgpu_default_params(ctypes.byref(p))
         p.num_ev = int(num_ev); p.lm = int(lm); p.ls = int(ls); p.tol = float(tol)
         p.num_history = int(num_history); p.precision = _prec(precision); p.refine = int(bool(refine))
         p.seed = int(seed); p.max_iter = int(max_iter)
+        p.reorth = {"default": 0, "full": 1, "partial": 2}[reorth]
         nev = min(int(num_ev), self.mdim)
         ev = np.zeros(nev, np.float64)
         vec = np.zeros((nev, self.mdim), np.float64) if want_vectors else None
@@ -178,7 +180,7 @@ class Hamiltonian:
                                            ptr(vec) if vec is not None else None, ctypes.byref(st)))
         stats = dict(iterations=st.iterations, n_hv=st.n_hv, n_hv64=st.n_hv64, n_restart=st.n_restart,
                      wall_ms=st.wall_ms, ritz=np.array(st.ritz[:nev]), n_reorth2=st.n_reorth2,
-                     n_jproj=st.n_jproj)
+                     n_jproj=st.n_jproj, n_reorth=st.n_reorth)
         return ev, vec, stats
 
     def trbl(self, num_ev, q, lm=300, ls=20, tol=1e-8, num_history=3, precision="fp64", refine=True, seed=1234,
@@ -205,7 +207,7 @@ class Hamiltonian:
                                      ctypes.byref(st)))
         stats = dict(iterations=st.iterations, n_hv=st.n_hv, n_hv64=st.n_hv64, n_restart=st.n_restart,
                      wall_ms=st.wall_ms, ritz=np.array(st.ritz[:nev]), n_reorth2=st.n_reorth2,
-                     n_jproj=st.n_jproj)
+                     n_jproj=st.n_jproj, n_reorth=st.n_reorth)
         return ev, vec, stats
 
     def __del__(self):

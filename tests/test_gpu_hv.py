@@ -148,6 +148,32 @@ def test_trl_vs_golden_arpack(golden, snt_dir, name, space, zv, nv, spe, sig):
     assert np.abs(en - ref[:len(en)]).max() < 1e-5
 
 
+@pytest.mark.parametrize("name,space,zv,nv,spe,sig", [("mg24", "sd", 4, 4, (-4, 0), 1.5),
+                                                      ("cr48", "pf", 4, 4, (-8, 0), 1.0)])
+def test_partial_reorth(golden, snt_dir, name, space, zv, nv, spe, sig):
+    """Partial re-orthogonalisation (omega recurrence): the mixed default runs a
+    projection pass on a fraction of the steps, full runs one every step, and both
+    reach the ARPACK goldens (1e-5 MeV mixed); FP64 with partial keeps 1e-7 MeV.
+    A restarted partial solve (lm 40) reaches the same values."""
+    g = _golden_cfg(golden, name)
+    path, nuc = synth.config_snt(f"{snt_dir}/{name}_pr.snt", space, zv, nv, g["seed"], spe, sig)
+    h = smgpu.Hamiltonian(smgpu.Problem(path, nuc))
+    ref = np.array(g["eigenvalues_MeV"])
+    k = len(ref)
+    en_p, _, st_p = h.trl(k, precision="mixed")
+    en_f, _, st_f = h.trl(k, precision="mixed", reorth="full")
+    print(name, "partial", st_p["n_reorth"], "/", st_p["n_hv"], np.abs(en_p - ref).max(),
+          "full", st_f["n_reorth"], "/", st_f["n_hv"], np.abs(en_f - ref).max(), flush=True)
+    assert np.abs(en_p - ref).max() < 1e-5 and np.abs(en_f - ref).max() < 1e-5
+    assert st_f["n_reorth"] == st_f["n_hv"]
+    assert 0 < st_p["n_reorth"] < st_p["n_hv"] // 2
+    en64, _, st64 = h.trl(k, precision="fp64", reorth="partial")
+    assert np.abs(en64 - ref).max() < 1e-7 and st64["n_reorth"] < st64["n_hv"]
+    en_r, _, st_r = h.trl(min(k, 5), lm=40, ls=10, precision="mixed")
+    assert st_r["n_restart"] > 0 and st_r["n_reorth"] < st_r["n_hv"]
+    assert np.abs(en_r - ref[:len(en_r)]).max() < 1e-5
+
+
 def test_lanczos_recurrence_matches_numpy(snt_dir):
     """alpha/beta of the device recurrence vs numpy CGS2 Lanczos with the same
     (device) H·v, at the Cr48 size: FP64 <= 1e-10, mixed <= 1e-5 (relative)."""
