@@ -97,8 +97,13 @@ def hv_only(ham, args):
     check(lib().smgpu_free(ham._h, a))
     check(lib().smgpu_free(ham._h, b))
     tb32 = ham.table_bytes("mixed")
-    print(json.dumps({"mode": "hv", "ms_fp32": t32, "ms_fp64": t64, "reps": args.hv_reps,
-                      "alg_bytes_fp32": sum(tb32.values()), "copy_bytes": nbytes, "table_bytes_fp32": tb32}))
+    out = {"mode": "hv", "ms_fp32": t32, "ms_fp64": t64, "reps": args.hv_reps,
+           "alg_bytes_fp32": sum(tb32.values()), "copy_bytes": nbytes, "table_bytes_fp32": tb32}
+    if args.hv_multi:
+        # bl_operate_H!: ms per vector when 2 or 4 vectors share a launch
+        out["ms_per_vector"] = {p: {str(k): ham.time_operate_H(p, args.hv_reps, nvec=k) / k for k in (1, 2, 4)}
+                                for p in ("mixed", "fp64")}
+    print(json.dumps(out))
 
 
 def _line(metric_value, unit, steps, warmup, t_all, config, extra, dtype="fp32 H·v / fp64 Krylov"):
@@ -123,7 +128,7 @@ def run_variants(args):
     tmp = tempfile.mkdtemp(prefix="smgpu_variants_")
     paths, nuc = synth.variant_snts(tmp, args.variants)
     nev = 3
-    srv = Server(max_inflight=args.inflight, host_threads=args.host_threads)
+    srv = Server(max_inflight=args.inflight, host_threads=args.host_threads, max_batch=args.batch)
     warm = [srv.submit(paths[i % len(paths)], nuc, nev, seed=900 + i) for i in range(args.inflight)]
     for rid in warm:
         srv.wait(rid)
@@ -156,7 +161,7 @@ def run_variants(args):
     _line(value, "solves/s", len(paths), args.inflight, t_all,
           {"workload": f"{len(paths)} interaction variants of {nuc} (pf 4p4n, mdim {res[0][1]['mdim']}), lowest {nev} "
                        f"each, one shared index, {args.inflight} in flight", "nucleus": nuc, "variants": len(paths),
-           "num_ev": nev, "inflight": args.inflight},
+           "num_ev": nev, "inflight": args.inflight, "batch": args.batch},
           {"latency_ms": {"p50": percentile(lat, 50), "p99": percentile(lat, 99)},
            "fp64": {"solves_per_s": fp64_rate, "variants_checked": n64},
            "speedup_vs_fp64": value / fp64_rate, "max_abs_dE_vs_fp64_MeV": max(dE), "within_tol": ok,
@@ -176,7 +181,7 @@ def run_stream(args):
     tmp = tempfile.mkdtemp(prefix="smgpu_stream_")
     lo, hi = (int(x) for x in args.particles.split(","))
     reqs = synth.serving_stream(tmp, args.requests, rate_per_s=args.rate, particles=(lo, hi))
-    srv = Server(max_inflight=args.inflight, host_threads=args.host_threads)
+    srv = Server(max_inflight=args.inflight, host_threads=args.host_threads, max_batch=args.batch)
     t0 = time.perf_counter()
     ids = []
     for r in reqs:
@@ -208,7 +213,7 @@ def run_stream(args):
     _line(value, "solves/s", len(reqs), 0, t_all,
           {"workload": f"{len(reqs)} requests, sd/pf, valence {lo}..{hi} per species, k 1..10, Poisson "
                        f"{args.rate}/s, {args.inflight} in flight", "requests": len(reqs), "rate_per_s": args.rate,
-           "particles": [lo, hi], "inflight": args.inflight},
+           "particles": [lo, hi], "inflight": args.inflight, "batch": args.batch},
           {"latency_ms": {"p50": percentile(lat, 50), "p99": percentile(lat, 99), "note": "arrival to result"},
            "sequential_fp64": {"solves_per_s": len(reqs) / t_seq, "wall_s": t_seq},
            "speedup_vs_fp64": value / (len(reqs) / t_seq), "max_abs_dE_vs_fp64_MeV": max(dE) if dE else None,
@@ -226,7 +231,10 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--hv-reps", type=int, default=20)
-    ap.add_argument("--inflight", type=int, default=4)
+    ap.add_argument("--hv-multi", action="store_true", help="--mode hv: also time 2- and 4-vector launches")
+    ap.add_argument("--inflight", type=int, default=2)
+    ap.add_argument("--batch", type=int, default=2,
+                    help="requests with identical Hamiltonians solved in lockstep per slot (server max_batch)")
     ap.add_argument("--host-threads", type=int, default=2)
     ap.add_argument("--mode", default="solve", choices=["solve", "hv"],
                     help="hv: only the H·v launches of the workload (for profiler runs) plus a 256 MiB "
@@ -291,7 +299,7 @@ def main():
 
     # serving form: K requests through the persistent loop, M in flight
     from smgpu.server import Server
-    srv = Server(max_inflight=args.inflight, host_threads=args.host_threads)
+    srv = Server(max_inflight=args.inflight, host_threads=args.host_threads, max_batch=args.batch)
     warm = [srv.submit(snt, nuc, nev, lm=lm, ls=ls, precision="mixed", seed=1000 + i)
             for i in range(max(args.warmup, args.inflight))]
     for rid in warm:
@@ -358,9 +366,11 @@ def main():
         "dtype": "fp32 H·v / fp64 Krylov",
         "data": "synthetic (seeded isospin interaction written as .snt)",
         "config": {"workload": f"{nuc} {space} {zv}p{nv}n 2M={prob.info.twoM} mdim {mdim}, lowest {nev}, "
-                               f"tol 1e-8, lm {lm}, ls {ls}; {args.steps} requests served {args.inflight} in flight",
+                               f"tol 1e-8, lm {lm}, ls {ls}; {args.steps} requests served in {args.inflight} slots, "
+                               f"batches of up to {args.batch}",
                    "nucleus": nuc, "mdim": mdim, "num_ev": nev, "lm": lm, "ls": ls, "precision": "mixed",
-                   "refine": "fp64 Rayleigh quotient", "inflight": args.inflight, "requests": args.steps},
+                   "refine": "fp64 Rayleigh quotient", "inflight": args.inflight, "batch": args.batch,
+                   "requests": args.steps},
         "latency_ms": {"p50": percentile(lat, 50), "p99": percentile(lat, 99),
                        "solve_p50": percentile(solve_ms, 50), "prep_p50": percentile(prep_ms, 50),
                        "note": "submit to result; all K requests submitted at t=0"},

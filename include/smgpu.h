@@ -143,6 +143,16 @@ int32_t smgpu_work_terms(smgpu_ham h, int64_t* terms5);
 int32_t smgpu_apply_H(smgpu_ham h, int32_t precision, const void* v_dev, void* w_dev, void* stream);
 /* Host-buffer convenience: FP64 in/out, computed in `precision` on the device. */
 int32_t smgpu_apply_H_host(smgpu_ham h, int32_t precision, const double* v, double* w);
+/* H·V on nvec vectors at once (bl_operate_H!, lanczos_methods.jl:435-548): w_b = H v_b
+ * for v_b = v_dev + b*ldv, w_b = w_dev + b*ldw (elements; ld >= mdim when nvec > 1).
+ * Launches of up to four vectors share every table load; each vector's result is
+ * bit-identical to smgpu_apply_H on it. */
+int32_t smgpu_apply_H_multi(smgpu_ham h, int32_t precision, int32_t nvec, const void* v_dev, int64_t ldv,
+                            void* w_dev, int64_t ldw, void* stream);
+/* Host-buffer convenience: v and w are nvec x mdim row-major FP64. */
+int32_t smgpu_apply_H_host_multi(smgpu_ham h, int32_t precision, int32_t nvec, const double* v, double* w);
+/* Timing of smgpu_apply_H_multi on resident vectors: milliseconds per call (all nvec). */
+int32_t smgpu_time_apply_H_multi(smgpu_ham h, int32_t precision, int32_t nvec, int32_t reps, double* ms);
 /* Timing: `reps` launches of H·v on resident device vectors, HIP events on the
  * launch stream; average milliseconds per launch in *ms. */
 int32_t smgpu_time_apply_H(smgpu_ham h, int32_t precision, int32_t reps, double* ms);
@@ -165,6 +175,18 @@ int32_t smgpu_trl_solve_v0(smgpu_ham h, const smgpu_solve_params* params, const 
  * than 1e-7 (Jlanczos' Jtol) is replaced by its exact projection on that J. */
 int32_t smgpu_trl_solve_j(smgpu_ham h, smgpu_ham j2, int32_t twoJ, const smgpu_solve_params* params,
                           const double* v0, double* evals, double* evecs, smgpu_solve_stats* stats);
+/* nb independent TRL solves on one Hamiltonian in lockstep (no counterpart in the
+ * reference, which solves one start vector per main_sm call): every round issues
+ * the H·v of all unfinished members in shared launches (smgpu_apply_H_multi's
+ * kernels), then each member runs the rest of its Lanczos step on its own stream.
+ * params, v0 (each NULL or mdim doubles; v0 itself may be NULL), evals (num_ev
+ * doubles each), evecs (each NULL or num_ev*mdim; evecs itself may be NULL), stats
+ * and status (may be NULL) are arrays of nb.  All members share one precision.
+ * Each member's results are bit-identical to smgpu_trl_solve_v0 with its params.
+ * Returns the first failing member's code (status[] holds every member's). */
+int32_t smgpu_trl_solve_batch(smgpu_ham h, int32_t nb, const smgpu_solve_params* params, const double* const* v0,
+                              double* const* evals, double* const* evecs, smgpu_solve_stats* stats,
+                              int32_t* status);
 /* Thick-restart block Lanczos: main_sm with is_block=true and block size q
  * (TRBL, lanczos_methods.jl:312-420, with bl_operate_H!, bl_ReORTH, bl_QR! and
  * bl_ThickRestart :422-617; shellmodel_main.jl:167-191, 211-224).  params->lm is the
@@ -209,6 +231,10 @@ typedef struct {
     int32_t host_threads;   /* requests prepared concurrently; <= 0: 2 */
     int32_t table_threads;  /* threads per table build; <= 0: cores / host_threads */
     int64_t keep_bytes;     /* an idle slot frees its buffers above this; <= 0: 16 GiB */
+    int32_t max_batch;      /* requests with identical Hamiltonians (same basis and coefficients,
+                               same precision) one slot solves in lockstep with shared H·v
+                               launches; <= 0: 1 (no batching) */
+    int32_t reserved_;
 } smgpu_server_config;
 
 typedef struct {
@@ -234,6 +260,9 @@ typedef struct {
     int64_t hbm_budget, hbm_in_use, hbm_peak;
     int64_t index_hits, index_misses, index_uploads, index_evictions;
     double clock_ms;        /* server clock now */
+    int64_t batched;        /* requests that ran in a batch of two or more */
+    int32_t peak_batch;     /* largest batch run */
+    int32_t reserved_;
 } smgpu_server_stats;
 
 void smgpu_default_server_config(smgpu_server_config* cfg);

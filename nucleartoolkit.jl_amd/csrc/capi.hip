@@ -302,6 +302,84 @@ int32_t smgpu_apply_H_host(smgpu_ham h, int32_t precision, const double* v, doub
     });
 }
 
+int32_t smgpu_apply_H_multi(smgpu_ham h, int32_t precision, int32_t nvec, const void* v_dev, int64_t ldv,
+                            void* w_dev, int64_t ldw, void* stream) {
+    return guard([&]() -> int32_t {
+        if (!h || !v_dev || !w_dev || nvec < 1) throw std::invalid_argument("bad argument");
+        if (nvec > 1 && (ldv < h->dev->mdim || ldw < h->dev->mdim)) throw std::invalid_argument("ld below mdim");
+        SM_HIP(hipSetDevice(h->dev->device));
+        hipStream_t s = stream ? (hipStream_t)stream : h->dev->stream;
+        launch_hv_multi(*h->dev, precision == SMGPU_FP64 ? 0 : 1, nvec, v_dev, ldv, w_dev, ldw, s);
+        return SMGPU_OK;
+    });
+}
+
+int32_t smgpu_apply_H_host_multi(smgpu_ham h, int32_t precision, int32_t nvec, const double* v, double* w) {
+    return guard([&]() -> int32_t {
+        if (!h || !v || !w || nvec < 1) throw std::invalid_argument("bad argument");
+        DeviceHam& d = *h->dev;
+        SM_HIP(hipSetDevice(d.device));
+        const int64_t n = d.mdim, tot = n * nvec;
+        DBuf dv, dw;
+        dv.alloc(tot * 8);
+        dw.alloc(tot * 8);
+        SM_HIP(hipMemcpyAsync(dv.p, v, tot * 8, hipMemcpyHostToDevice, d.stream));
+        const int g = (int)std::min<int64_t>(4096, (tot + 255) / 256);
+        if (precision == SMGPU_FP64) {
+            launch_hv_multi(d, 0, nvec, dv.p, n, dw.p, n, d.stream);
+        } else {
+            DBuf fv, fw;
+            fv.alloc(tot * 4);
+            fw.alloc(tot * 4);
+            hipLaunchKernelGGL((k_cast<double, float>), dim3(g), dim3(256), 0, d.stream, dv.as<double>(),
+                               fv.as<float>(), tot);
+            launch_hv_multi(d, 1, nvec, fv.p, n, fw.p, n, d.stream);
+            hipLaunchKernelGGL((k_cast<float, double>), dim3(g), dim3(256), 0, d.stream, fw.as<float>(),
+                               dw.as<double>(), tot);
+            SM_HIP(hipStreamSynchronize(d.stream));
+        }
+        SM_HIP(hipMemcpyAsync(w, dw.p, tot * 8, hipMemcpyDeviceToHost, d.stream));
+        SM_HIP(hipStreamSynchronize(d.stream));
+        return SMGPU_OK;
+    });
+}
+
+int32_t smgpu_time_apply_H_multi(smgpu_ham h, int32_t precision, int32_t nvec, int32_t reps, double* ms) {
+    return guard([&]() -> int32_t {
+        if (!h || !ms || reps < 1 || nvec < 1) throw std::invalid_argument("bad argument");
+        DeviceHam& d = *h->dev;
+        SM_HIP(hipSetDevice(d.device));
+        const int64_t n = d.mdim, tot = n * nvec;
+        const size_t s = precision == SMGPU_FP64 ? 8 : 4;
+        DBuf dv, dw;
+        dv.alloc(tot * s);
+        dw.alloc(tot * s);
+        std::vector<double> hv(tot);
+        for (int64_t i = 0; i < tot; i++)
+            hv[i] = 1.0 / std::sqrt((double)n) * (1.0 + 0.25 * std::sin(0.37 * (double)i));
+        if (s == 8) SM_HIP(hipMemcpy(dv.p, hv.data(), tot * 8, hipMemcpyHostToDevice));
+        else {
+            std::vector<float> f(hv.begin(), hv.end());
+            SM_HIP(hipMemcpy(dv.p, f.data(), tot * 4, hipMemcpyHostToDevice));
+        }
+        const int p = precision == SMGPU_FP64 ? 0 : 1;
+        for (int r = 0; r < 2; r++) launch_hv_multi(d, p, nvec, dv.p, n, dw.p, n, d.stream);
+        hipEvent_t e0, e1;
+        SM_HIP(hipEventCreate(&e0));
+        SM_HIP(hipEventCreate(&e1));
+        SM_HIP(hipEventRecord(e0, d.stream));
+        for (int r = 0; r < reps; r++) launch_hv_multi(d, p, nvec, dv.p, n, dw.p, n, d.stream);
+        SM_HIP(hipEventRecord(e1, d.stream));
+        SM_HIP(hipEventSynchronize(e1));
+        float t = 0.f;
+        SM_HIP(hipEventElapsedTime(&t, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *ms = (double)t / reps;
+        return SMGPU_OK;
+    });
+}
+
 int32_t smgpu_time_apply_H(smgpu_ham h, int32_t precision, int32_t reps, double* ms) {
     return guard([&]() -> int32_t {
         if (!h || !ms || reps < 1) throw std::invalid_argument("bad argument");
@@ -397,6 +475,48 @@ int32_t smgpu_trl_solve_j(smgpu_ham h, smgpu_ham j2, int32_t twoJ, const smgpu_s
         jp.twoJmax = j2->dev->twoJmax;
         trl_solve(*h->dev, p, evals, evecs, st, v0, nullptr, &jp);
         if (stats) *stats = st;
+        return SMGPU_OK;
+    });
+}
+
+int32_t smgpu_trl_solve_batch(smgpu_ham h, int32_t nb, const smgpu_solve_params* params, const double* const* v0,
+                              double* const* evals, double* const* evecs, smgpu_solve_stats* stats,
+                              int32_t* status) {
+    return guard([&]() -> int32_t {
+        if (!h || nb < 1 || !params || !evals) throw std::invalid_argument("null argument");
+        DeviceHam& d = *h->dev;
+        SM_HIP(hipSetDevice(d.device));
+        std::vector<smgpu_solve_params> p(params, params + nb);
+        for (auto& q : p) {
+            if (q.max_iter <= 0) q.max_iter = 100000;
+            if (q.num_history < 2) q.num_history = 3;
+            if (q.num_ev > 64) throw std::invalid_argument("num_ev above 64");
+            q.num_ev = (int32_t)std::min<int64_t>(q.num_ev, d.mdim);
+        }
+        std::vector<SolveCtx> ctx(nb);
+        std::vector<const SolveCtx*> cp(nb);
+        struct Streams {
+            std::vector<SolveCtx>& c;
+            ~Streams() {
+                for (auto& x : c) if (x.stream) { (void)hipStreamSynchronize(x.stream); (void)hipStreamDestroy(x.stream); }
+            }
+        } guard_streams{ctx};
+        for (int b = 0; b < nb; b++) {
+            SM_HIP(hipStreamCreateWithFlags(&ctx[b].stream, hipStreamNonBlocking));
+            cp[b] = &ctx[b];
+        }
+        std::vector<smgpu_solve_stats> st(nb);
+        std::memset(st.data(), 0, sizeof(smgpu_solve_stats) * nb);
+        std::vector<int32_t> stat(nb, SMGPU_OK);
+        std::vector<std::string> msgs(nb);
+        trl_solve_batch(d, nb, p.data(), v0, evals, evecs, st.data(), cp.data(), d.stream, stat.data(), msgs.data());
+        if (stats) for (int b = 0; b < nb; b++) stats[b] = st[b];
+        if (status) for (int b = 0; b < nb; b++) status[b] = stat[b];
+        for (int b = 0; b < nb; b++)
+            if (stat[b] != SMGPU_OK) {
+                set_last_error("batch member " + std::to_string(b) + ": " + msgs[b]);
+                return stat[b];
+            }
         return SMGPU_OK;
     });
 }

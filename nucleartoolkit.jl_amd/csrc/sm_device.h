@@ -135,6 +135,8 @@ std::shared_ptr<const HostCoefficients> pack_coefficients(const Basis& b, const 
 int64_t device_bytes(const HostCoefficients& c);     // HBM of the coefficients, both precisions
 int64_t coefficient_growth(const DeviceHam& h, const HostCoefficients& c);   // bind_ham's new allocation
 uint64_t pattern_of(const HostCoefficients& c);
+uint64_t content_hash(const HostCoefficients& c);                 // of the coefficient values
+bool same_coefficients(const HostCoefficients& a, const HostCoefficients& b);   // identical Hamiltonians
 uint64_t pattern_of(const HostIndex& x);
 // Point h at idx with coefficients c, reusing h's coefficient buffers when large
 // enough (no allocation in steady state).  Throws if the sparsity patterns differ.
@@ -148,6 +150,18 @@ std::unique_ptr<DeviceHam> attach_ham(std::shared_ptr<DeviceIndex> idx, const Ba
 std::unique_ptr<DeviceHam> upload_ham(const Basis& b, const Tables& t, int device);
 // w = H v for one vector; prec 0 = FP64 (double*), 1 = FP32 (float*)
 void launch_hv(const DeviceHam& h, int prec, const void* v, void* w, hipStream_t s);
+// input/output vectors of one H·v launch (up to four, each in the launch precision)
+struct HvVecs {
+    const void* v[4];
+    void* w[4];
+};
+// w[b] = H v[b] for nvec arbitrary vectors, in launches of up to four that share
+// the table loads (each result bit-identical to launch_hv)
+void launch_hv_set(const DeviceHam& h, int prec, int nvec, const void* const* v, void* const* w, hipStream_t s);
+// w_b = H v_b for nvec vectors at v + b*ldv -> w + b*ldw (elements), in launches of
+// up to four vectors that share the table loads
+void launch_hv_multi(const DeviceHam& h, int prec, int nvec, const void* v, int64_t ldv, void* w, int64_t ldw,
+                     hipStream_t s);
 // re-upload interaction-dependent coefficients (same basis and index tables)
 void update_coefficients(DeviceHam& h, const Basis& b, const Tables& t);
 

@@ -51,23 +51,27 @@ template <> struct Vec4<float> { using type = float4; };
 template <> struct Vec4<double> { using type = double4; };
 
 // One workgroup per group of RG consecutive proton rows of one block (lanes
-// along the neutron index, NPT columns per lane).  Every neutron-side table
-// entry a lane loads (nn and pn hop lists) is applied to all rows of the group.
+// along the neutron index, NPT columns per lane), for NV input vectors at once
+// (P.v[b] -> P.w[b], b < NV; the reference's bl_operate_H!,
+// lanczos_methods.jl:435-548, for NV > 1).  Every neutron-side table entry and
+// V_eff slot a lane loads (nn and pn hop lists) is applied to all rows of the
+// group and all NV vectors.
 // pn: per coupled input block, the proton hops of the group's rows are staged
 // four per row per round ("slots"), slot-interleaved in LDS:
-//   X[in][slot] = s_p v[bi: ip_slot, in],   Vs[col][slot] = V_eff[dM][ac_slot][col]
+//   X[in][b][slot] = s_p v_b[bi: ip_slot, in],   Vs[col][slot] = V_eff[dM][ac_slot][col]
 // so that a lane fetches the four hops of a row for its neutron hop (in, col)
-// with two 4-wide LDS loads at compile-time offsets.  Unused slots are zero.
-template <typename T, int NPT, int RG, int PCH>
-__global__ __launch_bounds__(256) void k_hv(HvTables A, const T* __restrict__ v, T* __restrict__ w) {
+// with one 4-wide LDS load per vector and one for V_eff, at compile-time offsets.
+// Unused slots are zero.
+template <typename T, int NPT, int RG, int PCH, int NV>
+__global__ __launch_bounds__(256) void k_hv(HvTables A, HvVecs P) {
     static_assert(PCH % 4 == 0, "slots come in groups of four");
     constexpr int NG = PCH / 4;       // slot groups per row and round
     constexpr int SL = RG * PCH;      // slots per round
     using T4 = typename Vec4<T>::type;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    T* X = reinterpret_cast<T*>(smem_raw);                      // nN_max * SL
-    T* Vs = X + (size_t)A.nN_max * SL;                           // cols2_max * SL
-    T* xs = Vs + (size_t)A.cols2_max * SL;                       // nN_max * RG  ([col][row])
+    T* X = reinterpret_cast<T*>(smem_raw);                      // nN_max * NV * SL
+    T* Vs = X + (size_t)A.nN_max * SL * NV;                      // cols2_max * SL
+    T* xs = Vs + (size_t)A.cols2_max * SL;                       // nN_max * RG * NV  ([col][row][vec])
     const int g = blockIdx.x;
     const int tid = threadIdx.x;
     const int bf = A.grp_block[g];
@@ -81,11 +85,14 @@ __global__ __launch_bounds__(256) void k_hv(HvTables A, const T* __restrict__ v,
     for (int i = tid; i < nN; i += 256) {
 #pragma unroll
         for (int r = 0; r < RG; r++)
-            xs[i * RG + r] = r < nr ? v[bstart + (int64_t)(fp0 + r) * nN + i] : T(0);
+#pragma unroll
+            for (int b = 0; b < NV; b++)
+                xs[(i * RG + r) * NV + b] =
+                    r < nr ? static_cast<const T*>(P.v[b])[bstart + (int64_t)(fp0 + r) * nN + i] : T(0);
     }
     __syncthreads();
 
-    T acc[NPT][RG];
+    T acc[NPT][RG][NV];
     T epr[RG];
 #pragma unroll
     for (int r = 0; r < RG; r++) epr[r] = r < nr ? ep[row0 + r] : T(0);
@@ -101,13 +108,17 @@ __global__ __launch_bounds__(256) void k_hv(HvTables A, const T* __restrict__ v,
 #pragma unroll
     for (int k = 0; k < NPT; k++) {
 #pragma unroll
-        for (int r = 0; r < RG; r++) acc[k][r] = T(0);
+        for (int r = 0; r < RG; r++)
+#pragma unroll
+            for (int b = 0; b < NV; b++) acc[k][r][b] = T(0);
         const int sk = wave + 4 * k;
         if (sk < nsl) {
             const int fnc = min(tid + k * 256, nN - 1);
             const T e_n = en[fnc];
 #pragma unroll
-            for (int r = 0; r < RG; r++) acc[k][r] = (epr[r] + e_n) * xs[fnc * RG + r];
+            for (int r = 0; r < RG; r++)
+#pragma unroll
+                for (int b = 0; b < NV; b++) acc[k][r][b] = (epr[r] + e_n) * xs[(fnc * RG + r) * NV + b];
             const int64_t s = nsb + sk;
             const int64_t off = A.nn_slice_off[s] + (tid & 63);
             const int len = __builtin_amdgcn_readfirstlane(A.nn_slice_len[s]);
@@ -123,20 +134,24 @@ __global__ __launch_bounds__(256) void k_hv(HvTables A, const T* __restrict__ v,
 #pragma unroll
                 for (int u = 0; u < 4; u++)
 #pragma unroll
-                    for (int r = 0; r < RG; r++) acc[k][r] += c[u] * xs[col[u] * RG + r];
+                    for (int r = 0; r < RG; r++)
+#pragma unroll
+                        for (int b = 0; b < NV; b++) acc[k][r][b] += c[u] * xs[(col[u] * RG + r) * NV + b];
             }
             for (; q < len; q++) {
                 const T c = nnv[off + (int64_t)q * 64];
                 const int col = A.nn_col[off + (int64_t)q * 64];
 #pragma unroll
-                for (int r = 0; r < RG; r++) acc[k][r] += c * xs[col * RG + r];
+                for (int r = 0; r < RG; r++)
+#pragma unroll
+                    for (int b = 0; b < NV; b++) acc[k][r][b] += c * xs[(col * RG + r) * NV + b];
             }
         }
     }
     // pp: rows of the same block; UP entries (uniform) per step, their row gathers
     // issued together as unconditional loads
     {
-        constexpr int UP = NPT <= 2 ? 8 : (NPT <= 4 ? 4 : 2);
+        constexpr int UP = NPT * NV <= 2 ? 8 : (NPT * NV <= 4 ? 4 : 2);
         const T* ppv = reinterpret_cast<const T*>(A.pp_val);
 #pragma unroll
         for (int r = 0; r < RG; r++) {
@@ -155,13 +170,16 @@ __global__ __launch_bounds__(256) void k_hv(HvTables A, const T* __restrict__ v,
                     for (int k = 0; k < NPT; k++) {
                         if (wave + 4 * k >= nsl) continue;          // wave-uniform
                         const int fnc = min(tid + k * 256, nN - 1);
-                        T x[UP];
 #pragma unroll
-                        for (int u = 0; u < UP; u++) x[u] = v[so[u] + fnc];
-                        T a = acc[k][r];
+                        for (int b = 0; b < NV; b++) {
+                            T x[UP];
 #pragma unroll
-                        for (int u = 0; u < UP; u++) a += c[u] * x[u];
-                        acc[k][r] = a;
+                            for (int u = 0; u < UP; u++) x[u] = static_cast<const T*>(P.v[b])[so[u] + fnc];
+                            T a = acc[k][r][b];
+#pragma unroll
+                            for (int u = 0; u < UP; u++) a += c[u] * x[u];
+                            acc[k][r][b] = a;
+                        }
                     }
                 }
             }
@@ -213,7 +231,7 @@ __global__ __launch_bounds__(256) void k_hv(HvTables A, const T* __restrict__ v,
         for (int ch = 0; ch < nch; ch++) {
             __syncthreads();
             // Stage the round's active slot groups (four hops of one row each):
-            // X[in][slot] = s v[bi: ip, in] and Vs[col][slot] = V_eff[op][col].
+            // X[in][b][slot] = s v_b[bi: ip, in] and Vs[col][slot] = V_eff[op][col].
             // Offsets, not pointers, so every gather is an unconditional global
             // load from a kernel argument (a group's unused slots read a valid
             // element times 0).  Groups past a row's hops are neither staged nor read.
@@ -240,12 +258,16 @@ __global__ __launch_bounds__(256) void k_hv(HvTables A, const T* __restrict__ v,
                     }
                     const int so = r * PCH + g * 4;
                     for (int i = tid; i < nNi; i += 256) {
-                        T4 x4;
-                        x4.x = sg[0] * v[xo[0] + i];
-                        x4.y = sg[1] * v[xo[1] + i];
-                        x4.z = sg[2] * v[xo[2] + i];
-                        x4.w = sg[3] * v[xo[3] + i];
-                        *reinterpret_cast<T4*>(X + (size_t)i * SL + so) = x4;
+#pragma unroll
+                        for (int b = 0; b < NV; b++) {
+                            const T* vb = static_cast<const T*>(P.v[b]);
+                            T4 x4;
+                            x4.x = sg[0] * vb[xo[0] + i];
+                            x4.y = sg[1] * vb[xo[1] + i];
+                            x4.z = sg[2] * vb[xo[2] + i];
+                            x4.w = sg[3] * vb[xo[3] + i];
+                            *reinterpret_cast<T4*>(X + ((size_t)i * NV + b) * SL + so) = x4;
+                        }
                     }
                     for (int c = tid; c < c2; c += 256) {
                         T4 v4;
@@ -298,14 +320,18 @@ __global__ __launch_bounds__(256) void k_hv(HvTables A, const T* __restrict__ v,
 #pragma unroll
                             for (int k = 0; k < NPT; k++) {
                                 if (qq >= len[k]) continue;          // wave-uniform
-                                const T4 xv = *reinterpret_cast<const T4*>(X + (size_t)in[d][k] * SL + so);
                                 const T4 vv = *reinterpret_cast<const T4*>(Vs + (size_t)col[d][k] * SL + so);
-                                T a = acc[k][r];
-                                a += vv.x * xv.x;
-                                a += vv.y * xv.y;
-                                a += vv.z * xv.z;
-                                a += vv.w * xv.w;
-                                acc[k][r] = a;
+#pragma unroll
+                                for (int b = 0; b < NV; b++) {
+                                    const T4 xv =
+                                        *reinterpret_cast<const T4*>(X + ((size_t)in[d][k] * NV + b) * SL + so);
+                                    T a = acc[k][r][b];
+                                    a += vv.x * xv.x;
+                                    a += vv.y * xv.y;
+                                    a += vv.z * xv.z;
+                                    a += vv.w * xv.w;
+                                    acc[k][r][b] = a;
+                                }
                             }
                         }
                 }
@@ -315,36 +341,79 @@ __global__ __launch_bounds__(256) void k_hv(HvTables A, const T* __restrict__ v,
 #pragma unroll
     for (int r = 0; r < RG; r++) {
         if (r < nr) {
-            T* dst = w + bstart + (int64_t)(fp0 + r) * nN;
 #pragma unroll
-            for (int k = 0; k < NPT; k++) {
-                const int fn = tid + k * 256;
-                if (fn < nN) dst[fn] = acc[k][r];
+            for (int b = 0; b < NV; b++) {
+                T* dst = static_cast<T*>(P.w[b]) + bstart + (int64_t)(fp0 + r) * nN;
+#pragma unroll
+                for (int k = 0; k < NPT; k++) {
+                    const int fn = tid + k * 256;
+                    if (fn < nN) dst[fn] = acc[k][r][b];
+                }
             }
         }
     }
 }
 
-// instantiated shapes (NPT, RG, PCH)
-#define HV_NPTS(X_, T_, R_, P_) X_(T_, 1, R_, P_) X_(T_, 2, R_, P_) X_(T_, 3, R_, P_) X_(T_, 4, R_, P_) \
-    X_(T_, 6, R_, P_) X_(T_, 8, R_, P_) X_(T_, 12, R_, P_) X_(T_, 16, R_, P_)
-#define HV_FOR_SHAPES(X_, T_) HV_NPTS(X_, T_, 2, 4) HV_NPTS(X_, T_, 1, 4)
+// The dynamic-LDS limit is a property of the kernel function, shared by every
+// index that launches it: only ever raise it (indices of different widths may
+// be resident and in use from several threads at once).
+static void raise_lds_limit(const void* fn, size_t bytes) {
+    static std::mutex mu;
+    static std::map<const void*, size_t> set;
+    std::lock_guard<std::mutex> lk(mu);
+    size_t& cur = set[fn];
+    if (bytes <= cur) return;
+    SM_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    cur = bytes;
+}
 
-template <typename T> static const void* hv_fn(int npt, int rg, int pch) {
-#define HV_CASE(T_, N_, R_, P_) if (npt == N_ && rg == R_ && pch == P_) return (const void*)k_hv<T_, N_, R_, P_>;
+// instantiated shapes (NPT, RG, PCH, NV): one vector for every width; two and
+// four vectors (NV = 4 up to 1,536 neutron determinants per block)
+#define HV_NPTS(X_, T_, R_, P_, V_) X_(T_, 1, R_, P_, V_) X_(T_, 2, R_, P_, V_) X_(T_, 3, R_, P_, V_) \
+    X_(T_, 4, R_, P_, V_) X_(T_, 6, R_, P_, V_) X_(T_, 8, R_, P_, V_) X_(T_, 12, R_, P_, V_) X_(T_, 16, R_, P_, V_)
+#define HV_NPTS_NARROW(X_, T_, R_, P_, V_) X_(T_, 1, R_, P_, V_) X_(T_, 2, R_, P_, V_) X_(T_, 3, R_, P_, V_) \
+    X_(T_, 4, R_, P_, V_) X_(T_, 6, R_, P_, V_)
+#define HV_FOR_SHAPES(X_, T_) HV_NPTS(X_, T_, 2, 4, 1) HV_NPTS(X_, T_, 1, 4, 1) \
+    HV_NPTS(X_, T_, 2, 4, 2) HV_NPTS(X_, T_, 1, 4, 2) HV_NPTS_NARROW(X_, T_, 2, 4, 4) HV_NPTS_NARROW(X_, T_, 1, 4, 4)
+
+template <typename T> static const void* hv_fn(int npt, int rg, int pch, int nv) {
+#define HV_CASE(T_, N_, R_, P_, V_) \
+    if (npt == N_ && rg == R_ && pch == P_ && nv == V_) return (const void*)k_hv<T_, N_, R_, P_, V_>;
     HV_FOR_SHAPES(HV_CASE, T)
 #undef HV_CASE
     return nullptr;
 }
 
+// LDS of one workgroup for nv vectors: X (nN_max x slots x nv), Vs (cols2_max x
+// slots), own rows (nN_max x rg x nv)
+static size_t hv_lds_bytes(const HvTables& A, int nv, size_t tsize) {
+    return ((size_t)A.nN_max * A.pch * A.rg * nv + (size_t)A.cols2_max * A.pch * A.rg + (size_t)A.nN_max * A.rg * nv) *
+           tsize;
+}
+
+// Widest instantiated vector count <= nv whose workgroup LDS stays within 32 KiB.
+// Measured per-vector times (FP32 / FP64): Cr48 (14.5 KiB per vector) 0.49 -> 0.34 ms
+// with two vectors and 0.36 with four (48 KiB); Mg24 0.044 -> 0.017 ms with four;
+// Mn50 and Cr48 FP64 (over 32 KiB at two) gain nothing and lose occupancy.
+static constexpr size_t HV_LDS_SHARED = 32 * 1024;
+template <typename T> static int hv_width(const HvTables& A, int nv) {
+    for (int c : {4, 2}) {
+        if (c > nv) continue;
+        if (hv_fn<T>(A.npt, A.rg, A.pch, c) && hv_lds_bytes(A, c, sizeof(T)) <= HV_LDS_SHARED) return c;
+    }
+    return 1;
+}
+
 template <typename T>
-static void launch_t(const HvTables& A, const T* v, T* w, hipStream_t s) {
+static void launch_t(const HvTables& A, int nv, const HvVecs& P, hipStream_t s) {
     dim3 grid(A.G), block(256);
+    const size_t lds = hv_lds_bytes(A, nv, sizeof(T));
+    if (nv > 1) raise_lds_limit(hv_fn<T>(A.npt, A.rg, A.pch, nv), lds);
     bool done = false;
-#define HV_LAUNCH(T_, N_, R_, P_)                                                              \
-    if (!done && A.npt == N_ && A.rg == R_ && A.pch == P_) {                                   \
-        hipLaunchKernelGGL((k_hv<T_, N_, R_, P_>), grid, block, A.lds_bytes, s, A, v, w);     \
-        done = true;                                                                           \
+#define HV_LAUNCH(T_, N_, R_, P_, V_)                                                                      \
+    if (!done && A.npt == N_ && A.rg == R_ && A.pch == P_ && nv == V_) {                                   \
+        hipLaunchKernelGGL((k_hv<T_, N_, R_, P_, V_>), grid, block, lds, s, A, P);                        \
+        done = true;                                                                                       \
     }
     HV_FOR_SHAPES(HV_LAUNCH, T)
 #undef HV_LAUNCH
@@ -354,8 +423,41 @@ static void launch_t(const HvTables& A, const T* v, T* w, hipStream_t s) {
 
 void launch_hv(const DeviceHam& h, int prec, const void* v, void* w, hipStream_t s) {
     if (h.idx->R == 0) return;
-    if (prec == 0) launch_t<double>(h.view[0], (const double*)v, (double*)w, s);
-    else launch_t<float>(h.view[1], (const float*)v, (float*)w, s);
+    HvVecs P{};
+    P.v[0] = v;
+    P.w[0] = w;
+    if (prec == 0) launch_t<double>(h.view[0], 1, P, s);
+    else launch_t<float>(h.view[1], 1, P, s);
+}
+
+template <typename T>
+static void launch_set_t(const HvTables& A, int nvec, const void* const* v, void* const* w, hipStream_t s) {
+    for (int b = 0; b < nvec;) {
+        const int c = hv_width<T>(A, nvec - b);
+        HvVecs P{};
+        for (int j = 0; j < c; j++) { P.v[j] = v[b + j]; P.w[j] = w[b + j]; }
+        launch_t<T>(A, c, P, s);
+        b += c;
+    }
+}
+
+void launch_hv_set(const DeviceHam& h, int prec, int nvec, const void* const* v, void* const* w, hipStream_t s) {
+    if (h.idx->R == 0 || nvec <= 0) return;
+    if (prec == 0) launch_set_t<double>(h.view[0], nvec, v, w, s);
+    else launch_set_t<float>(h.view[1], nvec, v, w, s);
+}
+
+void launch_hv_multi(const DeviceHam& h, int prec, int nvec, const void* v, int64_t ldv, void* w, int64_t ldw,
+                     hipStream_t s) {
+    if (h.idx->R == 0 || nvec <= 0) return;
+    const size_t es = prec == 0 ? 8 : 4;
+    std::vector<const void*> vp(nvec);
+    std::vector<void*> wp(nvec);
+    for (int b = 0; b < nvec; b++) {
+        vp[b] = static_cast<const char*>(v) + (size_t)b * ldv * es;
+        wp[b] = static_cast<char*>(w) + (size_t)b * ldw * es;
+    }
+    launch_hv_set(h, prec, nvec, vp.data(), wp.data(), s);
 }
 
 // ---------------------------------------------------------------- upload
@@ -579,18 +681,6 @@ void index_layout(const Basis& b, const Tables& t, HostLayout& L) {
     }
 }
 
-// The dynamic-LDS limit is a property of the kernel function, shared by every
-// index that launches it: only ever raise it (indices of different widths may
-// be resident and in use from several threads at once).
-void raise_lds_limit(const void* fn, size_t bytes) {
-    static std::mutex mu;
-    static std::map<const void*, size_t> set;
-    std::lock_guard<std::mutex> lk(mu);
-    size_t& cur = set[fn];
-    if (bytes <= cur) return;
-    SM_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
-    cur = bytes;
-}
 
 template <typename T> void set_view(DeviceIndex& h, int prec, const HostLayout& L) {
     HvTables& A = h.view[prec];
@@ -642,7 +732,7 @@ template <typename T> void set_view(DeviceIndex& h, int prec, const HostLayout& 
     A.G = (int)gb.size();
     A.grp_block = h.grp_block[prec].as<int32_t>();
     A.grp_fp0 = h.grp_fp0[prec].as<int32_t>();
-    const void* fn = hv_fn<T>(A.npt, A.rg, A.pch);
+    const void* fn = hv_fn<T>(A.npt, A.rg, A.pch, 1);
     if (!fn) throw std::runtime_error("no H·v kernel instance for this shape");
     raise_lds_limit(fn, A.lds_bytes);
 }
@@ -659,6 +749,25 @@ struct HostCoefficients {
     std::vector<float> ep32, en32, pp_val32, nn_val32, veff32;
     uint64_t pattern = 0;
 };
+
+// hash of the FP64 coefficient values (the FP32 copies derive from them)
+uint64_t content_hash(const HostCoefficients& c) {
+    uint64_t hsh = 1469598103934665603ull;
+    for (const std::vector<double>* v : {&c.ep, &c.en, &c.pp_val, &c.nn_val, &c.veff}) {
+        hsh = (hsh ^ (uint64_t)v->size()) * 1099511628211ull;
+        for (double x : *v) {
+            uint64_t u;
+            std::memcpy(&u, &x, 8);
+            hsh = (hsh ^ u) * 1099511628211ull;
+        }
+    }
+    return hsh;
+}
+
+bool same_coefficients(const HostCoefficients& a, const HostCoefficients& b) {
+    return a.pattern == b.pattern && a.ep == b.ep && a.en == b.en && a.pp_val == b.pp_val && a.nn_val == b.nn_val &&
+           a.veff == b.veff;
+}
 
 uint64_t coefficient_pattern(const Basis& b, const Tables& t) {
     uint64_t hsh = 1469598103934665603ull;

@@ -76,6 +76,7 @@ struct Request {
     std::string error;
     std::shared_ptr<IndexEntry> entry;
     std::shared_ptr<const HostCoefficients> coef;
+    uint64_t coef_hash = 0;   // content_hash(*coef): candidates for a shared batch
     int64_t mdim = 0, ws_need = 0, coef_need = 0;
     bool uploader = false, index_shared = false;
     int bypassed = 0;
@@ -89,10 +90,24 @@ struct Request {
 struct Slot {
     int id = 0;
     hipStream_t stream = nullptr;
-    SolveCtx ctx;
+    SolveCtx ctx;             // one request, or member 0 of a batch
+    // batches: the H·v stream and members 1.. (each its own stream and workspace)
+    hipStream_t hv_stream = nullptr;
+    std::vector<std::unique_ptr<SolveCtx>> more;
     DeviceHam ham;
-    std::shared_ptr<Request> job;
+    std::shared_ptr<Request> job;                  // first (or only) request
+    std::vector<std::shared_ptr<Request>> batch;   // every request of the slot's run
     int64_t cap = 0;          // HBM held by this slot's buffers (server mutex)
+    SolveCtx& member(int b) { return b == 0 ? ctx : *more[b - 1]; }
+    int64_t workspace_bytes() const {
+        int64_t c = workspace_capacity(ctx);
+        for (auto& x : more) c += workspace_capacity(*x);
+        return c;
+    }
+    void release_workspaces() {
+        release_workspace(ctx);
+        for (auto& x : more) release_workspace(*x);
+    }
     bool release = false;     // free the buffers before waiting again
     std::condition_variable cv;
     std::thread th;
@@ -198,6 +213,7 @@ struct Server {
                 throw std::invalid_argument("interaction has a different pp/nn sparsity pattern than the shared tables");
         }
         r.entry = e;
+        r.coef_hash = content_hash(*r.coef);
         r.mdim = e->basis.mdim;
         r.ws_need = solve_workspace_bytes(r.mdim, r.params);
         r.coef_need = device_bytes(*r.coef);
@@ -266,7 +282,7 @@ struct Server {
             if (!s->job && s->cap > 0) {
                 s->ham.idx.reset();
                 s->ham.release_coefficients();
-                release_workspace(s->ctx);
+                s->release_workspaces();
                 s->cap = 0;
                 return true;
             }
@@ -284,30 +300,59 @@ struct Server {
             finish_locked(*r, SMGPU_REQ_FAILED, SMGPU_ENOMEM, os.str());
             return true;
         }
+        // requests that can share this one's H·v launches: same index tables, identical
+        // coefficients and precision, in ready-list order
+        std::vector<std::shared_ptr<Request>> mates;
+        if (cfg.max_batch > 1)
+            for (auto& q : ready) {
+                if ((int)mates.size() + 1 >= cfg.max_batch) break;
+                if (q == r || q->cancel || q->entry != r->entry || q->coef_hash != r->coef_hash) continue;
+                if ((q->params.precision == 0) != (r->params.precision == 0)) continue;
+                if (!same_coefficients(*q->coef, *r->coef)) continue;
+                mates.push_back(q);
+            }
         for (;;) {
             Slot* best = nullptr;
             int64_t best_g = 0;
+            int best_n = 1;
             for (auto& s : slots) {
                 if (s->job) continue;
-                int64_t g = workspace_growth(s->ctx, r->mdim, r->params) + coefficient_growth(s->ham, *r->coef);
-                if (!best || g < best_g) { best = s.get(); best_g = g; }
+                // the largest batch (up to the mates found) whose growth fits the budget
+                int64_t g = coefficient_growth(s->ham, *r->coef) + workspace_growth(s->ctx, r->mdim, r->params);
+                int n = 1;
+                for (size_t i = 0; i < mates.size(); i++) {
+                    const int64_t gi = workspace_growth(s->member((int)i + 1), mates[i]->mdim, mates[i]->params);
+                    if (in_use() + need_index + g + gi > cfg.hbm_budget) break;
+                    g += gi;
+                    n++;
+                }
+                if (!best || n > best_n || (n == best_n && g < best_g)) { best = s.get(); best_g = g; best_n = n; }
             }
             if (!best) return false;
             if (in_use() + need_index + best_g <= cfg.hbm_budget) {
-                ready.remove(r);
-                r->state = SMGPU_REQ_RUNNING;
-                r->slot = best->id;
-                r->t_admit = now_ms();
-                r->index_shared = e.didx != nullptr || e.uploading;
+                std::vector<std::shared_ptr<Request>> run{r};
+                for (int i = 0; i + 1 < best_n; i++) run.push_back(mates[i]);
+                for (auto& q : run) {
+                    ready.remove(q);
+                    q->state = SMGPU_REQ_RUNNING;
+                    q->slot = best->id;
+                    q->t_admit = now_ms();
+                    q->index_shared = e.didx != nullptr || e.uploading;
+                    e.users++;
+                }
                 if (need_index) { e.uploading = true; r->uploader = true; index_resident += e.index_bytes; }
-                e.users++;
                 e.last_use = ++use_clock;
                 r->reserved = need_index + best_g;
                 best->cap += best_g;
                 best->job = r;
+                best->batch = run;
+                if (run.size() > 1) {
+                    st.batched += (int64_t)run.size();
+                    st.peak_batch = std::max(st.peak_batch, (int32_t)run.size());
+                }
                 hbm_peak = std::max(hbm_peak, in_use());
                 int running = 0;
-                for (auto& s : slots) running += s->job ? 1 : 0;
+                for (auto& s : slots) running += s->job ? (int)s->batch.size() : 0;
                 st.peak_running = std::max(st.peak_running, running);
                 best->cv.notify_one();
                 return true;
@@ -319,16 +364,21 @@ struct Server {
     void schedule_locked() {
         if (stopping) return;
         const int max_bypass = 4 * (int)slots.size();
-        bool head_blocked = false;
-        for (auto it = ready.begin(); it != ready.end();) {
-            auto r = *it;
-            ++it;   // try_admit may remove r
-            if (head_blocked && ready.front()->bypassed >= max_bypass) break;
-            if (try_admit_locked(r)) {
-                if (head_blocked && r != ready.front()) ready.front()->bypassed++;
-                continue;
+        // An admission removes the request and any batch mates from the ready list,
+        // so every admission ends the pass and the scan starts again from the front.
+        for (bool admitted = true; admitted;) {
+            admitted = false;
+            bool head_blocked = false;
+            for (auto it = ready.begin(); it != ready.end(); ++it) {
+                if (head_blocked && ready.front()->bypassed >= max_bypass) break;
+                auto r = *it;
+                if (try_admit_locked(r)) {
+                    if (head_blocked && !ready.empty() && r != ready.front()) ready.front()->bypassed++;
+                    admitted = true;
+                    break;
+                }
+                head_blocked = true;
             }
-            if (!head_blocked) head_blocked = true;
         }
     }
 
@@ -362,14 +412,44 @@ struct Server {
             idx = e.didx;
         }
         bind_ham(s.ham, idx, *r.coef);
-        r.evals.assign(std::max(1, r.params.num_ev), 0.0);
-        s.ctx.cancel = &r.cancel;
+        const int nb = (int)s.batch.size();
+        for (auto& q : s.batch) q->evals.assign(std::max(1, q->params.num_ev), 0.0);
         struct Unhook {
-            SolveCtx& c;
-            ~Unhook() { c.cancel = nullptr; }
-        } unhook{s.ctx};
-        trl_solve(s.ham, r.params, r.evals.data(), nullptr, r.st, nullptr, &s.ctx);
-        r.evals.resize((size_t)std::min<int64_t>(r.params.num_ev, r.mdim));
+            Slot& s;
+            int nb;
+            ~Unhook() { for (int b = 0; b < nb; b++) s.member(b).cancel = nullptr; }
+        } unhook{s, nb};
+        for (int b = 0; b < nb; b++) s.member(b).cancel = &s.batch[b]->cancel;
+        if (nb == 1) {
+            trl_solve(s.ham, r.params, r.evals.data(), nullptr, r.st, nullptr, &s.ctx);
+            r.evals.resize((size_t)std::min<int64_t>(r.params.num_ev, r.mdim));
+            return;
+        }
+        // lockstep batch: each member's result is published as soon as it finishes
+        std::vector<SolveParams> p(nb);
+        std::vector<double*> ev(nb);
+        std::vector<SolveStats> sst(nb);
+        std::vector<const SolveCtx*> cx(nb);
+        std::vector<int32_t> status(nb, SMGPU_OK);
+        std::vector<std::string> msgs(nb);
+        for (int b = 0; b < nb; b++) {
+            p[b] = s.batch[b]->params;
+            p[b].num_ev = (int32_t)std::min<int64_t>(p[b].num_ev, r.mdim);
+            ev[b] = s.batch[b]->evals.data();
+            cx[b] = &s.member(b);
+        }
+        trl_solve_batch(s.ham, nb, p.data(), nullptr, ev.data(), nullptr, sst.data(), cx.data(), s.hv_stream,
+                        status.data(), msgs.data(), [&](int b) {
+                            Request& q = *s.batch[b];
+                            q.st = sst[b];
+                            q.evals.resize((size_t)p[b].num_ev);
+                            std::lock_guard<std::mutex> lk(m);
+                            q.entry->users--;
+                            q.entry->last_use = ++use_clock;
+                            if (status[b] == SMGPU_OK) finish_locked(q, SMGPU_REQ_DONE, SMGPU_OK, "");
+                            else if (status[b] == SMGPU_ECANCELED) finish_locked(q, SMGPU_REQ_CANCELED, status[b], msgs[b]);
+                            else finish_locked(q, SMGPU_REQ_FAILED, status[b], msgs[b]);
+                        });
     }
 
     void slot_loop(Slot& s) {
@@ -385,24 +465,39 @@ struct Server {
             int32_t code = SMGPU_OK;
             std::string msg;
             try {
-                if (r->cancel) throw Canceled();
+                // (a batch member's own cancel flag is checked at its first step)
+                if (r->cancel && s.batch.size() == 1) throw Canceled();
                 run_job(s, *r);
             } catch (...) {
                 code = status_of_current_exception(msg);
             }
             std::lock_guard<std::mutex> lk(m);
             s.ham.idx.reset();
-            r->entry->users--;
-            r->entry->last_use = ++use_clock;
+            // requests the batch driver has not finished (one request alone, or a run
+            // that failed before the driver started)
+            for (size_t b = 1; b < s.batch.size(); b++) {
+                Request& q = *s.batch[b];
+                if (finished(q)) continue;
+                q.entry->users--;
+                if (code == SMGPU_ECANCELED || q.cancel) finish_locked(q, SMGPU_REQ_CANCELED, SMGPU_ECANCELED, "canceled");
+                else finish_locked(q, SMGPU_REQ_FAILED, code == SMGPU_OK ? SMGPU_EINVAL : code, msg);
+            }
+            const bool open = !finished(*r);
+            if (open) {
+                r->entry->users--;
+                r->entry->last_use = ++use_clock;
+            }
             // charge what the slot really holds now
-            s.cap = workspace_capacity(s.ctx) + s.ham.coefficient_capacity();
+            s.cap = s.workspace_bytes() + s.ham.coefficient_capacity();
             if (s.cap > cfg.keep_bytes) {
                 s.ham.release_coefficients();
-                release_workspace(s.ctx);
+                s.release_workspaces();
                 s.cap = 0;
             }
             s.job.reset();
-            if (code == SMGPU_OK) finish_locked(*r, SMGPU_REQ_DONE, SMGPU_OK, "");
+            s.batch.clear();
+            if (!open) {
+            } else if (code == SMGPU_OK) finish_locked(*r, SMGPU_REQ_DONE, SMGPU_OK, "");
             else if (code == SMGPU_ECANCELED) finish_locked(*r, SMGPU_REQ_CANCELED, code, msg);
             else finish_locked(*r, SMGPU_REQ_FAILED, code, msg);
             schedule_locked();
@@ -418,6 +513,8 @@ struct Server {
         if (cfg.table_threads <= 0)
             cfg.table_threads = std::max(1, (int)std::thread::hardware_concurrency() / cfg.host_threads);
         if (cfg.keep_bytes <= 0) cfg.keep_bytes = (int64_t)16 << 30;
+        if (cfg.max_batch <= 0) cfg.max_batch = 1;
+        if (cfg.max_batch > 16) throw std::invalid_argument("max_batch above 16");
         if (cfg.hbm_budget <= 0) {
             size_t fr = 0, tot = 0;
             SM_HIP(hipMemGetInfo(&fr, &tot));
@@ -428,6 +525,14 @@ struct Server {
             s->id = i;
             SM_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
             s->ctx.stream = s->stream;
+            if (cfg.max_batch > 1) {
+                SM_HIP(hipStreamCreateWithFlags(&s->hv_stream, hipStreamNonBlocking));
+                for (int b = 1; b < cfg.max_batch; b++) {
+                    auto c = std::make_unique<SolveCtx>();
+                    SM_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+                    s->more.push_back(std::move(c));
+                }
+            }
             slots.push_back(std::move(s));
         }
         for (auto& s : slots) {
@@ -455,8 +560,10 @@ struct Server {
         ready.clear();
         for (auto& s : slots) {
             s->ham.idx.reset();
-            release_workspace(s->ctx);
+            s->release_workspaces();
             if (s->stream) (void)hipStreamDestroy(s->stream);
+            if (s->hv_stream) (void)hipStreamDestroy(s->hv_stream);
+            for (auto& c : s->more) if (c->stream) (void)hipStreamDestroy(c->stream);
         }
         cache.clear();
     }
@@ -532,7 +639,7 @@ struct Server {
         s.ready = (int32_t)ready.size();
         s.running = 0;
         for (auto& kv : reqs) if (kv.second->state == SMGPU_REQ_PREPARING) s.preparing++;
-        for (auto& sl : slots) if (sl->job) s.running++;
+        for (auto& sl : slots) if (sl->job) s.running += (int32_t)sl->batch.size();
         s.slots = (int32_t)slots.size();
         s.hbm_budget = cfg.hbm_budget;
         s.hbm_in_use = in_use();
@@ -561,6 +668,7 @@ void smgpu_default_server_config(smgpu_server_config* cfg) {
     cfg->host_threads = 2;
     cfg->table_threads = 0;
     cfg->keep_bytes = 0;
+    cfg->max_batch = 1;
 }
 
 int32_t smgpu_server_create(const smgpu_server_config* cfg, smgpu_server* out) {

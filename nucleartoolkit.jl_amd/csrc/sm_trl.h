@@ -4,7 +4,9 @@
 #include "sm_device.h"
 
 #include <atomic>
+#include <functional>
 #include <stdexcept>
+#include <string>
 
 namespace smgpu {
 
@@ -44,6 +46,14 @@ struct JProj {
 
 void trl_solve(DeviceHam& h, const SolveParams& p, double* evals, double* evecs, SolveStats& st,
                const double* v0 = nullptr, const SolveCtx* ctx = nullptr, const JProj* jp = nullptr);
+// nb TRL solves on one Hamiltonian in lockstep with shared H·v launches on hv_stream;
+// ctx[b] gives member b its stream and workspace (distinct streams).  status[b] gets
+// each member's SMGPU_* code and msgs[b] (may be null) its error text (one member's
+// failure does not stop the others); on_done(b), if set, runs when member b has
+// finished (its results written) or failed.
+void trl_solve_batch(DeviceHam& h, int nb, const SolveParams* p, const double* const* v0, double* const* evals,
+                     double* const* evecs, SolveStats* st, const SolveCtx* const* ctx, hipStream_t hv_stream,
+                     int32_t* status, std::string* msgs = nullptr, const std::function<void(int)>& on_done = {});
 // thick-restart block Lanczos with block size q (TRBL); v0: q start vectors (q*mdim) or null
 void trbl_solve(DeviceHam& h, int q, const SolveParams& p, double* evals, double* evecs, SolveStats& st,
                 const double* v0 = nullptr, const JProj* jp = nullptr);

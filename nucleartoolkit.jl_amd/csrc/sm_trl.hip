@@ -32,12 +32,15 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <functional>
+#include <memory>
 #include <random>
 #include <stdexcept>
 #include <string>
 
 #include "dense_eig.h"
 #include "sm_device.h"
+#include "sm_status.h"
 #include "sm_trl.h"
 #include "tridiag.h"
 
@@ -682,11 +685,15 @@ struct Workspace {
     DBuf basis, wH, w64, part, npart, npart2, h1, h2, alpha, beta, enring, flags, S, swork, U, scal, tvals, Y, Q,
         R, hv64, loc, gate, jy, jz, om;
     int* hflag = nullptr;          // pinned mirror of the convergence-flag ring
-    hipEvent_t ev[4] = {};
+    hipEvent_t ev[4] = {};         // side stream: a step's convergence flag is in hflag
+    hipEvent_t evm[4] = {};        // main stream: a step's alpha/beta are written
+    hipStream_t side = nullptr;    // Ritz values and convergence test, off the H·v stream
     void host_sync_objects() {
         if (hflag) return;
         SM_HIP(hipHostMalloc((void**)&hflag, 4 * sizeof(int), hipHostMallocDefault));
         for (int q = 0; q < 4; q++) SM_HIP(hipEventCreateWithFlags(&ev[q], hipEventDisableTiming));
+        for (int q = 0; q < 4; q++) SM_HIP(hipEventCreateWithFlags(&evm[q], hipEventDisableTiming));
+        SM_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
     }
     int64_t capacity() const {
         int64_t c = 0;
@@ -696,6 +703,8 @@ struct Workspace {
         return c;
     }
     ~Workspace() {
+        if (side) { (void)hipStreamSynchronize(side); (void)hipStreamDestroy(side); }
+        for (int q = 0; q < 4; q++) if (evm[q]) (void)hipEventDestroy(evm[q]);
         for (int q = 0; q < 4; q++) if (ev[q]) (void)hipEventDestroy(ev[q]);
         if (hflag) (void)hipHostFree(hflag);
     }
@@ -734,7 +743,7 @@ static void workspace_plan(int64_t n, const SolveParams& p, F&& f) {
     f(&Workspace::Y, 32 * lm * 8);
     f(&Workspace::Q, 32 * 32 * 8);
     f(&Workspace::R, nev * ld * 8);
-    f(&Workspace::hv64, mixed && p.refine ? ld * 8 : 0);
+    f(&Workspace::hv64, mixed && p.refine ? nev * ld * 8 : 0);   // FP64 H·v of the Ritz vectors
     f(&Workspace::jy, ld * 8);     // J projection scratch (FP64), small next to the basis
     f(&Workspace::jz, ld * 8);
 }
@@ -902,8 +911,12 @@ struct Solver {
     }
 
     void enqueue_iteration(int it, int step, int nprev) {
-        if (prec_hv == 0) launch_hv(H, 0, V(it), W.wH.p, s);
-        else launch_hv(H, 1, V(it), W.wH.p, s);
+        launch_hv(H, prec_hv, V(it), W.wH.p, s);
+        after_hv(it, step, nprev);
+    }
+
+    // the rest of a Lanczos step once w = H v_it is in W.wH
+    void after_hv(int it, int step, int nprev) {
         st.n_hv++;
         if (reorth_cgs2) {
             const int nv = it + 1;
@@ -918,10 +931,16 @@ struct Solver {
             if (prec_hv == 0) reorth_local<double>(it, W.wH.as<double>());
             else reorth_local<float>(it, W.wH.as<float>());
         }
+        // Ritz values of T[0..it] and diagonalize_T!'s test run on the side stream once
+        // this step's alpha/beta exist; the next H·v does not wait for them (later steps
+        // write other entries of alpha/beta; a restart starts after the host has waited
+        // for the last test)
         const int nn = std::min(nev, it + 1);
-        hipLaunchKernelGGL(k_ritz, dim3(nn), dim3(TB), 0, s, W.alpha.as<double>(), W.beta.as<double>(), it + 1,
+        SM_HIP(hipEventRecord(W.evm[(step + 1) & 3], s));
+        SM_HIP(hipStreamWaitEvent(W.side, W.evm[(step + 1) & 3], 0));
+        hipLaunchKernelGGL(k_ritz, dim3(nn), dim3(TB), 0, W.side, W.alpha.as<double>(), W.beta.as<double>(), it + 1,
                            en(step - 1), nprev, en(step));
-        hipLaunchKernelGGL(k_conv, dim3(1), dim3(64), 0, s, en(step - 1), en(step), nev, nn, P.tol, flag(step));
+        hipLaunchKernelGGL(k_conv, dim3(1), dim3(64), 0, W.side, en(step - 1), en(step), nev, nn, P.tol, flag(step));
         if (it + 1 < lm)
             hipLaunchKernelGGL((k_store<TBv>), dim3(gstore), dim3(TB), 0, s, W.w64.as<double>(),
                                W.beta.as<double>() + it, V(it + 1), n);
@@ -931,6 +950,7 @@ struct Solver {
         // grow-only buffers cached on the handle (no per-solve hipMalloc / memset)
         workspace_plan<TBv>(n, P, [&](DBuf Workspace::*m, int64_t b) { (W.*m).ensure((size_t)b); });
         W.host_sync_objects();
+        SM_HIP(hipStreamSynchronize(W.side));     // a canceled solve may have left tests queued
         std::vector<double> e0(4 * 64, 1.0e4);
         SM_HIP(hipMemcpyAsync(W.enring.p, e0.data(), e0.size() * 8, hipMemcpyHostToDevice, s));
         SM_HIP(hipMemsetAsync(W.loc.p, 0, W.loc.bytes, s));
@@ -966,76 +986,101 @@ struct Solver {
         setup(v0_in);
         for (int it = 0; it < nsteps; it++) enqueue_iteration(it, it, it == 0 ? 0 : std::min(nev, it));
         SM_HIP(hipStreamSynchronize(s));
+        SM_HIP(hipStreamSynchronize(W.side));
         SM_HIP(hipMemcpy(alpha_out, W.alpha.p, nsteps * 8, hipMemcpyDeviceToHost));
         SM_HIP(hipMemcpy(beta_out, W.beta.p, nsteps * 8, hipMemcpyDeviceToHost));
     }
 
-    void run(double* evals, double* evecs, const double* v0_in = nullptr) {
-        auto t0 = std::chrono::steady_clock::now();
+    // ---- driver: start(), then per step H·v of hv_in() into hv_out() on the solve's
+    // stream followed by advance(), until advance() reports the end; then end().
+    // run() is the one-solve loop; a batch driver issues the H·v of several solves
+    // on one Hamiltonian in shared launches between their advance() calls.
+    int d_k = 0, d_it = 0, d_step = 0, d_first = 0, d_elit = 1, d_conv_step = 0;
+    int64_t d_total = 0;
+    bool d_conv = false, d_after_restart = false;
+    std::chrono::steady_clock::time_point d_t0;
+
+    void start(const double* v0_in) {
+        d_t0 = std::chrono::steady_clock::now();
         if (lm < 3 || ls < 1 || ls > 32 || ls >= lm - 1 || nev < 1 || nev > ls)
             throw std::invalid_argument("need 1 <= num_ev <= ls <= 32 and ls < lm - 1");
         if (lm > 1000) throw std::invalid_argument("lm above 1000 is not supported");
         setup(v0_in);
         if (jp) jfilter(0, 0, true);
-        int* hflag = W.hflag;
-        hipEvent_t* ev = W.ev;
-        int k = 0;
-        bool conv = false;
-        int elit = 1, conv_step = 0;
-        int step = 0;
-        int64_t total_steps = 0;
+        d_k = 0; d_it = 0; d_step = 0; d_first = 0; d_elit = 1; d_conv_step = 0;
+        d_total = 0; d_conv = false; d_after_restart = false;
         st.n_hv = 0; st.n_restart = 0;
-        bool after_restart = false;
-        auto check = [&](int stp) -> bool {
-            SM_HIP(hipEventSynchronize(ev[(stp + 1) & 3]));
-            return hflag[(stp + 1) & 3] != 0;
-        };
-        while (!conv) {
-            int first_step = step;
-            int it = k;
-            for (; it < lm - 1; it++, step++) {
-                if (++total_steps > P.max_iter) throw std::runtime_error("TRL: iteration limit reached");
-                if (cancel && cancel->load(std::memory_order_relaxed)) throw Canceled();
-                // interlacing brackets need T[0..it-1] to be the previous step's matrix
-                const int nprev = (it == k && after_restart) || it == 0 ? 0 : std::min(nev, it);
-                enqueue_iteration(it, step, nprev);
-                SM_HIP(hipMemcpyAsync(hflag + ((step + 1) & 3), flag(step), sizeof(int), hipMemcpyDeviceToHost, s));
-                SM_HIP(hipEventRecord(ev[(step + 1) & 3], s));
-                if (jp && it + 1 < lm && it + 1 < n) {
-                    // the J subspace is exhausted when beta vanishes (TRL: tbeta < Jtol stops)
-                    double b = 0.0;
-                    SM_HIP(hipMemcpyAsync(&b, W.beta.as<double>() + it, 8, hipMemcpyDeviceToHost, s));
-                    SM_HIP(hipStreamSynchronize(s));
-                    // ... or when the projection of the next vector keeps almost nothing of
-                    // it: the residual was rounding noise outside the (invariant) J space
-                    if (b < jp->tol || jfilter(it + 1, it + 1, false) < J_EXHAUSTED) {
-                        conv = true; conv_step = step; elit = it + 1;
-                        step++;
-                        break;
-                    }
-                }
-                if (it + 1 == n) {                         // Krylov space exhausted (it == mdim)
-                    if (step > first_step && check(step - 1)) { conv_step = step - 1; elit = it; }
-                    else { conv_step = step; elit = it + 1; }
-                    conv = true;
-                    step++;
-                    break;
-                }
-                if (step > first_step && check(step - 1)) {
-                    conv = true; conv_step = step - 1; elit = it;
-                    step++;
-                    break;
-                }
+        begin_step();
+    }
+
+    // checks before the H·v of step d_step (iteration d_it)
+    void begin_step() {
+        if (++d_total > P.max_iter) throw std::runtime_error("TRL: iteration limit reached");
+        if (cancel && cancel->load(std::memory_order_relaxed)) throw Canceled();
+    }
+
+    const TBv* hv_in() { return V(d_it); }
+    void* hv_out() { return W.wH.p; }
+    hipStream_t stream() const { return s; }
+
+    bool check(int stp) {
+        SM_HIP(hipEventSynchronize(W.ev[(stp + 1) & 3]));
+        return W.hflag[(stp + 1) & 3] != 0;
+    }
+
+    // after the H·v of the current step: enqueue the rest of it, run the lag-1
+    // convergence test, restart at the end of a sweep.  true when the solve is done.
+    bool advance() {
+        const int it = d_it, step = d_step, k = d_k;
+        // interlacing brackets need T[0..it-1] to be the previous step's matrix
+        const int nprev = (it == k && d_after_restart) || it == 0 ? 0 : std::min(nev, it);
+        after_hv(it, step, nprev);
+        SM_HIP(hipMemcpyAsync(W.hflag + ((step + 1) & 3), flag(step), sizeof(int), hipMemcpyDeviceToHost, W.side));
+        SM_HIP(hipEventRecord(W.ev[(step + 1) & 3], W.side));
+        bool done = false;
+        if (jp && it + 1 < lm && it + 1 < n) {
+            // the J subspace is exhausted when beta vanishes (TRL: tbeta < Jtol stops)
+            double b = 0.0;
+            SM_HIP(hipMemcpyAsync(&b, W.beta.as<double>() + it, 8, hipMemcpyDeviceToHost, s));
+            SM_HIP(hipStreamSynchronize(s));
+            // ... or when the projection of the next vector keeps almost nothing of
+            // it: the residual was rounding noise outside the (invariant) J space
+            if (b < jp->tol || jfilter(it + 1, it + 1, false) < J_EXHAUSTED) {
+                d_conv_step = step; d_elit = it + 1; done = true;
             }
-            if (conv) break;
-            // last iteration of the sweep: its flag decides between stop and restart
-            if (check(step - 1)) { conv = true; conv_step = step - 1; elit = lm - 1; break; }
-            restart();
-            st.n_restart++;
-            k = ls;
-            after_restart = true;
         }
+        if (!done && it + 1 == n) {                         // Krylov space exhausted (it == mdim)
+            if (step > d_first && check(step - 1)) { d_conv_step = step - 1; d_elit = it; }
+            else { d_conv_step = step; d_elit = it + 1; }
+            done = true;
+        }
+        if (!done && step > d_first && check(step - 1)) {
+            d_conv_step = step - 1; d_elit = it; done = true;
+        }
+        d_it = it + 1;
+        d_step = step + 1;
+        if (!done && d_it == lm - 1) {
+            // last iteration of the sweep: its flag decides between stop and restart
+            if (check(d_step - 1)) {
+                d_conv_step = d_step - 1; d_elit = lm - 1; done = true;
+            } else {
+                restart();
+                st.n_restart++;
+                d_k = ls;
+                d_it = ls;
+                d_first = d_step;
+                d_after_restart = true;
+            }
+        }
+        d_conv = done;
+        if (!done) begin_step();
+        return done;
+    }
+
+    void end(double* evals, double* evecs) {
         SM_HIP(hipStreamSynchronize(s));
+        SM_HIP(hipStreamSynchronize(W.side));
+        const int elit = d_elit, conv_step = d_conv_step;
         if (elit < nev) throw std::invalid_argument("the Krylov space holds fewer states than num_ev (J space too small)");
         st.iterations = elit;
         std::vector<double> en_last(nev);
@@ -1046,7 +1091,15 @@ struct Solver {
         st.n_reorth = partial ? (int32_t)lc[5] : st.n_hv;
         for (int i = 0; i < nev; i++) { evals[i] = en_last[i]; st.ritz[i] = en_last[i]; }
         finish(elit, conv_step, evals, evecs);
-        st.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        st.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - d_t0).count();
+    }
+
+    void run(double* evals, double* evecs, const double* v0_in = nullptr) {
+        start(v0_in);
+        do {
+            launch_hv(H, prec_hv, hv_in(), hv_out(), s);
+        } while (!advance());
+        end(evals, evecs);
     }
 
     // <y|J^2|y> / <y|y> with z = J^2 y (synchronous)
@@ -1234,10 +1287,25 @@ struct Solver {
         bool need_vecs = evecs != nullptr || (prec_hv == 1 && P.refine);
         if (!need_vecs) return;
         // eigenvectors of T[0..elit-1] for the converged Ritz values
+        // (inverse iteration on the host: a few thousand dependent operations, which
+        // one GPU thread would spend milliseconds on in global-memory latency)
         double* Y = W.Y.as<double>();
-        hipLaunchKernelGGL(k_tri_vecs, dim3(1), dim3(64), 0, s, (const double*)W.alpha.as<double>(),
-                           (const double*)W.beta.as<double>(), elit, (const double*)en(conv_step), nev, Y,
-                           W.swork.as<double>());
+        {
+            std::vector<double> a(elit), b(elit), vals(nev), y((size_t)nev * elit), work(5 * (size_t)elit + 8);
+            SM_HIP(hipMemcpyAsync(a.data(), W.alpha.p, elit * 8, hipMemcpyDeviceToHost, s));
+            SM_HIP(hipMemcpyAsync(b.data(), W.beta.p, elit * 8, hipMemcpyDeviceToHost, s));
+            SM_HIP(hipMemcpyAsync(vals.data(), en(conv_step), nev * 8, hipMemcpyDeviceToHost, s));
+            SM_HIP(hipStreamSynchronize(s));
+            const double tn = tri_norm(a.data(), b.data(), elit);
+            const double ortol = 1e-3 * tn;
+            for (int j = 0; j < nev; j++) {
+                const int first = tri_cluster_first(vals.data(), j, ortol);
+                tri_eigvec(a.data(), b.data(), elit, vals[j], y.data() + (size_t)j * elit, work.data(),
+                           y.data() + (size_t)first * elit, j - first, tn);
+            }
+            SM_HIP(hipMemcpyAsync(Y, y.data(), y.size() * 8, hipMemcpyHostToDevice, s));
+            SM_HIP(hipStreamSynchronize(s));
+        }
         DBuf& R = W.R;
         int gb = (int)std::min<int64_t>(2048, (n + TB - 1) / TB);
         size_t sh = (size_t)nev * elit * 8;
@@ -1257,12 +1325,16 @@ struct Solver {
             hipLaunchKernelGGL(k_sqrt1, dim3(1), dim3(1), 0, s, sc + 2 * i, sc + 2 * i + 1);
             hipLaunchKernelGGL((k_store<double>), dim3(gstore), dim3(TB), 0, s, (const double*)r, sc + 2 * i + 1,
                                r, n);
-            if (refine) {
-                // FP64 Rayleigh quotient with the FP64 H·v
-                launch_hv(H, 0, r, hv64.p, s);
-                st.n_hv64++;
-                hipLaunchKernelGGL((k_dotsw<double, double>), dim3(Gd), dim3(TB), 0, s, (const double*)r, ld, 1,
-                                   hv64.as<double>(), n, W.part.as<double>(), ldp, 0, (const int*)nullptr);
+        }
+        if (refine) {
+            // FP64 Rayleigh quotients with the FP64 H·v, all Ritz vectors in shared launches
+            launch_hv_multi(H, 0, nev, R.p, ld, hv64.p, ld, s);
+            st.n_hv64 += nev;
+            for (int i = 0; i < nev; i++) {
+                const double* r = R.as<double>() + (int64_t)i * ld;
+                hipLaunchKernelGGL((k_dotsw<double, double>), dim3(Gd), dim3(TB), 0, s, r, ld, 1,
+                                   (const double*)(hv64.as<double>() + (int64_t)i * ld), n, W.part.as<double>(), ldp,
+                                   0, (const int*)nullptr);
                 hipLaunchKernelGGL(k_reduce, dim3(1), dim3(TB), 0, s, W.part.as<double>(), (int)Gd, ldp,
                                    W.h1.as<double>() + i, (const int*)nullptr);
             }
@@ -1367,7 +1439,7 @@ struct BlockSolver {
         U.alloc((size_t)lsv * ld * sizeof(TBv));
         Sd.alloc((size_t)lmT * 32 * 8);
         R.alloc((size_t)nev * ld * 8);
-        if (prec_hv == 1 && P.refine) hv64.alloc((size_t)ld * 8);
+        if (prec_hv == 1 && P.refine) hv64.alloc((size_t)nev * ld * 8);
         if (jp) jz.alloc((size_t)ld * 8);
     }
 
@@ -1540,10 +1612,8 @@ struct BlockSolver {
                 total += qe;
                 if (total > P.max_iter) throw std::runtime_error("TRBL: iteration limit reached");
                 const int j0 = it * q;
-                for (int b = 0; b < qe; b++) {
-                    launch_hv(H, prec_hv, V(j0 + b), HVb(b), s);
-                    st.n_hv++;
-                }
+                launch_hv_multi(H, prec_hv, qe, V(j0), ld, HVb(0), ld, s);     // bl_operate_H!
+                st.n_hv += qe;
                 const int k = j0 + qe;
                 for (int b = 0; b < qe; b++)
                     cgs2_into<TBv>(k, (const TBv*)HVb(b), Wb(b), h1.as<double>() + (int64_t)b * ldp,
@@ -1703,6 +1773,88 @@ void trl_solve(DeviceHam& h, const SolveParams& p, double* evals, double* evecs,
 }
 
 // device run of the restart small problem (tests compare it with the host build)
+// Several TRL solves on one Hamiltonian in lockstep: each round issues the H·v of
+// every unfinished member in shared launches (launch_hv_set, up to four vectors
+// per launch) on hv_stream, after which each member runs the rest of its step on
+// its own stream.  Members keep their own basis, workspace and convergence; each
+// result is bit-identical to the member solved alone with the same parameters.
+template <typename TBv>
+static void batch_run(DeviceHam& h, int nb, const SolveParams* p, const double* const* v0, double* const* evals,
+                      double* const* evecs, SolveStats* st, const SolveCtx* const* ctx, hipStream_t hv_stream,
+                      int32_t* status, std::string* msgs, const std::function<void(int)>& on_done) {
+    std::vector<std::unique_ptr<Solver<TBv>>> S(nb);
+    std::vector<int> active;
+    std::vector<hipEvent_t> ready(nb, nullptr);
+    hipEvent_t evh = nullptr;
+    SM_HIP(hipEventCreateWithFlags(&evh, hipEventDisableTiming));
+    auto fail = [&](int b) {
+        std::string m;
+        status[b] = status_of_current_exception(m);
+        if (msgs) msgs[b] = m;
+        if (on_done) on_done(b);
+    };
+    for (int b = 0; b < nb; b++) {
+        status[b] = SMGPU_OK;
+        SM_HIP(hipEventCreateWithFlags(&ready[b], hipEventDisableTiming));
+        try {
+            S[b] = std::make_unique<Solver<TBv>>(h, p[b], st[b], ctx[b]);
+            S[b]->start(v0 ? v0[b] : nullptr);
+            active.push_back(b);
+        } catch (...) {
+            fail(b);
+        }
+    }
+    std::vector<const void*> vin;
+    std::vector<void*> vout;
+    while (!active.empty()) {
+        vin.clear();
+        vout.clear();
+        for (int b : active) {
+            SM_HIP(hipEventRecord(ready[b], S[b]->stream()));
+            SM_HIP(hipStreamWaitEvent(hv_stream, ready[b], 0));
+            vin.push_back(S[b]->hv_in());
+            vout.push_back(S[b]->hv_out());
+        }
+        launch_hv_set(h, sizeof(TBv) == 4 ? 1 : 0, (int)active.size(), vin.data(), vout.data(), hv_stream);
+        SM_HIP(hipEventRecord(evh, hv_stream));
+        std::vector<int> next;
+        for (int b : active) {
+            SM_HIP(hipStreamWaitEvent(S[b]->stream(), evh, 0));
+            try {
+                if (S[b]->advance()) {
+                    S[b]->end(evals[b], evecs ? evecs[b] : nullptr);
+                    if (on_done) on_done(b);
+                } else {
+                    next.push_back(b);
+                }
+            } catch (...) {
+                fail(b);
+            }
+        }
+        active.swap(next);
+    }
+    SM_HIP(hipStreamSynchronize(hv_stream));
+    for (int b = 0; b < nb; b++) {
+        if (S[b]) SM_HIP(hipStreamSynchronize(S[b]->stream()));
+        (void)hipEventDestroy(ready[b]);
+    }
+    (void)hipEventDestroy(evh);
+}
+
+void trl_solve_batch(DeviceHam& h, int nb, const SolveParams* p, const double* const* v0, double* const* evals,
+                     double* const* evecs, SolveStats* st, const SolveCtx* const* ctx, hipStream_t hv_stream,
+                     int32_t* status, std::string* msgs, const std::function<void(int)>& on_done) {
+    SM_HIP(hipSetDevice(h.device));
+    if (nb < 1) return;
+    for (int b = 1; b < nb; b++)
+        if ((p[b].precision == 0) != (p[0].precision == 0))
+            throw std::invalid_argument("a batch shares one H·v precision");
+    for (int b = 0; b < nb; b++)
+        if (!ctx[b] || !ctx[b]->stream) throw std::invalid_argument("each batch member needs its own stream");
+    if (p[0].precision == 0) batch_run<double>(h, nb, p, v0, evals, evecs, st, ctx, hv_stream, status, msgs, on_done);
+    else batch_run<float>(h, nb, p, v0, evals, evecs, st, ctx, hv_stream, status, msgs, on_done);
+}
+
 void trbl_solve(DeviceHam& h, int q, const SolveParams& p, double* evals, double* evecs, SolveStats& st,
                 const double* v0, const JProj* jp) {
     SM_HIP(hipSetDevice(h.device));

@@ -54,7 +54,8 @@ class SolveStats(ctypes.Structure):
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("max_inflight", ctypes.c_int32), ("hbm_budget", ctypes.c_int64),
-                ("host_threads", ctypes.c_int32), ("table_threads", ctypes.c_int32), ("keep_bytes", ctypes.c_int64)]
+                ("host_threads", ctypes.c_int32), ("table_threads", ctypes.c_int32), ("keep_bytes", ctypes.c_int64),
+                ("max_batch", ctypes.c_int32), ("reserved_", ctypes.c_int32)]
 
 
 class RequestInfo(ctypes.Structure):
@@ -72,7 +73,8 @@ class ServerStats(ctypes.Structure):
                 ("ready", ctypes.c_int32), ("running", ctypes.c_int32), ("peak_running", ctypes.c_int32),
                 ("slots", ctypes.c_int32), ("hbm_budget", ctypes.c_int64), ("hbm_in_use", ctypes.c_int64),
                 ("hbm_peak", ctypes.c_int64), ("index_hits", ctypes.c_int64), ("index_misses", ctypes.c_int64),
-                ("index_uploads", ctypes.c_int64), ("index_evictions", ctypes.c_int64), ("clock_ms", ctypes.c_double)]
+                ("index_uploads", ctypes.c_int64), ("index_evictions", ctypes.c_int64), ("clock_ms", ctypes.c_double),
+                ("batched", ctypes.c_int64), ("peak_batch", ctypes.c_int32), ("reserved_", ctypes.c_int32)]
 
 
 # exported symbol -> (restype, argtypes)
@@ -99,11 +101,15 @@ SIGNATURES = {
     "smgpu_apply_H": (I32, [P, I32, P, P, P]),
     "smgpu_apply_H_host": (I32, [P, I32, P, P]),
     "smgpu_time_apply_H": (I32, [P, I32, I32, P]),
+    "smgpu_apply_H_multi": (I32, [P, I32, I32, P, I64, P, I64, P]),
+    "smgpu_apply_H_host_multi": (I32, [P, I32, I32, P, P]),
+    "smgpu_time_apply_H_multi": (I32, [P, I32, I32, I32, P]),
     "smgpu_default_params": (None, [P]),
     "smgpu_trl_solve": (I32, [P, P, P, P, P]),
     "smgpu_trl_solve_v0": (I32, [P, P, P, P, P, P]),
     "smgpu_trl_solve_j": (I32, [P, P, I32, P, P, P, P, P]),
     "smgpu_trbl_solve": (I32, [P, P, I32, I32, P, P, P, P, P]),
+    "smgpu_trl_solve_batch": (I32, [P, I32, P, P, P, P, P, P]),
     "smgpu_lanczos_steps": (I32, [P, I32, I32, P, P, P]),
     "smgpu_malloc": (I32, [P, I64, P]),
     "smgpu_free": (I32, [P, P]),

@@ -103,7 +103,7 @@ def solve_ec(Hs, target_nuc, samples: ECSamples, num_ev_target, *, device=0, pre
         if k > 0:
             prob.set_interaction(sntf)
             ham.update_interaction(prob)
-        HV = np.array([ham.operate_H(v, precision) for v in V])
+        HV = ham.operate_H_multi(V, precision)
         Hmat = V @ HV.T
         Hmat = 0.5 * (Hmat + Hmat.T)
         tildH = Linv @ Hmat @ Linv.T

@@ -125,9 +125,22 @@ class Hamiltonian:
         check(lib().smgpu_apply_H_host(self._h, _prec(precision), ptr(v), ptr(w)))
         return w
 
-    def time_operate_H(self, precision="mixed", reps=20):
+    def operate_H_multi(self, V, precision="fp64"):
+        """W[b] = H V[b] for the rows of V (nvec x mdim), several vectors per launch
+        (bl_operate_H!)."""
+        V = np.ascontiguousarray(np.atleast_2d(V), dtype=np.float64)
+        assert V.shape[1] == self.mdim
+        W = np.zeros_like(V)
+        check(lib().smgpu_apply_H_host_multi(self._h, _prec(precision), V.shape[0], ptr(V), ptr(W)))
+        return W
+
+    def time_operate_H(self, precision="mixed", reps=20, nvec=1):
+        """Milliseconds per H·v launch (nvec > 1: per call on nvec vectors)."""
         ms = ctypes.c_double()
-        check(lib().smgpu_time_apply_H(self._h, _prec(precision), int(reps), ctypes.byref(ms)))
+        if nvec == 1:
+            check(lib().smgpu_time_apply_H(self._h, _prec(precision), int(reps), ctypes.byref(ms)))
+        else:
+            check(lib().smgpu_time_apply_H_multi(self._h, _prec(precision), int(nvec), int(reps), ctypes.byref(ms)))
         return ms.value
 
     def table_bytes(self, precision="mixed"):
@@ -183,6 +196,36 @@ class Hamiltonian:
                      n_jproj=st.n_jproj, n_reorth=st.n_reorth)
         return ev, vec, stats
 
+    def trl_batch(self, num_ev, seeds, lm=300, ls=20, tol=1e-8, num_history=3, precision="fp64", refine=True,
+                  want_vectors=False, max_iter=0, reorth="default"):
+        """len(seeds) TRL solves in lockstep with shared H·v launches
+        (smgpu_trl_solve_batch); returns [(evals, evecs or None, stats)] per seed."""
+        nb = len(seeds)
+        params = (SolveParams * nb)()
+        nev = min(int(num_ev), self.mdim)
+        evs = [np.zeros(nev, np.float64) for _ in range(nb)]
+        vecs = [np.zeros((nev, self.mdim), np.float64) for _ in range(nb)] if want_vectors else None
+        for b, sd in enumerate(seeds):
+            p = params[b]
+            lib().smgpu_default_params(ctypes.byref(p))
+            p.num_ev = int(num_ev); p.lm = int(lm); p.ls = int(ls); p.tol = float(tol)
+            p.num_history = int(num_history); p.precision = _prec(precision); p.refine = int(bool(refine))
+            p.seed = int(sd); p.max_iter = int(max_iter)
+            p.reorth = {"default": 0, "full": 1, "partial": 2}[reorth]
+        ev_ptrs = (ctypes.c_void_p * nb)(*[ptr(e) for e in evs])
+        vec_ptrs = (ctypes.c_void_p * nb)(*[ptr(v) for v in vecs]) if vecs is not None else None
+        stats = (SolveStats * nb)()
+        status = (ctypes.c_int32 * nb)()
+        check(lib().smgpu_trl_solve_batch(self._h, nb, params, None, ev_ptrs, vec_ptrs, stats, status))
+        out = []
+        for b in range(nb):
+            st = stats[b]
+            sd = dict(iterations=st.iterations, n_hv=st.n_hv, n_hv64=st.n_hv64, n_restart=st.n_restart,
+                      wall_ms=st.wall_ms, ritz=np.array(st.ritz[:nev]), n_reorth2=st.n_reorth2,
+                      n_jproj=st.n_jproj, n_reorth=st.n_reorth)
+            out.append((evs[b], vecs[b] if vecs is not None else None, sd))
+        return out
+
     def trbl(self, num_ev, q, lm=300, ls=20, tol=1e-8, num_history=3, precision="fp64", refine=True, seed=1234,
              want_vectors=False, max_iter=0, v0=None, j2=None, twoJ=None):
         """Thick-restart block Lanczos with block size q (TRBL); returns (evals,
@@ -230,7 +273,9 @@ def j_from_jj1(jj, tol=1e-6):
 def expect_j2(problem: Problem, vecs, device=0):
     """<v|J^2|v> of each row of `vecs` (main_sm's Js, shellmodel_main.jl:226-230)."""
     hj = problem.j2_hamiltonian(device)
-    return np.array([float(v @ hj.operate_H(v, "fp64")) for v in np.atleast_2d(vecs)])
+    V = np.atleast_2d(vecs)
+    W = hj.operate_H_multi(V, "fp64")
+    return np.einsum("ij,ij->i", V, W)
 
 
 def main_sm(sntf, target_nuc, num_ev, target_J=(), *, lm=300, ls=20, tol=1e-8, num_history=3,

@@ -62,6 +62,24 @@ def test_hv_parity_cr48(snt_dir):
     assert np.abs(hsum - (hu + 2.0 * w64)).max() <= 1e-11 * scale
 
 
+@pytest.mark.parametrize("space,zv,nv,seed", [("sd", 4, 4, 11), ("pf", 3, 3, 12), ("pf", 4, 2, 13)])
+def test_hv_multi_bit_identical(snt_dir, space, zv, nv, seed):
+    """smgpu_apply_H_multi (bl_operate_H!: 2 or 4 vectors per launch sharing the
+    table loads) gives each vector bit for bit what the one-vector kernel gives,
+    for 1..9 vectors (launches of 4, 2 and 1), FP64 and FP32, on one-row and
+    two-row group shapes."""
+    path, nuc = synth.config_snt(f"{snt_dir}/multi_{space}{zv}{nv}.snt", space, zv, nv, synth.BASE_SEED + seed,
+                                 (-6, 0), 1.2)
+    h = smgpu.Hamiltonian(smgpu.Problem(path, nuc))
+    rng = np.random.default_rng(seed)
+    V = rng.standard_normal((9, h.mdim))
+    for prec in ("fp64", "mixed"):
+        ref = np.array([h.operate_H(v, prec) for v in V])
+        for k in (1, 2, 3, 4, 5, 9):
+            W = h.operate_H_multi(V[:k], prec)
+            assert np.array_equal(W, ref[:k]), (prec, k, np.abs(W - ref[:k]).max())
+
+
 def test_dense_equals_brute_force(golden):
     H, _ = bf.brute_hamiltonian(f"{golden}/ckpot.snt", "Be8", 4, 4)
     p = smgpu.Problem(f"{golden}/ckpot.snt", "Be8")

@@ -152,3 +152,40 @@ def test_server_interaction_variants(snt_dir):
         assert np.abs(res[i][0] - ev64).max() <= TOL_FP64, (i, res[i][0] - ev64)
         grounds.append(ev64[0])
     assert len(set(np.round(grounds, 6))) == len(paths)       # the variants differ
+
+
+def test_server_batches_identical_hamiltonians(snt_dir):
+    """max_batch: ready requests with the same basis and identical coefficients (and
+    precision) run in one slot in lockstep with shared H·v launches.  Every result
+    equals the request solved alone (1e-9 MeV; the batched solve is bit-identical),
+    requests on other interactions or bases are served beside them, a request
+    canceled while its batch runs ends canceled and its mates still finish."""
+    a, nuc_a = synth.config_snt(f"{snt_dir}/bat_a.snt", "sd", 4, 4, 8101, (-6, 0), 1.2)
+    b, nuc_b = synth.config_snt(f"{snt_dir}/bat_b.snt", "sd", 4, 4, 8102, (-6, 0), 1.2)   # same basis
+    c, nuc_c = synth.config_snt(f"{snt_dir}/bat_c.snt", "pf", 2, 3, 8103, (-6, 0), 1.2)
+    reqs = [(a, nuc_a, 5, 300 + i) for i in range(6)] + [(b, nuc_b, 3, 310), (b, nuc_b, 3, 311), (c, nuc_c, 4, 312)]
+    alone = [_alone(p, n, k, s, "mixed") for p, n, k, s in reqs]
+    with Server(max_inflight=2, max_batch=3) as srv:
+        ids = [srv.submit(p, n, k, precision="mixed", seed=s) for p, n, k, s in reqs]
+        for rid in ids:
+            assert srv.wait(rid, 120000)
+        res = [srv.result(rid) for rid in ids]
+        st = srv.stats()
+    for (ev, info), ref in zip(res, alone):
+        assert info["state"] == "done"
+        np.testing.assert_allclose(ev, ref, atol=TOL_MIX)
+    assert st["batched"] >= 3 and 2 <= st["peak_batch"] <= 3, st
+    # cancel one member of a running batch: the others complete
+    with Server(max_inflight=1, max_batch=3) as srv:
+        ids = [srv.submit(a, nuc_a, 5, precision="mixed", seed=400 + i) for i in range(3)]
+        t0 = time.time()
+        while srv.poll(ids[1]) not in ("running", "done", "canceled", "failed") and time.time() - t0 < 60:
+            time.sleep(0.002)
+        srv.cancel(ids[1])
+        for rid in ids:
+            assert srv.wait(rid, 120000)
+        out = [srv.result(rid, raise_on_error=False) for rid in ids]
+        assert out[1][1]["state"] in ("canceled", "done")
+        for i in (0, 2):
+            assert out[i][1]["state"] == "done"
+            np.testing.assert_allclose(out[i][0], _alone(a, nuc_a, 5, 400 + i, "mixed"), atol=TOL_MIX)
