@@ -301,7 +301,7 @@ def main():
     from smgpu.server import Server
     srv = Server(max_inflight=args.inflight, host_threads=args.host_threads, max_batch=args.batch)
     warm = [srv.submit(snt, nuc, nev, lm=lm, ls=ls, precision="mixed", seed=1000 + i)
-            for i in range(max(args.warmup, args.inflight))]
+            for i in range(max(args.warmup, args.inflight * args.batch))]
     for rid in warm:
         srv.wait(rid)
         srv.result(rid)

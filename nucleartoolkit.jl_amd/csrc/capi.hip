@@ -509,7 +509,15 @@ int32_t smgpu_trl_solve_batch(smgpu_ham h, int32_t nb, const smgpu_solve_params*
         std::memset(st.data(), 0, sizeof(smgpu_solve_stats) * nb);
         std::vector<int32_t> stat(nb, SMGPU_OK);
         std::vector<std::string> msgs(nb);
-        trl_solve_batch(d, nb, p.data(), v0, evals, evecs, st.data(), cp.data(), d.stream, stat.data(), msgs.data());
+        BatchIO io;
+        io.p = p.data();
+        io.v0 = v0;
+        io.evals = evals;
+        io.evecs = evecs;
+        io.st = st.data();
+        io.status = stat.data();
+        io.msgs = msgs.data();
+        trl_solve_batch(d, nb, nb, io, cp.data(), d.stream);
         if (stats) for (int b = 0; b < nb; b++) stats[b] = st[b];
         if (status) for (int b = 0; b < nb; b++) status[b] = stat[b];
         for (int b = 0; b < nb; b++)

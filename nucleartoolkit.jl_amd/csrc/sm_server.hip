@@ -96,7 +96,8 @@ struct Slot {
     std::vector<std::unique_ptr<SolveCtx>> more;
     DeviceHam ham;
     std::shared_ptr<Request> job;                  // first (or only) request
-    std::vector<std::shared_ptr<Request>> batch;   // every request of the slot's run
+    std::vector<std::shared_ptr<Request>> batch;   // request per member slot of the run (null: free)
+    std::shared_ptr<const HostCoefficients> bound; // the run's Hamiltonian (for requests joining it)
     int64_t cap = 0;          // HBM held by this slot's buffers (server mutex)
     SolveCtx& member(int b) { return b == 0 ? ctx : *more[b - 1]; }
     int64_t workspace_bytes() const {
@@ -352,7 +353,7 @@ struct Server {
                 }
                 hbm_peak = std::max(hbm_peak, in_use());
                 int running = 0;
-                for (auto& s : slots) running += s->job ? (int)s->batch.size() : 0;
+                for (auto& s : slots) running += running_in(*s);
                 st.peak_running = std::max(st.peak_running, running);
                 best->cv.notify_one();
                 return true;
@@ -414,42 +415,108 @@ struct Server {
         bind_ham(s.ham, idx, *r.coef);
         const int nb = (int)s.batch.size();
         for (auto& q : s.batch) q->evals.assign(std::max(1, q->params.num_ev), 0.0);
+        const int cap = std::max(nb, cfg.max_batch);
         struct Unhook {
             Slot& s;
-            int nb;
-            ~Unhook() { for (int b = 0; b < nb; b++) s.member(b).cancel = nullptr; }
-        } unhook{s, nb};
+            int cap;
+            ~Unhook() {
+                for (int b = 0; b < cap; b++) s.member(b).cancel = nullptr;
+                s.bound.reset();
+            }
+        } unhook{s, cap};
         for (int b = 0; b < nb; b++) s.member(b).cancel = &s.batch[b]->cancel;
-        if (nb == 1) {
+        if (cap == 1) {
             trl_solve(s.ham, r.params, r.evals.data(), nullptr, r.st, nullptr, &s.ctx);
             r.evals.resize((size_t)std::min<int64_t>(r.params.num_ev, r.mdim));
             return;
         }
-        // lockstep batch: each member's result is published as soon as it finishes
-        std::vector<SolveParams> p(nb);
-        std::vector<double*> ev(nb);
-        std::vector<SolveStats> sst(nb);
-        std::vector<const SolveCtx*> cx(nb);
-        std::vector<int32_t> status(nb, SMGPU_OK);
-        std::vector<std::string> msgs(nb);
-        for (int b = 0; b < nb; b++) {
-            p[b] = s.batch[b]->params;
-            p[b].num_ev = (int32_t)std::min<int64_t>(p[b].num_ev, r.mdim);
-            ev[b] = s.batch[b]->evals.data();
-            cx[b] = &s.member(b);
-        }
-        trl_solve_batch(s.ham, nb, p.data(), nullptr, ev.data(), nullptr, sst.data(), cx.data(), s.hv_stream,
-                        status.data(), msgs.data(), [&](int b) {
-                            Request& q = *s.batch[b];
-                            q.st = sst[b];
-                            q.evals.resize((size_t)p[b].num_ev);
-                            std::lock_guard<std::mutex> lk(m);
-                            q.entry->users--;
-                            q.entry->last_use = ++use_clock;
-                            if (status[b] == SMGPU_OK) finish_locked(q, SMGPU_REQ_DONE, SMGPU_OK, "");
-                            else if (status[b] == SMGPU_ECANCELED) finish_locked(q, SMGPU_REQ_CANCELED, status[b], msgs[b]);
-                            else finish_locked(q, SMGPU_REQ_FAILED, status[b], msgs[b]);
-                        });
+        // lockstep batch with member slots; later requests with the same Hamiltonian
+        // join free member slots between rounds; each result is published as soon as
+        // its solve finishes
+        s.bound = r.coef;
+        const uint64_t hash = r.coef_hash;
+        const bool fp64 = r.params.precision == 0;
+        s.batch.resize(cap);
+        std::vector<SolveParams> p(cap);
+        std::vector<double*> ev(cap, nullptr);
+        std::vector<SolveStats> sst(cap);
+        std::vector<const SolveCtx*> cx(cap);
+        std::vector<int32_t> status(cap, SMGPU_OK);
+        std::vector<std::string> msgs(cap);
+        auto fill = [&](int b) {
+            Request& q = *s.batch[b];
+            p[b] = q.params;
+            p[b].num_ev = (int32_t)std::min<int64_t>(p[b].num_ev, q.mdim);
+            ev[b] = q.evals.data();
+            sst[b] = SolveStats{};
+            status[b] = SMGPU_OK;
+            msgs[b].clear();
+        };
+        for (int b = 0; b < cap; b++) cx[b] = &s.member(b);
+        for (int b = 0; b < nb; b++) fill(b);
+        auto join = [&](int b) -> bool {
+            std::lock_guard<std::mutex> lk(m);
+            if (stopping) return false;
+            IndexEntry& e = *r.entry;
+            for (auto& q : ready) {
+                if (q->cancel || q->entry != r.entry || q->coef_hash != hash || (q->params.precision == 0) != fp64)
+                    continue;
+                if (!same_coefficients(*q->coef, *s.bound)) continue;
+                const int64_t g = workspace_growth(s.member(b), q->mdim, q->params);
+                if (in_use() + g > cfg.hbm_budget && !(reclaim_locked(&e) && in_use() + g <= cfg.hbm_budget))
+                    return false;
+                auto keep = q;          // ready.remove drops the list's reference
+                ready.remove(keep);
+                keep->state = SMGPU_REQ_RUNNING;
+                keep->slot = s.id;
+                keep->t_admit = now_ms();
+                keep->index_shared = true;
+                keep->reserved = g;
+                e.users++;
+                e.last_use = ++use_clock;
+                s.cap += g;
+                hbm_peak = std::max(hbm_peak, in_use());
+                keep->evals.assign(std::max(1, keep->params.num_ev), 0.0);
+                s.batch[b] = keep;
+                s.member(b).cancel = &keep->cancel;
+                fill(b);
+                st.batched++;
+                int live = 0;
+                for (auto& x : s.batch) live += (x && !finished(*x)) ? 1 : 0;
+                st.peak_batch = std::max(st.peak_batch, (int32_t)live);
+                int running = 0;
+                for (auto& sl : slots) running += running_in(*sl);
+                st.peak_running = std::max(st.peak_running, running);
+                return true;
+            }
+            return false;
+        };
+        BatchIO io;
+        io.p = p.data();
+        io.evals = ev.data();
+        io.st = sst.data();
+        io.status = status.data();
+        io.msgs = msgs.data();
+        trl_solve_batch(s.ham, cap, nb, io, cx.data(), s.hv_stream, join, [&](int b) {
+            Request& q = *s.batch[b];
+            q.st = sst[b];
+            q.evals.resize((size_t)p[b].num_ev);
+            std::lock_guard<std::mutex> lk(m);
+            q.entry->users--;
+            q.entry->last_use = ++use_clock;
+            s.member(b).cancel = nullptr;
+            if (status[b] == SMGPU_OK) finish_locked(q, SMGPU_REQ_DONE, SMGPU_OK, "");
+            else if (status[b] == SMGPU_ECANCELED) finish_locked(q, SMGPU_REQ_CANCELED, status[b], msgs[b]);
+            else finish_locked(q, SMGPU_REQ_FAILED, status[b], msgs[b]);
+        });
+    }
+
+    // requests running in a slot (server mutex)
+    static int running_in(const Slot& s) {
+        if (!s.job) return 0;
+        int n = 0;
+        for (auto& x : s.batch) n += (x && !finished(*x)) ? 1 : 0;
+        return std::max(n, 1);
     }
 
     void slot_loop(Slot& s) {
@@ -466,7 +533,7 @@ struct Server {
             std::string msg;
             try {
                 // (a batch member's own cancel flag is checked at its first step)
-                if (r->cancel && s.batch.size() == 1) throw Canceled();
+                if (r->cancel && cfg.max_batch == 1) throw Canceled();
                 run_job(s, *r);
             } catch (...) {
                 code = status_of_current_exception(msg);
@@ -476,6 +543,7 @@ struct Server {
             // requests the batch driver has not finished (one request alone, or a run
             // that failed before the driver started)
             for (size_t b = 1; b < s.batch.size(); b++) {
+                if (!s.batch[b]) continue;
                 Request& q = *s.batch[b];
                 if (finished(q)) continue;
                 q.entry->users--;
@@ -639,7 +707,7 @@ struct Server {
         s.ready = (int32_t)ready.size();
         s.running = 0;
         for (auto& kv : reqs) if (kv.second->state == SMGPU_REQ_PREPARING) s.preparing++;
-        for (auto& sl : slots) if (sl->job) s.running += (int32_t)sl->batch.size();
+        for (auto& sl : slots) s.running += running_in(*sl);
         s.slots = (int32_t)slots.size();
         s.hbm_budget = cfg.hbm_budget;
         s.hbm_in_use = in_use();

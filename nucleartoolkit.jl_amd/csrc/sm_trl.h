@@ -46,14 +46,26 @@ struct JProj {
 
 void trl_solve(DeviceHam& h, const SolveParams& p, double* evals, double* evecs, SolveStats& st,
                const double* v0 = nullptr, const SolveCtx* ctx = nullptr, const JProj* jp = nullptr);
-// nb TRL solves on one Hamiltonian in lockstep with shared H·v launches on hv_stream;
-// ctx[b] gives member b its stream and workspace (distinct streams).  status[b] gets
-// each member's SMGPU_* code and msgs[b] (may be null) its error text (one member's
-// failure does not stop the others); on_done(b), if set, runs when member b has
-// finished (its results written) or failed.
-void trl_solve_batch(DeviceHam& h, int nb, const SolveParams* p, const double* const* v0, double* const* evals,
-                     double* const* evecs, SolveStats* st, const SolveCtx* const* ctx, hipStream_t hv_stream,
-                     int32_t* status, std::string* msgs = nullptr, const std::function<void(int)>& on_done = {});
+// Lockstep TRL solves on one Hamiltonian with shared H·v launches on hv_stream.
+// Per member slot b < cap (ctx[b]: its own stream and workspace): params p[b],
+// start vector v0[b] (v0 or the entry may be null), evals[b], evecs[b] (evecs or
+// the entry may be null), stats st[b], status[b] (SMGPU_* code) and msgs[b] (error
+// text; msgs may be null).  Slots [0, nb) are filled at the call; a member's
+// failure does not stop the others.  on_done(b) runs when slot b's member has
+// finished (results written) or failed; afterwards, and between rounds, join(b)
+// may fill io at a free slot b with a new member (same precision) and return true.
+struct BatchIO {
+    SolveParams* p = nullptr;
+    const double* const* v0 = nullptr;
+    double* const* evals = nullptr;
+    double* const* evecs = nullptr;
+    SolveStats* st = nullptr;
+    int32_t* status = nullptr;
+    std::string* msgs = nullptr;
+};
+void trl_solve_batch(DeviceHam& h, int cap, int nb, const BatchIO& io, const SolveCtx* const* ctx,
+                     hipStream_t hv_stream, const std::function<bool(int)>& join = {},
+                     const std::function<void(int)>& on_done = {});
 // thick-restart block Lanczos with block size q (TRBL); v0: q start vectors (q*mdim) or null
 void trbl_solve(DeviceHam& h, int q, const SolveParams& p, double* evals, double* evecs, SolveStats& st,
                 const double* v0 = nullptr, const JProj* jp = nullptr);

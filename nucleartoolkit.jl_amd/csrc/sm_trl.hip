@@ -1774,39 +1774,58 @@ void trl_solve(DeviceHam& h, const SolveParams& p, double* evals, double* evecs,
 
 // device run of the restart small problem (tests compare it with the host build)
 // Several TRL solves on one Hamiltonian in lockstep: each round issues the H·v of
-// every unfinished member in shared launches (launch_hv_set, up to four vectors
-// per launch) on hv_stream, after which each member runs the rest of its step on
-// its own stream.  Members keep their own basis, workspace and convergence; each
+// every active member in shared launches (launch_hv_set, up to four vectors per
+// launch) on hv_stream, after which each member runs the rest of its step on its
+// own stream.  Members keep their own basis, workspace and convergence; each
 // result is bit-identical to the member solved alone with the same parameters.
+// Member slots (stream + workspace, ctx[b]) are reused: a member that finishes or
+// fails frees its slot, and before every round join(b) may place a new solve in a
+// free slot b (filling io at b), which then starts at its own first step.
 template <typename TBv>
-static void batch_run(DeviceHam& h, int nb, const SolveParams* p, const double* const* v0, double* const* evals,
-                      double* const* evecs, SolveStats* st, const SolveCtx* const* ctx, hipStream_t hv_stream,
-                      int32_t* status, std::string* msgs, const std::function<void(int)>& on_done) {
-    std::vector<std::unique_ptr<Solver<TBv>>> S(nb);
+static void batch_run(DeviceHam& h, int cap, int nb, const BatchIO& io, const SolveCtx* const* ctx,
+                      hipStream_t hv_stream, const std::function<bool(int)>& join,
+                      const std::function<void(int)>& on_done) {
+    std::vector<std::unique_ptr<Solver<TBv>>> S(cap);
+    std::vector<char> busy(cap, 0);
     std::vector<int> active;
-    std::vector<hipEvent_t> ready(nb, nullptr);
+    std::vector<hipEvent_t> ready(cap, nullptr);
     hipEvent_t evh = nullptr;
+    struct Events {
+        std::vector<hipEvent_t>& r;
+        hipEvent_t& h;
+        ~Events() {
+            for (auto e : r) if (e) (void)hipEventDestroy(e);
+            if (h) (void)hipEventDestroy(h);
+        }
+    } events{ready, evh};
     SM_HIP(hipEventCreateWithFlags(&evh, hipEventDisableTiming));
+    for (int b = 0; b < cap; b++) SM_HIP(hipEventCreateWithFlags(&ready[b], hipEventDisableTiming));
     auto fail = [&](int b) {
         std::string m;
-        status[b] = status_of_current_exception(m);
-        if (msgs) msgs[b] = m;
+        io.status[b] = status_of_current_exception(m);
+        if (io.msgs) io.msgs[b] = m;
+        busy[b] = 0;
         if (on_done) on_done(b);
     };
-    for (int b = 0; b < nb; b++) {
-        status[b] = SMGPU_OK;
-        SM_HIP(hipEventCreateWithFlags(&ready[b], hipEventDisableTiming));
+    auto begin = [&](int b) {
+        io.status[b] = SMGPU_OK;
+        busy[b] = 1;
         try {
-            S[b] = std::make_unique<Solver<TBv>>(h, p[b], st[b], ctx[b]);
-            S[b]->start(v0 ? v0[b] : nullptr);
+            S[b] = std::make_unique<Solver<TBv>>(h, io.p[b], io.st[b], ctx[b]);
+            S[b]->start(io.v0 ? io.v0[b] : nullptr);
             active.push_back(b);
         } catch (...) {
             fail(b);
         }
-    }
+    };
+    for (int b = 0; b < nb; b++) begin(b);
     std::vector<const void*> vin;
     std::vector<void*> vout;
-    while (!active.empty()) {
+    for (;;) {
+        if (join)
+            for (int b = 0; b < cap; b++)
+                if (!busy[b] && join(b)) begin(b);
+        if (active.empty()) break;
         vin.clear();
         vout.clear();
         for (int b : active) {
@@ -1822,7 +1841,8 @@ static void batch_run(DeviceHam& h, int nb, const SolveParams* p, const double* 
             SM_HIP(hipStreamWaitEvent(S[b]->stream(), evh, 0));
             try {
                 if (S[b]->advance()) {
-                    S[b]->end(evals[b], evecs ? evecs[b] : nullptr);
+                    S[b]->end(io.evals[b], io.evecs ? io.evecs[b] : nullptr);
+                    busy[b] = 0;
                     if (on_done) on_done(b);
                 } else {
                     next.push_back(b);
@@ -1834,25 +1854,23 @@ static void batch_run(DeviceHam& h, int nb, const SolveParams* p, const double* 
         active.swap(next);
     }
     SM_HIP(hipStreamSynchronize(hv_stream));
-    for (int b = 0; b < nb; b++) {
+    for (int b = 0; b < cap; b++)
         if (S[b]) SM_HIP(hipStreamSynchronize(S[b]->stream()));
-        (void)hipEventDestroy(ready[b]);
-    }
-    (void)hipEventDestroy(evh);
 }
 
-void trl_solve_batch(DeviceHam& h, int nb, const SolveParams* p, const double* const* v0, double* const* evals,
-                     double* const* evecs, SolveStats* st, const SolveCtx* const* ctx, hipStream_t hv_stream,
-                     int32_t* status, std::string* msgs, const std::function<void(int)>& on_done) {
+void trl_solve_batch(DeviceHam& h, int cap, int nb, const BatchIO& io, const SolveCtx* const* ctx,
+                     hipStream_t hv_stream, const std::function<bool(int)>& join,
+                     const std::function<void(int)>& on_done) {
     SM_HIP(hipSetDevice(h.device));
-    if (nb < 1) return;
+    if (cap < 1 || nb < 0 || nb > cap) throw std::invalid_argument("bad batch size");
     for (int b = 1; b < nb; b++)
-        if ((p[b].precision == 0) != (p[0].precision == 0))
+        if ((io.p[b].precision == 0) != (io.p[0].precision == 0))
             throw std::invalid_argument("a batch shares one H·v precision");
-    for (int b = 0; b < nb; b++)
+    for (int b = 0; b < cap; b++)
         if (!ctx[b] || !ctx[b]->stream) throw std::invalid_argument("each batch member needs its own stream");
-    if (p[0].precision == 0) batch_run<double>(h, nb, p, v0, evals, evecs, st, ctx, hv_stream, status, msgs, on_done);
-    else batch_run<float>(h, nb, p, v0, evals, evecs, st, ctx, hv_stream, status, msgs, on_done);
+    const bool fp64 = nb > 0 ? io.p[0].precision == 0 : false;
+    if (fp64) batch_run<double>(h, cap, nb, io, ctx, hv_stream, join, on_done);
+    else batch_run<float>(h, cap, nb, io, ctx, hv_stream, join, on_done);
 }
 
 void trbl_solve(DeviceHam& h, int q, const SolveParams& p, double* evals, double* evecs, SolveStats& st,
