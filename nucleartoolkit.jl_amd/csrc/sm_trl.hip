@@ -28,6 +28,7 @@
 // the final Ritz vectors are assembled in FP64 and their eigenvalues refined by
 // an FP64 Rayleigh quotient (one FP64 H·v per eigenpair).
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -37,6 +38,7 @@
 #include <random>
 #include <stdexcept>
 #include <string>
+#include <thread>
 
 #include "dense_eig.h"
 #include "sm_device.h"
@@ -931,17 +933,27 @@ struct Solver {
             if (prec_hv == 0) reorth_local<double>(it, W.wH.as<double>());
             else reorth_local<float>(it, W.wH.as<float>());
         }
-        // Ritz values of T[0..it] and diagonalize_T!'s test run on the side stream once
-        // this step's alpha/beta exist; the next H·v does not wait for them (later steps
-        // write other entries of alpha/beta; a restart starts after the host has waited
-        // for the last test)
+        // Ritz values of T[0..it] and diagonalize_T!'s test run off the path to the next
+        // H·v (later steps write other entries of alpha/beta; a restart starts after the
+        // host has waited for the last test): alone, on the side stream once this step's
+        // alpha/beta exist; in a batch, on the solve's own stream after step_ready, the
+        // event the batch's next H·v waits for, so they overlap that H·v.
         const int nn = std::min(nev, it + 1);
-        SM_HIP(hipEventRecord(W.evm[(step + 1) & 3], s));
-        SM_HIP(hipStreamWaitEvent(W.side, W.evm[(step + 1) & 3], 0));
-        hipLaunchKernelGGL(k_ritz, dim3(nn), dim3(TB), 0, W.side, W.alpha.as<double>(), W.beta.as<double>(), it + 1,
+        hipStream_t rs = W.side;
+        if (step_ready) {
+            if (it + 1 < lm)
+                hipLaunchKernelGGL((k_store<TBv>), dim3(gstore), dim3(TB), 0, s, W.w64.as<double>(),
+                                   W.beta.as<double>() + it, V(it + 1), n);
+            SM_HIP(hipEventRecord(step_ready, s));
+            rs = s;
+        } else {
+            SM_HIP(hipEventRecord(W.evm[(step + 1) & 3], s));
+            SM_HIP(hipStreamWaitEvent(W.side, W.evm[(step + 1) & 3], 0));
+        }
+        hipLaunchKernelGGL(k_ritz, dim3(nn), dim3(TB), 0, rs, W.alpha.as<double>(), W.beta.as<double>(), it + 1,
                            en(step - 1), nprev, en(step));
-        hipLaunchKernelGGL(k_conv, dim3(1), dim3(64), 0, W.side, en(step - 1), en(step), nev, nn, P.tol, flag(step));
-        if (it + 1 < lm)
+        hipLaunchKernelGGL(k_conv, dim3(1), dim3(64), 0, rs, en(step - 1), en(step), nev, nn, P.tol, flag(step));
+        if (!step_ready && it + 1 < lm)
             hipLaunchKernelGGL((k_store<TBv>), dim3(gstore), dim3(TB), 0, s, W.w64.as<double>(),
                                W.beta.as<double>() + it, V(it + 1), n);
     }
@@ -1019,6 +1031,10 @@ struct Solver {
         if (cancel && cancel->load(std::memory_order_relaxed)) throw Canceled();
     }
 
+    // set by a batch driver: recorded on the solve's stream when the next H·v's input
+    // vector is ready (the Ritz test then runs on the solve's stream behind it)
+    hipEvent_t step_ready = nullptr;
+
     const TBv* hv_in() { return V(d_it); }
     void* hv_out() { return W.wH.p; }
     hipStream_t stream() const { return s; }
@@ -1035,9 +1051,11 @@ struct Solver {
         // interlacing brackets need T[0..it-1] to be the previous step's matrix
         const int nprev = (it == k && d_after_restart) || it == 0 ? 0 : std::min(nev, it);
         after_hv(it, step, nprev);
-        SM_HIP(hipMemcpyAsync(W.hflag + ((step + 1) & 3), flag(step), sizeof(int), hipMemcpyDeviceToHost, W.side));
-        SM_HIP(hipEventRecord(W.ev[(step + 1) & 3], W.side));
+        hipStream_t rs = step_ready ? s : W.side;
+        SM_HIP(hipMemcpyAsync(W.hflag + ((step + 1) & 3), flag(step), sizeof(int), hipMemcpyDeviceToHost, rs));
+        SM_HIP(hipEventRecord(W.ev[(step + 1) & 3], rs));
         bool done = false;
+        bool moved = false;     // the next input vector changed after step_ready
         if (jp && it + 1 < lm && it + 1 < n) {
             // the J subspace is exhausted when beta vanishes (TRL: tbeta < Jtol stops)
             double b = 0.0;
@@ -1048,6 +1066,7 @@ struct Solver {
             if (b < jp->tol || jfilter(it + 1, it + 1, false) < J_EXHAUSTED) {
                 d_conv_step = step; d_elit = it + 1; done = true;
             }
+            moved = true;
         }
         if (!done && it + 1 == n) {                         // Krylov space exhausted (it == mdim)
             if (step > d_first && check(step - 1)) { d_conv_step = step - 1; d_elit = it; }
@@ -1070,9 +1089,11 @@ struct Solver {
                 d_it = ls;
                 d_first = d_step;
                 d_after_restart = true;
+                moved = true;
             }
         }
         d_conv = done;
+        if (moved && step_ready && !done) SM_HIP(hipEventRecord(step_ready, s));
         if (!done) begin_step();
         return done;
     }
@@ -1788,6 +1809,16 @@ static void batch_run(DeviceHam& h, int cap, int nb, const BatchIO& io, const So
     std::vector<std::unique_ptr<Solver<TBv>>> S(cap);
     std::vector<char> busy(cap, 0);
     std::vector<int> active;
+    // a member that converged finishes (Ritz vectors, FP64 refinement, results) on a
+    // thread of its own, so the other members' rounds do not wait for it; its slot
+    // is reused once that thread has ended
+    std::vector<std::thread> fin(cap);
+    std::unique_ptr<std::atomic<bool>[]> fin_done(new std::atomic<bool>[cap]);
+    for (int b = 0; b < cap; b++) fin_done[b] = false;
+    struct Finishers {
+        std::vector<std::thread>& t;
+        ~Finishers() { for (auto& x : t) if (x.joinable()) x.join(); }
+    } finishers{fin};
     std::vector<hipEvent_t> ready(cap, nullptr);
     hipEvent_t evh = nullptr;
     struct Events {
@@ -1812,24 +1843,40 @@ static void batch_run(DeviceHam& h, int cap, int nb, const BatchIO& io, const So
         busy[b] = 1;
         try {
             S[b] = std::make_unique<Solver<TBv>>(h, io.p[b], io.st[b], ctx[b]);
+            S[b]->step_ready = ready[b];
             S[b]->start(io.v0 ? io.v0[b] : nullptr);
+            SM_HIP(hipEventRecord(ready[b], S[b]->stream()));      // start vector ready
             active.push_back(b);
         } catch (...) {
             fail(b);
+        }
+    };
+    auto reap = [&](int b) {          // a finisher thread that has ended frees its slot
+        if (fin[b].joinable() && fin_done[b]) {
+            fin[b].join();
+            busy[b] = 0;
         }
     };
     for (int b = 0; b < nb; b++) begin(b);
     std::vector<const void*> vin;
     std::vector<void*> vout;
     for (;;) {
+        for (int b = 0; b < cap; b++) reap(b);
         if (join)
             for (int b = 0; b < cap; b++)
                 if (!busy[b] && join(b)) begin(b);
-        if (active.empty()) break;
+        if (active.empty()) {
+            bool pending = false;
+            for (int b = 0; b < cap; b++) pending = pending || fin[b].joinable();
+            if (!pending) break;
+            // nothing to step: wait for a finisher (a request may join its slot)
+            for (int b = 0; b < cap; b++)
+                if (fin[b].joinable()) { fin[b].join(); busy[b] = 0; break; }
+            continue;
+        }
         vin.clear();
         vout.clear();
         for (int b : active) {
-            SM_HIP(hipEventRecord(ready[b], S[b]->stream()));
             SM_HIP(hipStreamWaitEvent(hv_stream, ready[b], 0));
             vin.push_back(S[b]->hv_in());
             vout.push_back(S[b]->hv_out());
@@ -1841,9 +1888,21 @@ static void batch_run(DeviceHam& h, int cap, int nb, const BatchIO& io, const So
             SM_HIP(hipStreamWaitEvent(S[b]->stream(), evh, 0));
             try {
                 if (S[b]->advance()) {
-                    S[b]->end(io.evals[b], io.evecs ? io.evecs[b] : nullptr);
-                    busy[b] = 0;
-                    if (on_done) on_done(b);
+                    fin_done[b] = false;
+                    const int dev = h.device;
+                    fin[b] = std::thread([&, b, dev] {
+                        (void)hipSetDevice(dev);
+                        try {
+                            S[b]->end(io.evals[b], io.evecs ? io.evecs[b] : nullptr);
+                            if (on_done) on_done(b);
+                        } catch (...) {
+                            std::string m;
+                            io.status[b] = status_of_current_exception(m);
+                            if (io.msgs) io.msgs[b] = m;
+                            if (on_done) on_done(b);
+                        }
+                        fin_done[b] = true;
+                    });
                 } else {
                     next.push_back(b);
                 }
