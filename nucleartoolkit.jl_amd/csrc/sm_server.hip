@@ -96,7 +96,10 @@ struct Slot {
     std::vector<std::unique_ptr<SolveCtx>> more;
     DeviceHam ham;
     std::shared_ptr<Request> job;                  // first (or only) request
-    std::vector<std::shared_ptr<Request>> batch;   // request per member slot of the run (null: free)
+    // request per member slot of the run (null: free), sized at admission; changed
+    // only under the server mutex (other threads count a slot's requests)
+    std::vector<std::shared_ptr<Request>> batch;
+    int admitted = 0;                              // requests placed at admission
     std::shared_ptr<const HostCoefficients> bound; // the run's Hamiltonian (for requests joining it)
     int64_t cap = 0;          // HBM held by this slot's buffers (server mutex)
     SolveCtx& member(int b) { return b == 0 ? ctx : *more[b - 1]; }
@@ -347,6 +350,8 @@ struct Server {
                 best->cap += best_g;
                 best->job = r;
                 best->batch = run;
+                best->admitted = (int)run.size();
+                best->batch.resize(std::max<size_t>(run.size(), (size_t)cfg.max_batch));
                 if (run.size() > 1) {
                     st.batched += (int64_t)run.size();
                     st.peak_batch = std::max(st.peak_batch, (int32_t)run.size());
@@ -413,9 +418,9 @@ struct Server {
             idx = e.didx;
         }
         bind_ham(s.ham, idx, *r.coef);
-        const int nb = (int)s.batch.size();
-        for (auto& q : s.batch) q->evals.assign(std::max(1, q->params.num_ev), 0.0);
-        const int cap = std::max(nb, cfg.max_batch);
+        const int nb = s.admitted;
+        for (int b = 0; b < nb; b++) s.batch[b]->evals.assign(std::max(1, s.batch[b]->params.num_ev), 0.0);
+        const int cap = (int)s.batch.size();
         struct Unhook {
             Slot& s;
             int cap;
@@ -436,7 +441,6 @@ struct Server {
         s.bound = r.coef;
         const uint64_t hash = r.coef_hash;
         const bool fp64 = r.params.precision == 0;
-        s.batch.resize(cap);
         std::vector<SolveParams> p(cap);
         std::vector<double*> ev(cap, nullptr);
         std::vector<SolveStats> sst(cap);

@@ -1809,16 +1809,6 @@ static void batch_run(DeviceHam& h, int cap, int nb, const BatchIO& io, const So
     std::vector<std::unique_ptr<Solver<TBv>>> S(cap);
     std::vector<char> busy(cap, 0);
     std::vector<int> active;
-    // a member that converged finishes (Ritz vectors, FP64 refinement, results) on a
-    // thread of its own, so the other members' rounds do not wait for it; its slot
-    // is reused once that thread has ended
-    std::vector<std::thread> fin(cap);
-    std::unique_ptr<std::atomic<bool>[]> fin_done(new std::atomic<bool>[cap]);
-    for (int b = 0; b < cap; b++) fin_done[b] = false;
-    struct Finishers {
-        std::vector<std::thread>& t;
-        ~Finishers() { for (auto& x : t) if (x.joinable()) x.join(); }
-    } finishers{fin};
     std::vector<hipEvent_t> ready(cap, nullptr);
     hipEvent_t evh = nullptr;
     struct Events {
@@ -1851,29 +1841,14 @@ static void batch_run(DeviceHam& h, int cap, int nb, const BatchIO& io, const So
             fail(b);
         }
     };
-    auto reap = [&](int b) {          // a finisher thread that has ended frees its slot
-        if (fin[b].joinable() && fin_done[b]) {
-            fin[b].join();
-            busy[b] = 0;
-        }
-    };
     for (int b = 0; b < nb; b++) begin(b);
     std::vector<const void*> vin;
     std::vector<void*> vout;
     for (;;) {
-        for (int b = 0; b < cap; b++) reap(b);
         if (join)
             for (int b = 0; b < cap; b++)
                 if (!busy[b] && join(b)) begin(b);
-        if (active.empty()) {
-            bool pending = false;
-            for (int b = 0; b < cap; b++) pending = pending || fin[b].joinable();
-            if (!pending) break;
-            // nothing to step: wait for a finisher (a request may join its slot)
-            for (int b = 0; b < cap; b++)
-                if (fin[b].joinable()) { fin[b].join(); busy[b] = 0; break; }
-            continue;
-        }
+        if (active.empty()) break;
         vin.clear();
         vout.clear();
         for (int b : active) {
@@ -1888,21 +1863,9 @@ static void batch_run(DeviceHam& h, int cap, int nb, const BatchIO& io, const So
             SM_HIP(hipStreamWaitEvent(S[b]->stream(), evh, 0));
             try {
                 if (S[b]->advance()) {
-                    fin_done[b] = false;
-                    const int dev = h.device;
-                    fin[b] = std::thread([&, b, dev] {
-                        (void)hipSetDevice(dev);
-                        try {
-                            S[b]->end(io.evals[b], io.evecs ? io.evecs[b] : nullptr);
-                            if (on_done) on_done(b);
-                        } catch (...) {
-                            std::string m;
-                            io.status[b] = status_of_current_exception(m);
-                            if (io.msgs) io.msgs[b] = m;
-                            if (on_done) on_done(b);
-                        }
-                        fin_done[b] = true;
-                    });
+                    S[b]->end(io.evals[b], io.evecs ? io.evecs[b] : nullptr);
+                    busy[b] = 0;
+                    if (on_done) on_done(b);
                 } else {
                     next.push_back(b);
                 }
