@@ -129,6 +129,11 @@ int32_t smgpu_ham_destroy(smgpu_ham h);
 int32_t smgpu_ham_create_j2(smgpu_problem p, int32_t device, smgpu_ham* out);
 /* Re-upload the coefficients after smgpu_problem_set_interaction (index tables stay). */
 int32_t smgpu_ham_update_interaction(smgpu_ham h, smgpu_problem p);
+/* A handle for another interaction on base's basis: it shares base's device index
+ * tables and uploads only p's coefficients (the per-sample interaction tables of
+ * prepEC on one basis, eigenvector_continuation.jl:140-175).  p must have base's
+ * basis and pp / nn sparsity pattern.  The index lives while any handle holds it. */
+int32_t smgpu_ham_create_variant(smgpu_ham base, smgpu_problem p, smgpu_ham* out);
 
 /* Bytes of every array the H·v kernel streams per call (roofline accounting).
  * names: newline-separated names written into names_buf (may be NULL). */
@@ -153,6 +158,19 @@ int32_t smgpu_apply_H_multi(smgpu_ham h, int32_t precision, int32_t nvec, const 
 int32_t smgpu_apply_H_host_multi(smgpu_ham h, int32_t precision, int32_t nvec, const double* v, double* w);
 /* Timing of smgpu_apply_H_multi on resident vectors: milliseconds per call (all nvec). */
 int32_t smgpu_time_apply_H_multi(smgpu_ham h, int32_t precision, int32_t nvec, int32_t reps, double* ms);
+/* w_b = H_b v_b for handles that share one index (smgpu_ham_create_variant; a handle
+ * may repeat): vectors of one handle share launches of up to four, vectors of
+ * different handles are paired into launches that share the index loads and entry
+ * decodes with per-vector coefficients.  Each result is bit-identical to
+ * smgpu_apply_H(hs[b], ...) on v_b.  Device pointers in the launch precision; no
+ * counterpart in the reference (one interaction per operate_H! call). */
+int32_t smgpu_apply_H_variants(const smgpu_ham* hs, int32_t precision, int32_t nvec, const void* const* v_dev,
+                               void* const* w_dev, void* stream);
+/* Host-buffer convenience: v and w are nvec x mdim row-major FP64. */
+int32_t smgpu_apply_H_host_variants(const smgpu_ham* hs, int32_t precision, int32_t nvec, const double* v,
+                                    double* w);
+/* Timing of smgpu_apply_H_variants on resident vectors: milliseconds per call (all nvec). */
+int32_t smgpu_time_apply_H_variants(const smgpu_ham* hs, int32_t precision, int32_t nvec, int32_t reps, double* ms);
 /* Timing: `reps` launches of H·v on resident device vectors, HIP events on the
  * launch stream; average milliseconds per launch in *ms. */
 int32_t smgpu_time_apply_H(smgpu_ham h, int32_t precision, int32_t reps, double* ms);
@@ -187,6 +205,14 @@ int32_t smgpu_trl_solve_j(smgpu_ham h, smgpu_ham j2, int32_t twoJ, const smgpu_s
 int32_t smgpu_trl_solve_batch(smgpu_ham h, int32_t nb, const smgpu_solve_params* params, const double* const* v0,
                               double* const* evals, double* const* evecs, smgpu_solve_stats* stats,
                               int32_t* status);
+/* smgpu_trl_solve_batch over interaction variants on one basis: member b solves
+ * hs[b] (handles sharing one index, smgpu_ham_create_variant; a handle may repeat),
+ * with each round's H·v in smgpu_apply_H_variants' launches (prepEC's samples,
+ * eigenvector_continuation.jl:140-175, solved together instead of one after
+ * another).  Each member's results are bit-identical to smgpu_trl_solve_v0(hs[b]). */
+int32_t smgpu_trl_solve_variants(const smgpu_ham* hs, int32_t nb, const smgpu_solve_params* params,
+                                 const double* const* v0, double* const* evals, double* const* evecs,
+                                 smgpu_solve_stats* stats, int32_t* status);
 /* Thick-restart block Lanczos: main_sm with is_block=true and block size q
  * (TRBL, lanczos_methods.jl:312-420, with bl_operate_H!, bl_ReORTH, bl_QR! and
  * bl_ThickRestart :422-617; shellmodel_main.jl:167-191, 211-224).  params->lm is the

@@ -234,6 +234,19 @@ int32_t smgpu_ham_update_interaction(smgpu_ham h, smgpu_problem p) {
     });
 }
 
+int32_t smgpu_ham_create_variant(smgpu_ham base, smgpu_problem p, smgpu_ham* out) {
+    return guard([&]() -> int32_t {
+        if (!base || !p || !out) throw std::invalid_argument("null argument");
+        *out = nullptr;
+        if (base->dev->mdim != p->basis.mdim) throw std::invalid_argument("problem does not match the handle's basis");
+        SM_HIP(hipSetDevice(base->dev->device));
+        auto h = std::make_unique<smgpu_ham_s>();
+        h->dev = attach_ham(base->dev->idx, p->basis, p->tables);
+        *out = h.release();
+        return SMGPU_OK;
+    });
+}
+
 int32_t smgpu_table_bytes(smgpu_ham h, int32_t precision, int32_t max_arrays, int64_t* bytes, int32_t* n_arrays,
                           char* names_buf, int32_t names_len) {
     return guard([&]() -> int32_t {
@@ -380,6 +393,113 @@ int32_t smgpu_time_apply_H_multi(smgpu_ham h, int32_t precision, int32_t nvec, i
     });
 }
 
+// the handles' device Hamiltonians after checking that they share one index
+static std::vector<const DeviceHam*> variant_hams(const smgpu_ham* hs, int32_t nvec) {
+    if (!hs || nvec < 1) throw std::invalid_argument("bad argument");
+    std::vector<const DeviceHam*> d(nvec);
+    for (int b = 0; b < nvec; b++) {
+        if (!hs[b]) throw std::invalid_argument("null handle");
+        d[b] = hs[b]->dev.get();
+        if (d[b]->idx != d[0]->idx) throw std::invalid_argument("handles do not share one index");
+    }
+    return d;
+}
+
+int32_t smgpu_apply_H_variants(const smgpu_ham* hs, int32_t precision, int32_t nvec, const void* const* v_dev,
+                               void* const* w_dev, void* stream) {
+    return guard([&]() -> int32_t {
+        auto d = variant_hams(hs, nvec);
+        if (!v_dev || !w_dev) throw std::invalid_argument("null argument");
+        SM_HIP(hipSetDevice(d[0]->device));
+        hipStream_t s = stream ? (hipStream_t)stream : d[0]->stream;
+        launch_hv_variants(d.data(), precision == SMGPU_FP64 ? 0 : 1, nvec, v_dev, w_dev, s);
+        return SMGPU_OK;
+    });
+}
+
+int32_t smgpu_apply_H_host_variants(const smgpu_ham* hs, int32_t precision, int32_t nvec, const double* v,
+                                    double* w) {
+    return guard([&]() -> int32_t {
+        auto d = variant_hams(hs, nvec);
+        if (!v || !w) throw std::invalid_argument("null argument");
+        const DeviceHam& h0 = *d[0];
+        SM_HIP(hipSetDevice(h0.device));
+        hipStream_t s = h0.stream;
+        const int64_t n = h0.mdim, tot = n * nvec;
+        const bool fp64 = precision == SMGPU_FP64;
+        const size_t es = fp64 ? 8 : 4;
+        DBuf dv, dw, fv, fw;
+        dv.alloc(tot * 8);
+        dw.alloc(tot * 8);
+        SM_HIP(hipMemcpyAsync(dv.p, v, tot * 8, hipMemcpyHostToDevice, s));
+        const int g = (int)std::min<int64_t>(4096, (tot + 255) / 256);
+        if (!fp64) {
+            fv.alloc(tot * 4);
+            fw.alloc(tot * 4);
+            hipLaunchKernelGGL((k_cast<double, float>), dim3(g), dim3(256), 0, s, dv.as<double>(), fv.as<float>(), tot);
+        }
+        char* vin = static_cast<char*>(fp64 ? dv.p : fv.p);
+        char* wout = static_cast<char*>(fp64 ? dw.p : fw.p);
+        std::vector<const void*> vp(nvec);
+        std::vector<void*> wp(nvec);
+        for (int b = 0; b < nvec; b++) {
+            vp[b] = vin + (size_t)b * n * es;
+            wp[b] = wout + (size_t)b * n * es;
+        }
+        launch_hv_variants(d.data(), fp64 ? 0 : 1, nvec, vp.data(), wp.data(), s);
+        if (!fp64)
+            hipLaunchKernelGGL((k_cast<float, double>), dim3(g), dim3(256), 0, s, fw.as<float>(), dw.as<double>(), tot);
+        SM_HIP(hipMemcpyAsync(w, dw.p, tot * 8, hipMemcpyDeviceToHost, s));
+        SM_HIP(hipStreamSynchronize(s));
+        return SMGPU_OK;
+    });
+}
+
+int32_t smgpu_time_apply_H_variants(const smgpu_ham* hs, int32_t precision, int32_t nvec, int32_t reps, double* ms) {
+    return guard([&]() -> int32_t {
+        auto d = variant_hams(hs, nvec);
+        if (!ms || reps < 1) throw std::invalid_argument("bad argument");
+        const DeviceHam& h0 = *d[0];
+        SM_HIP(hipSetDevice(h0.device));
+        hipStream_t st = h0.stream;
+        const int64_t n = h0.mdim, tot = n * nvec;
+        const bool fp64 = precision == SMGPU_FP64;
+        const size_t es = fp64 ? 8 : 4;
+        DBuf dv, dw;
+        dv.alloc(tot * es);
+        dw.alloc(tot * es);
+        std::vector<double> hv(tot);
+        for (int64_t i = 0; i < tot; i++)
+            hv[i] = 1.0 / std::sqrt((double)n) * (1.0 + 0.25 * std::sin(0.37 * (double)i));
+        if (fp64) SM_HIP(hipMemcpy(dv.p, hv.data(), tot * 8, hipMemcpyHostToDevice));
+        else {
+            std::vector<float> f(hv.begin(), hv.end());
+            SM_HIP(hipMemcpy(dv.p, f.data(), tot * 4, hipMemcpyHostToDevice));
+        }
+        std::vector<const void*> vp(nvec);
+        std::vector<void*> wp(nvec);
+        for (int b = 0; b < nvec; b++) {
+            vp[b] = static_cast<const char*>(dv.p) + (size_t)b * n * es;
+            wp[b] = static_cast<char*>(dw.p) + (size_t)b * n * es;
+        }
+        const int p = fp64 ? 0 : 1;
+        for (int r = 0; r < 2; r++) launch_hv_variants(d.data(), p, nvec, vp.data(), wp.data(), st);
+        hipEvent_t e0, e1;
+        SM_HIP(hipEventCreate(&e0));
+        SM_HIP(hipEventCreate(&e1));
+        SM_HIP(hipEventRecord(e0, st));
+        for (int r = 0; r < reps; r++) launch_hv_variants(d.data(), p, nvec, vp.data(), wp.data(), st);
+        SM_HIP(hipEventRecord(e1, st));
+        SM_HIP(hipEventSynchronize(e1));
+        float t = 0.f;
+        SM_HIP(hipEventElapsedTime(&t, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *ms = (double)t / reps;
+        return SMGPU_OK;
+    });
+}
+
 int32_t smgpu_time_apply_H(smgpu_ham h, int32_t precision, int32_t reps, double* ms) {
     return guard([&]() -> int32_t {
         if (!h || !ms || reps < 1) throw std::invalid_argument("bad argument");
@@ -479,53 +599,74 @@ int32_t smgpu_trl_solve_j(smgpu_ham h, smgpu_ham j2, int32_t twoJ, const smgpu_s
     });
 }
 
+// lockstep batch on d; hams: per-member Hamiltonians on d's index, or null
+static int32_t solve_batch(DeviceHam& d, DeviceHam* const* hams, int32_t nb, const smgpu_solve_params* params,
+                           const double* const* v0, double* const* evals, double* const* evecs,
+                           smgpu_solve_stats* stats, int32_t* status) {
+    SM_HIP(hipSetDevice(d.device));
+    std::vector<smgpu_solve_params> p(params, params + nb);
+    for (auto& q : p) {
+        if (q.max_iter <= 0) q.max_iter = 100000;
+        if (q.num_history < 2) q.num_history = 3;
+        if (q.num_ev > 64) throw std::invalid_argument("num_ev above 64");
+        q.num_ev = (int32_t)std::min<int64_t>(q.num_ev, d.mdim);
+    }
+    std::vector<SolveCtx> ctx(nb);
+    std::vector<const SolveCtx*> cp(nb);
+    struct Streams {
+        std::vector<SolveCtx>& c;
+        ~Streams() {
+            for (auto& x : c) if (x.stream) { (void)hipStreamSynchronize(x.stream); (void)hipStreamDestroy(x.stream); }
+        }
+    } guard_streams{ctx};
+    for (int b = 0; b < nb; b++) {
+        SM_HIP(hipStreamCreateWithFlags(&ctx[b].stream, hipStreamNonBlocking));
+        cp[b] = &ctx[b];
+    }
+    std::vector<smgpu_solve_stats> st(nb);
+    std::memset(st.data(), 0, sizeof(smgpu_solve_stats) * nb);
+    std::vector<int32_t> stat(nb, SMGPU_OK);
+    std::vector<std::string> msgs(nb);
+    BatchIO io;
+    io.hams = hams;
+    io.p = p.data();
+    io.v0 = v0;
+    io.evals = evals;
+    io.evecs = evecs;
+    io.st = st.data();
+    io.status = stat.data();
+    io.msgs = msgs.data();
+    trl_solve_batch(d, nb, nb, io, cp.data(), d.stream);
+    if (stats) for (int b = 0; b < nb; b++) stats[b] = st[b];
+    if (status) for (int b = 0; b < nb; b++) status[b] = stat[b];
+    for (int b = 0; b < nb; b++)
+        if (stat[b] != SMGPU_OK) {
+            set_last_error("batch member " + std::to_string(b) + ": " + msgs[b]);
+            return stat[b];
+        }
+    return SMGPU_OK;
+}
+
 int32_t smgpu_trl_solve_batch(smgpu_ham h, int32_t nb, const smgpu_solve_params* params, const double* const* v0,
                               double* const* evals, double* const* evecs, smgpu_solve_stats* stats,
                               int32_t* status) {
     return guard([&]() -> int32_t {
         if (!h || nb < 1 || !params || !evals) throw std::invalid_argument("null argument");
-        DeviceHam& d = *h->dev;
-        SM_HIP(hipSetDevice(d.device));
-        std::vector<smgpu_solve_params> p(params, params + nb);
-        for (auto& q : p) {
-            if (q.max_iter <= 0) q.max_iter = 100000;
-            if (q.num_history < 2) q.num_history = 3;
-            if (q.num_ev > 64) throw std::invalid_argument("num_ev above 64");
-            q.num_ev = (int32_t)std::min<int64_t>(q.num_ev, d.mdim);
-        }
-        std::vector<SolveCtx> ctx(nb);
-        std::vector<const SolveCtx*> cp(nb);
-        struct Streams {
-            std::vector<SolveCtx>& c;
-            ~Streams() {
-                for (auto& x : c) if (x.stream) { (void)hipStreamSynchronize(x.stream); (void)hipStreamDestroy(x.stream); }
-            }
-        } guard_streams{ctx};
-        for (int b = 0; b < nb; b++) {
-            SM_HIP(hipStreamCreateWithFlags(&ctx[b].stream, hipStreamNonBlocking));
-            cp[b] = &ctx[b];
-        }
-        std::vector<smgpu_solve_stats> st(nb);
-        std::memset(st.data(), 0, sizeof(smgpu_solve_stats) * nb);
-        std::vector<int32_t> stat(nb, SMGPU_OK);
-        std::vector<std::string> msgs(nb);
-        BatchIO io;
-        io.p = p.data();
-        io.v0 = v0;
-        io.evals = evals;
-        io.evecs = evecs;
-        io.st = st.data();
-        io.status = stat.data();
-        io.msgs = msgs.data();
-        trl_solve_batch(d, nb, nb, io, cp.data(), d.stream);
-        if (stats) for (int b = 0; b < nb; b++) stats[b] = st[b];
-        if (status) for (int b = 0; b < nb; b++) status[b] = stat[b];
+        return solve_batch(*h->dev, nullptr, nb, params, v0, evals, evecs, stats, status);
+    });
+}
+
+int32_t smgpu_trl_solve_variants(const smgpu_ham* hs, int32_t nb, const smgpu_solve_params* params,
+                                 const double* const* v0, double* const* evals, double* const* evecs,
+                                 smgpu_solve_stats* stats, int32_t* status) {
+    return guard([&]() -> int32_t {
+        if (nb < 1 || !params || !evals) throw std::invalid_argument("null argument");
+        variant_hams(hs, nb);
+        std::vector<DeviceHam*> d(nb);
+        for (int b = 0; b < nb; b++) d[b] = hs[b]->dev.get();
         for (int b = 0; b < nb; b++)
-            if (stat[b] != SMGPU_OK) {
-                set_last_error("batch member " + std::to_string(b) + ": " + msgs[b]);
-                return stat[b];
-            }
-        return SMGPU_OK;
+            if (d[b]->mdim != d[0]->mdim) throw std::invalid_argument("handles do not share one basis");
+        return solve_batch(*d[0], d.data(), nb, params, v0, evals, evecs, stats, status);
     });
 }
 

@@ -154,10 +154,22 @@ void launch_hv(const DeviceHam& h, int prec, const void* v, void* w, hipStream_t
 struct HvVecs {
     const void* v[4];
     void* w[4];
+    // per-vector coefficients (interaction variants on one index; used by the PV
+    // kernels): e_p, e_n, pp values, nn values and V_eff of vector b's Hamiltonian
+    const void* ep[4];
+    const void* en[4];
+    const void* pp[4];
+    const void* nn[4];
+    const void* veff[4];
 };
 // w[b] = H v[b] for nvec arbitrary vectors, in launches of up to four that share
 // the table loads (each result bit-identical to launch_hv)
 void launch_hv_set(const DeviceHam& h, int prec, int nvec, const void* const* v, void* const* w, hipStream_t s);
+// w[b] = H_b v[b] for Hamiltonians that share one index (interaction variants on one
+// basis): launches of up to four share the index loads and entry decodes, each vector
+// with its own coefficients; results are bit-identical to launch_hv(*hams[b], ...)
+void launch_hv_variants(const DeviceHam* const* hams, int prec, int nvec, const void* const* v, void* const* w,
+                        hipStream_t s);
 // w_b = H v_b for nvec vectors at v + b*ldv -> w + b*ldw (elements), in launches of
 // up to four vectors that share the table loads
 void launch_hv_multi(const DeviceHam& h, int prec, int nvec, const void* v, int64_t ldv, void* w, int64_t ldw,

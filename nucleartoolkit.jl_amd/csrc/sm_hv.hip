@@ -62,16 +62,17 @@ template <> struct Vec4<double> { using type = double4; };
 // so that a lane fetches the four hops of a row for its neutron hop (in, col)
 // with one 4-wide LDS load per vector and one for V_eff, at compile-time offsets.
 // Unused slots are zero.
-template <typename T, int NPT, int RG, int PCH, int NV>
+template <typename T, int NPT, int RG, int PCH, int NV, bool PV>
 __global__ __launch_bounds__(256) void k_hv(HvTables A, HvVecs P) {
     static_assert(PCH % 4 == 0, "slots come in groups of four");
     constexpr int NG = PCH / 4;       // slot groups per row and round
     constexpr int SL = RG * PCH;      // slots per round
     using T4 = typename Vec4<T>::type;
     extern __shared__ __align__(16) unsigned char smem_raw[];
+    constexpr int NC = PV ? NV : 1;  // coefficient sets (PV: one per vector)
     T* X = reinterpret_cast<T*>(smem_raw);                      // nN_max * NV * SL
-    T* Vs = X + (size_t)A.nN_max * SL * NV;                      // cols2_max * SL
-    T* xs = Vs + (size_t)A.cols2_max * SL;                       // nN_max * RG * NV  ([col][row][vec])
+    T* Vs = X + (size_t)A.nN_max * SL * NV;                      // NC * cols2_max * SL
+    T* xs = Vs + (size_t)A.cols2_max * SL * NC;                  // nN_max * RG * NV  ([col][row][vec])
     const int g = blockIdx.x;
     const int tid = threadIdx.x;
     const int bf = A.grp_block[g];
@@ -80,8 +81,12 @@ __global__ __launch_bounds__(256) void k_hv(HvTables A, HvVecs P) {
     const int nr = min(RG, A.nP[bf] - fp0);
     const int row0 = A.block_row0[bf] + fp0;          // global row index of fp0
     const int64_t bstart = A.tdims[bf];
-    const T* ep = reinterpret_cast<const T*>(A.ep);
-    const T* en = reinterpret_cast<const T*>(A.en) + A.en_off[bf];
+    // coefficient set of vector b (the launch's Hamiltonian, or vector b's own for PV)
+    auto cep = [&](int b) { return static_cast<const T*>(PV ? P.ep[b] : A.ep); };
+    auto cen = [&](int b) { return static_cast<const T*>(PV ? P.en[b] : A.en) + A.en_off[bf]; };
+    auto cpp = [&](int b) { return static_cast<const T*>(PV ? P.pp[b] : A.pp_val); };
+    auto cnn = [&](int b) { return static_cast<const T*>(PV ? P.nn[b] : A.nn_val); };
+    auto cve = [&](int b) { return static_cast<const T*>(PV ? P.veff[b] : A.veff); };
     for (int i = tid; i < nN; i += 256) {
 #pragma unroll
         for (int r = 0; r < RG; r++)
@@ -93,10 +98,11 @@ __global__ __launch_bounds__(256) void k_hv(HvTables A, HvVecs P) {
     __syncthreads();
 
     T acc[NPT][RG][NV];
-    T epr[RG];
+    T epr[RG][NC];
 #pragma unroll
-    for (int r = 0; r < RG; r++) epr[r] = r < nr ? ep[row0 + r] : T(0);
-    const T* nnv = reinterpret_cast<const T*>(A.nn_val);
+    for (int r = 0; r < RG; r++)
+#pragma unroll
+        for (int c = 0; c < NC; c++) epr[r][c] = r < nr ? cep(c)[row0 + r] : T(0);
     const int64_t nsb = A.nn_slice_base[bf];
     // Column k of a lane is fn = tid + 256 k, in slice (tid >> 6) + 4 k: the same for
     // the whole wave, so slice tests are wave-uniform.  Lanes past nN inside the
@@ -114,21 +120,25 @@ __global__ __launch_bounds__(256) void k_hv(HvTables A, HvVecs P) {
         const int sk = wave + 4 * k;
         if (sk < nsl) {
             const int fnc = min(tid + k * 256, nN - 1);
-            const T e_n = en[fnc];
+            T e_n[NC];
+#pragma unroll
+            for (int c = 0; c < NC; c++) e_n[c] = cen(c)[fnc];
 #pragma unroll
             for (int r = 0; r < RG; r++)
 #pragma unroll
-                for (int b = 0; b < NV; b++) acc[k][r][b] = (epr[r] + e_n) * xs[(fnc * RG + r) * NV + b];
+                for (int b = 0; b < NV; b++)
+                    acc[k][r][b] = (epr[r][PV ? b : 0] + e_n[PV ? b : 0]) * xs[(fnc * RG + r) * NV + b];
             const int64_t s = nsb + sk;
             const int64_t off = A.nn_slice_off[s] + (tid & 63);
             const int len = __builtin_amdgcn_readfirstlane(A.nn_slice_len[s]);
             int q = 0;
             for (; q + 4 <= len; q += 4) {
-                T c[4];
+                T c[4][NC];
                 int col[4];
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
-                    c[u] = nnv[off + (int64_t)(q + u) * 64];
+#pragma unroll
+                    for (int cc = 0; cc < NC; cc++) c[u][cc] = cnn(cc)[off + (int64_t)(q + u) * 64];
                     col[u] = A.nn_col[off + (int64_t)(q + u) * 64];
                 }
 #pragma unroll
@@ -136,15 +146,18 @@ __global__ __launch_bounds__(256) void k_hv(HvTables A, HvVecs P) {
 #pragma unroll
                     for (int r = 0; r < RG; r++)
 #pragma unroll
-                        for (int b = 0; b < NV; b++) acc[k][r][b] += c[u] * xs[(col[u] * RG + r) * NV + b];
+                        for (int b = 0; b < NV; b++)
+                            acc[k][r][b] += c[u][PV ? b : 0] * xs[(col[u] * RG + r) * NV + b];
             }
             for (; q < len; q++) {
-                const T c = nnv[off + (int64_t)q * 64];
+                T c[NC];
+#pragma unroll
+                for (int cc = 0; cc < NC; cc++) c[cc] = cnn(cc)[off + (int64_t)q * 64];
                 const int col = A.nn_col[off + (int64_t)q * 64];
 #pragma unroll
                 for (int r = 0; r < RG; r++)
 #pragma unroll
-                    for (int b = 0; b < NV; b++) acc[k][r][b] += c * xs[(col * RG + r) * NV + b];
+                    for (int b = 0; b < NV; b++) acc[k][r][b] += c[PV ? b : 0] * xs[(col * RG + r) * NV + b];
             }
         }
     }
@@ -152,18 +165,18 @@ __global__ __launch_bounds__(256) void k_hv(HvTables A, HvVecs P) {
     // issued together as unconditional loads
     {
         constexpr int UP = NPT * NV <= 2 ? 8 : (NPT * NV <= 4 ? 4 : 2);
-        const T* ppv = reinterpret_cast<const T*>(A.pp_val);
 #pragma unroll
         for (int r = 0; r < RG; r++) {
             if (r < nr) {
                 const int64_t pe0 = A.pp_rowptr[row0 + r], pe1 = A.pp_rowptr[row0 + r + 1];
                 for (int64_t e = pe0; e < pe1; e += UP) {
-                    T c[UP];
+                    T c[UP][NC];
                     int64_t so[UP];
 #pragma unroll
                     for (int u = 0; u < UP; u++) {
                         const int64_t eu = min(e + u, pe1 - 1);
-                        c[u] = e + u < pe1 ? ppv[eu] : T(0);
+#pragma unroll
+                        for (int cc = 0; cc < NC; cc++) c[u][cc] = e + u < pe1 ? cpp(cc)[eu] : T(0);
                         so[u] = bstart + (int64_t)A.pp_col[eu] * nN;
                     }
 #pragma unroll
@@ -177,7 +190,7 @@ __global__ __launch_bounds__(256) void k_hv(HvTables A, HvVecs P) {
                             for (int u = 0; u < UP; u++) x[u] = static_cast<const T*>(P.v[b])[so[u] + fnc];
                             T a = acc[k][r][b];
 #pragma unroll
-                            for (int u = 0; u < UP; u++) a += c[u] * x[u];
+                            for (int u = 0; u < UP; u++) a += c[u][PV ? b : 0] * x[u];
                             acc[k][r][b] = a;
                         }
                     }
@@ -186,7 +199,6 @@ __global__ __launch_bounds__(256) void k_hv(HvTables A, HvVecs P) {
         }
     }
     // pn
-    const T* veff = reinterpret_cast<const T*>(A.veff);
     const int q1 = A.pair_ptr[bf + 1];
     for (int q = A.pair_ptr[bf]; q < q1; q++) {
         const int pt = A.pair_ptab[q];
@@ -204,7 +216,7 @@ __global__ __launch_bounds__(256) void k_hv(HvTables A, HvVecs P) {
         const uint32_t* pent = A.ptab_ent + A.ptab_ent_base[pt];
         const int nNi = A.nN[bi];
         const int64_t ibase = A.tdims[bi];
-        const T* V2 = veff + A.veff_off[dm];
+        const int64_t vbase = A.veff_off[dm];
         const int c2 = A.veff_cols2[dm];
         const int64_t sb = A.ntab_slice_base[nt];
         // column k of a lane is fn = tid + 256 k: slice (tid >> 6) + 4 k, the same
@@ -269,13 +281,18 @@ __global__ __launch_bounds__(256) void k_hv(HvTables A, HvVecs P) {
                             *reinterpret_cast<T4*>(X + ((size_t)i * NV + b) * SL + so) = x4;
                         }
                     }
-                    for (int c = tid; c < c2; c += 256) {
-                        T4 v4;
-                        v4.x = vm[0] * V2[vo[0] + c];
-                        v4.y = vm[1] * V2[vo[1] + c];
-                        v4.z = vm[2] * V2[vo[2] + c];
-                        v4.w = vm[3] * V2[vo[3] + c];
-                        *reinterpret_cast<T4*>(Vs + (size_t)c * SL + so) = v4;
+#pragma unroll
+                    for (int cc = 0; cc < NC; cc++) {
+                        const T* V2 = cve(cc) + vbase;
+                        T* Vd = Vs + (size_t)cc * A.cols2_max * SL;
+                        for (int c = tid; c < c2; c += 256) {
+                            T4 v4;
+                            v4.x = vm[0] * V2[vo[0] + c];
+                            v4.y = vm[1] * V2[vo[1] + c];
+                            v4.z = vm[2] * V2[vo[2] + c];
+                            v4.w = vm[3] * V2[vo[3] + c];
+                            *reinterpret_cast<T4*>(Vd + (size_t)c * SL + so) = v4;
+                        }
                     }
                 }
             __syncthreads();
@@ -320,9 +337,11 @@ __global__ __launch_bounds__(256) void k_hv(HvTables A, HvVecs P) {
 #pragma unroll
                             for (int k = 0; k < NPT; k++) {
                                 if (qq >= len[k]) continue;          // wave-uniform
-                                const T4 vv = *reinterpret_cast<const T4*>(Vs + (size_t)col[d][k] * SL + so);
+                                T4 vv = *reinterpret_cast<const T4*>(Vs + (size_t)col[d][k] * SL + so);
 #pragma unroll
                                 for (int b = 0; b < NV; b++) {
+                                    if (PV && b > 0)
+                                        vv = *reinterpret_cast<const T4*>(Vs + ((size_t)b * A.cols2_max + col[d][k]) * SL + so);
                                     const T4 xv =
                                         *reinterpret_cast<const T4*>(X + ((size_t)in[d][k] * NV + b) * SL + so);
                                     T a = acc[k][r][b];
@@ -367,28 +386,32 @@ static void raise_lds_limit(const void* fn, size_t bytes) {
     cur = bytes;
 }
 
-// instantiated shapes (NPT, RG, PCH, NV): one vector for every width; two and
-// four vectors (NV = 4 up to 1,536 neutron determinants per block)
-#define HV_NPTS(X_, T_, R_, P_, V_) X_(T_, 1, R_, P_, V_) X_(T_, 2, R_, P_, V_) X_(T_, 3, R_, P_, V_) \
-    X_(T_, 4, R_, P_, V_) X_(T_, 6, R_, P_, V_) X_(T_, 8, R_, P_, V_) X_(T_, 12, R_, P_, V_) X_(T_, 16, R_, P_, V_)
-#define HV_NPTS_NARROW(X_, T_, R_, P_, V_) X_(T_, 1, R_, P_, V_) X_(T_, 2, R_, P_, V_) X_(T_, 3, R_, P_, V_) \
-    X_(T_, 4, R_, P_, V_) X_(T_, 6, R_, P_, V_)
-#define HV_FOR_SHAPES(X_, T_) HV_NPTS(X_, T_, 2, 4, 1) HV_NPTS(X_, T_, 1, 4, 1) \
-    HV_NPTS(X_, T_, 2, 4, 2) HV_NPTS(X_, T_, 1, 4, 2) HV_NPTS_NARROW(X_, T_, 2, 4, 4) HV_NPTS_NARROW(X_, T_, 1, 4, 4)
+// instantiated shapes (NPT, RG, PCH, NV, PV): one vector for every width; two and
+// four vectors on one Hamiltonian (four up to 1,536 neutron determinants per block);
+// two interaction variants on one index (PV)
+#define HV_NPTS(X_, T_, R_, P_, V_, W_) X_(T_, 1, R_, P_, V_, W_) X_(T_, 2, R_, P_, V_, W_) \
+    X_(T_, 3, R_, P_, V_, W_) X_(T_, 4, R_, P_, V_, W_) X_(T_, 6, R_, P_, V_, W_) X_(T_, 8, R_, P_, V_, W_) \
+    X_(T_, 12, R_, P_, V_, W_) X_(T_, 16, R_, P_, V_, W_)
+#define HV_NPTS_NARROW(X_, T_, R_, P_, V_, W_) X_(T_, 1, R_, P_, V_, W_) X_(T_, 2, R_, P_, V_, W_) \
+    X_(T_, 3, R_, P_, V_, W_) X_(T_, 4, R_, P_, V_, W_) X_(T_, 6, R_, P_, V_, W_)
+#define HV_FOR_SHAPES(X_, T_) HV_NPTS(X_, T_, 2, 4, 1, false) HV_NPTS(X_, T_, 1, 4, 1, false) \
+    HV_NPTS(X_, T_, 2, 4, 2, false) HV_NPTS(X_, T_, 1, 4, 2, false) \
+    HV_NPTS_NARROW(X_, T_, 2, 4, 4, false) HV_NPTS_NARROW(X_, T_, 1, 4, 4, false) \
+    HV_NPTS(X_, T_, 2, 4, 2, true) HV_NPTS(X_, T_, 1, 4, 2, true)
 
-template <typename T> static const void* hv_fn(int npt, int rg, int pch, int nv) {
-#define HV_CASE(T_, N_, R_, P_, V_) \
-    if (npt == N_ && rg == R_ && pch == P_ && nv == V_) return (const void*)k_hv<T_, N_, R_, P_, V_>;
+template <typename T> static const void* hv_fn(int npt, int rg, int pch, int nv, bool pv) {
+#define HV_CASE(T_, N_, R_, P_, V_, W_) \
+    if (npt == N_ && rg == R_ && pch == P_ && nv == V_ && pv == W_) return (const void*)k_hv<T_, N_, R_, P_, V_, W_>;
     HV_FOR_SHAPES(HV_CASE, T)
 #undef HV_CASE
     return nullptr;
 }
 
 // LDS of one workgroup for nv vectors: X (nN_max x slots x nv), Vs (cols2_max x
-// slots), own rows (nN_max x rg x nv)
-static size_t hv_lds_bytes(const HvTables& A, int nv, size_t tsize) {
-    return ((size_t)A.nN_max * A.pch * A.rg * nv + (size_t)A.cols2_max * A.pch * A.rg + (size_t)A.nN_max * A.rg * nv) *
-           tsize;
+// slots, per vector for variants), own rows (nN_max x rg x nv)
+static size_t hv_lds_bytes(const HvTables& A, int nv, bool pv, size_t tsize) {
+    return ((size_t)A.nN_max * A.pch * A.rg * nv + (size_t)A.cols2_max * A.pch * A.rg * (pv ? nv : 1) +
+            (size_t)A.nN_max * A.rg * nv) * tsize;
 }
 
 // Widest instantiated vector count <= nv whose workgroup LDS stays within 32 KiB.
@@ -396,24 +419,25 @@ static size_t hv_lds_bytes(const HvTables& A, int nv, size_t tsize) {
 // with two vectors and 0.36 with four (48 KiB); Mg24 0.044 -> 0.017 ms with four;
 // Mn50 and Cr48 FP64 (over 32 KiB at two) gain nothing and lose occupancy.
 static constexpr size_t HV_LDS_SHARED = 32 * 1024;
-template <typename T> static int hv_width(const HvTables& A, int nv) {
+template <typename T> static int hv_width(const HvTables& A, int nv, bool pv) {
     for (int c : {4, 2}) {
         if (c > nv) continue;
-        if (hv_fn<T>(A.npt, A.rg, A.pch, c) && hv_lds_bytes(A, c, sizeof(T)) <= HV_LDS_SHARED) return c;
+        if (hv_fn<T>(A.npt, A.rg, A.pch, c, pv) && hv_lds_bytes(A, c, pv, sizeof(T)) <= HV_LDS_SHARED) return c;
     }
     return 1;
 }
 
 template <typename T>
-static void launch_t(const HvTables& A, int nv, const HvVecs& P, hipStream_t s) {
+static void launch_t(const HvTables& A, int nv, bool pv, const HvVecs& P, hipStream_t s) {
     dim3 grid(A.G), block(256);
-    const size_t lds = hv_lds_bytes(A, nv, sizeof(T));
-    if (nv > 1) raise_lds_limit(hv_fn<T>(A.npt, A.rg, A.pch, nv), lds);
+    if (nv == 1) pv = false;
+    const size_t lds = hv_lds_bytes(A, nv, pv, sizeof(T));
+    if (nv > 1) raise_lds_limit(hv_fn<T>(A.npt, A.rg, A.pch, nv, pv), lds);
     bool done = false;
-#define HV_LAUNCH(T_, N_, R_, P_, V_)                                                                      \
-    if (!done && A.npt == N_ && A.rg == R_ && A.pch == P_ && nv == V_) {                                   \
-        hipLaunchKernelGGL((k_hv<T_, N_, R_, P_, V_>), grid, block, lds, s, A, P);                        \
-        done = true;                                                                                       \
+#define HV_LAUNCH(T_, N_, R_, P_, V_, W_)                                                                    \
+    if (!done && A.npt == N_ && A.rg == R_ && A.pch == P_ && nv == V_ && pv == W_) {                         \
+        hipLaunchKernelGGL((k_hv<T_, N_, R_, P_, V_, W_>), grid, block, lds, s, A, P);                      \
+        done = true;                                                                                         \
     }
     HV_FOR_SHAPES(HV_LAUNCH, T)
 #undef HV_LAUNCH
@@ -426,17 +450,17 @@ void launch_hv(const DeviceHam& h, int prec, const void* v, void* w, hipStream_t
     HvVecs P{};
     P.v[0] = v;
     P.w[0] = w;
-    if (prec == 0) launch_t<double>(h.view[0], 1, P, s);
-    else launch_t<float>(h.view[1], 1, P, s);
+    if (prec == 0) launch_t<double>(h.view[0], 1, false, P, s);
+    else launch_t<float>(h.view[1], 1, false, P, s);
 }
 
 template <typename T>
 static void launch_set_t(const HvTables& A, int nvec, const void* const* v, void* const* w, hipStream_t s) {
     for (int b = 0; b < nvec;) {
-        const int c = hv_width<T>(A, nvec - b);
+        const int c = hv_width<T>(A, nvec - b, false);
         HvVecs P{};
         for (int j = 0; j < c; j++) { P.v[j] = v[b + j]; P.w[j] = w[b + j]; }
-        launch_t<T>(A, c, P, s);
+        launch_t<T>(A, c, false, P, s);
         b += c;
     }
 }
@@ -445,6 +469,47 @@ void launch_hv_set(const DeviceHam& h, int prec, int nvec, const void* const* v,
     if (h.idx->R == 0 || nvec <= 0) return;
     if (prec == 0) launch_set_t<double>(h.view[0], nvec, v, w, s);
     else launch_set_t<float>(h.view[1], nvec, v, w, s);
+}
+
+// Runs of one Hamiltonian (same handle) go to the shared-coefficient kernels (up to
+// four vectors); the rest are paired into per-vector-coefficient launches.
+template <typename T>
+static void launch_variants_t(const DeviceHam* const* hams, int prec, int nvec, const void* const* v,
+                              void* const* w, hipStream_t s) {
+    const HvTables& A = hams[0]->view[prec];
+    for (int b = 0; b < nvec;) {
+        int run = 1;
+        while (b + run < nvec && hams[b + run] == hams[b]) run++;
+        if (run >= 2) {
+            const int c = hv_width<T>(A, run, false);
+            launch_set_t<T>(hams[b]->view[prec], c, v + b, w + b, s);
+            b += c;
+            continue;
+        }
+        const int c = hv_width<T>(A, nvec - b, true);
+        HvVecs P{};
+        for (int j = 0; j < c; j++) {
+            const HvTables& Aj = hams[b + j]->view[prec];
+            P.v[j] = v[b + j];
+            P.w[j] = w[b + j];
+            P.ep[j] = Aj.ep;
+            P.en[j] = Aj.en;
+            P.pp[j] = Aj.pp_val;
+            P.nn[j] = Aj.nn_val;
+            P.veff[j] = Aj.veff;
+        }
+        launch_t<T>(c == 1 ? hams[b]->view[prec] : A, c, c > 1, P, s);
+        b += c;
+    }
+}
+
+void launch_hv_variants(const DeviceHam* const* hams, int prec, int nvec, const void* const* v, void* const* w,
+                        hipStream_t s) {
+    if (nvec <= 0 || hams[0]->idx->R == 0) return;
+    for (int b = 1; b < nvec; b++)
+        if (hams[b]->idx != hams[0]->idx) throw std::invalid_argument("variants must share one index");
+    if (prec == 0) launch_variants_t<double>(hams, prec, nvec, v, w, s);
+    else launch_variants_t<float>(hams, prec, nvec, v, w, s);
 }
 
 void launch_hv_multi(const DeviceHam& h, int prec, int nvec, const void* v, int64_t ldv, void* w, int64_t ldw,
@@ -732,7 +797,7 @@ template <typename T> void set_view(DeviceIndex& h, int prec, const HostLayout& 
     A.G = (int)gb.size();
     A.grp_block = h.grp_block[prec].as<int32_t>();
     A.grp_fp0 = h.grp_fp0[prec].as<int32_t>();
-    const void* fn = hv_fn<T>(A.npt, A.rg, A.pch, 1);
+    const void* fn = hv_fn<T>(A.npt, A.rg, A.pch, 1, false);
     if (!fn) throw std::runtime_error("no H·v kernel instance for this shape");
     raise_lds_limit(fn, A.lds_bytes);
 }

@@ -54,7 +54,11 @@ void trl_solve(DeviceHam& h, const SolveParams& p, double* evals, double* evecs,
 // failure does not stop the others.  on_done(b) runs when slot b's member has
 // finished (results written) or failed; afterwards, and between rounds, join(b)
 // may fill io at a free slot b with a new member (same precision) and return true.
+// hams (optional): member b solves *hams[b], a Hamiltonian on h's index tables
+// (interaction variants on one basis; null: every member solves h); the shared
+// launches then give each vector its member's coefficients.
 struct BatchIO {
+    DeviceHam* const* hams = nullptr;
     SolveParams* p = nullptr;
     const double* const* v0 = nullptr;
     double* const* evals = nullptr;

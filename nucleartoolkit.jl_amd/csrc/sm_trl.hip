@@ -1797,7 +1797,10 @@ void trl_solve(DeviceHam& h, const SolveParams& p, double* evals, double* evecs,
 // Several TRL solves on one Hamiltonian in lockstep: each round issues the H·v of
 // every active member in shared launches (launch_hv_set, up to four vectors per
 // launch) on hv_stream, after which each member runs the rest of its step on its
-// own stream.  Members keep their own basis, workspace and convergence; each
+// own stream.  With per-member Hamiltonians (io.hams, interaction variants on one
+// index) the round's vectors are ordered by Hamiltonian and issued through
+// launch_hv_variants: members of one Hamiltonian share launches of up to four,
+// the others pair up with their own coefficients.  Members keep their own basis, workspace and convergence; each
 // result is bit-identical to the member solved alone with the same parameters.
 // Member slots (stream + workspace, ctx[b]) are reused: a member that finishes or
 // fails frees its slot, and before every round join(b) may place a new solve in a
@@ -1810,6 +1813,7 @@ static void batch_run(DeviceHam& h, int cap, int nb, const BatchIO& io, const So
     std::vector<char> busy(cap, 0);
     std::vector<int> active;
     std::vector<hipEvent_t> ready(cap, nullptr);
+    std::vector<const DeviceHam*> mh(cap, &h);
     hipEvent_t evh = nullptr;
     struct Events {
         std::vector<hipEvent_t>& r;
@@ -1832,7 +1836,10 @@ static void batch_run(DeviceHam& h, int cap, int nb, const BatchIO& io, const So
         io.status[b] = SMGPU_OK;
         busy[b] = 1;
         try {
-            S[b] = std::make_unique<Solver<TBv>>(h, io.p[b], io.st[b], ctx[b]);
+            DeviceHam& hb = io.hams ? *io.hams[b] : h;
+            if (hb.idx != h.idx) throw std::invalid_argument("a batch member's Hamiltonian is on other index tables");
+            S[b] = std::make_unique<Solver<TBv>>(hb, io.p[b], io.st[b], ctx[b]);
+            mh[b] = &hb;
             S[b]->step_ready = ready[b];
             S[b]->start(io.v0 ? io.v0[b] : nullptr);
             SM_HIP(hipEventRecord(ready[b], S[b]->stream()));      // start vector ready
@@ -1844,19 +1851,28 @@ static void batch_run(DeviceHam& h, int cap, int nb, const BatchIO& io, const So
     for (int b = 0; b < nb; b++) begin(b);
     std::vector<const void*> vin;
     std::vector<void*> vout;
+    std::vector<const DeviceHam*> hin;
+    std::vector<int> order;
+    const int prec = sizeof(TBv) == 4 ? 1 : 0;
     for (;;) {
         if (join)
             for (int b = 0; b < cap; b++)
                 if (!busy[b] && join(b)) begin(b);
         if (active.empty()) break;
+        order = active;
+        if (io.hams)
+            std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return mh[a] < mh[b]; });
         vin.clear();
         vout.clear();
-        for (int b : active) {
+        hin.clear();
+        for (int b : order) {
             SM_HIP(hipStreamWaitEvent(hv_stream, ready[b], 0));
             vin.push_back(S[b]->hv_in());
             vout.push_back(S[b]->hv_out());
+            hin.push_back(mh[b]);
         }
-        launch_hv_set(h, sizeof(TBv) == 4 ? 1 : 0, (int)active.size(), vin.data(), vout.data(), hv_stream);
+        if (io.hams) launch_hv_variants(hin.data(), prec, (int)order.size(), vin.data(), vout.data(), hv_stream);
+        else launch_hv_set(h, prec, (int)order.size(), vin.data(), vout.data(), hv_stream);
         SM_HIP(hipEventRecord(evh, hv_stream));
         std::vector<int> next;
         for (int b : active) {

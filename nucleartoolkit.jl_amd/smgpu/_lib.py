@@ -96,6 +96,7 @@ SIGNATURES = {
     "smgpu_ham_destroy": (I32, [P]),
     "smgpu_ham_create_j2": (I32, [P, I32, P]),
     "smgpu_ham_update_interaction": (I32, [P, P]),
+    "smgpu_ham_create_variant": (I32, [P, P, P]),
     "smgpu_table_bytes": (I32, [P, I32, I32, P, P, ctypes.c_char_p, I32]),
     "smgpu_work_terms": (I32, [P, P]),
     "smgpu_apply_H": (I32, [P, I32, P, P, P]),
@@ -104,12 +105,16 @@ SIGNATURES = {
     "smgpu_apply_H_multi": (I32, [P, I32, I32, P, I64, P, I64, P]),
     "smgpu_apply_H_host_multi": (I32, [P, I32, I32, P, P]),
     "smgpu_time_apply_H_multi": (I32, [P, I32, I32, I32, P]),
+    "smgpu_apply_H_variants": (I32, [P, I32, I32, P, P, P]),
+    "smgpu_apply_H_host_variants": (I32, [P, I32, I32, P, P]),
+    "smgpu_time_apply_H_variants": (I32, [P, I32, I32, I32, P]),
     "smgpu_default_params": (None, [P]),
     "smgpu_trl_solve": (I32, [P, P, P, P, P]),
     "smgpu_trl_solve_v0": (I32, [P, P, P, P, P, P]),
     "smgpu_trl_solve_j": (I32, [P, P, I32, P, P, P, P, P]),
     "smgpu_trbl_solve": (I32, [P, P, I32, I32, P, P, P, P, P]),
     "smgpu_trl_solve_batch": (I32, [P, I32, P, P, P, P, P, P]),
+    "smgpu_trl_solve_variants": (I32, [P, I32, P, P, P, P, P, P]),
     "smgpu_lanczos_steps": (I32, [P, I32, I32, P, P, P]),
     "smgpu_malloc": (I32, [P, I64, P]),
     "smgpu_free": (I32, [P, P]),

@@ -200,31 +200,21 @@ class Hamiltonian:
                   want_vectors=False, max_iter=0, reorth="default"):
         """len(seeds) TRL solves in lockstep with shared H·v launches
         (smgpu_trl_solve_batch); returns [(evals, evecs or None, stats)] per seed."""
-        nb = len(seeds)
-        params = (SolveParams * nb)()
-        nev = min(int(num_ev), self.mdim)
-        evs = [np.zeros(nev, np.float64) for _ in range(nb)]
-        vecs = [np.zeros((nev, self.mdim), np.float64) for _ in range(nb)] if want_vectors else None
-        for b, sd in enumerate(seeds):
-            p = params[b]
-            lib().smgpu_default_params(ctypes.byref(p))
-            p.num_ev = int(num_ev); p.lm = int(lm); p.ls = int(ls); p.tol = float(tol)
-            p.num_history = int(num_history); p.precision = _prec(precision); p.refine = int(bool(refine))
-            p.seed = int(sd); p.max_iter = int(max_iter)
-            p.reorth = {"default": 0, "full": 1, "partial": 2}[reorth]
-        ev_ptrs = (ctypes.c_void_p * nb)(*[ptr(e) for e in evs])
-        vec_ptrs = (ctypes.c_void_p * nb)(*[ptr(v) for v in vecs]) if vecs is not None else None
-        stats = (SolveStats * nb)()
-        status = (ctypes.c_int32 * nb)()
-        check(lib().smgpu_trl_solve_batch(self._h, nb, params, None, ev_ptrs, vec_ptrs, stats, status))
-        out = []
-        for b in range(nb):
-            st = stats[b]
-            sd = dict(iterations=st.iterations, n_hv=st.n_hv, n_hv64=st.n_hv64, n_restart=st.n_restart,
-                      wall_ms=st.wall_ms, ritz=np.array(st.ritz[:nev]), n_reorth2=st.n_reorth2,
-                      n_jproj=st.n_jproj, n_reorth=st.n_reorth)
-            out.append((evs[b], vecs[b] if vecs is not None else None, sd))
-        return out
+        return _solve_batch([self] * len(seeds), False, num_ev, seeds, lm, ls, tol, num_history, precision, refine,
+                            want_vectors, max_iter, reorth)
+
+    def variant(self, problem: Problem):
+        """A Hamiltonian for `problem`'s interaction on this handle's basis that shares
+        this handle's device index tables (smgpu_ham_create_variant): only the
+        coefficients are uploaded, as for prepEC's samples on one basis
+        (eigenvector_continuation.jl:140-175)."""
+        h = ctypes.c_void_p()
+        check(lib().smgpu_ham_create_variant(self._h, problem._h, ctypes.byref(h)))
+        obj = Hamiltonian.__new__(Hamiltonian)
+        obj._h = h
+        obj.problem = problem
+        obj.mdim = self.mdim
+        return obj
 
     def trbl(self, num_ev, q, lm=300, ls=20, tol=1e-8, num_history=3, precision="fp64", refine=True, seed=1234,
              want_vectors=False, max_iter=0, v0=None, j2=None, twoJ=None):
@@ -258,6 +248,76 @@ class Hamiltonian:
         if h:
             lib().smgpu_ham_destroy(h)
             self._h = None
+
+
+def _solve_batch(hams, variants, num_ev, seeds, lm, ls, tol, num_history, precision, refine, want_vectors,
+                 max_iter, reorth):
+    nb = len(seeds)
+    mdim = hams[0].mdim
+    params = (SolveParams * nb)()
+    nev = min(int(num_ev), mdim)
+    evs = [np.zeros(nev, np.float64) for _ in range(nb)]
+    vecs = [np.zeros((nev, mdim), np.float64) for _ in range(nb)] if want_vectors else None
+    for b, sd in enumerate(seeds):
+        p = params[b]
+        lib().smgpu_default_params(ctypes.byref(p))
+        p.num_ev = int(num_ev); p.lm = int(lm); p.ls = int(ls); p.tol = float(tol)
+        p.num_history = int(num_history); p.precision = _prec(precision); p.refine = int(bool(refine))
+        p.seed = int(sd); p.max_iter = int(max_iter)
+        p.reorth = {"default": 0, "full": 1, "partial": 2}[reorth]
+    ev_ptrs = (ctypes.c_void_p * nb)(*[ptr(e) for e in evs])
+    vec_ptrs = (ctypes.c_void_p * nb)(*[ptr(v) for v in vecs]) if vecs is not None else None
+    stats = (SolveStats * nb)()
+    status = (ctypes.c_int32 * nb)()
+    if variants:
+        hs = (ctypes.c_void_p * nb)(*[h._h.value for h in hams])
+        check(lib().smgpu_trl_solve_variants(hs, nb, params, None, ev_ptrs, vec_ptrs, stats, status))
+    else:
+        check(lib().smgpu_trl_solve_batch(hams[0]._h, nb, params, None, ev_ptrs, vec_ptrs, stats, status))
+    out = []
+    for b in range(nb):
+        st = stats[b]
+        sd = dict(iterations=st.iterations, n_hv=st.n_hv, n_hv64=st.n_hv64, n_restart=st.n_restart,
+                  wall_ms=st.wall_ms, ritz=np.array(st.ritz[:nev]), n_reorth2=st.n_reorth2,
+                  n_jproj=st.n_jproj, n_reorth=st.n_reorth)
+        out.append((evs[b], vecs[b] if vecs is not None else None, sd))
+    return out
+
+
+def _handles(hams):
+    assert len(hams) >= 1 and all(h.mdim == hams[0].mdim for h in hams)
+    return (ctypes.c_void_p * len(hams))(*[h._h.value for h in hams])
+
+
+def operate_H_variants(hams, V, precision="fp64"):
+    """W[b] = H_b V[b] for Hamiltonians on one index (Hamiltonian.variant; a handle may
+    repeat), the vectors of different interactions sharing index loads in one launch
+    (smgpu_apply_H_host_variants); each row bit-identical to hams[b].operate_H(V[b])."""
+    V = np.ascontiguousarray(np.atleast_2d(V), dtype=np.float64)
+    assert V.shape == (len(hams), hams[0].mdim)
+    W = np.zeros_like(V)
+    check(lib().smgpu_apply_H_host_variants(_handles(hams), _prec(precision), len(hams), ptr(V), ptr(W)))
+    return W
+
+
+def time_operate_H_variants(hams, precision="mixed", reps=20):
+    """Milliseconds per smgpu_apply_H_variants call on len(hams) resident vectors."""
+    ms = ctypes.c_double()
+    check(lib().smgpu_time_apply_H_variants(_handles(hams), _prec(precision), len(hams), int(reps),
+                                            ctypes.byref(ms)))
+    return ms.value
+
+
+def trl_variants(hams, num_ev, seeds, lm=300, ls=20, tol=1e-8, num_history=3, precision="fp64", refine=True,
+                 want_vectors=False, max_iter=0, reorth="default"):
+    """Lockstep TRL solves of interaction variants on one basis: member b solves
+    hams[b] from seed seeds[b] (smgpu_trl_solve_variants; prepEC's samples,
+    eigenvector_continuation.jl:140-175, solved together).  Returns
+    [(evals, evecs or None, stats)] per member, each bit-identical to hams[b].trl."""
+    assert len(hams) == len(seeds)
+    _handles(hams)
+    return _solve_batch(list(hams), True, num_ev, seeds, lm, ls, tol, num_history, precision, refine,
+                        want_vectors, max_iter, reorth)
 
 
 def j_from_jj1(jj, tol=1e-6):
