@@ -257,10 +257,12 @@ typedef struct {
     int32_t host_threads;   /* requests prepared concurrently; <= 0: 2 */
     int32_t table_threads;  /* threads per table build; <= 0: cores / host_threads */
     int64_t keep_bytes;     /* an idle slot frees its buffers above this; <= 0: 16 GiB */
-    int32_t max_batch;      /* requests with identical Hamiltonians (same basis and coefficients,
-                               same precision) one slot solves in lockstep with shared H·v
-                               launches; <= 0: 1 (no batching) */
-    int32_t reserved_;
+    int32_t max_batch;      /* requests on one basis and precision that one slot solves in
+                               lockstep with shared H·v launches; <= 0: 1 (no batching) */
+    int32_t batch_variants; /* nonzero: a batch also takes requests with other interactions on
+                               its basis (each member with its own coefficients, H·v launches
+                               sharing the index loads); 0: identical Hamiltonians only.
+                               smgpu_default_server_config sets 1 */
 } smgpu_server_config;
 
 typedef struct {

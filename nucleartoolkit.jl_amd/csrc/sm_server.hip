@@ -79,6 +79,7 @@ struct Request {
     uint64_t coef_hash = 0;   // content_hash(*coef): candidates for a shared batch
     int64_t mdim = 0, ws_need = 0, coef_need = 0;
     bool uploader = false, index_shared = false;
+    bool own_coef = false;    // a batch member whose coefficients differ from the run's
     int bypassed = 0;
     int slot = -1;
     int64_t reserved = 0;     // HBM charged at admission
@@ -95,6 +96,9 @@ struct Slot {
     hipStream_t hv_stream = nullptr;
     std::vector<std::unique_ptr<SolveCtx>> more;
     DeviceHam ham;
+    // members 1.. whose interaction differs from the run's: their own coefficients on
+    // the run's index (grow-only buffers like ham's; member b uses mham[b - 1])
+    std::vector<std::unique_ptr<DeviceHam>> mham;
     std::shared_ptr<Request> job;                  // first (or only) request
     // request per member slot of the run (null: free), sized at admission; changed
     // only under the server mutex (other threads count a slot's requests)
@@ -111,6 +115,19 @@ struct Slot {
     void release_workspaces() {
         release_workspace(ctx);
         for (auto& x : more) release_workspace(*x);
+    }
+    int64_t coefficient_bytes() const {
+        int64_t c = ham.coefficient_capacity();
+        for (auto& h : mham) c += h->coefficient_capacity();
+        return c;
+    }
+    void release_coefficients() {
+        ham.release_coefficients();
+        for (auto& h : mham) h->release_coefficients();
+    }
+    void drop_index() {
+        ham.idx.reset();
+        for (auto& h : mham) h->idx.reset();
     }
     bool release = false;     // free the buffers before waiting again
     std::condition_variable cv;
@@ -284,8 +301,8 @@ struct Server {
         }
         for (auto& s : slots)
             if (!s->job && s->cap > 0) {
-                s->ham.idx.reset();
-                s->ham.release_coefficients();
+                s->drop_index();
+                s->release_coefficients();
                 s->release_workspaces();
                 s->cap = 0;
                 return true;
@@ -304,16 +321,20 @@ struct Server {
             finish_locked(*r, SMGPU_REQ_FAILED, SMGPU_ENOMEM, os.str());
             return true;
         }
-        // requests that can share this one's H·v launches: same index tables, identical
-        // coefficients and precision, in ready-list order
+        // requests that can share this one's H·v launches: same index tables and
+        // precision, identical coefficients (or any interaction on the basis with
+        // batch_variants; those bring their own coefficients), in ready-list order
         std::vector<std::shared_ptr<Request>> mates;
+        std::vector<char> own;    // mate i needs its own coefficient buffers
         if (cfg.max_batch > 1)
             for (auto& q : ready) {
                 if ((int)mates.size() + 1 >= cfg.max_batch) break;
-                if (q == r || q->cancel || q->entry != r->entry || q->coef_hash != r->coef_hash) continue;
+                if (q == r || q->cancel || q->entry != r->entry) continue;
                 if ((q->params.precision == 0) != (r->params.precision == 0)) continue;
-                if (!same_coefficients(*q->coef, *r->coef)) continue;
+                const bool same = q->coef_hash == r->coef_hash && same_coefficients(*q->coef, *r->coef);
+                if (!same && !cfg.batch_variants) continue;
                 mates.push_back(q);
+                own.push_back(same ? 0 : 1);
             }
         for (;;) {
             Slot* best = nullptr;
@@ -325,7 +346,8 @@ struct Server {
                 int64_t g = coefficient_growth(s->ham, *r->coef) + workspace_growth(s->ctx, r->mdim, r->params);
                 int n = 1;
                 for (size_t i = 0; i < mates.size(); i++) {
-                    const int64_t gi = workspace_growth(s->member((int)i + 1), mates[i]->mdim, mates[i]->params);
+                    int64_t gi = workspace_growth(s->member((int)i + 1), mates[i]->mdim, mates[i]->params);
+                    if (own[i]) gi += coefficient_growth(*s->mham[i + 1], *mates[i]->coef);
                     if (in_use() + need_index + g + gi > cfg.hbm_budget) break;
                     g += gi;
                     n++;
@@ -335,7 +357,11 @@ struct Server {
             if (!best) return false;
             if (in_use() + need_index + best_g <= cfg.hbm_budget) {
                 std::vector<std::shared_ptr<Request>> run{r};
-                for (int i = 0; i + 1 < best_n; i++) run.push_back(mates[i]);
+                r->own_coef = false;
+                for (int i = 0; i + 1 < best_n; i++) {
+                    run.push_back(mates[i]);
+                    mates[i]->own_coef = own[i] != 0;
+                }
                 for (auto& q : run) {
                     ready.remove(q);
                     q->state = SMGPU_REQ_RUNNING;
@@ -435,12 +461,19 @@ struct Server {
             r.evals.resize((size_t)std::min<int64_t>(r.params.num_ev, r.mdim));
             return;
         }
-        // lockstep batch with member slots; later requests with the same Hamiltonian
-        // join free member slots between rounds; each result is published as soon as
-        // its solve finishes
+        // lockstep batch with member slots; later requests on the same basis (the same
+        // Hamiltonian, or any interaction with batch_variants) join free member slots
+        // between rounds; each result is published as soon as its solve finishes.
+        // Members with the run's coefficients use s.ham, the others their slot's mham.
         s.bound = r.coef;
         const uint64_t hash = r.coef_hash;
         const bool fp64 = r.params.precision == 0;
+        std::vector<DeviceHam*> mh(cap, &s.ham);
+        for (int b = 1; b < nb; b++)
+            if (s.batch[b]->own_coef) {
+                bind_ham(*s.mham[b], idx, *s.batch[b]->coef);
+                mh[b] = s.mham[b].get();
+            }
         std::vector<SolveParams> p(cap);
         std::vector<double*> ev(cap, nullptr);
         std::vector<SolveStats> sst(cap);
@@ -459,17 +492,21 @@ struct Server {
         for (int b = 0; b < cap; b++) cx[b] = &s.member(b);
         for (int b = 0; b < nb; b++) fill(b);
         auto join = [&](int b) -> bool {
+            std::shared_ptr<Request> keep;
+            {
             std::lock_guard<std::mutex> lk(m);
             if (stopping) return false;
             IndexEntry& e = *r.entry;
             for (auto& q : ready) {
-                if (q->cancel || q->entry != r.entry || q->coef_hash != hash || (q->params.precision == 0) != fp64)
-                    continue;
-                if (!same_coefficients(*q->coef, *s.bound)) continue;
-                const int64_t g = workspace_growth(s.member(b), q->mdim, q->params);
+                if (q->cancel || q->entry != r.entry || (q->params.precision == 0) != fp64) continue;
+                const bool same = q->coef_hash == hash && same_coefficients(*q->coef, *s.bound);
+                if (!same && !cfg.batch_variants) continue;
+                int64_t g = workspace_growth(s.member(b), q->mdim, q->params);
+                if (!same) g += coefficient_growth(*s.mham[b], *q->coef);
                 if (in_use() + g > cfg.hbm_budget && !(reclaim_locked(&e) && in_use() + g <= cfg.hbm_budget))
                     return false;
-                auto keep = q;          // ready.remove drops the list's reference
+                keep = q;               // ready.remove drops the list's reference
+                keep->own_coef = !same;
                 ready.remove(keep);
                 keep->state = SMGPU_REQ_RUNNING;
                 keep->slot = s.id;
@@ -480,10 +517,8 @@ struct Server {
                 e.last_use = ++use_clock;
                 s.cap += g;
                 hbm_peak = std::max(hbm_peak, in_use());
-                keep->evals.assign(std::max(1, keep->params.num_ev), 0.0);
                 s.batch[b] = keep;
                 s.member(b).cancel = &keep->cancel;
-                fill(b);
                 st.batched++;
                 int live = 0;
                 for (auto& x : s.batch) live += (x && !finished(*x)) ? 1 : 0;
@@ -491,11 +526,33 @@ struct Server {
                 int running = 0;
                 for (auto& sl : slots) running += running_in(*sl);
                 st.peak_running = std::max(st.peak_running, running);
-                return true;
+                break;
             }
-            return false;
+            if (!keep) return false;
+            }
+            // the upload of a variant's coefficients runs outside the server mutex
+            mh[b] = &s.ham;
+            if (keep->own_coef) {
+                try {
+                    bind_ham(*s.mham[b], idx, *keep->coef);
+                } catch (...) {
+                    std::string msg;
+                    const int32_t code = status_of_current_exception(msg);
+                    std::lock_guard<std::mutex> lk(m);
+                    keep->entry->users--;
+                    s.member(b).cancel = nullptr;
+                    s.batch[b].reset();
+                    finish_locked(*keep, SMGPU_REQ_FAILED, code, msg);
+                    return false;
+                }
+                mh[b] = s.mham[b].get();
+            }
+            keep->evals.assign(std::max(1, keep->params.num_ev), 0.0);
+            fill(b);
+            return true;
         };
         BatchIO io;
+        io.hams = mh.data();
         io.p = p.data();
         io.evals = ev.data();
         io.st = sst.data();
@@ -543,7 +600,7 @@ struct Server {
                 code = status_of_current_exception(msg);
             }
             std::lock_guard<std::mutex> lk(m);
-            s.ham.idx.reset();
+            s.drop_index();
             // requests the batch driver has not finished (one request alone, or a run
             // that failed before the driver started)
             for (size_t b = 1; b < s.batch.size(); b++) {
@@ -560,9 +617,9 @@ struct Server {
                 r->entry->last_use = ++use_clock;
             }
             // charge what the slot really holds now
-            s.cap = s.workspace_bytes() + s.ham.coefficient_capacity();
+            s.cap = s.workspace_bytes() + s.coefficient_bytes();
             if (s.cap > cfg.keep_bytes) {
-                s.ham.release_coefficients();
+                s.release_coefficients();
                 s.release_workspaces();
                 s.cap = 0;
             }
@@ -598,6 +655,7 @@ struct Server {
             SM_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
             s->ctx.stream = s->stream;
             if (cfg.max_batch > 1) {
+                for (int b = 0; b < cfg.max_batch; b++) s->mham.push_back(std::make_unique<DeviceHam>());
                 SM_HIP(hipStreamCreateWithFlags(&s->hv_stream, hipStreamNonBlocking));
                 for (int b = 1; b < cfg.max_batch; b++) {
                     auto c = std::make_unique<SolveCtx>();
@@ -631,7 +689,7 @@ struct Server {
         prep_q.clear();
         ready.clear();
         for (auto& s : slots) {
-            s->ham.idx.reset();
+            s->drop_index();
             s->release_workspaces();
             if (s->stream) (void)hipStreamDestroy(s->stream);
             if (s->hv_stream) (void)hipStreamDestroy(s->hv_stream);
@@ -741,6 +799,7 @@ void smgpu_default_server_config(smgpu_server_config* cfg) {
     cfg->table_threads = 0;
     cfg->keep_bytes = 0;
     cfg->max_batch = 1;
+    cfg->batch_variants = 1;
 }
 
 int32_t smgpu_server_create(const smgpu_server_config* cfg, smgpu_server* out) {

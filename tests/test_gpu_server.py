@@ -189,3 +189,36 @@ def test_server_batches_identical_hamiltonians(snt_dir):
         for i in (0, 2):
             assert out[i][1]["state"] == "done"
             np.testing.assert_allclose(out[i][0], _alone(a, nuc_a, 5, 400 + i, "mixed"), atol=TOL_MIX)
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_server_batches_interaction_variants(snt_dir, precision):
+    """batch_variants: ready requests on one basis and precision with different
+    interactions share a slot's lockstep batch, each member with its own
+    coefficients on the shared index (admitted together and joining between
+    rounds).  Every result equals the request solved alone on its own handle;
+    with batch_variants off no two of them share a batch."""
+    paths, nuc = synth.variant_snts(snt_dir, 6, space="sd", zv=4, nv=4)
+    reqs = [(paths[i % 6], 4 + i % 3, 500 + i) for i in range(10)]
+    alone = [_alone(p, nuc, k, s, precision) for p, k, s in reqs]
+    with Server(max_inflight=1, max_batch=4) as srv:
+        ids = [srv.submit(p, nuc, k, precision=precision, seed=s) for p, k, s in reqs]
+        for rid in ids:
+            assert srv.wait(rid, 120000)
+        res = [srv.result(rid) for rid in ids]
+        st = srv.stats()
+    for (ev, info), ref in zip(res, alone):
+        assert info["state"] == "done"
+        np.testing.assert_allclose(ev, ref, atol=TOL_MIX)
+    assert st["batched"] >= 6 and st["peak_batch"] >= 3, st
+    assert st["index_uploads"] == 1, st
+    distinct = reqs[:6]
+    with Server(max_inflight=1, max_batch=4, batch_variants=False) as srv:
+        ids = [srv.submit(p, nuc, k, precision=precision, seed=s) for p, k, s in distinct]
+        for rid in ids:
+            assert srv.wait(rid, 120000)
+        res = [srv.result(rid) for rid in ids]
+        st = srv.stats()
+    for (ev, info), ref in zip(res, alone[:6]):
+        np.testing.assert_allclose(ev, ref, atol=TOL_MIX)
+    assert st["batched"] == 0, st
