@@ -296,63 +296,131 @@ __global__ __launch_bounds__(256) void k_hv(HvTables A, HvVecs P) {
                     }
                 }
             __syncthreads();
-            // neutron-hop entries in chunks of PF steps, double-buffered: a chunk is
-            // decoded into registers, the next chunk's loads are issued into the
-            // raw buffer, and the decoded chunk is computed while they are in
-            // flight.  Loads are unconditional (a column past its slice length
-            // re-reads its last entry; every slice has at least one); the LDS reads
-            // and FMAs of a column stop at its slice length (wave-uniform).
-            constexpr int PF = NPT <= 4 ? 4 : (NPT <= 8 ? 2 : 1);
-            uint32_t raw[PF][NPT];
+            if constexpr (NPT <= 4) {
+                // Column by column (k outer): a chunk of PF entries of one column is
+                // decoded, the next chunk's entry loads are issued, then the chunk's LDS
+                // reads are all issued before its FMAs (one LDS wait per chunk).  Per
+                // accumulator the order of entries and slots is that of one entry at a
+                // time.  PF trades LDS reads in flight against VGPRs (occupancy):
+                // measured on Cr48, four entries suit one FP32 vector (0.478 -> 0.447 ms),
+                // two everything else (two vectors 0.331 -> 0.315 ms per vector, FP64
+                // 0.72 -> 0.70 ms).
+                constexpr int PF = (NV == 1 && sizeof(T) == 4) ? 4 : 2;
 #pragma unroll
-            for (int d = 0; d < PF; d++)
+                for (int k = 0; k < NPT; k++) {
+                    const int lk = len[k];
+                    if (lk == 0) continue;                                // wave-uniform
+                    uint32_t raw[PF];
 #pragma unroll
-                for (int k = 0; k < NPT; k++)
-                    raw[d][k] = A.ntab_ent[off[k] + (int64_t)min(d, max(len[k] - 1, 0)) * 64];
-            for (int q0 = 0; q0 < maxlen; q0 += PF) {
-                int in[PF][NPT], col[PF][NPT];
+                    for (int d = 0; d < PF; d++) raw[d] = A.ntab_ent[off[k] + (int64_t)min(d, lk - 1) * 64];
+                    for (int q0 = 0; q0 < lk; q0 += PF) {
+                        int in[PF], col[PF];
+#pragma unroll
+                        for (int d = 0; d < PF; d++) {
+                            in[d] = (int)(raw[d] & 0xFFFFFu);
+                            col[d] = (int)(raw[d] >> 20);
+                        }
+                        if (q0 + PF < lk) {
+#pragma unroll
+                            for (int d = 0; d < PF; d++)
+                                raw[d] = A.ntab_ent[off[k] + (int64_t)min(q0 + PF + d, lk - 1) * 64];
+                        }
+#pragma unroll
+                        for (int r = 0; r < RG; r++)
+#pragma unroll
+                            for (int g = 0; g < NG; g++) {
+                                if (rb[r] + ch * PCH + g * 4 >= rb[r + 1]) continue;   // uniform: group not staged
+                                const int so = r * PCH + g * 4;
+                                T4 vv[PF][NC], xv[PF][NV];
+#pragma unroll
+                                for (int d = 0; d < PF; d++) {
+                                    if (q0 + d >= lk) continue;                  // wave-uniform
+#pragma unroll
+                                    for (int c = 0; c < NC; c++)
+                                        vv[d][c] = *reinterpret_cast<const T4*>(
+                                            Vs + ((size_t)c * A.cols2_max + col[d]) * SL + so);
+#pragma unroll
+                                    for (int b = 0; b < NV; b++)
+                                        xv[d][b] = *reinterpret_cast<const T4*>(X + ((size_t)in[d] * NV + b) * SL + so);
+                                }
+#pragma unroll
+                                for (int d = 0; d < PF; d++) {
+                                    if (q0 + d >= lk) continue;                  // wave-uniform
+#pragma unroll
+                                    for (int b = 0; b < NV; b++) {
+                                        const T4 v4 = vv[d][PV ? b : 0];
+                                        T a = acc[k][r][b];
+                                        a += v4.x * xv[d][b].x;
+                                        a += v4.y * xv[d][b].y;
+                                        a += v4.z * xv[d][b].z;
+                                        a += v4.w * xv[d][b].w;
+                                        acc[k][r][b] = a;
+                                    }
+                                }
+                            }
+                    }
+                }
+            } else {
+                // Wide blocks (NPT > 4), all columns per entry step (measured on Mn50
+                // FP64: 13.2 ms against 14.3 ms column by column).
+                // Neutron-hop entries in chunks of PF steps, double-buffered: a chunk is
+                // decoded into registers, the next chunk's loads are issued into the
+                // raw buffer, and the decoded chunk is computed while they are in
+                // flight.  Loads are unconditional (a column past its slice length
+                // re-reads its last entry; every slice has at least one); the LDS reads
+                // and FMAs of a column stop at its slice length (wave-uniform).
+                constexpr int PF = NPT <= 4 ? 4 : (NPT <= 8 ? 2 : 1);
+                uint32_t raw[PF][NPT];
 #pragma unroll
                 for (int d = 0; d < PF; d++)
 #pragma unroll
-                    for (int k = 0; k < NPT; k++) {
-                        in[d][k] = (int)(raw[d][k] & 0xFFFFFu);
-                        col[d][k] = (int)(raw[d][k] >> 20);
-                    }
-                if (q0 + PF < maxlen) {
+                    for (int k = 0; k < NPT; k++)
+                        raw[d][k] = A.ntab_ent[off[k] + (int64_t)min(d, max(len[k] - 1, 0)) * 64];
+                for (int q0 = 0; q0 < maxlen; q0 += PF) {
+                    int in[PF][NPT], col[PF][NPT];
 #pragma unroll
                     for (int d = 0; d < PF; d++)
 #pragma unroll
-                        for (int k = 0; k < NPT; k++)
-                            raw[d][k] = A.ntab_ent[off[k] + (int64_t)min(q0 + PF + d, max(len[k] - 1, 0)) * 64];
-                }
+                        for (int k = 0; k < NPT; k++) {
+                            in[d][k] = (int)(raw[d][k] & 0xFFFFFu);
+                            col[d][k] = (int)(raw[d][k] >> 20);
+                        }
+                    if (q0 + PF < maxlen) {
 #pragma unroll
-                for (int d = 0; d < PF; d++) {
-                    const int qq = q0 + d;
+                        for (int d = 0; d < PF; d++)
 #pragma unroll
-                    for (int r = 0; r < RG; r++)
+                            for (int k = 0; k < NPT; k++)
+                                raw[d][k] = A.ntab_ent[off[k] + (int64_t)min(q0 + PF + d, max(len[k] - 1, 0)) * 64];
+                    }
 #pragma unroll
-                        for (int g = 0; g < NG; g++) {
-                            if (rb[r] + ch * PCH + g * 4 >= rb[r + 1]) continue;   // uniform: group not staged
-                            const int so = r * PCH + g * 4;
+                    for (int d = 0; d < PF; d++) {
+                        const int qq = q0 + d;
 #pragma unroll
-                            for (int k = 0; k < NPT; k++) {
-                                if (qq >= len[k]) continue;          // wave-uniform
-                                T4 vv = *reinterpret_cast<const T4*>(Vs + (size_t)col[d][k] * SL + so);
+                        for (int r = 0; r < RG; r++)
 #pragma unroll
-                                for (int b = 0; b < NV; b++) {
-                                    if (PV && b > 0)
-                                        vv = *reinterpret_cast<const T4*>(Vs + ((size_t)b * A.cols2_max + col[d][k]) * SL + so);
-                                    const T4 xv =
-                                        *reinterpret_cast<const T4*>(X + ((size_t)in[d][k] * NV + b) * SL + so);
-                                    T a = acc[k][r][b];
-                                    a += vv.x * xv.x;
-                                    a += vv.y * xv.y;
-                                    a += vv.z * xv.z;
-                                    a += vv.w * xv.w;
-                                    acc[k][r][b] = a;
+                            for (int g = 0; g < NG; g++) {
+                                if (rb[r] + ch * PCH + g * 4 >= rb[r + 1]) continue;   // uniform: group not staged
+                                const int so = r * PCH + g * 4;
+#pragma unroll
+                                for (int k = 0; k < NPT; k++) {
+                                    if (qq >= len[k]) continue;          // wave-uniform
+                                    T4 vv = *reinterpret_cast<const T4*>(Vs + (size_t)col[d][k] * SL + so);
+#pragma unroll
+                                    for (int b = 0; b < NV; b++) {
+                                        if (PV && b > 0)
+                                            vv = *reinterpret_cast<const T4*>(Vs + ((size_t)b * A.cols2_max + col[d][k]) * SL + so);
+                                        const T4 xv =
+                                            *reinterpret_cast<const T4*>(X + ((size_t)in[d][k] * NV + b) * SL + so);
+                                        T a = acc[k][r][b];
+                                        a += vv.x * xv.x;
+                                        a += vv.y * xv.y;
+                                        a += vv.z * xv.z;
+                                        a += vv.w * xv.w;
+                                        acc[k][r][b] = a;
+                                    }
                                 }
                             }
-                        }
+                    }
                 }
             }
         }
