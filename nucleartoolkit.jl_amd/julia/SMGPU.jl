@@ -103,6 +103,18 @@ function HamJ2(p::Problem; device::Integer=0)
     return h
 end
 
+# Another interaction on base's basis that shares base's device index tables and
+# uploads only p's coefficients (prepEC's samples on one basis,
+# src/ShellModel/eigenvector_continuation.jl:140-175); p must have base's basis.
+function HamVariant(base::Ham, p::Problem)
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:smgpu_ham_create_variant, libsmgpu), Int32, (Ptr{Cvoid}, Ptr{Cvoid}, Ref{Ptr{Cvoid}}),
+                base.h, p.h, out))
+    h = Ham(out[], base.mdim)
+    finalizer(x -> ccall((:smgpu_ham_destroy, libsmgpu), Int32, (Ptr{Cvoid},), x.h), h)
+    return h
+end
+
 # twf += H wf  (operate_H!'s contract; the device computes H wf in `precision`)
 function operate_H!(wf::Vector{Float64}, twf::Vector{Float64}, H::Ham; precision::Int32=FP64)
     @assert length(wf) == H.mdim && length(twf) == H.mdim
@@ -166,6 +178,31 @@ function trbl(H::Ham, num_ev::Integer, q::Integer; lm=300, ls=20, tol=1e-8, num_
                     evecs === nothing ? C_NULL : pointer(evecs), st))
     end
     return evals, evecs, st[]
+end
+
+# TRL solves of interaction variants on one basis (HamVariant handles; a handle may
+# repeat) in lockstep, the H·v of all members in shared launches each round
+# (smgpu_trl_solve_variants); member b starts from seed seeds[b].  Returns
+# [(evals, stats)] per member, each equal to trl(Hs[b], ...) with that seed.
+function trl_variants(Hs::Vector{Ham}, num_ev::Integer; seeds=[1234 + b for b in 0:length(Hs)-1], lm=300,
+                      ls=20, tol=1e-8, num_history=3, precision::Int32=MIXED, refine::Bool=true)
+    nb = length(Hs)
+    @assert length(seeds) == nb
+    d = default_params()
+    ps = [SolveParams(num_ev, lm, ls, num_history, tol, precision, refine, d.max_iter, UInt64(sd)) for sd in seeds]
+    nev = min(num_ev, Hs[1].mdim)
+    evals = [zeros(Float64, nev) for _ in 1:nb]
+    stats = Vector{SolveStats}(undef, nb)
+    status = zeros(Int32, nb)
+    hs = [H.h for H in Hs]
+    GC.@preserve evals begin
+        evp = [pointer(e) for e in evals]
+        check(ccall((:smgpu_trl_solve_variants, libsmgpu), Int32,
+                    (Ptr{Ptr{Cvoid}}, Int32, Ptr{SolveParams}, Ptr{Ptr{Float64}}, Ptr{Ptr{Float64}},
+                     Ptr{Ptr{Float64}}, Ptr{SolveStats}, Ptr{Int32}),
+                    hs, nb, ps, C_NULL, evp, C_NULL, stats, status))
+    end
+    return [(evals[b], stats[b]) for b in 1:nb]
 end
 
 # main_sm(sntf, target_nuc, num_ev, target_J; ...): en[1].  target_J = [2J] runs the
