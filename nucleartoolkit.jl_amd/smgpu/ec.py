@@ -10,8 +10,11 @@ Mirrors prepEC's "sample" mode and solveEC
   smgpu_ham_update_interaction).  With tJ given, every solve is J-projected
   Lanczos (smgpu_trl_solve_j, as prepEC's TRL with doubleLanczos), so the
   num_ev lowest states of J = tJ/2 are the EC basis; without it, the lowest
-  num_ev states of 2M = A % 2 are.  Optionally each sample is written as a
-  .wav file like prepEC's save_wav.
+  num_ev states of 2M = A % 2 are.  Without tJ the interactions are solved in
+  lockstep groups of `batch` (variant handles on the one index,
+  smgpu_trl_solve_variants: the H.v of a group shares launches), each solve
+  identical to the one-at-a-time solve with the same seed.  Optionally each
+  sample is written as a .wav file like prepEC's save_wav.
 * `solve_ec(Hs, target_nuc, samples, num_ev_target)` builds, for every target
   interaction, H_ij = <psi_i|H|psi_j> and N_ij = <psi_i|psi_j> and solves
   H c = E N c through the Cholesky factor of N, with solveEC's epsilon shift
@@ -27,7 +30,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .shellmodel import Hamiltonian, Problem, expect_j2, j_from_jj1
+from .shellmodel import Hamiltonian, Problem, j_from_jj1, trl_variants
 from .wav import write_ritzvecs
 
 
@@ -42,22 +45,40 @@ class ECSamples:
 
 
 def prep_ec(Hs, target_nuc, num_ev, tJ=None, *, lm=300, ls=20, tol=1e-8, precision="fp64", device=0,
-            seed=1234, save_wav=False, path_to_samplewav=""):
+            seed=1234, save_wav=False, path_to_samplewav="", batch=4):
     """prepEC(Hs, target_nuc, num_ev, num_ECsample, tJ, "sample"): sample vectors
     of every interaction in Hs on the basis of 2M = tJ (default A % 2)."""
     prob = Problem(Hs[0], target_nuc, twoM=-1 if tJ is None else int(tJ))
     twoM = int(prob.info.twoM)
     ham = Hamiltonian(prob, device)
     j2 = prob.j2_hamiltonian(device) if tJ is not None else None
+    nev = min(int(num_ev), prob.mdim)
+    # (energies, Ritz vectors) per interaction, in the order of Hs
+    solved = []
+    if tJ is None and batch > 1:
+        for g in range(0, len(Hs), batch):
+            group = []
+            for k in range(g, min(g + batch, len(Hs))):
+                if k == 0:
+                    group.append(ham)
+                else:
+                    prob.set_interaction(Hs[k])
+                    group.append(ham.variant(prob))
+            out = trl_variants(group, nev, [seed] * len(group), lm=lm, ls=ls, tol=tol, precision=precision,
+                               want_vectors=True)
+            solved.extend((ev, R) for ev, R, _ in out)
+    else:
+        for k, sntf in enumerate(Hs):
+            if k > 0:
+                prob.set_interaction(sntf)
+                ham.update_interaction(prob)
+            ev, R, _ = ham.trl(nev, lm=lm, ls=ls, tol=tol, precision=precision, seed=seed, want_vectors=True,
+                               j2=j2, twoJ=None if tJ is None else int(tJ))
+            solved.append((ev, R))
     vecs, src, energies, Js = [], [], [], []
-    for k, sntf in enumerate(Hs):
-        if k > 0:
-            prob.set_interaction(sntf)
-            ham.update_interaction(prob)
-        nev = min(int(num_ev), prob.mdim)
-        ev, R, _ = ham.trl(nev, lm=lm, ls=ls, tol=tol, precision=precision, seed=seed, want_vectors=True,
-                           j2=j2, twoJ=None if tJ is None else int(tJ))
-        J = np.array([j_from_jj1(x) for x in expect_j2(prob, R, device)])
+    hj = j2 if j2 is not None else prob.j2_hamiltonian(device)     # J^2 depends on the basis only
+    for sntf, (ev, R) in zip(Hs, solved):
+        J = np.array([j_from_jj1(x) for x in np.einsum("ij,ij->i", R, hj.operate_H_multi(R, "fp64"))])
         energies.append(ev)
         Js.append(J)
         for i in range(nev):

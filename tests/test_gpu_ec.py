@@ -53,3 +53,17 @@ def test_ec_singular_norm_matrix(snt_dir):
     samples.vectors = np.vstack([samples.vectors, samples.vectors[:1]])   # exact duplicate
     (_, _, v), = ec.solve_ec([paths[0]], nuc, samples, 1)
     assert abs(v[0] - en[0]) < 1e-6
+
+
+def test_prep_ec_batched_equals_sequential(snt_dir):
+    """Without tJ the samples are solved in lockstep groups of variant handles on one
+    index; every energy and sample vector equals the one-at-a-time solve (same seed)."""
+    paths, nuc = _variants(snt_dir, 5)
+    seq = ec.prep_ec(paths, nuc, 3, batch=1)
+    bat = ec.prep_ec(paths, nuc, 3, batch=3)
+    assert np.array_equal(seq.vectors, bat.vectors)
+    for a, b in zip(seq.energies, bat.energies):
+        assert np.array_equal(a, b)
+    assert seq.source == bat.source
+    (_, _, v), = ec.solve_ec([paths[1]], nuc, bat, 1)
+    assert abs(v[0] - bat.energies[1][0]) < 1e-8
