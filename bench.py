@@ -232,9 +232,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--hv-reps", type=int, default=20)
     ap.add_argument("--hv-multi", action="store_true", help="--mode hv: also time 2- and 4-vector launches")
-    ap.add_argument("--inflight", type=int, default=2)
-    ap.add_argument("--batch", type=int, default=2,
-                    help="requests with identical Hamiltonians solved in lockstep per slot (server max_batch)")
+    ap.add_argument("--inflight", type=int, default=None,
+                    help="server slots (default 2; 4 for --workload stream)")
+    ap.add_argument("--batch", type=int, default=None,
+                    help="requests on one basis solved in lockstep per slot (server max_batch; default 2, "
+                         "1 for --workload stream, whose requests rarely share a basis)")
     ap.add_argument("--host-threads", type=int, default=2)
     ap.add_argument("--mode", default="solve", choices=["solve", "hv"],
                     help="hv: only the H·v launches of the workload (for profiler runs) plus a 256 MiB "
@@ -249,6 +251,13 @@ def main():
     ap.add_argument("--rate", type=float, default=50.0)
     ap.add_argument("--particles", default="2,4", help="valence range per species, e.g. 2,6")
     args = ap.parse_args()
+    # measured best per workload: the stream's mostly distinct bases gain from more slots
+    # (4 slots, no batching: 49.9 solves/s, p50 120 ms; 2 slots with batches of 2: 47.2, 215 ms),
+    # the Cr48 workloads from two slots with batches of two
+    if args.inflight is None:
+        args.inflight = 4 if args.workload == "stream" else 2
+    if args.batch is None:
+        args.batch = 1 if args.workload == "stream" else 2
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); independent solves shard "
               "one per GPU and are measured per GPU", file=sys.stderr)
