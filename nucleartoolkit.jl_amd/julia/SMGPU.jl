@@ -8,6 +8,8 @@
 # Errors come back as Julia exceptions with the library's message.
 module SMGPU
 
+using Printf
+
 const libsmgpu = get(ENV, "SMGPU_LIB", joinpath(@__DIR__, "..", "build", "libsmgpu.so"))
 
 const FP64 = Int32(0)
@@ -207,12 +209,28 @@ end
 
 # main_sm(sntf, target_nuc, num_ev, target_J; ...): en[1].  target_J = [2J] runs the
 # J-projected solve at 2M = 2J (:109-116); is_block=true the block Lanczos with block size q.
+# main_sm's dimension line (mdim_print, src/ShellModel/shellmodel_main.jl:1316-1321)
+function mdim_print(target_nuc, inf::ProblemInfo, tJ=-1)
+    s0 = @sprintf "%6s %14s %10s %10s" target_nuc "Z,N=($(inf.Z),$(inf.N))" "c($(inf.cp),$(inf.cn))" "v($(inf.vp),$(inf.vn))"
+    if tJ != -1; s0 *= @sprintf(" 2J = %3i", tJ); end
+    s1 = @sprintf "%s %12i %s %5.2f %s " "  mdim:" inf.mdim "(" log10(max(inf.mdim, 1)) ")"
+    print(s0, s1)
+end
+
+# mdimmode=true prints the dimension line and returns nothing (:129-130)
 function main_sm(sntf, target_nuc, num_ev, target_J; num_history=3, lm=300, ls=20, tol=1e-8,
-                 precision::Int32=MIXED, device::Integer=0, q::Integer=1, is_block::Bool=false)
+                 precision::Int32=MIXED, device::Integer=0, q::Integer=1, is_block::Bool=false,
+                 mdimmode::Bool=false)
     length(target_J) <= 1 || error("Multiple J is not supported")
     tJ = isempty(target_J) ? -1 : Int(target_J[1])
     prob = Problem(sntf, target_nuc; twoM=tJ)
-    mdim = info(prob).mdim
+    inf = info(prob)
+    if mdimmode
+        mdim_print(target_nuc, inf, tJ)
+        println()
+        return nothing
+    end
+    mdim = inf.mdim
     mdim == 0 && return Float64[]
     H = Ham(prob; device=device)
     j2 = tJ < 0 ? nothing : HamJ2(prob; device=device)

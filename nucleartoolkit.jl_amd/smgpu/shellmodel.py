@@ -12,6 +12,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import time
 
 import numpy as np
 
@@ -338,9 +339,30 @@ def expect_j2(problem: Problem, vecs, device=0):
     return np.einsum("ij,ij->i", V, W)
 
 
+def mdim_print(target_nuc, info, tJ=-1):
+    """The dimension line main_sm prints (mdim_print, shellmodel_main.jl:1316-1321)."""
+    s0 = "%6s %14s %10s %10s" % (target_nuc, f"Z,N=({info.Z},{info.N})", f"c({info.cp},{info.cn})",
+                                 f"v({info.vp},{info.vn})")
+    if tJ != -1:
+        s0 += " 2J = %3i" % tJ
+    s1 = "%s %12i %s %5.2f %s " % ("  mdim:", info.mdim, "(", np.log10(max(info.mdim, 1)), ")")
+    print(s0 + s1, end="")          # the J / En lines follow on the same line, as in main_sm
+
+
+def _print_levels(ev, totJs, tJ):
+    """main_sm's result lines (shellmodel_main.jl:256-266)."""
+    if tJ is None:
+        print(f"J {list(totJs)}")
+        print("En. " + "".join("%10.4f " % x for x in ev))
+        print("Ex. " + "".join("%10.4f " % (x - ev[0]) for x in ev))
+    else:
+        print(f"2J= {int(round(2 * totJs[0]))}  En." + "".join("%12.5f " % x for x in ev))
+
+
 def main_sm(sntf, target_nuc, num_ev, target_J=(), *, lm=300, ls=20, tol=1e-8, num_history=3,
             precision="fp64", refine=True, device=0, seed=1234, return_stats=False, return_J=False,
-            save_wav=False, wav_path=None, in_wf="", q=1, is_block=False):
+            save_wav=False, wav_path=None, in_wf="", q=1, is_block=False, mdimmode=False, verbose=False,
+            is_show=False):
     """Lowest `num_ev` eigenvalues (MeV) of `target_nuc` with the interaction `sntf`.
 
     Mirrors main_sm (shellmodel_main.jl:89-281): with target_J = [] the plain
@@ -352,17 +374,29 @@ def main_sm(sntf, target_nuc, num_ev, target_J=(), *, lm=300, ls=20, tol=1e-8, n
     ./<nucleus>_<interaction>.wav (:241-246; ..._j<tJ>.wav with target_J).  in_wf
     starts the Lanczos from the first vector of a .wav file (read_appwav, :194-198).
     is_block=True runs the thick-restart block Lanczos with block size q instead
-    (TRBL, :167-191; in_wf then supplies the first q vectors).
+    (TRBL, :167-191; in_wf then supplies the first q vectors).  mdimmode=True
+    prints the dimension line and returns None after the basis (:129-130);
+    verbose=True prints main_sm's dimension and J / En / Ex lines; is_show=True
+    prints the wall time of the table build, upload and solve (main_sm's
+    TimerOutput summary, :279).
     """
     if len(target_J) > 1:
         raise ValueError("Multiple J is not supported (main_sm asserts length(target_J) <= 1)")
     tJ = int(target_J[0]) if len(target_J) else None
+    t0 = time.perf_counter()
     prob = Problem(sntf, target_nuc, twoM=-1 if tJ is None else tJ)
+    t1 = time.perf_counter()
+    if verbose or mdimmode:
+        mdim_print(target_nuc, prob.info, -1 if tJ is None else tJ)
+    if mdimmode:
+        print()
+        return None
     if prob.mdim == 0:
         return (np.zeros(0), {}) if return_stats else np.zeros(0)
     ham = Hamiltonian(prob, device)
+    t2 = time.perf_counter()
     nev = min(int(num_ev), prob.mdim)
-    need_vecs = return_J or save_wav
+    need_vecs = return_J or save_wav or verbose
     v0 = None
     if in_wf:
         from .wav import read_appwav
@@ -374,9 +408,12 @@ def main_sm(sntf, target_nuc, num_ev, target_J=(), *, lm=300, ls=20, tol=1e-8, n
     else:
         ev, vecs, st = ham.trl(nev, lm=lm, ls=ls, tol=tol, num_history=num_history, precision=precision,
                                refine=refine, seed=seed, want_vectors=need_vecs, v0=v0, j2=j2, twoJ=tJ)
+    t3 = time.perf_counter()
     out = [ev]
     if need_vecs:
         totJs = np.array([j_from_jj1(x) for x in expect_j2(prob, vecs, device)])
+        if verbose:
+            _print_levels(ev, totJs, tJ)
         if save_wav:
             from .wav import write_ritzvecs
             if wav_path is None:
@@ -385,6 +422,9 @@ def main_sm(sntf, target_nuc, num_ev, target_J=(), *, lm=300, ls=20, tol=1e-8, n
             write_ritzvecs(wav_path, prob.info.twoM, ev, totJs, vecs)
         if return_J:
             out.append(totJs)
+    if is_show:
+        print(f"tables {1e3 * (t1 - t0):.1f} ms, upload {1e3 * (t2 - t1):.1f} ms, solve {1e3 * (t3 - t2):.1f} ms "
+              f"({st['iterations']} iterations, {st['n_hv']} H.v, {st['n_restart']} restarts)")
     if return_stats:
         out.append(st)
     return out[0] if len(out) == 1 else tuple(out)

@@ -203,3 +203,13 @@ def test_restart_small_problem_host():
         coup = b[m - 1] * St[m - 1, :]
         assert np.abs(coup[:-1]).max() < 1e-10 and abs(abs(coup[-1]) - bo[-1]) < 1e-10
         assert np.abs(np.linalg.eigvalsh(Tn) - np.linalg.eigvalsh(T)[:ls]).max() < 1e-10
+
+
+def test_main_sm_mdimmode(golden, capsys):
+    """mdimmode: main_sm prints its dimension line and stops after the basis (host
+    only).  usdb Mg24 gives the line of the reference's example log
+    (example/log_sample_script.txt:323, up to the J list that follows it there)."""
+    out = smgpu.main_sm(f"{golden}/usdb.snt", "Mg24", 10, mdimmode=True)
+    assert out is None
+    line = capsys.readouterr().out
+    assert line.startswith("  Mg24    Z,N=(12,12)     c(8,8)     v(4,4)  mdim:        28503 (  4.45 ) "), repr(line)
