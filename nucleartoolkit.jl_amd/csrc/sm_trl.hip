@@ -49,6 +49,9 @@
 namespace smgpu {
 
 static constexpr int TB = 256;
+// Largest vector (amplitudes) whose mixed-precision Lanczos step (partial
+// re-orthogonalisation) runs as the one-workgroup k_step_small
+static constexpr int64_t SMALL_STEP_N = 8192;
 static constexpr int EPL = 8;                 // elements per lane in the streaming passes
 static constexpr int CHUNK = 64 * EPL;        // x per block in k_dotsw
 static constexpr int UCHUNK = TB * 4;         // x per block in k_updatew
@@ -435,9 +438,10 @@ __global__ __launch_bounds__(TB) void k_local_step(const TW* __restrict__ win, c
 // gate[1] = 1 runs the projection pass of this step: when max_k |om_{j+1,k}| >
 // sqrt(eps) (and then also for the next step, gate[2] counts forced steps), after
 // which om_{j+1,k} = eps.  loc[5] counts the steps that ran it.
-__global__ void k_omega(const double* __restrict__ alpha, const double* __restrict__ beta, int j,
-                        const double* __restrict__ npart, int G, double* __restrict__ om, int ldo,
-                        double* __restrict__ loc, int* __restrict__ gate, double eps) {
+// (one workgroup of TB threads; returns the gate of this step's pass in every thread)
+__device__ int omega_block(const double* __restrict__ alpha, const double* __restrict__ beta, int j,
+                           const double* __restrict__ npart, int G, double* __restrict__ om, int ldo,
+                           double* __restrict__ loc, int* __restrict__ gate, double eps) {
     __shared__ double red[TB / 64];
     __shared__ double mx[TB / 64];
     const int forced = gate[2];        // read before the barriers below; thread 0 rewrites it last
@@ -479,6 +483,106 @@ __global__ void k_omega(const double* __restrict__ alpha, const double* __restri
         gate[2] = forced > 0 ? forced - 1 : (run ? 1 : 0);
         loc[5] += run;
     }
+    return run;
+}
+
+__global__ void k_omega(const double* __restrict__ alpha, const double* __restrict__ beta, int j,
+                        const double* __restrict__ npart, int G, double* __restrict__ om, int ldo,
+                        double* __restrict__ loc, int* __restrict__ gate, double eps) {
+    omega_block(alpha, beta, j, npart, G, om, ldo, loc, gate, eps);
+}
+
+// One projection pass of w (FP64, in place) against the basis B[0..nv) by one
+// workgroup: h[j] = B_j . w (wave j mod 4, lanes along the vector), then
+// w -= sum_j h[j] B_j; returns |w|^2 after the pass (every thread).
+template <typename TBv>
+__device__ double small_pass(const TBv* __restrict__ B, int64_t ldb, int nv, double* __restrict__ w, int64_t n,
+                             double* __restrict__ h, double* hs, double* red) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int j = wv; j < nv; j += TB / 64) {
+        const TBv* b = B + (int64_t)j * ldb;
+        double d = 0.0;
+        #pragma unroll 4
+        for (int64_t x = lane; x < n; x += 64) d += (double)b[x] * w[x];
+        d = wave_sum(d);
+        if (lane == 0) hs[j] = d;
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < nv; j += TB) h[j] = hs[j];
+    double nr = 0.0;
+    #pragma unroll 4
+    for (int64_t x = threadIdx.x; x < n; x += TB) {
+        double t = w[x];
+        for (int j = 0; j < nv; j++) t -= hs[j] * (double)B[(int64_t)j * ldb + x];
+        w[x] = t;
+        nr += t * t;
+    }
+    nr = block_sum(nr, red);
+    __shared__ double nsh;
+    if (threadIdx.x == 0) nsh = nr;
+    __syncthreads();
+    return nsh;
+}
+
+// The whole Lanczos step after H·v for a small vector (n <= SMALL_STEP_N), in one
+// workgroup, with partial re-orthogonalisation: alpha = v_it . Hv, the local step
+// w = Hv - alpha v_it - beta_{it-1} v_{it-1} (FP64), the omega recurrence and its
+// gated projection pass with the DGKS second pass, alpha[it] / beta[it], and
+// vnext = w / beta[it] (when set).  The same operations as the multi-kernel step
+// (k_dot2 .. k_store), with one launch instead of about a dozen dependent ones.
+template <typename TBv, typename TW>
+__global__ __launch_bounds__(TB) void k_step_small(const TW* __restrict__ win, const TBv* __restrict__ B,
+                                                   int64_t ldb, int it, int64_t n, double* __restrict__ w,
+                                                   double* __restrict__ loc, int* __restrict__ gate,
+                                                   double* __restrict__ alpha, double* __restrict__ beta,
+                                                   double* __restrict__ om, int ldo, double eps,
+                                                   double* __restrict__ h1, double* __restrict__ h2,
+                                                   TBv* __restrict__ vnext) {
+    __shared__ double red[TB / 64];
+    __shared__ double hs[1024];
+    __shared__ double sh[2];
+    const TBv* vc = B + (int64_t)it * ldb;
+    const TBv* vp = it > 0 ? B + (int64_t)(it - 1) * ldb : nullptr;
+    double s = 0.0;
+    #pragma unroll 4
+    for (int64_t x = threadIdx.x; x < n; x += TB) s += (double)vc[x] * (double)win[x];
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) sh[0] = s;
+    __syncthreads();
+    const double al = sh[0];
+    const double be = vp ? beta[it - 1] : 0.0;
+    double nr = 0.0;
+    #pragma unroll 4
+    for (int64_t x = threadIdx.x; x < n; x += TB) {
+        double t = (double)win[x] - al * (double)vc[x];
+        if (vp) t -= be * (double)vp[x];
+        w[x] = t;
+        nr += t * t;
+    }
+    nr = block_sum(nr, red);
+    if (threadIdx.x == 0) { sh[1] = nr; loc[0] = al; }
+    __syncthreads();
+    const double nb = sh[1];                 // |w'|^2
+    const int run = omega_block(alpha, beta, it, sh + 1, 1, om, ldo, loc, gate, eps);
+    double nf = nb;
+    int second = 0;
+    if (run) {
+        const double na = small_pass<TBv>(B, ldb, it + 1, w, n, h1, hs, red);
+        second = na < 0.5 * nb ? 1 : 0;      // DGKS: the pass removed more than half of |w'|^2
+        if (threadIdx.x == 0) { loc[1] = nb; loc[2] = na; }
+        nf = second ? small_pass<TBv>(B, ldb, it + 1, w, n, h2, hs, red) : na;
+    }
+    if (threadIdx.x == 0) {
+        gate[0] = second;
+        loc[4] = second;
+        loc[3] += second;
+        alpha[it] = al;
+        beta[it] = sqrt(nf);
+    }
+    if (!vnext) return;
+    const double inv = 1.0 / sqrt(nf);
+    #pragma unroll 4
+    for (int64_t x = threadIdx.x; x < n; x += TB) vnext[x] = (TBv)(w[x] * inv);
 }
 
 // omega rows after a thick restart at step j: the kept vectors and v_j are
@@ -766,6 +870,7 @@ struct Solver {
     const std::atomic<bool>* cancel = nullptr;
     bool reorth_cgs2 = false;   // SMGPU_REORTH=cgs2: unconditional CGS2 instead of the local step
     bool partial = false;       // partial re-orthogonalisation (omega recurrence)
+    bool small = false;         // partial, n <= SMALL_STEP_N: the step is one k_step_small
     double eps_basis = 0.0;     // unit of the basis precision for the omega recurrence
     const JProj* jp = nullptr;  // J projection (doubleLanczos), or none
 
@@ -795,6 +900,7 @@ struct Solver {
         Gu = (n + UCHUNK - 1) / UCHUNK;
         Gx = (n + GsShape<TBv>::SEG - 1) / GsShape<TBv>::SEG;
         gstore = (int)std::min<int64_t>(4096, (n + TB - 1) / TB);
+        small = partial && n <= SMALL_STEP_N && lm + 1 <= 1024;
     }
 
     TBv* V(int j) { return W.basis.as<TBv>() + (int64_t)j * ld; }
@@ -881,6 +987,13 @@ struct Solver {
         int* gate = W.gate.as<int>();
         const TBv* vp = it > 0 ? (const TBv*)V(it - 1) : nullptr;
         const double* bp = it > 0 ? W.beta.as<double>() + (it - 1) : nullptr;
+        if (small) {
+            hipLaunchKernelGGL((k_step_small<TBv, TW>), dim3(1), dim3(TB), 0, s, win, (const TBv*)B, ld, it, n,
+                               W.w64.as<double>(), loc, gate, W.alpha.as<double>(), W.beta.as<double>(),
+                               W.om.as<double>(), lm + 2, eps_basis, W.h1.as<double>(), W.h2.as<double>(),
+                               it + 1 < lm ? V(it + 1) : (TBv*)nullptr);
+            return;
+        }
         const int G2 = (int)std::min<int64_t>(1024, Gd);
         hipLaunchKernelGGL((k_dot2<TBv, TW>), dim3(G2), dim3(TB), 0, s, (const TBv*)V(it), win, n, part, ldp);
         hipLaunchKernelGGL(k_reduce, dim3(1), dim3(TB), 0, s, part, G2, ldp, loc, (const int*)nullptr);
@@ -941,7 +1054,7 @@ struct Solver {
         const int nn = std::min(nev, it + 1);
         hipStream_t rs = W.side;
         if (step_ready) {
-            if (it + 1 < lm)
+            if (it + 1 < lm && !small)      // (k_step_small stores v_{it+1} itself)
                 hipLaunchKernelGGL((k_store<TBv>), dim3(gstore), dim3(TB), 0, s, W.w64.as<double>(),
                                    W.beta.as<double>() + it, V(it + 1), n);
             SM_HIP(hipEventRecord(step_ready, s));
@@ -953,7 +1066,7 @@ struct Solver {
         hipLaunchKernelGGL(k_ritz, dim3(nn), dim3(TB), 0, rs, W.alpha.as<double>(), W.beta.as<double>(), it + 1,
                            en(step - 1), nprev, en(step));
         hipLaunchKernelGGL(k_conv, dim3(1), dim3(64), 0, rs, en(step - 1), en(step), nev, nn, P.tol, flag(step));
-        if (!step_ready && it + 1 < lm)
+        if (!step_ready && it + 1 < lm && !small)
             hipLaunchKernelGGL((k_store<TBv>), dim3(gstore), dim3(TB), 0, s, W.w64.as<double>(),
                                W.beta.as<double>() + it, V(it + 1), n);
     }

@@ -42,3 +42,19 @@ def test_j_projected_levels(golden, fname, nuc, Z, N, twoJ, k, precision, tol):
     np.testing.assert_allclose(en, ref, atol=tol)
     assert all(x == 0.5 * twoJ for x in J), J
     assert st["n_jproj"] >= 1
+
+
+def test_main_sm_verbose_lines(golden, capsys):
+    """verbose: main_sm's printed lines (shellmodel_main.jl:256-266): the dimension
+    line, J of every state, En. and Ex. with %10.4f, or 2J= and En. with %12.5f for
+    a J-projected solve; the printed values are the returned ones."""
+    en, J = smgpu.main_sm(f"{golden}/ckpot.snt", "Be8", 4, verbose=True, return_J=True)
+    out = capsys.readouterr().out
+    assert out.startswith("   Be8      Z,N=(4,4)     c(2,2)     v(2,2)  mdim:           51 (  1.71 ) J ["), repr(out)
+    lines = out.split("\n")
+    assert lines[1] == "En. " + "".join("%10.4f " % x for x in en)
+    assert lines[2] == "Ex. " + "".join("%10.4f " % (x - en[0]) for x in en)
+    assert J[0] == 0.0
+    en2 = smgpu.main_sm(f"{golden}/ckpot.snt", "Be8", 2, [4], verbose=True)
+    out = capsys.readouterr().out
+    assert "2J =   4" in out and out.rstrip("\n").endswith("2J= 4  En." + "".join("%12.5f " % x for x in en2))
