@@ -637,39 +637,75 @@ __global__ __launch_bounds__(TB) void k_ritz(const double* __restrict__ alpha, c
     __shared__ double sa[1024], sb[1024];
     __shared__ int firstbig;
     __shared__ double slo, shi;
+    __shared__ double rl[TB / 64], rh[TB / 64], rm[TB / 64];
+    __shared__ int ok[2];
     const int j = blockIdx.x;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int i = threadIdx.x; i < n; i += TB) { sa[i] = alpha[i]; sb[i] = i < n - 1 ? beta[i] : 0.0; }
     __syncthreads();
-    const double pm = tri_pivmin(sb, n);
-    if (threadIdx.x == 0) {
-        double lo, hi;
-        tri_bounds(sa, sb, n, &lo, &hi);
-        double l2 = lo, h2 = hi;
-        if (nprev > 0) {
-            const double pad = 1e-9 * fmax(1.0, fmax(fabs(lo), fabs(hi)));
-            if (j >= 1 && j - 1 < nprev) l2 = fmax(lo, prev[j - 1] - pad);
-            if (j < nprev) h2 = fmin(hi, prev[j] + pad);
-            // keep the interlacing bracket only if the Sturm counts confirm it
-            if (!(l2 < h2 && tri_count(sa, sb, n, l2, pm) <= j && tri_count(sa, sb, n, h2, pm) > j)) {
-                l2 = lo; h2 = hi;
-            }
-        }
-        slo = l2; shi = h2;
+    // Gershgorin bounds and the pivot floor (tri_bounds, tri_pivmin) as block reductions
+    double l = 1e300, h = -1e300, m = 1.0;
+    for (int i = threadIdx.x; i < n; i += TB) {
+        const double r = (i > 0 ? fabs(sb[i - 1]) : 0.0) + (i < n - 1 ? fabs(sb[i]) : 0.0);
+        l = fmin(l, sa[i] - r);
+        h = fmax(h, sa[i] + r);
+        if (i < n - 1) m = fmax(m, sb[i] * sb[i]);
     }
+    for (int o = 32; o > 0; o >>= 1) {
+        l = fmin(l, __shfl_xor(l, o));
+        h = fmax(h, __shfl_xor(h, o));
+        m = fmax(m, __shfl_xor(m, o));
+    }
+    if (lane == 0) { rl[wv] = l; rh[wv] = h; rm[wv] = m; }
     __syncthreads();
+    l = rl[0]; h = rh[0]; m = rm[0];
+    for (int w = 1; w < TB / 64; w++) { l = fmin(l, rl[w]); h = fmax(h, rh[w]); m = fmax(m, rm[w]); }
+    const double bpad = 1e-12 * fmax(1.0, fmax(fabs(l), fabs(h)));
+    const double lo = l - bpad, hi = h + bpad;
+    const double pm = 2.2250738585072014e-308 / 2.220446049250313e-16 * m;
+    // Cauchy interlacing bracket from the previous step's values, kept only if the
+    // Sturm counts at its ends (two threads) confirm it
+    double l2 = lo, h2 = hi;
+    bool geo = false;
+    if (nprev > 0) {
+        const double pad = 1e-9 * fmax(1.0, fmax(fabs(lo), fabs(hi)));
+        if (j >= 1 && j - 1 < nprev) l2 = fmax(lo, prev[j - 1] - pad);
+        if (j < nprev) h2 = fmin(hi, prev[j] + pad);
+        if (threadIdx.x == 0) ok[0] = l2 < h2 && tri_count(sa, sb, n, l2, pm) <= j;
+        if (threadIdx.x == 1) ok[1] = l2 < h2 && tri_count(sa, sb, n, h2, pm) > j;
+        __syncthreads();
+        if (!(ok[0] && ok[1])) { l2 = lo; h2 = hi; }
+        else geo = j < nprev && h2 - l2 > 2.0 * pad;
+    }
+    if (threadIdx.x == 0) { slo = l2; shi = h2; }
+    __syncthreads();
+    // First round over the interlacing bracket [l2, h2 = prev_j + pad]: the eigenvalue
+    // is at or below prev_j and close to it once the solve converges, so the points
+    // are spaced geometrically in their distance from h2 (from the bracket width down
+    // to pad / 2); later rounds are uniform multisection.
+    const double gpad = 0.5e-9 * fmax(1.0, fmax(fabs(lo), fabs(hi)));
+    const double gr = geo ? pow(gpad / (h2 - l2), 1.0 / (double)(TB - 1)) : 0.0;
     for (int round = 0; round < 12; round++) {
-        const double lo = slo, hi = shi;
-        if (hi - lo <= 1e-12 * fmax(1.0, fmax(fabs(lo), fabs(hi)))) break;
+        const double rlo = slo, rhi = shi;
+        if (rhi - rlo <= 1e-12 * fmax(1.0, fmax(fabs(rlo), fabs(rhi)))) break;
+        const bool g = geo && round == 0;
         if (threadIdx.x == 0) firstbig = TB;
         __syncthreads();
-        const double x = lo + (hi - lo) * (double)(threadIdx.x + 1) / (double)(TB + 1);
+        const double x = g ? rhi - (rhi - rlo) * pow(gr, (double)threadIdx.x)
+                           : rlo + (rhi - rlo) * (double)(threadIdx.x + 1) / (double)(TB + 1);
         const int c = tri_count(sa, sb, n, x, pm);
         if (c > j) atomicMin(&firstbig, (int)threadIdx.x);
         __syncthreads();
         if (threadIdx.x == 0) {
             const int t = firstbig;
-            const double nlo = t == 0 ? lo : lo + (hi - lo) * (double)t / (double)(TB + 1);
-            const double nhi = t == TB ? hi : lo + (hi - lo) * (double)(t + 1) / (double)(TB + 1);
+            double nlo, nhi;
+            if (g) {
+                nlo = t == 0 ? rlo : rhi - (rhi - rlo) * pow(gr, (double)(t - 1));
+                nhi = t == TB ? rhi : rhi - (rhi - rlo) * pow(gr, (double)t);
+            } else {
+                nlo = t == 0 ? rlo : rlo + (rhi - rlo) * (double)t / (double)(TB + 1);
+                nhi = t == TB ? rhi : rlo + (rhi - rlo) * (double)(t + 1) / (double)(TB + 1);
+            }
             slo = nlo; shi = nhi;
         }
         __syncthreads();
