@@ -631,9 +631,14 @@ __global__ __launch_bounds__(TB) void k_convert(const TA* __restrict__ a, TBv* _
 // Ritz values: block j computes eigenvalue j of T[0..n-1] by 256-way multisection.
 // prev (nprev values) are the eigenvalues of the leading (n-1)x(n-1) block when
 // nprev > 0: lambda_j(T_n) lies in [prev[j-1], prev[j]] (Cauchy interlacing).
+// With a convergence slot (flag != null) the last block to finish also runs the
+// test of diagonalize_T! (lanczos_methods.jl:250-258: all(prev - out < tol) over
+// num_ev values; entries past the blocks keep prev) and writes its result to
+// *flag and to the host-visible *hflag (ticket: zero between launches).
 __global__ __launch_bounds__(TB) void k_ritz(const double* __restrict__ alpha, const double* __restrict__ beta,
                                              int n, const double* __restrict__ prev, int nprev,
-                                             double* __restrict__ out) {
+                                             double* __restrict__ out, int num_ev, double tol,
+                                             int* __restrict__ flag, int* hflag, int* __restrict__ ticket) {
     __shared__ double sa[1024], sb[1024];
     __shared__ int firstbig;
     __shared__ double slo, shi;
@@ -711,17 +716,26 @@ __global__ __launch_bounds__(TB) void k_ritz(const double* __restrict__ alpha, c
         __syncthreads();
     }
     if (threadIdx.x == 0) out[j] = 0.5 * (slo + shi);
+    if (!flag) return;
+    __shared__ int is_last;
+    if (threadIdx.x == 0) {
+        __threadfence();
+        is_last = atomicAdd(ticket, 1) == (int)gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!is_last || threadIdx.x != 0) return;
+    __threadfence();
+    for (int k = (int)gridDim.x; k < num_ev; k++) out[k] = prev[k];
+    int okc = 1;
+    for (int k = 0; k < num_ev; k++) okc &= (prev[k] - out[k] < tol) ? 1 : 0;
+    *flag = okc;
+    *hflag = okc;
+    __threadfence_system();
+    *ticket = 0;
 }
 
 // diagonalize_T! bookkeeping: entries n..num_ev-1 keep the previous values,
 // flag = all(prev - cur < tol)
-__global__ void k_conv(const double* prev, double* cur, int num_ev, int n, double tol, int* flag) {
-    if (threadIdx.x != 0) return;
-    for (int k = n; k < num_ev; k++) cur[k] = prev[k];
-    int ok = 1;
-    for (int k = 0; k < num_ev; k++) ok &= (prev[k] - cur[k] < tol) ? 1 : 0;
-    *flag = ok;
-}
 
 // Eigenvectors of T[0..n-1] for the k eigenvalues vals (ascending) by inverse
 // iteration: one thread per cluster of close eigenvalues (chained gaps below
@@ -811,7 +825,7 @@ __global__ __launch_bounds__(TB) void k_rotate(const TBv* __restrict__ B, int64_
 static void restart_small_on_stream(double* alpha, double* beta, int m, int ls, double* S, double* vals,
                                     double* Y, double* Qrev, double* work, hipStream_t st) {
     hipLaunchKernelGGL(k_ritz, dim3(ls), dim3(TB), 0, st, (const double*)alpha, (const double*)beta, m,
-                       (const double*)nullptr, 0, vals);
+                       (const double*)nullptr, 0, vals, 0, 0.0, (int*)nullptr, (int*)nullptr, (int*)nullptr);
     hipLaunchKernelGGL(k_tri_vecs, dim3(1), dim3(64), 0, st, (const double*)alpha, (const double*)beta, m,
                        (const double*)vals, ls, Y, work);
     hipLaunchKernelGGL(k_restart_q, dim3(1), dim3(64), 0, st, alpha, beta, m, ls, (const double*)vals,
@@ -826,13 +840,14 @@ namespace {
 struct Workspace {
     DBuf basis, wH, w64, part, npart, npart2, h1, h2, alpha, beta, enring, flags, S, swork, U, scal, tvals, Y, Q,
         R, hv64, loc, gate, jy, jz, om;
-    int* hflag = nullptr;          // pinned mirror of the convergence-flag ring
+    int* hflag = nullptr;          // host ring of convergence flags (coherent; k_ritz writes it)
     hipEvent_t ev[4] = {};         // side stream: a step's convergence flag is in hflag
     hipEvent_t evm[4] = {};        // main stream: a step's alpha/beta are written
     hipStream_t side = nullptr;    // Ritz values and convergence test, off the H·v stream
     void host_sync_objects() {
         if (hflag) return;
-        SM_HIP(hipHostMalloc((void**)&hflag, 4 * sizeof(int), hipHostMallocDefault));
+        // coherent: k_ritz writes the convergence flags here directly
+        SM_HIP(hipHostMalloc((void**)&hflag, 4 * sizeof(int), hipHostMallocCoherent));
         for (int q = 0; q < 4; q++) SM_HIP(hipEventCreateWithFlags(&ev[q], hipEventDisableTiming));
         for (int q = 0; q < 4; q++) SM_HIP(hipEventCreateWithFlags(&evm[q], hipEventDisableTiming));
         SM_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
@@ -876,7 +891,7 @@ static void workspace_plan(int64_t n, const SolveParams& p, F&& f) {
     f(&Workspace::alpha, (lm + 1) * 8);
     f(&Workspace::beta, (lm + 1) * 8);
     f(&Workspace::enring, 4 * 64 * 8);
-    f(&Workspace::flags, 4 * (int64_t)sizeof(int));
+    f(&Workspace::flags, 8 * (int64_t)sizeof(int));   // convergence flags [0,4), k_ritz's ticket [4]
     f(&Workspace::S, lm * 32 * 8);
     f(&Workspace::swork, (32 * 5 * lm + 4096) * 8);
     f(&Workspace::U, ls * ld * (int64_t)sizeof(TBv));
@@ -1099,9 +1114,10 @@ struct Solver {
             SM_HIP(hipEventRecord(W.evm[(step + 1) & 3], s));
             SM_HIP(hipStreamWaitEvent(W.side, W.evm[(step + 1) & 3], 0));
         }
+        // Ritz values and the convergence test; the flag also lands in the host ring
         hipLaunchKernelGGL(k_ritz, dim3(nn), dim3(TB), 0, rs, W.alpha.as<double>(), W.beta.as<double>(), it + 1,
-                           en(step - 1), nprev, en(step));
-        hipLaunchKernelGGL(k_conv, dim3(1), dim3(64), 0, rs, en(step - 1), en(step), nev, nn, P.tol, flag(step));
+                           en(step - 1), nprev, en(step), nev, P.tol, flag(step), W.hflag + ((step + 1) & 3),
+                           W.flags.as<int>() + 4);
         if (!step_ready && it + 1 < lm && !small)
             hipLaunchKernelGGL((k_store<TBv>), dim3(gstore), dim3(TB), 0, s, W.w64.as<double>(),
                                W.beta.as<double>() + it, V(it + 1), n);
@@ -1117,6 +1133,7 @@ struct Solver {
         SM_HIP(hipMemsetAsync(W.loc.p, 0, W.loc.bytes, s));
         SM_HIP(hipMemsetAsync(W.gate.p, 0, W.gate.bytes, s));
         SM_HIP(hipMemsetAsync(W.om.p, 0, W.om.bytes, s));
+        SM_HIP(hipMemsetAsync(W.flags.p, 0, W.flags.bytes, s));     // flags and k_ritz's ticket
         {
             const double one = 1.0;          // om_{0,0}
             SM_HIP(hipMemcpyAsync(W.om.p, &one, 8, hipMemcpyHostToDevice, s));
@@ -1201,8 +1218,7 @@ struct Solver {
         const int nprev = (it == k && d_after_restart) || it == 0 ? 0 : std::min(nev, it);
         after_hv(it, step, nprev);
         hipStream_t rs = step_ready ? s : W.side;
-        SM_HIP(hipMemcpyAsync(W.hflag + ((step + 1) & 3), flag(step), sizeof(int), hipMemcpyDeviceToHost, rs));
-        SM_HIP(hipEventRecord(W.ev[(step + 1) & 3], rs));
+        SM_HIP(hipEventRecord(W.ev[(step + 1) & 3], rs));       // k_ritz has written the flag
         bool done = false;
         bool moved = false;     // the next input vector changed after step_ready
         if (jp && it + 1 < lm && it + 1 < n) {
