@@ -96,3 +96,23 @@ def test_trl_variants_dense_levels(tmp_path):
     for p, (ev, _, _) in zip(paths, out64):
         H, _ = bf.brute_hamiltonian(p, nuc, 10, 10)
         np.testing.assert_allclose(ev, np.linalg.eigvalsh(H)[:6], atol=1e-7)
+
+
+def test_variant_errors(mg24, tmp_path):
+    """A variant needs the base's basis; variant launches and solves need handles on one
+    index.  Both fail with EINVAL, and the handles stay usable."""
+    paths, nuc, hams = mg24
+    other_path, other_nuc = synth.config_snt(str(tmp_path / "o.snt"), "sd", 3, 3, synth.BASE_SEED + 93, (-6, 0),
+                                             1.2)
+    other = smgpu.Hamiltonian(smgpu.Problem(other_path, other_nuc))
+    with pytest.raises(smgpu.SmgpuError):
+        hams[0].variant(smgpu.Problem(other_path, other_nuc))
+    fresh = smgpu.Hamiltonian(smgpu.Problem(paths[1], nuc))          # same basis, its own index
+    V = np.ones((2, hams[0].mdim))
+    with pytest.raises(smgpu.SmgpuError):
+        smgpu.operate_H_variants([hams[0], fresh], V)
+    with pytest.raises(smgpu.SmgpuError):
+        smgpu.trl_variants([hams[0], fresh], 2, [1, 2])
+    assert other.mdim != hams[0].mdim
+    w = smgpu.operate_H_variants([hams[0], hams[1]], V)
+    assert np.array_equal(w[1], hams[1].operate_H(V[1]))
