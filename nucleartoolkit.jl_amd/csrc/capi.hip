@@ -38,6 +38,26 @@ __global__ void k_cast(const TA* a, TB* b, int64_t n) {
     for (int64_t x = (int64_t)blockIdx.x * 256 + threadIdx.x; x < n; x += (int64_t)gridDim.x * 256) b[x] = (TB)a[x];
 }
 
+// two timing events, destroyed on every exit path
+struct TimingEvents {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    TimingEvents() {
+        SM_HIP(hipEventCreate(&e0));
+        try {
+            SM_HIP(hipEventCreate(&e1));
+        } catch (...) {
+            (void)hipEventDestroy(e0);
+            throw;
+        }
+    }
+    ~TimingEvents() {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    TimingEvents(const TimingEvents&) = delete;
+    TimingEvents& operator=(const TimingEvents&) = delete;
+};
+
 static thread_local std::string g_err;
 
 void smgpu::set_last_error(const std::string& msg) { g_err = msg; }
@@ -377,17 +397,14 @@ int32_t smgpu_time_apply_H_multi(smgpu_ham h, int32_t precision, int32_t nvec, i
         }
         const int p = precision == SMGPU_FP64 ? 0 : 1;
         for (int r = 0; r < 2; r++) launch_hv_multi(d, p, nvec, dv.p, n, dw.p, n, d.stream);
-        hipEvent_t e0, e1;
-        SM_HIP(hipEventCreate(&e0));
-        SM_HIP(hipEventCreate(&e1));
+        TimingEvents ev;
+        hipEvent_t e0 = ev.e0, e1 = ev.e1;
         SM_HIP(hipEventRecord(e0, d.stream));
         for (int r = 0; r < reps; r++) launch_hv_multi(d, p, nvec, dv.p, n, dw.p, n, d.stream);
         SM_HIP(hipEventRecord(e1, d.stream));
         SM_HIP(hipEventSynchronize(e1));
         float t = 0.f;
         SM_HIP(hipEventElapsedTime(&t, e0, e1));
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
         *ms = (double)t / reps;
         return SMGPU_OK;
     });
@@ -484,17 +501,14 @@ int32_t smgpu_time_apply_H_variants(const smgpu_ham* hs, int32_t precision, int3
         }
         const int p = fp64 ? 0 : 1;
         for (int r = 0; r < 2; r++) launch_hv_variants(d.data(), p, nvec, vp.data(), wp.data(), st);
-        hipEvent_t e0, e1;
-        SM_HIP(hipEventCreate(&e0));
-        SM_HIP(hipEventCreate(&e1));
+        TimingEvents ev;
+        hipEvent_t e0 = ev.e0, e1 = ev.e1;
         SM_HIP(hipEventRecord(e0, st));
         for (int r = 0; r < reps; r++) launch_hv_variants(d.data(), p, nvec, vp.data(), wp.data(), st);
         SM_HIP(hipEventRecord(e1, st));
         SM_HIP(hipEventSynchronize(e1));
         float t = 0.f;
         SM_HIP(hipEventElapsedTime(&t, e0, e1));
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
         *ms = (double)t / reps;
         return SMGPU_OK;
     });
@@ -519,17 +533,14 @@ int32_t smgpu_time_apply_H(smgpu_ham h, int32_t precision, int32_t reps, double*
         }
         int p = precision == SMGPU_FP64 ? 0 : 1;
         for (int r = 0; r < 2; r++) launch_hv(d, p, dv.p, dw.p, d.stream);
-        hipEvent_t e0, e1;
-        SM_HIP(hipEventCreate(&e0));
-        SM_HIP(hipEventCreate(&e1));
+        TimingEvents ev;
+        hipEvent_t e0 = ev.e0, e1 = ev.e1;
         SM_HIP(hipEventRecord(e0, d.stream));
         for (int r = 0; r < reps; r++) launch_hv(d, p, dv.p, dw.p, d.stream);
         SM_HIP(hipEventRecord(e1, d.stream));
         SM_HIP(hipEventSynchronize(e1));
         float t = 0.f;
         SM_HIP(hipEventElapsedTime(&t, e0, e1));
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
         *ms = (double)t / reps;
         return SMGPU_OK;
     });
