@@ -67,3 +67,19 @@ def test_prep_ec_batched_equals_sequential(snt_dir):
     assert seq.source == bat.source
     (_, _, v), = ec.solve_ec([paths[1]], nuc, bat, 1)
     assert abs(v[0] - bat.energies[1][0]) < 1e-8
+
+
+def test_ec_reference_setup_o20(golden):
+    """The reference's EC test (test/ShellModel_test.jl:40-68): O20, num_ev 3, 2J = 0,
+    samples from the five sd interactions random_snts/tmp_0..4, continuation to
+    usdb.  The reference checks only that it runs; here, for an interaction of the
+    sample set the EC ground state is its exact J = 0 ground state (1e-8 MeV), and for
+    usdb the EC levels are variational upper bounds of the J-projected solve's."""
+    Hs = [f"{golden}/random_snts/tmp_{i}.snt" for i in range(5)]
+    samples = ec.prep_ec(Hs, "O20", 3, tJ=0)
+    assert samples.vectors.shape[0] >= 5
+    (_, _, v3), (_, _, vu) = ec.solve_ec([Hs[3], f"{golden}/usdb.snt"], "O20", samples, 3)
+    e3 = smgpu.main_sm(Hs[3], "O20", 3, [0])
+    eu = smgpu.main_sm(f"{golden}/usdb.snt", "O20", 3, [0])
+    assert abs(v3[0] - e3[0]) < 1e-8, (v3[0], e3[0])
+    assert np.all(vu >= eu - 1e-8), (vu, eu)
