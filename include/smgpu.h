@@ -327,7 +327,9 @@ int32_t smgpu_selftest_tridiag(const double* alpha, const double* beta, int32_t 
 /* Self-test of the host dense symmetric eigen-solver of the block Lanczos matrix
  * (Householder reduction + tridiagonal bisection / inverse iteration): lowest k
  * eigenvalues of the n x n row-major symmetric A, and their unit eigenvectors
- * (vector j at vecs + j*n) unless vecs is NULL. */
+ * (vector j at vecs + j*n) unless vecs is NULL.  Without vecs, a banded A (lower
+ * bandwidth below n / 4) goes through the Givens band-to-tridiagonal reduction the
+ * solver uses for its per-step eigenvalues. */
 int32_t smgpu_selftest_symeig(const double* A, int32_t n, int32_t k, double* vals, double* vecs);
 
 /* Self-test of the thick-restart small problem (tri_restart) for the tridiagonal

@@ -774,9 +774,12 @@ int32_t smgpu_selftest_symeig(const double* A, int32_t n, int32_t k, double* val
     return guard([&]() -> int32_t {
         if (!A || !vals || n < 1 || k < 1 || k > n) throw std::invalid_argument("bad argument");
         DenseEig e;
-        e.reduce(A, n, n);
-        if (vecs) e.lowest_pairs(k, vals, vecs);
-        else e.lowest(k, vals);
+        if (vecs) {
+            e.reduce(A, n, n);
+            e.lowest_pairs(k, vals, vecs);
+        } else {
+            e.lowest_values(A, n, n, k, vals);     // the band reduction when A is banded
+        }
         return SMGPU_OK;
     });
 }

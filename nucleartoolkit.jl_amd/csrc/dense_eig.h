@@ -9,6 +9,7 @@
 // (tridiag.h), back-transformed by Q.  The reference calls LAPACK's
 // eigvals/eigen on the same matrix; the eigenvalues agree to rounding.
 #pragma once
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -71,13 +72,99 @@ struct DenseEig {
         if (n >= 2) e[n - 2] = a[(size_t)(n - 1) * n + (n - 2)];
     }
 
-    // lowest k eigenvalues (ascending)
-    void lowest(int k, double* vals) const {
+    // Lowest nv eigenvalues (ascending) of the leading k x k block of T (row stride ldt)
+    // without eigenvectors.  A banded T (lower bandwidth bw < k / 4: the block
+    // tridiagonal projection of the block Lanczos method, bw ~ q, or ~ ls + q with the
+    // arrowhead of a thick restart) is reduced to tridiagonal form by Givens rotations
+    // in O(k^2 bw) (band_reduce); otherwise by the O(k^3) Householder reduction.  Both
+    // are orthogonal similarities, so the values agree to rounding.
+    // hint (optional): the previous step's values, upper bounds of these by Cauchy
+    // interlacing (rows were only added), used to bracket the bisections.
+    void lowest_values(const double* T, int ldt, int k, int nv, double* vals, const double* hint = nullptr) {
+        int bw = 0;
+        for (int i = 0; i < k; i++)
+            for (int j = 0; j < i - bw; j++)
+                if (T[(size_t)i * ldt + j] != 0.0) { bw = i - j; break; }
+        if (k >= 8 && 4 * bw < k) band_reduce(T, ldt, k, bw);
+        else reduce(T, ldt, k);
+        lowest(nv, vals, hint);
+    }
+
+    // Band to tridiagonal (Rutishauser 1963, Schwarz 1968): for m = bw .. 2, the
+    // outermost diagonal is annihilated column by column with a rotation of rows and
+    // columns (c + m - 1, c + m); each rotation fills one element outside the band at
+    // distance m + 1 further down, which the next rotation (m rows lower) annihilates in
+    // turn.  Columns are swept in increasing order, so the element left of each
+    // rotation is already zero and no other fill appears.  Only the band window of the
+    // two rows and columns is updated.  Sets d, e (tau = 0: no eigenvectors).
+    void band_reduce(const double* T, int ldt, int k, int bw) {
+        n = k;
+        a.assign((size_t)n * n, 0.0);
+        for (int i = 0; i < n; i++)
+            for (int j = std::max(0, i - bw); j <= i; j++)
+                a[(size_t)i * n + j] = a[(size_t)j * n + i] = T[(size_t)i * ldt + j];
+        auto A = [&](int i, int j) -> double& { return a[(size_t)i * n + j]; };
+        for (int m = bw; m > 1; m--) {
+            for (int c0 = 0; c0 + m < n; c0++) {
+                int col = c0, p = c0 + m - 1, q = c0 + m;
+                for (;;) {
+                    const double x = A(p, col), y = A(q, col);
+                    if (y != 0.0) {
+                        const double r = std::hypot(x, y), c = x / r, sn = y / r;
+                        const int lo = std::max(0, p - m - 1), hi = std::min(n - 1, q + m + 1);
+                        for (int t = lo; t <= hi; t++) {
+                            const double ap = A(p, t), aq = A(q, t);
+                            A(p, t) = c * ap + sn * aq;
+                            A(q, t) = -sn * ap + c * aq;
+                        }
+                        for (int t = lo; t <= hi; t++) {
+                            const double ap = A(t, p), aq = A(t, q);
+                            A(t, p) = c * ap + sn * aq;
+                            A(t, q) = -sn * ap + c * aq;
+                        }
+                        A(q, col) = 0.0;
+                        A(col, q) = 0.0;
+                    }
+                    if (q + m >= n) break;
+                    col = p;
+                    p += m;
+                    q += m;
+                }
+            }
+        }
+        d.assign(n, 0.0);
+        e.assign(n > 1 ? n - 1 : 1, 0.0);
+        tau.assign(n, 0.0);
+        for (int i = 0; i < n; i++) d[i] = A(i, i);
+        for (int i = 0; i + 1 < n; i++) e[i] = A(i + 1, i);
+    }
+
+    // lowest k eigenvalues (ascending).  With hint[j] an upper bound of eigenvalue j
+    // (checked by a Sturm count), its bisection starts from [hint - delta, hint + pad]
+    // with delta grown until the count confirms the lower end; otherwise from the
+    // Gershgorin interval.
+    void lowest(int k, double* vals, const double* hint = nullptr) const {
         if (n == 1) { vals[0] = d[0]; return; }
         double lo, hi;
         tri_bounds(d.data(), e.data(), n, &lo, &hi);
         const double pm = tri_pivmin(e.data(), n);
-        for (int j = 0; j < k; j++) vals[j] = tri_bisect(d.data(), e.data(), n, j, lo, hi, pm);
+        const double scale = std::fmax(1.0, std::fmax(std::fabs(lo), std::fabs(hi)));
+        for (int j = 0; j < k; j++) {
+            double l = lo, h = hi;
+            if (hint && hint[j] > lo && hint[j] < hi) {
+                const double hh = std::fmin(hi, hint[j] + 1e-9 * scale);
+                if (tri_count(d.data(), e.data(), n, hh, pm) > j) {
+                    double del = 1e-8 * scale;
+                    for (int t = 0; t < 24; t++) {
+                        const double ll = std::fmax(lo, hh - del);
+                        if (tri_count(d.data(), e.data(), n, ll, pm) <= j) { l = ll; h = hh; break; }
+                        if (ll <= lo) break;
+                        del *= 16.0;
+                    }
+                }
+            }
+            vals[j] = tri_bisect(d.data(), e.data(), n, j, l, h, pm);
+        }
     }
 
     // lowest k eigenpairs; vector j at vecs + j*n (unit 2-norm)

@@ -1713,10 +1713,10 @@ struct BlockSolver {
     // lowest min(nev, k) eigenvalues of T[0..k-1] into cur (entries past keep prev);
     // returns diagonalize_T!'s convergence flag all(prev - cur < tol)
     bool converged(int k, std::vector<double>& prev, std::vector<double>& cur) {
-        eig.reduce(T.data(), lmT, k);
         const int m = std::min(nev, k);
         cur = prev;
-        eig.lowest(m, cur.data());
+        // band reduction while T is banded; the previous values bound these from above
+        eig.lowest_values(T.data(), lmT, k, m, cur.data(), prev.data());
         bool ok = true;
         for (int i = 0; i < nev; i++) ok = ok && (prev[i] - cur[i] < P.tol);
         prev = cur;
