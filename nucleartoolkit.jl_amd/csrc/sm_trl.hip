@@ -437,7 +437,8 @@ __global__ __launch_bounds__(TB) void k_local_step(const TW* __restrict__ win, c
 // Rows om_{j-1}, om_j, om_{j+1} rotate through three rows of om (stride ldo).
 // gate[1] = 1 runs the projection pass of this step: when max_k |om_{j+1,k}| >
 // sqrt(eps) (and then also for the next step, gate[2] counts forced steps), after
-// which om_{j+1,k} = eps.  loc[5] counts the steps that ran it.
+// which om_{j+1,k} = eps.  loc[5] counts the steps that ran it.  A step without the
+// pass also has no DGKS second pass: gate[0] = 0, loc[4] = 0.
 // (one workgroup of TB threads; returns the gate of this step's pass in every thread)
 __device__ int omega_block(const double* __restrict__ alpha, const double* __restrict__ beta, int j,
                            const double* __restrict__ npart, int G, double* __restrict__ om, int ldo,
@@ -482,14 +483,18 @@ __device__ int omega_block(const double* __restrict__ alpha, const double* __res
         gate[1] = run;
         gate[2] = forced > 0 ? forced - 1 : (run ? 1 : 0);
         loc[5] += run;
+        if (!run) { gate[0] = 0; loc[4] = 0.0; }
     }
     return run;
 }
 
+// ... and, when hgate is set, the gate also to the host-visible *hgate (coherent host
+// memory): the host then launches the projection pass only for steps that run it
 __global__ void k_omega(const double* __restrict__ alpha, const double* __restrict__ beta, int j,
                         const double* __restrict__ npart, int G, double* __restrict__ om, int ldo,
-                        double* __restrict__ loc, int* __restrict__ gate, double eps) {
-    omega_block(alpha, beta, j, npart, G, om, ldo, loc, gate, eps);
+                        double* __restrict__ loc, int* __restrict__ gate, double eps, int* hgate) {
+    const int run = omega_block(alpha, beta, j, npart, G, om, ldo, loc, gate, eps);
+    if (hgate && threadIdx.x == 0) *hgate = run;
 }
 
 // One projection pass of w (FP64, in place) against the basis B[0..nv) by one
@@ -840,14 +845,17 @@ namespace {
 struct Workspace {
     DBuf basis, wH, w64, part, npart, npart2, h1, h2, alpha, beta, enring, flags, S, swork, U, scal, tvals, Y, Q,
         R, hv64, loc, gate, jy, jz, om;
-    int* hflag = nullptr;          // host ring of convergence flags (coherent; k_ritz writes it)
+    int* hflag = nullptr;          // host ring of convergence flags (coherent; k_ritz writes it),
+                                   // hflag[4]: the projection gate of the step (k_omega writes it)
     hipEvent_t ev[4] = {};         // side stream: a step's convergence flag is in hflag
+    hipEvent_t evg = nullptr;      // main stream: k_omega has written hflag[4]
     hipEvent_t evm[4] = {};        // main stream: a step's alpha/beta are written
     hipStream_t side = nullptr;    // Ritz values and convergence test, off the H·v stream
     void host_sync_objects() {
         if (hflag) return;
         // coherent: k_ritz writes the convergence flags here directly
-        SM_HIP(hipHostMalloc((void**)&hflag, 4 * sizeof(int), hipHostMallocCoherent));
+        SM_HIP(hipHostMalloc((void**)&hflag, 8 * sizeof(int), hipHostMallocCoherent));
+        SM_HIP(hipEventCreateWithFlags(&evg, hipEventDisableTiming));
         for (int q = 0; q < 4; q++) SM_HIP(hipEventCreateWithFlags(&ev[q], hipEventDisableTiming));
         for (int q = 0; q < 4; q++) SM_HIP(hipEventCreateWithFlags(&evm[q], hipEventDisableTiming));
         SM_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
@@ -863,6 +871,7 @@ struct Workspace {
         if (side) { (void)hipStreamSynchronize(side); (void)hipStreamDestroy(side); }
         for (int q = 0; q < 4; q++) if (evm[q]) (void)hipEventDestroy(evm[q]);
         for (int q = 0; q < 4; q++) if (ev[q]) (void)hipEventDestroy(ev[q]);
+        if (evg) (void)hipEventDestroy(evg);
         if (hflag) (void)hipHostFree(hflag);
     }
 };
@@ -1029,9 +1038,10 @@ struct Solver {
     // Lanczos step with partial re-orthogonalisation: the local step, then the omega
     // recurrence decides on the device whether this step runs one CGS pass against
     // V[0..it] (plus the DGKS second pass when that removes more than half of |w|^2).
+    // This half enqueues the local step and k_omega, which also hands its gate to the
+    // host; reorth_partial_end enqueues the rest.  (Small bases: all of it in k_step_small.)
     template <typename TW>
-    void reorth_partial(int it, const TW* win) {
-        const int nv = it + 1;
+    void reorth_partial_begin(int it, const TW* win) {
         const TBv* B = V(0);
         double* part = W.part.as<double>();
         double* loc = W.loc.as<double>();
@@ -1052,28 +1062,67 @@ struct Solver {
                            (const double*)loc, bp, W.w64.as<double>(), n, W.npart.as<double>(), W.npart2.as<double>());
         hipLaunchKernelGGL(k_omega, dim3(1), dim3(TB), 0, s, (const double*)W.alpha.as<double>(),
                            (const double*)W.beta.as<double>(), it, (const double*)W.npart.as<double>(), (int)Gx,
-                           W.om.as<double>(), lm + 2, loc, gate, eps_basis);
-        const int* g1 = gate + 1;
-        hipLaunchKernelGGL((k_gs_dots<TBv, double, false>), dim3(Gx), dim3(TB), 0, s, B, ld, nv,
-                           (const double*)W.w64.as<double>(), n, part, ldp, g1, (const TBv*)nullptr,
-                           (const TBv*)nullptr, (const double*)nullptr, (const double*)nullptr, (double*)nullptr,
-                           (double*)nullptr);
-        hipLaunchKernelGGL(k_reduce, dim3(nv), dim3(TB), 0, s, part, (int)Gx, ldp, W.h1.as<double>(), g1);
-        hipLaunchKernelGGL((k_gs_upd<TBv, double, false>), dim3(Gx), dim3(TB), 0, s, B, ld, nv, W.h1.as<double>(),
-                           W.w64.as<double>(), W.w64.as<double>(), n, part, ldp, W.npart.as<double>(), g1);
-        hipLaunchKernelGGL(k_dgks, dim3(1), dim3(TB), 0, s, (const double*)W.npart.as<double>(),
-                           (const double*)W.npart2.as<double>(), (int)Gx, loc, gate, g1);
-        hipLaunchKernelGGL((k_gs_dots<TBv, double, false>), dim3(Gx), dim3(TB), 0, s, B, ld, nv,
-                           (const double*)W.w64.as<double>(), n, part, ldp, (const int*)gate, (const TBv*)nullptr,
-                           (const TBv*)nullptr, (const double*)nullptr, (const double*)nullptr, (double*)nullptr,
-                           (double*)nullptr);
-        hipLaunchKernelGGL(k_reduce, dim3(nv), dim3(TB), 0, s, part, (int)Gx, ldp, W.h2.as<double>(),
-                           (const int*)gate);
-        hipLaunchKernelGGL((k_gs_upd<TBv, double, false>), dim3(Gx), dim3(TB), 0, s, B, ld, nv, W.h2.as<double>(),
-                           W.w64.as<double>(), W.w64.as<double>(), n, part, ldp, W.npart.as<double>(),
-                           (const int*)gate);
+                           W.om.as<double>(), lm + 2, loc, gate, eps_basis, step_ready ? W.hflag + 4 : nullptr);
+        if (step_ready) SM_HIP(hipEventRecord(W.evg, s));
+    }
+
+    // The rest of the step: the projection pass and its DGKS second pass (both gated on
+    // the device), then alpha/beta.  A batch member first waits for the gate of k_omega
+    // and launches the projection kernels only on steps that run it: those kernels do
+    // nothing on the other steps (about nine in ten), so leaving them out changes no
+    // result and saves seven dependent dispatches, each of which waits for a free CU
+    // slot behind the other solves' H·v; the wait itself overlaps its mates' steps and
+    // the other H·v.  A solve alone keeps them queued behind k_omega instead, so the
+    // host stays a step ahead of the device.
+    void reorth_partial_end(int it) {
+        if (small) return;
+        const int nv = it + 1;
+        const TBv* B = V(0);
+        double* part = W.part.as<double>();
+        double* loc = W.loc.as<double>();
+        int* gate = W.gate.as<int>();
+        bool run = true;
+        if (step_ready) {
+            SM_HIP(hipEventSynchronize(W.evg));
+            run = static_cast<volatile int*>(W.hflag)[4] != 0;
+        }
+        if (run) {
+            const int* g1 = gate + 1;
+            hipLaunchKernelGGL((k_gs_dots<TBv, double, false>), dim3(Gx), dim3(TB), 0, s, B, ld, nv,
+                               (const double*)W.w64.as<double>(), n, part, ldp, g1, (const TBv*)nullptr,
+                               (const TBv*)nullptr, (const double*)nullptr, (const double*)nullptr, (double*)nullptr,
+                               (double*)nullptr);
+            hipLaunchKernelGGL(k_reduce, dim3(nv), dim3(TB), 0, s, part, (int)Gx, ldp, W.h1.as<double>(), g1);
+            hipLaunchKernelGGL((k_gs_upd<TBv, double, false>), dim3(Gx), dim3(TB), 0, s, B, ld, nv,
+                               W.h1.as<double>(), W.w64.as<double>(), W.w64.as<double>(), n, part, ldp,
+                               W.npart.as<double>(), g1);
+            hipLaunchKernelGGL(k_dgks, dim3(1), dim3(TB), 0, s, (const double*)W.npart.as<double>(),
+                               (const double*)W.npart2.as<double>(), (int)Gx, loc, gate, g1);
+            hipLaunchKernelGGL((k_gs_dots<TBv, double, false>), dim3(Gx), dim3(TB), 0, s, B, ld, nv,
+                               (const double*)W.w64.as<double>(), n, part, ldp, (const int*)gate, (const TBv*)nullptr,
+                               (const TBv*)nullptr, (const double*)nullptr, (const double*)nullptr, (double*)nullptr,
+                               (double*)nullptr);
+            hipLaunchKernelGGL(k_reduce, dim3(nv), dim3(TB), 0, s, part, (int)Gx, ldp, W.h2.as<double>(),
+                               (const int*)gate);
+            hipLaunchKernelGGL((k_gs_upd<TBv, double, false>), dim3(Gx), dim3(TB), 0, s, B, ld, nv,
+                               W.h2.as<double>(), W.w64.as<double>(), W.w64.as<double>(), n, part, ldp,
+                               W.npart.as<double>(), (const int*)gate);
+        }
         hipLaunchKernelGGL(k_finish_local, dim3(1), dim3(TB), 0, s, loc, it,
                            (const double*)W.npart.as<double>(), (int)Gx, W.alpha.as<double>(), W.beta.as<double>());
+    }
+
+    // the first half of the next advance() (partial path): a batch driver calls it for
+    // every member before their advance() calls, so their k_omega waits overlap
+    bool d_pre = false;
+    void partial_begin(int it) {
+        if (prec_hv == 0) reorth_partial_begin<double>(it, W.wH.as<double>());
+        else reorth_partial_begin<float>(it, W.wH.as<float>());
+    }
+    void pre_advance() {
+        if (d_pre || reorth_cgs2 || !partial) return;
+        partial_begin(d_it);
+        d_pre = true;
     }
 
     void enqueue_iteration(int it, int step, int nprev) {
@@ -1091,8 +1140,9 @@ struct Solver {
             hipLaunchKernelGGL(k_finish, dim3(1), dim3(TB), 0, s, W.h1.as<double>(), it, W.npart.as<double>(),
                                (int)Gx, W.alpha.as<double>(), W.beta.as<double>());
         } else if (partial) {
-            if (prec_hv == 0) reorth_partial<double>(it, W.wH.as<double>());
-            else reorth_partial<float>(it, W.wH.as<float>());
+            if (!d_pre) partial_begin(it);
+            d_pre = false;
+            reorth_partial_end(it);
         } else {
             if (prec_hv == 0) reorth_local<double>(it, W.wH.as<double>());
             else reorth_local<float>(it, W.wH.as<float>());
@@ -2039,9 +2089,17 @@ static void batch_run(DeviceHam& h, int cap, int nb, const BatchIO& io, const So
         if (io.hams) launch_hv_variants(hin.data(), prec, (int)order.size(), vin.data(), vout.data(), hv_stream);
         else launch_hv_set(h, prec, (int)order.size(), vin.data(), vout.data(), hv_stream);
         SM_HIP(hipEventRecord(evh, hv_stream));
-        std::vector<int> next;
+        std::vector<int> next, live;
         for (int b : active) {
-            SM_HIP(hipStreamWaitEvent(S[b]->stream(), evh, 0));
+            try {
+                SM_HIP(hipStreamWaitEvent(S[b]->stream(), evh, 0));
+                S[b]->pre_advance();
+                live.push_back(b);
+            } catch (...) {
+                fail(b);
+            }
+        }
+        for (int b : live) {
             try {
                 if (S[b]->advance()) {
                     S[b]->end(io.evals[b], io.evecs ? io.evecs[b] : nullptr);
