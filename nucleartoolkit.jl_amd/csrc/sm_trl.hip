@@ -442,7 +442,8 @@ __global__ __launch_bounds__(TB) void k_local_step(const TW* __restrict__ win, c
 // (one workgroup of TB threads; returns the gate of this step's pass in every thread)
 __device__ int omega_block(const double* __restrict__ alpha, const double* __restrict__ beta, int j,
                            const double* __restrict__ npart, int G, double* __restrict__ om, int ldo,
-                           double* __restrict__ loc, int* __restrict__ gate, double eps) {
+                           double* __restrict__ loc, int* __restrict__ gate, double eps,
+                           double* bj_out = nullptr) {
     __shared__ double red[TB / 64];
     __shared__ double mx[TB / 64];
     const int forced = gate[2];        // read before the barriers below; thread 0 rewrites it last
@@ -484,17 +485,24 @@ __device__ int omega_block(const double* __restrict__ alpha, const double* __res
         gate[2] = forced > 0 ? forced - 1 : (run ? 1 : 0);
         loc[5] += run;
         if (!run) { gate[0] = 0; loc[4] = 0.0; }
+        if (bj_out) *bj_out = bj;
     }
     return run;
 }
 
 // ... and, when hgate is set, the gate also to the host-visible *hgate (coherent host
-// memory): the host then launches the projection pass only for steps that run it
-__global__ void k_omega(const double* __restrict__ alpha, const double* __restrict__ beta, int j,
+// memory): the host then launches the projection pass only for steps that run it.  A
+// step without the pass also gets alpha[j] = loc[0] and beta[j] = |w'| here, the
+// values k_finish_local would write (the same sum of the same partials).
+__global__ void k_omega(double* __restrict__ alpha, double* __restrict__ beta, int j,
                         const double* __restrict__ npart, int G, double* __restrict__ om, int ldo,
                         double* __restrict__ loc, int* __restrict__ gate, double eps, int* hgate) {
-    const int run = omega_block(alpha, beta, j, npart, G, om, ldo, loc, gate, eps);
-    if (hgate && threadIdx.x == 0) *hgate = run;
+    __shared__ double bj;
+    const int run = omega_block(alpha, beta, j, npart, G, om, ldo, loc, gate, eps, &bj);
+    if (threadIdx.x == 0) {
+        if (!run) { alpha[j] = loc[0]; beta[j] = bj; }
+        if (hgate) *hgate = run;
+    }
 }
 
 // One projection pass of w (FP64, in place) against the basis B[0..nv) by one
@@ -1060,20 +1068,20 @@ struct Solver {
         hipLaunchKernelGGL(k_reduce, dim3(1), dim3(TB), 0, s, part, G2, ldp, loc, (const int*)nullptr);
         hipLaunchKernelGGL((k_local_step<TBv, TW>), dim3(Gx), dim3(TB), 0, s, win, (const TBv*)V(it), vp,
                            (const double*)loc, bp, W.w64.as<double>(), n, W.npart.as<double>(), W.npart2.as<double>());
-        hipLaunchKernelGGL(k_omega, dim3(1), dim3(TB), 0, s, (const double*)W.alpha.as<double>(),
-                           (const double*)W.beta.as<double>(), it, (const double*)W.npart.as<double>(), (int)Gx,
+        hipLaunchKernelGGL(k_omega, dim3(1), dim3(TB), 0, s, W.alpha.as<double>(),
+                           W.beta.as<double>(), it, (const double*)W.npart.as<double>(), (int)Gx,
                            W.om.as<double>(), lm + 2, loc, gate, eps_basis, step_ready ? W.hflag + 4 : nullptr);
         if (step_ready) SM_HIP(hipEventRecord(W.evg, s));
     }
 
     // The rest of the step: the projection pass and its DGKS second pass (both gated on
     // the device), then alpha/beta.  A batch member first waits for the gate of k_omega
-    // and launches the projection kernels only on steps that run it: those kernels do
-    // nothing on the other steps (about nine in ten), so leaving them out changes no
-    // result and saves seven dependent dispatches, each of which waits for a free CU
-    // slot behind the other solves' H·v; the wait itself overlaps its mates' steps and
-    // the other H·v.  A solve alone keeps them queued behind k_omega instead, so the
-    // host stays a step ahead of the device.
+    // and launches these kernels only on steps that run the pass: on the other steps
+    // (about nine in ten) they do nothing, or rewrite what k_omega wrote, so leaving
+    // them out changes no result and saves eight dependent dispatches, each of which
+    // waits for a free CU slot behind the other solves' H·v; the wait itself overlaps
+    // its mates' steps and the other H·v.  A solve alone keeps them queued behind
+    // k_omega instead, so the host stays a step ahead of the device.
     void reorth_partial_end(int it) {
         if (small) return;
         const int nv = it + 1;
@@ -1107,9 +1115,11 @@ struct Solver {
             hipLaunchKernelGGL((k_gs_upd<TBv, double, false>), dim3(Gx), dim3(TB), 0, s, B, ld, nv,
                                W.h2.as<double>(), W.w64.as<double>(), W.w64.as<double>(), n, part, ldp,
                                W.npart.as<double>(), (const int*)gate);
+            hipLaunchKernelGGL(k_finish_local, dim3(1), dim3(TB), 0, s, loc, it,
+                               (const double*)W.npart.as<double>(), (int)Gx, W.alpha.as<double>(),
+                               W.beta.as<double>());
         }
-        hipLaunchKernelGGL(k_finish_local, dim3(1), dim3(TB), 0, s, loc, it,
-                           (const double*)W.npart.as<double>(), (int)Gx, W.alpha.as<double>(), W.beta.as<double>());
+        // (no pass: k_omega has written alpha[it] and beta[it])
     }
 
     // the first half of the next advance() (partial path): a batch driver calls it for
