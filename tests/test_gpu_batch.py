@@ -7,6 +7,11 @@ the same solve run alone (the shared launches give each vector the one-vector
 kernel's result, and the rest of a step is the member's own): sd and pf bases
 (two-row and one-row kernel shapes), FP64 and mixed, 1..5 members, and members
 that restart at different steps (lm 40) or converge at different steps.
+
+Mixed batches take the projection passes of partial re-orthogonalisation only on
+the steps whose omega gate runs them (the host reads the gate), while a solve
+alone queues them gated on the device: the pass counts (n_reorth, and the DGKS
+second passes n_reorth2) must agree too, with passes on some steps and not on all.
 """
 import numpy as np
 import pytest
@@ -30,5 +35,8 @@ def test_batch_equals_solo(snt_dir, space, zv, nv, seed, precision):
             assert np.array_equal(vec, vec1)
             assert (st["iterations"], st["n_hv"], st["n_restart"]) == (st1["iterations"], st1["n_hv"],
                                                                         st1["n_restart"])
+            assert (st["n_reorth"], st["n_reorth2"]) == (st1["n_reorth"], st1["n_reorth2"])
+            if precision == "mixed":
+                assert 0 < st["n_reorth"] < st["n_hv"], st
         if "lm" in kw:
             assert all(o[2]["n_restart"] > 0 for o in out)
