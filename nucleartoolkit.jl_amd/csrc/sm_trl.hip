@@ -1070,18 +1070,19 @@ struct Solver {
                            (const double*)loc, bp, W.w64.as<double>(), n, W.npart.as<double>(), W.npart2.as<double>());
         hipLaunchKernelGGL(k_omega, dim3(1), dim3(TB), 0, s, W.alpha.as<double>(),
                            W.beta.as<double>(), it, (const double*)W.npart.as<double>(), (int)Gx,
-                           W.om.as<double>(), lm + 2, loc, gate, eps_basis, step_ready ? W.hflag + 4 : nullptr);
-        if (step_ready) SM_HIP(hipEventRecord(W.evg, s));
+                           W.om.as<double>(), lm + 2, loc, gate, eps_basis, d_hg ? W.hflag + 4 : nullptr);
+        if (d_hg) SM_HIP(hipEventRecord(W.evg, s));
     }
 
     // The rest of the step: the projection pass and its DGKS second pass (both gated on
-    // the device), then alpha/beta.  A batch member first waits for the gate of k_omega
-    // and launches these kernels only on steps that run the pass: on the other steps
-    // (about nine in ten) they do nothing, or rewrite what k_omega wrote, so leaving
-    // them out changes no result and saves eight dependent dispatches, each of which
-    // waits for a free CU slot behind the other solves' H·v; the wait itself overlaps
-    // its mates' steps and the other H·v.  A solve alone keeps them queued behind
-    // k_omega instead, so the host stays a step ahead of the device.
+    // the device), then alpha/beta.  A batch member in a round of one Hamiltonian (d_hg)
+    // first waits for the gate of k_omega and launches these kernels only on steps that
+    // run the pass: on the other steps (about nine in ten) they do nothing, or rewrite
+    // what k_omega wrote, so leaving them out changes no result and saves eight
+    // dependent dispatches, each of which waits for a free CU slot behind the other
+    // solves' H·v; the wait itself overlaps its mates' steps and the other H·v.  A solve
+    // alone, and a round of interaction variants, keeps them queued behind k_omega
+    // instead, so the host stays a step ahead of the device.
     void reorth_partial_end(int it) {
         if (small) return;
         const int nv = it + 1;
@@ -1090,7 +1091,7 @@ struct Solver {
         double* loc = W.loc.as<double>();
         int* gate = W.gate.as<int>();
         bool run = true;
-        if (step_ready) {
+        if (d_hg) {
             SM_HIP(hipEventSynchronize(W.evg));
             run = static_cast<volatile int*>(W.hflag)[4] != 0;
         }
@@ -1125,12 +1126,14 @@ struct Solver {
     // the first half of the next advance() (partial path): a batch driver calls it for
     // every member before their advance() calls, so their k_omega waits overlap
     bool d_pre = false;
+    bool d_hg = false;          // this step reads the gate on the host (set by pre_advance)
     void partial_begin(int it) {
         if (prec_hv == 0) reorth_partial_begin<double>(it, W.wH.as<double>());
         else reorth_partial_begin<float>(it, W.wH.as<float>());
     }
-    void pre_advance() {
+    void pre_advance(bool host_gate) {
         if (d_pre || reorth_cgs2 || !partial) return;
+        d_hg = host_gate;
         partial_begin(d_it);
         d_pre = true;
     }
@@ -1150,7 +1153,7 @@ struct Solver {
             hipLaunchKernelGGL(k_finish, dim3(1), dim3(TB), 0, s, W.h1.as<double>(), it, W.npart.as<double>(),
                                (int)Gx, W.alpha.as<double>(), W.beta.as<double>());
         } else if (partial) {
-            if (!d_pre) partial_begin(it);
+            if (!d_pre) { d_hg = false; partial_begin(it); }
             d_pre = false;
             reorth_partial_end(it);
         } else {
@@ -2096,6 +2099,12 @@ static void batch_run(DeviceHam& h, int cap, int nb, const BatchIO& io, const So
             vout.push_back(S[b]->hv_out());
             hin.push_back(mh[b]);
         }
+        // Members read the omega gate on the host in rounds of one Hamiltonian only.
+        // Measured (served, two slots of two): Cr48 10.69 -> 10.87 solves/s with the
+        // host-read gate; 64 interaction variants 13.69 -> 13.20 with it in every round
+        // and 13.48 with it only in rounds of one Hamiltonian.
+        bool same_h = true;
+        for (const DeviceHam* x : hin) same_h = same_h && x == hin[0];
         if (io.hams) launch_hv_variants(hin.data(), prec, (int)order.size(), vin.data(), vout.data(), hv_stream);
         else launch_hv_set(h, prec, (int)order.size(), vin.data(), vout.data(), hv_stream);
         SM_HIP(hipEventRecord(evh, hv_stream));
@@ -2103,7 +2112,7 @@ static void batch_run(DeviceHam& h, int cap, int nb, const BatchIO& io, const So
         for (int b : active) {
             try {
                 SM_HIP(hipStreamWaitEvent(S[b]->stream(), evh, 0));
-                S[b]->pre_advance();
+                S[b]->pre_advance(same_h);
                 live.push_back(b);
             } catch (...) {
                 fail(b);
