@@ -860,6 +860,27 @@ template <typename T> void set_view(DeviceIndex& h, int prec, const HostLayout& 
     std::stable_sort(grp.begin(), grp.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
     std::vector<int32_t> gb, gf;
     for (auto& x : grp) { gb.push_back(x.second.first); gf.push_back(x.second.second); }
+    // Workgroup b runs on XCD b % 8, and each XCD has its own L2.  Deal the LPT list to
+    // the XCDs in chunks of C consecutive groups (chunk j to XCD j % 8), so that an XCD
+    // runs runs of neighbouring groups, which stage many of the same input rows from its
+    // L2, while every XCD still takes its share of the list from the most expensive
+    // groups down.  The groups past the last full deal keep their place.  Measured
+    // (ms per launch, FP32): Cr48 one / two vectors 0.449 / 0.629 -> 0.438 / 0.610 and
+    // Mn50 6.01 -> 5.85 with C = 64; C = 128 and 256 were slower on Cr48 (G = 4,845).
+    {
+        const int G = (int)gb.size(), X = 8;
+        const int C = std::max(1, std::min(64, G / 64));
+        const int nfull = (G / (X * C)) * X * C;
+        std::vector<int32_t> b2(gb), f2(gf);
+        for (int b = 0; b < nfull; b++) {
+            const int x = b % X, k = b / X;
+            const int g = ((k / C) * X + x) * C + (k % C);
+            b2[b] = gb[g];
+            f2[b] = gf[g];
+        }
+        gb.swap(b2);
+        gf.swap(f2);
+    }
     h.grp_block[prec].upload(gb);
     h.grp_fp0[prec].upload(gf);
     A.G = (int)gb.size();
