@@ -604,7 +604,6 @@ template <typename T> std::vector<T> cast_vec(const std::vector<double>& v) {
 
 struct HostLayout {
     std::vector<int32_t> row_block, block_row0, nP, nN;
-    std::vector<double> row_cost;    // estimated FMAs per proton row (workgroup ordering)
     std::vector<int64_t> tdims, en_off;
     std::vector<double> ep, en;
     std::vector<int64_t> pp_rowptr;
@@ -792,26 +791,6 @@ void index_layout(const Basis& b, const Tables& t, HostLayout& L) {
         }
         L.pair_ptr.push_back((int32_t)L.pair_bi.size());
     }
-    // per-row cost estimate: pn hops x neutron-list entries, pp entries x nN, nn entries
-    L.row_cost.assign(L.row_block.size(), 0.0);
-    for (int bf = 0; bf < b.nb; bf++) {
-        const int nN = L.nN[bf];
-        for (int q = L.pair_ptr[bf]; q < L.pair_ptr[bf + 1]; q++) {
-            int pt = L.pair_ptab[q], nt = L.pair_ntab[q];
-            int64_t ne = 0;
-            int nsl = (nN + 63) / 64;
-            for (int s0 = 0; s0 < nsl; s0++) ne += 64 * (int64_t)L.ntab_slice_len[L.ntab_slice_base[nt] + s0];
-            const int32_t* prp = L.ptab_rowptr.data() + L.ptab_rowptr_base[pt];
-            for (int fp = 0; fp < L.nP[bf]; fp++) {
-                int nh = prp[fp + 1] - prp[fp];
-                L.row_cost[L.block_row0[bf] + fp] += (double)((nh + 3) / 4 * 4) * (double)ne;
-            }
-        }
-        for (int fp = 0; fp < L.nP[bf]; fp++) {
-            int r = L.block_row0[bf] + fp;
-            L.row_cost[r] += (double)(L.pp_rowptr[r + 1] - L.pp_rowptr[r]) * nN + 64.0 * nN;
-        }
-    }
 }
 
 
@@ -849,24 +828,21 @@ template <typename T> void set_view(DeviceIndex& h, int prec, const HostLayout& 
     A.npt = 0;
     for (int x : npts) if (x >= need) { A.npt = x; break; }
     if (A.npt == 0) throw std::runtime_error("neutron block wider than 4096 determinants");
-    // row groups, most expensive first (longest-processing-time order)
-    std::vector<std::pair<double, std::pair<int, int>>> grp;
+    // Row groups in (block, row) order: the rows of one block stage many of the same
+    // input rows.  Workgroup b runs on XCD b % 8, and each XCD has its own L2, so the
+    // list is dealt to the XCDs in chunks of C consecutive groups (chunk j to XCD j % 8):
+    // an XCD runs neighbouring rows of a block together.  The groups past the last full
+    // deal keep their place.  Measured (ms per launch, FP32, Cr48 one / two vectors;
+    // Mn50 one): longest-processing-time order (the reference's make_distribute)
+    // 0.449 / 0.629; 6.01; dealt in chunks of 64: 0.438 / 0.610; 5.85; block order
+    // dealt in chunks of 64: 0.411 / 0.572; 5.49.  C = 128 and 256 were slower on Cr48
+    // (G = 4,845) with the cost order.
+    std::vector<int32_t> gb, gf;
     for (int k = 0; k < h.nb; k++)
         for (int i = 0; i < L.nP[k]; i += A.rg) {
-            double c = 0.0;
-            for (int r = i; r < std::min(L.nP[k], i + A.rg); r++) c += L.row_cost[L.block_row0[k] + r];
-            grp.push_back({c, {k, i}});
+            gb.push_back(k);
+            gf.push_back(i);
         }
-    std::stable_sort(grp.begin(), grp.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
-    std::vector<int32_t> gb, gf;
-    for (auto& x : grp) { gb.push_back(x.second.first); gf.push_back(x.second.second); }
-    // Workgroup b runs on XCD b % 8, and each XCD has its own L2.  Deal the LPT list to
-    // the XCDs in chunks of C consecutive groups (chunk j to XCD j % 8), so that an XCD
-    // runs runs of neighbouring groups, which stage many of the same input rows from its
-    // L2, while every XCD still takes its share of the list from the most expensive
-    // groups down.  The groups past the last full deal keep their place.  Measured
-    // (ms per launch, FP32): Cr48 one / two vectors 0.449 / 0.629 -> 0.438 / 0.610 and
-    // Mn50 6.01 -> 5.85 with C = 64; C = 128 and 256 were slower on Cr48 (G = 4,845).
     {
         const int G = (int)gb.size(), X = 8;
         const int C = std::max(1, std::min(64, G / 64));
