@@ -174,64 +174,75 @@ struct Server {
         return k.str();
     }
 
+    // The index tables are cached per (orbits, Z_v, N_v, 2M).  The pp / nn tables
+    // keep only operators with a nonzero coefficient, so their sparsity depends on
+    // the interaction: a request whose pattern differs from the cached entry's gets
+    // an entry of its own, keyed by the basis and its pattern.
     void prepare(Request& r) {
         const int nthr = std::max(1, cfg.table_threads);
         Snt snt = read_snt(r.snt_path, r.nucleus, false);
         NucleusSpec ns = resolve_nucleus(snt, r.nucleus, r.twoM);
-        const std::string key = key_of(snt, ns);
+        const std::string base_key = key_of(snt, ns);
+        std::string key = base_key;
         std::shared_ptr<IndexEntry> e;
-        bool builder = false;
-        {
-            std::unique_lock<std::mutex> lk(m);
-            auto it = cache.find(key);
-            if (it == cache.end()) {
-                e = std::make_shared<IndexEntry>();
-                e->key = key;
-                cache[key] = e;
-                builder = true;
-                st.index_misses++;
-            } else {
-                e = it->second;
-                st.index_hits++;
-                entry_cv.wait(lk, [&] { return e->ready; });
-                if (e->code != SMGPU_OK) throw std::invalid_argument("index tables failed: " + e->error);
+        for (int pass = 0;; pass++) {
+            bool builder = false;
+            {
+                std::unique_lock<std::mutex> lk(m);
+                auto it = cache.find(key);
+                if (it == cache.end()) {
+                    e = std::make_shared<IndexEntry>();
+                    e->key = key;
+                    cache[key] = e;
+                    builder = true;
+                    st.index_misses++;
+                } else {
+                    e = it->second;
+                    st.index_hits++;
+                    entry_cv.wait(lk, [&] { return e->ready; });
+                    if (e->code != SMGPU_OK) throw std::invalid_argument("index tables failed: " + e->error);
+                }
             }
-        }
-        if (builder) {
-            try {
-                Basis b = build_basis(snt, ns.vp, ns.vn, ns.twoM);
-                if (b.mdim == 0) throw std::invalid_argument("empty M-scheme space (mdim = 0)");
-                Tables t = build_tables(snt, b, nthr);
-                auto hidx = pack_index(b, t);
-                auto coef = pack_coefficients(b, t);
-                std::lock_guard<std::mutex> lk(m);
-                e->ob[0] = t.ob[0];
-                e->ob[1] = t.ob[1];
-                e->basis = std::move(b);
-                e->hidx = hidx;
-                e->index_bytes = device_bytes(*hidx);
-                e->ready = true;
-                r.coef = coef;
-            } catch (...) {
-                std::string msg;
-                int32_t c = status_of_current_exception(msg);
-                std::lock_guard<std::mutex> lk(m);
-                e->code = c;
-                e->error = msg;
-                e->ready = true;
-                cache.erase(key);   // a later request retries
+            if (builder) {
+                try {
+                    Basis b = build_basis(snt, ns.vp, ns.vn, ns.twoM);
+                    if (b.mdim == 0) throw std::invalid_argument("empty M-scheme space (mdim = 0)");
+                    Tables t = build_tables(snt, b, nthr);
+                    auto hidx = pack_index(b, t);
+                    auto coef = pack_coefficients(b, t);
+                    std::lock_guard<std::mutex> lk(m);
+                    e->ob[0] = t.ob[0];
+                    e->ob[1] = t.ob[1];
+                    e->basis = std::move(b);
+                    e->hidx = hidx;
+                    e->index_bytes = device_bytes(*hidx);
+                    e->ready = true;
+                    r.coef = coef;
+                } catch (...) {
+                    std::string msg;
+                    int32_t c = status_of_current_exception(msg);
+                    std::lock_guard<std::mutex> lk(m);
+                    e->code = c;
+                    e->error = msg;
+                    e->ready = true;
+                    cache.erase(key);   // a later request retries
+                    entry_cv.notify_all();
+                    throw;
+                }
                 entry_cv.notify_all();
-                throw;
+                break;
             }
-            entry_cv.notify_all();
-        } else {
             Tables t;
             t.ob[0] = e->ob[0];
             t.ob[1] = e->ob[1];
             build_coefficients(snt, e->basis, t, nthr);
             r.coef = pack_coefficients(e->basis, t);
-            if (pattern_of(*r.coef) != pattern_of(*e->hidx))
-                throw std::invalid_argument("interaction has a different pp/nn sparsity pattern than the shared tables");
+            if (pattern_of(*r.coef) == pattern_of(*e->hidx)) break;
+            if (pass > 0) throw std::logic_error("index entry keyed by a sparsity pattern holds another pattern");
+            std::ostringstream k2;
+            k2 << base_key << "#pattern=" << std::hex << pattern_of(*r.coef);
+            key = k2.str();
+            r.coef.reset();
         }
         r.entry = e;
         r.coef_hash = content_hash(*r.coef);

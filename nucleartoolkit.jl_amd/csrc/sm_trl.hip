@@ -1191,16 +1191,16 @@ struct Solver {
         workspace_plan<TBv>(n, P, [&](DBuf Workspace::*m, int64_t b) { (W.*m).ensure((size_t)b); });
         W.host_sync_objects();
         SM_HIP(hipStreamSynchronize(W.side));     // a canceled solve may have left tests queued
+        // host sources of the asynchronous copies below live until the stream
+        // synchronisation at the end of setup()
         std::vector<double> e0(4 * 64, 1.0e4);
+        const double one = 1.0;              // om_{0,0}
         SM_HIP(hipMemcpyAsync(W.enring.p, e0.data(), e0.size() * 8, hipMemcpyHostToDevice, s));
         SM_HIP(hipMemsetAsync(W.loc.p, 0, W.loc.bytes, s));
         SM_HIP(hipMemsetAsync(W.gate.p, 0, W.gate.bytes, s));
         SM_HIP(hipMemsetAsync(W.om.p, 0, W.om.bytes, s));
         SM_HIP(hipMemsetAsync(W.flags.p, 0, W.flags.bytes, s));     // flags and k_ritz's ticket
-        {
-            const double one = 1.0;          // om_{0,0}
-            SM_HIP(hipMemcpyAsync(W.om.p, &one, 8, hipMemcpyHostToDevice, s));
-        }
+        SM_HIP(hipMemcpyAsync(W.om.p, &one, 8, hipMemcpyHostToDevice, s));
         // start vector: caller's, or seeded normal deviates generated on the device
         // (initialize_wf, :939-948); normalised in FP64
         if (v0_in) {

@@ -135,7 +135,7 @@ end
 
 # TRL + Ritz assembly; returns (en[1], Ritz vectors as mdim x num_ev or nothing, stats).
 # j2 (HamJ2 on the same basis, built with twoM = twoJ) and twoJ: J-projected TRL (doubleLanczos).
-function trl(H::Ham, num_ev::Integer; lm=300, ls=20, tol=1e-8, num_history=3, precision::Int32=MIXED,
+function trl(H::Ham, num_ev::Integer; lm=300, ls=20, tol=1e-8, num_history=3, precision::Int32=FP64,
              refine::Bool=true, seed::Integer=1234, vectors::Bool=false, in_wf=nothing,
              j2::Union{Ham,Nothing}=nothing, twoJ::Integer=0)
     d = default_params()
@@ -162,7 +162,7 @@ end
 
 # Thick-restart block Lanczos (TRBL) with block size q; in_wf: mdim x q start block
 function trbl(H::Ham, num_ev::Integer, q::Integer; lm=300, ls=20, tol=1e-8, num_history=3,
-              precision::Int32=MIXED, refine::Bool=true, seed::Integer=1234, vectors::Bool=false,
+              precision::Int32=FP64, refine::Bool=true, seed::Integer=1234, vectors::Bool=false,
               in_wf=nothing, j2::Union{Ham,Nothing}=nothing, twoJ::Integer=0)
     d = default_params()
     p = SolveParams(num_ev, lm, ls, num_history, tol, precision, refine, d.max_iter, UInt64(seed))
@@ -187,7 +187,7 @@ end
 # (smgpu_trl_solve_variants); member b starts from seed seeds[b].  Returns
 # [(evals, stats)] per member, each equal to trl(Hs[b], ...) with that seed.
 function trl_variants(Hs::Vector{Ham}, num_ev::Integer; seeds=[1234 + b for b in 0:length(Hs)-1], lm=300,
-                      ls=20, tol=1e-8, num_history=3, precision::Int32=MIXED, refine::Bool=true)
+                      ls=20, tol=1e-8, num_history=3, precision::Int32=FP64, refine::Bool=true)
     nb = length(Hs)
     @assert length(seeds) == nb
     d = default_params()
@@ -219,7 +219,7 @@ end
 
 # mdimmode=true prints the dimension line and returns nothing (:129-130)
 function main_sm(sntf, target_nuc, num_ev, target_J; num_history=3, lm=300, ls=20, tol=1e-8,
-                 precision::Int32=MIXED, device::Integer=0, q::Integer=1, is_block::Bool=false,
+                 precision::Int32=FP64, device::Integer=0, q::Integer=1, is_block::Bool=false,
                  mdimmode::Bool=false)
     length(target_J) <= 1 || error("Multiple J is not supported")
     tJ = isempty(target_J) ? -1 : Int(target_J[1])

@@ -22,6 +22,7 @@ from smgpu import synth  # noqa: E402
 CFG = {
     "mg24": ("sd", 4, 4, 1, (-4.0, 0.0), 1.5, 5),
     "cr48": ("pf", 4, 4, 2, (-8.0, 0.0), 1.0, 10),
+    "mn50": ("pf", 5, 5, 3, (-8.0, 0.0), 1.0, 5),
 }
 
 

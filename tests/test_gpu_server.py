@@ -222,3 +222,46 @@ def test_server_batches_interaction_variants(snt_dir, precision):
     for (ev, info), ref in zip(res, alone[:6]):
         np.testing.assert_allclose(ev, ref, atol=TOL_MIX)
     assert st["batched"] == 0, st
+
+
+def _zero_like_particle(src, dst):
+    """Copy of a synthetic interaction file with every pp and nn TBME set to zero (pn
+    and SPEs kept): on the same basis its like-particle pair tables are empty, a
+    different sparsity pattern from the original's.  TBME lines are the lines of six
+    fields (a b c d J V); the first line without a comment gives lp."""
+    lines = open(src).read().splitlines()
+    lp = int(next(ln for ln in lines if not ln.lstrip().startswith("!")).split()[0])
+    out = []
+    for ln in lines:
+        f = ln.split()
+        if len(f) == 6 and not ln.lstrip().startswith("!"):
+            a, b, c, d = (int(x) for x in f[:4])
+            if (a <= lp) == (b <= lp) == (c <= lp) == (d <= lp):
+                f[5] = "0.0000000000e+00"
+                ln = "  " + "  ".join(f)
+        out.append(ln)
+    open(dst, "w").write("\n".join(out) + "\n")
+    return dst
+
+
+def test_server_interactions_with_other_sparsity_on_one_basis(snt_dir):
+    """Two interactions on one basis whose pp / nn tables differ in sparsity (one has
+    every like-particle TBME zero): both requests are served, each through index
+    tables of its own pattern, and each equals the request solved alone."""
+    path, nuc = synth.config_snt(f"{snt_dir}/pat_a.snt", "sd", 2, 3, 8100, (-6, 0), 1.2)
+    zero = _zero_like_particle(path, f"{snt_dir}/pat_b.snt")
+    pa, pb = smgpu.Problem(path, nuc), smgpu.Problem(zero, nuc)
+    assert pa.mdim == pb.mdim
+    assert pb.pair_matrix(0)[0].size == 0 and pa.pair_matrix(0)[0].size > 0
+    alone = [_alone(p, nuc, 4, 5, "mixed") for p in (path, zero)]
+    with Server(max_inflight=2) as srv:
+        ids = [srv.submit(p, nuc, 4, precision="mixed", seed=5) for p in (path, zero, zero, path)]
+        for rid in ids:
+            assert srv.wait(rid, 60000)
+        res = [srv.result(rid) for rid in ids]
+        st = srv.stats()
+    assert st["failed"] == 0 and st["index_misses"] == 2, st
+    for (ev, info), a in zip(res, [alone[0], alone[1], alone[1], alone[0]]):
+        assert info["state"] == "done"
+        assert np.abs(ev - a).max() <= TOL_MIX, ev - a
+    assert np.abs(alone[0] - alone[1]).max() > 1e-3
