@@ -83,7 +83,8 @@ struct HvTables {
     int pch;        // hop slots per row and staging round (multiple of 4)
     int nN_max;
     int cols2_max;
-    int npt;        // ceil(nN_max / 256)
+    int npt;        // columns per lane: ceil(nN_max / tpb), rounded up to an instance
+    int tpb;        // threads per workgroup (256, 512 or 1024)
     size_t lds_bytes;
 };
 

@@ -17,6 +17,7 @@
 // The reference's loop nest for the same sums is operate_H!
 // (src/ShellModel/lanczos_methods.jl:153-248).
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -62,14 +63,15 @@ template <> struct Vec4<double> { using type = double4; };
 // so that a lane fetches the four hops of a row for its neutron hop (in, col)
 // with one 4-wide LDS load per vector and one for V_eff, at compile-time offsets.
 // Unused slots are zero.
-template <typename T, int NPT, int RG, int PCH, int NV, bool PV>
-__global__ __launch_bounds__(256) void k_hv(HvTables A, HvVecs P) {
+template <typename T, int NPT, int RG, int PCH, int NV, bool PV, int TPB>
+__global__ __launch_bounds__(TPB) void k_hv(HvTables A, HvVecs P) {
     static_assert(PCH % 4 == 0, "slots come in groups of four");
     constexpr int NG = PCH / 4;       // slot groups per row and round
     constexpr int SL = RG * PCH;      // slots per round
     using T4 = typename Vec4<T>::type;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     constexpr int NC = PV ? NV : 1;  // coefficient sets (PV: one per vector)
+    constexpr int NW = TPB / 64;     // waves per workgroup
     T* X = reinterpret_cast<T*>(smem_raw);                      // nN_max * NV * SL
     T* Vs = X + (size_t)A.nN_max * SL * NV;                      // NC * cols2_max * SL
     T* xs = Vs + (size_t)A.cols2_max * SL * NC;                  // nN_max * RG * NV  ([col][row][vec])
@@ -87,7 +89,7 @@ __global__ __launch_bounds__(256) void k_hv(HvTables A, HvVecs P) {
     auto cpp = [&](int b) { return static_cast<const T*>(PV ? P.pp[b] : A.pp_val); };
     auto cnn = [&](int b) { return static_cast<const T*>(PV ? P.nn[b] : A.nn_val); };
     auto cve = [&](int b) { return static_cast<const T*>(PV ? P.veff[b] : A.veff); };
-    for (int i = tid; i < nN; i += 256) {
+    for (int i = tid; i < nN; i += TPB) {
 #pragma unroll
         for (int r = 0; r < RG; r++)
 #pragma unroll
@@ -104,7 +106,7 @@ __global__ __launch_bounds__(256) void k_hv(HvTables A, HvVecs P) {
 #pragma unroll
         for (int c = 0; c < NC; c++) epr[r][c] = r < nr ? cep(c)[row0 + r] : T(0);
     const int64_t nsb = A.nn_slice_base[bf];
-    // Column k of a lane is fn = tid + 256 k, in slice (tid >> 6) + 4 k: the same for
+    // Column k of a lane is fn = tid + TPB k, in slice (tid >> 6) + NW k: the same for
     // the whole wave, so slice tests are wave-uniform.  Lanes past nN inside the
     // last slice work on the clamped column nN - 1 (valid addresses; their results
     // are never stored) and read the zero padding of the ELL tables.
@@ -117,9 +119,9 @@ __global__ __launch_bounds__(256) void k_hv(HvTables A, HvVecs P) {
         for (int r = 0; r < RG; r++)
 #pragma unroll
             for (int b = 0; b < NV; b++) acc[k][r][b] = T(0);
-        const int sk = wave + 4 * k;
+        const int sk = wave + NW * k;
         if (sk < nsl) {
-            const int fnc = min(tid + k * 256, nN - 1);
+            const int fnc = min(tid + k * TPB, nN - 1);
             T e_n[NC];
 #pragma unroll
             for (int c = 0; c < NC; c++) e_n[c] = cen(c)[fnc];
@@ -181,8 +183,8 @@ __global__ __launch_bounds__(256) void k_hv(HvTables A, HvVecs P) {
                     }
 #pragma unroll
                     for (int k = 0; k < NPT; k++) {
-                        if (wave + 4 * k >= nsl) continue;          // wave-uniform
-                        const int fnc = min(tid + k * 256, nN - 1);
+                        if (wave + NW * k >= nsl) continue;         // wave-uniform
+                        const int fnc = min(tid + k * TPB, nN - 1);
 #pragma unroll
                         for (int b = 0; b < NV; b++) {
                             T x[UP];
@@ -219,7 +221,7 @@ __global__ __launch_bounds__(256) void k_hv(HvTables A, HvVecs P) {
         const int64_t vbase = A.veff_off[dm];
         const int c2 = A.veff_cols2[dm];
         const int64_t sb = A.ntab_slice_base[nt];
-        // column k of a lane is fn = tid + 256 k: slice (tid >> 6) + 4 k, the same
+        // column k of a lane is fn = tid + TPB k: slice (tid >> 6) + NW k, the same
         // for the whole wave, so slice lengths (and every test on them) are
         // wave-uniform; lanes past nN inside the last slice read its zero-column
         // padding, waves past the last slice get length 0 and skip all work
@@ -228,7 +230,7 @@ __global__ __launch_bounds__(256) void k_hv(HvTables A, HvVecs P) {
         int maxlen = 0;
 #pragma unroll
         for (int k = 0; k < NPT; k++) {
-            const int sk = wave + 4 * k;
+            const int sk = wave + NW * k;
             if (sk < nsl) {
                 const int64_t s = sb + sk;
                 off[k] = A.ntab_slice_off[s] + (tid & 63);
@@ -269,7 +271,7 @@ __global__ __launch_bounds__(256) void k_hv(HvTables A, HvVecs P) {
                         }
                     }
                     const int so = r * PCH + g * 4;
-                    for (int i = tid; i < nNi; i += 256) {
+                    for (int i = tid; i < nNi; i += TPB) {
 #pragma unroll
                         for (int b = 0; b < NV; b++) {
                             const T* vb = static_cast<const T*>(P.v[b]);
@@ -285,7 +287,7 @@ __global__ __launch_bounds__(256) void k_hv(HvTables A, HvVecs P) {
                     for (int cc = 0; cc < NC; cc++) {
                         const T* V2 = cve(cc) + vbase;
                         T* Vd = Vs + (size_t)cc * A.cols2_max * SL;
-                        for (int c = tid; c < c2; c += 256) {
+                        for (int c = tid; c < c2; c += TPB) {
                             T4 v4;
                             v4.x = vm[0] * V2[vo[0] + c];
                             v4.y = vm[1] * V2[vo[1] + c];
@@ -433,7 +435,7 @@ __global__ __launch_bounds__(256) void k_hv(HvTables A, HvVecs P) {
                 T* dst = static_cast<T*>(P.w[b]) + bstart + (int64_t)(fp0 + r) * nN;
 #pragma unroll
                 for (int k = 0; k < NPT; k++) {
-                    const int fn = tid + k * 256;
+                    const int fn = tid + k * TPB;
                     if (fn < nN) dst[fn] = acc[k][r][b];
                 }
             }
@@ -454,22 +456,30 @@ static void raise_lds_limit(const void* fn, size_t bytes) {
     cur = bytes;
 }
 
-// instantiated shapes (NPT, RG, PCH, NV, PV): one vector for every width; two and
-// four vectors on one Hamiltonian (four up to 1,536 neutron determinants per block);
-// two interaction variants on one index (PV)
-#define HV_NPTS(X_, T_, R_, P_, V_, W_) X_(T_, 1, R_, P_, V_, W_) X_(T_, 2, R_, P_, V_, W_) \
-    X_(T_, 3, R_, P_, V_, W_) X_(T_, 4, R_, P_, V_, W_) X_(T_, 6, R_, P_, V_, W_) X_(T_, 8, R_, P_, V_, W_) \
-    X_(T_, 12, R_, P_, V_, W_) X_(T_, 16, R_, P_, V_, W_)
-#define HV_NPTS_NARROW(X_, T_, R_, P_, V_, W_) X_(T_, 1, R_, P_, V_, W_) X_(T_, 2, R_, P_, V_, W_) \
-    X_(T_, 3, R_, P_, V_, W_) X_(T_, 4, R_, P_, V_, W_) X_(T_, 6, R_, P_, V_, W_)
+// instantiated shapes (NPT, RG, PCH, NV, PV, TPB): 256-thread workgroups for every
+// width, one vector, two and four vectors on one Hamiltonian (four up to 1,536
+// neutron determinants per block), two interaction variants on one index (PV);
+// 512- and 1,024-thread workgroups (one row each) for wide neutron blocks, where
+// the staged input rows of one workgroup fill most of a CU's LDS and more waves
+// then share them
+#define HV_NPTS(X_, T_, R_, P_, V_, W_) X_(T_, 1, R_, P_, V_, W_, 256) X_(T_, 2, R_, P_, V_, W_, 256) \
+    X_(T_, 3, R_, P_, V_, W_, 256) X_(T_, 4, R_, P_, V_, W_, 256) X_(T_, 6, R_, P_, V_, W_, 256) \
+    X_(T_, 8, R_, P_, V_, W_, 256) X_(T_, 12, R_, P_, V_, W_, 256) X_(T_, 16, R_, P_, V_, W_, 256)
+#define HV_NPTS_NARROW(X_, T_, R_, P_, V_, W_) X_(T_, 1, R_, P_, V_, W_, 256) X_(T_, 2, R_, P_, V_, W_, 256) \
+    X_(T_, 3, R_, P_, V_, W_, 256) X_(T_, 4, R_, P_, V_, W_, 256) X_(T_, 6, R_, P_, V_, W_, 256)
+#define HV_NPTS_WIDE(X_, T_, V_, W_) X_(T_, 2, 1, 4, V_, W_, 512) X_(T_, 3, 1, 4, V_, W_, 512) \
+    X_(T_, 4, 1, 4, V_, W_, 512) X_(T_, 6, 1, 4, V_, W_, 512) X_(T_, 8, 1, 4, V_, W_, 512) \
+    X_(T_, 2, 1, 4, V_, W_, 1024) X_(T_, 3, 1, 4, V_, W_, 1024) X_(T_, 4, 1, 4, V_, W_, 1024)
 #define HV_FOR_SHAPES(X_, T_) HV_NPTS(X_, T_, 2, 4, 1, false) HV_NPTS(X_, T_, 1, 4, 1, false) \
     HV_NPTS(X_, T_, 2, 4, 2, false) HV_NPTS(X_, T_, 1, 4, 2, false) \
     HV_NPTS_NARROW(X_, T_, 2, 4, 4, false) HV_NPTS_NARROW(X_, T_, 1, 4, 4, false) \
-    HV_NPTS(X_, T_, 2, 4, 2, true) HV_NPTS(X_, T_, 1, 4, 2, true)
+    HV_NPTS(X_, T_, 2, 4, 2, true) HV_NPTS(X_, T_, 1, 4, 2, true) \
+    HV_NPTS_WIDE(X_, T_, 1, false) HV_NPTS_WIDE(X_, T_, 2, false) HV_NPTS_WIDE(X_, T_, 2, true)
 
-template <typename T> static const void* hv_fn(int npt, int rg, int pch, int nv, bool pv) {
-#define HV_CASE(T_, N_, R_, P_, V_, W_) \
-    if (npt == N_ && rg == R_ && pch == P_ && nv == V_ && pv == W_) return (const void*)k_hv<T_, N_, R_, P_, V_, W_>;
+template <typename T> static const void* hv_fn(int npt, int rg, int pch, int nv, bool pv, int tpb) {
+#define HV_CASE(T_, N_, R_, P_, V_, W_, B_)                                                       \
+    if (npt == N_ && rg == R_ && pch == P_ && nv == V_ && pv == W_ && tpb == B_)                   \
+        return (const void*)k_hv<T_, N_, R_, P_, V_, W_, B_>;
     HV_FOR_SHAPES(HV_CASE, T)
 #undef HV_CASE
     return nullptr;
@@ -490,21 +500,22 @@ static constexpr size_t HV_LDS_SHARED = 32 * 1024;
 template <typename T> static int hv_width(const HvTables& A, int nv, bool pv) {
     for (int c : {4, 2}) {
         if (c > nv) continue;
-        if (hv_fn<T>(A.npt, A.rg, A.pch, c, pv) && hv_lds_bytes(A, c, pv, sizeof(T)) <= HV_LDS_SHARED) return c;
+        if (hv_fn<T>(A.npt, A.rg, A.pch, c, pv, A.tpb) && hv_lds_bytes(A, c, pv, sizeof(T)) <= HV_LDS_SHARED)
+            return c;
     }
     return 1;
 }
 
 template <typename T>
 static void launch_t(const HvTables& A, int nv, bool pv, const HvVecs& P, hipStream_t s) {
-    dim3 grid(A.G), block(256);
+    dim3 grid(A.G), block(A.tpb);
     if (nv == 1) pv = false;
     const size_t lds = hv_lds_bytes(A, nv, pv, sizeof(T));
-    if (nv > 1) raise_lds_limit(hv_fn<T>(A.npt, A.rg, A.pch, nv, pv), lds);
+    if (nv > 1) raise_lds_limit(hv_fn<T>(A.npt, A.rg, A.pch, nv, pv, A.tpb), lds);
     bool done = false;
-#define HV_LAUNCH(T_, N_, R_, P_, V_, W_)                                                                    \
-    if (!done && A.npt == N_ && A.rg == R_ && A.pch == P_ && nv == V_ && pv == W_) {                         \
-        hipLaunchKernelGGL((k_hv<T_, N_, R_, P_, V_, W_>), grid, block, lds, s, A, P);                      \
+#define HV_LAUNCH(T_, N_, R_, P_, V_, W_, B_)                                                                \
+    if (!done && A.npt == N_ && A.rg == R_ && A.pch == P_ && nv == V_ && pv == W_ && A.tpb == B_) {          \
+        hipLaunchKernelGGL((k_hv<T_, N_, R_, P_, V_, W_, B_>), grid, block, lds, s, A, P);                  \
         done = true;                                                                                         \
     }
     HV_FOR_SHAPES(HV_LAUNCH, T)
@@ -821,13 +832,37 @@ template <typename T> void set_view(DeviceIndex& h, int prec, const HostLayout& 
     // staging rounds, are no faster and cost LDS occupancy)
     A.rg = lds_for(2, 4) <= HV_LDS_TWO_ROWS ? 2 : 1;
     A.pch = 4;
+    const int npts[] = {1, 2, 3, 4, 6, 8, 12, 16};
+    auto npt_for = [&](int tpb) {
+        const int need = (A.nN_max + tpb - 1) / tpb;
+        for (int x : npts) if (x >= need) return x;
+        return 0;
+    };
+    // Workgroup size: blocks up to 1,024 neutron determinants run 256 threads (the
+    // LDS of several workgroups fits a CU); wider blocks run one row per 512- or
+    // 1,024-thread workgroup, whose 8 or 16 waves share one copy of the staged input
+    // rows (with 256 threads the FP64 staging of a 3,952-wide block leaves one 4-wave
+    // workgroup per CU).  Measured, ms per launch FP32 / FP64 at 256, 512, 1024
+    // threads (results bit-identical): Mn50 (widest block 1,651) 5.49 5.42 6.92 /
+    // 12.4 8.84 11.6; Mn51 (3,952) 22.5 14.0 14.8 / 50.1 33.9 27.8; Fe52 (3,952)
+    // 65.0 40.0 41.8 / 144 96.0 76.7; Cr48 (565) 0.432 0.538 - / 0.708 0.816 -.
+    // SMGPU_HV_TPB = 256 / 512 / 1024 overrides the choice where an instance exists.
+    A.tpb = 256;
+    if (A.nN_max > 1024) A.tpb = (sizeof(T) == 8 && A.nN_max > 2048) ? 1024 : 512;
+    if (const char* e = std::getenv("SMGPU_HV_TPB")) {
+        const int want = std::atoi(e);
+        if (want == 256 || want == 512 || want == 1024) A.tpb = want;
+    }
+    if (A.tpb > 256) A.rg = 1;
+    A.npt = npt_for(A.tpb);
+    if (A.tpb > 256 && !hv_fn<T>(A.npt, 1, 4, 1, false, A.tpb)) {
+        A.tpb = 256;
+        A.rg = lds_for(2, 4) <= HV_LDS_TWO_ROWS ? 2 : 1;
+        A.npt = npt_for(256);
+    }
+    if (A.npt == 0) throw std::runtime_error("neutron block wider than 4096 determinants");
     A.lds_bytes = lds_for(A.rg, A.pch);
     if (A.lds_bytes > 160 * 1024) throw std::runtime_error("neutron block too wide for the LDS of one CU");
-    int need = (A.nN_max + 255) / 256;
-    const int npts[] = {1, 2, 3, 4, 6, 8, 12, 16};
-    A.npt = 0;
-    for (int x : npts) if (x >= need) { A.npt = x; break; }
-    if (A.npt == 0) throw std::runtime_error("neutron block wider than 4096 determinants");
     // Row groups in (block, row) order: the rows of one block stage many of the same
     // input rows.  Workgroup b runs on XCD b % 8, and each XCD has its own L2, so the
     // list is dealt to the XCDs in chunks of C consecutive groups (chunk j to XCD j % 8):
@@ -862,7 +897,7 @@ template <typename T> void set_view(DeviceIndex& h, int prec, const HostLayout& 
     A.G = (int)gb.size();
     A.grp_block = h.grp_block[prec].as<int32_t>();
     A.grp_fp0 = h.grp_fp0[prec].as<int32_t>();
-    const void* fn = hv_fn<T>(A.npt, A.rg, A.pch, 1, false);
+    const void* fn = hv_fn<T>(A.npt, A.rg, A.pch, 1, false, A.tpb);
     if (!fn) throw std::runtime_error("no H·v kernel instance for this shape");
     raise_lds_limit(fn, A.lds_bytes);
 }
