@@ -1,24 +1,39 @@
 #!/usr/bin/env python3
-"""Benchmark: converged eigensolves/s (+ p50/p99 latency) of the MI355X shell-model
-eigensolver on the pf-shell Cr48 workload (4 protons + 4 neutrons, 2M = 0,
-mdim 1,963,461, lowest 10 eigenpairs, tol 1e-8, lm 300, ls 20).
+"""Benchmark: converged eigensolves/s + p50/p99 latency of the MI355X shell-model
+eigensolver against the plain FP64 TRL, one solve at a time.
 
-Measured form: the serving loop (smgpu_server_*) with FP32 H·v and FP64
-recurrence / reorthogonalisation / eigensolves / Rayleigh-quotient refinement,
-`--inflight` solves overlapping on the device.  One step = one request: parse
-the interaction file, build its coefficients (index tables shared and resident
-after the warm-up), solve to convergence; the K requests are submitted at once
-and timed from the first submit to the last completion.  Every result is
-checked against the plain FP64 solve (|ΔE| <= 1e-5 MeV); only those within the
-tolerance count.  Beside it: the plain FP64 GPU path one solve at a time
-(tables resident, solve only), H·v timing and roofline, and the oracle's CPU
-restatement timed on a bounded sample.
+Default workload (the serving stream the metric is quoted on): 512 eigensolve
+requests with Poisson arrivals at 50/s; each request picks the sd or pf space
+uniformly, proton and neutron valence numbers uniformly in 2..6, 2M = A % 2,
+k uniformly in 1..10, and carries its own seeded synthetic interaction (SPEs
+U[-6,0] MeV, TBMEs N(0,1.2 MeV)); bases range from mdim 640 (sd 2p2n) to
+109,954,620 (pf 6p6n).  Each request is one main_sm solve (lm 300, ls 20,
+tol 1e-8).
 
-usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config cr48|mg24|mn50] [--inflight M]
+  served leg : the serving loop (smgpu_server_*) with FP32 H·v and FP64
+               recurrence / small eigensolves / Rayleigh-quotient refinement,
+               requests submitted at their arrival times, `--inflight` solves
+               overlapping on the device.  `value` = requests whose every
+               eigenvalue agrees with the FP64 leg within 1e-5 MeV, divided by
+               the time from the first arrival to the last result.
+  FP64 leg   : the same requests, same arrival times, through a one-slot server
+               in plain FP64 (every vector FP64, full reorthogonalisation every
+               step): the plain FP64 TRL one solve at a time.
+Latencies of both legs are arrival-to-result.  Beside them: the H·v kernel's
+timing and rooflines on the stream's largest basis (HIP events on the launch
+stream) and the oracle's CPU restatement of operate_H!/TRL timed on this host.
+
+The stream's size is fixed by the workload (512 requests); --steps only sizes
+the other workloads.  Other workloads: --workload served (one configuration
+served in a burst), variants (64 interaction variants of Cr48), --mode hv
+(H·v launches only, for profiler runs).
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload stream|served|variants]
 """
 import argparse
 import json
 import os
+import platform
 import sys
 import tempfile
 import time
@@ -30,12 +45,16 @@ import numpy as np  # noqa: E402
 
 METRIC = "converged eigensolves/s + p50/p99 latency vs plain FP64 TRL at |ΔE|≤1e-5 MeV"
 HBM_PEAK_GBS = 8000.0
+# LDS: 256 CUs x 128 bytes per clock x 2.4 GHz
+LDS_PEAK_GBS = 256 * 128 * 2.4
+TOL_MEV = 1e-5
 
 CONFIGS = {
     # name: (space, zv, nv, seed offset, spe range, sigma, num_ev, lm, ls)
     "mg24": ("sd", 4, 4, 1, (-4.0, 0.0), 1.5, 5, 300, 20),
     "cr48": ("pf", 4, 4, 2, (-8.0, 0.0), 1.0, 10, 300, 20),
     "mn50": ("pf", 5, 5, 3, (-8.0, 0.0), 1.0, 5, 100, 20),
+    "fe52": ("pf", 6, 6, 6, (-6.0, 0.0), 1.2, 5, 300, 20),
 }
 
 
@@ -43,85 +62,369 @@ def percentile(x, q):
     return float(np.percentile(np.array(x), q)) if len(x) else None
 
 
-def cpu_baseline(snt, nuc, n_iter_fp64, avg_basis, budget_s=20.0):
-    """Oracle (C operate_H! + numpy MGS ReORTH) timed on this host, extrapolated
-    to one full FP64 solve with the iteration count of the GPU FP64 run."""
+def host_cpu():
+    model = platform.processor() or ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"model": model, "cores_visible": len(os.sched_getaffinity(0)), "cores_online": os.cpu_count()}
+
+
+# ----------------------------------------------------------------------------- CPU
+def _oracle():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import sm_oracle as orc
-    threads = min(16, os.cpu_count() or 1)
+    return orc
+
+
+def _cpu_hv_seconds(orc, snt, nuc, reps, threads, warmups=3):
     orc.set_threads(threads)
-    t0 = time.perf_counter()
     o = orc.OracleHamiltonian(snt, nuc)
-    t_build = time.perf_counter() - t0
-    rng = np.random.default_rng(0)
-    v = rng.standard_normal(o.mdim)
-    o.operate_H(v)                      # warm-up
-    n_hv = 0
-    t0 = time.perf_counter()
-    while True:
+    v = np.random.default_rng(0).standard_normal(o.mdim)
+    for _ in range(warmups):
         o.operate_H(v)
-        n_hv += 1
-        if time.perf_counter() - t0 > budget_s / 2 or n_hv >= 20:
-            break
-    t_hv = (time.perf_counter() - t0) / n_hv
-    # ReORTH sample: MGS of one vector against k basis vectors (shellmodel_main.jl:1005-1012)
-    k = 16
-    V = rng.standard_normal((k, o.mdim))
-    w = rng.standard_normal(o.mdim)
-    t0 = time.perf_counter()
-    for l in range(k - 1, -1, -1):
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        o.operate_H(v)
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), o
+
+
+def _mgs_seconds_per_element(n=2_000_000, k=16):
+    rng = np.random.default_rng(1)
+    V = rng.standard_normal((k, n))
+    w = rng.standard_normal(n)
+    for l in range(k - 1, -1, -1):           # warm-up
         w -= np.dot(V[l], w) * V[l]
-    t_re_per_vec = (time.perf_counter() - t0) / k
-    t_iter = t_hv + t_re_per_vec * avg_basis
-    t_solve = n_iter_fp64 * t_iter
-    return {"value": 1.0 / t_solve, "unit": "solves/s", "cores": threads, "kind": "port",
-            "sample": (f"oracle C operate_H! x{n_hv} ({t_hv * 1e3:.1f} ms each, {threads} OpenMP threads) + "
-                       f"numpy MGS ReORTH against {k} vectors ({t_re_per_vec * 1e3:.2f} ms/vector), "
-                       f"extrapolated to {n_iter_fp64} Lanczos iterations with {avg_basis:.0f} basis vectors "
-                       f"on average (the GPU FP64 run's counts); table build {t_build:.1f} s not included"),
-            "hv_per_s": 1.0 / t_hv}
+    t0 = time.perf_counter()
+    for l in range(k - 1, -1, -1):           # ReORTH (shellmodel_main.jl:1005-1012)
+        w -= np.dot(V[l], w) * V[l]
+    return (time.perf_counter() - t0) / (k * n)
 
 
-def hv_only(ham, args):
-    """H·v launches only (FP32 then FP64), and a 256 MiB D2D copy x3."""
-    import ctypes
-    from smgpu._lib import check, lib
-    t32 = ham.time_operate_H("mixed", args.hv_reps)
-    t64 = ham.time_operate_H("fp64", args.hv_reps)
-    nbytes = 256 << 20
-    a, b = ctypes.c_void_p(), ctypes.c_void_p()
-    check(lib().smgpu_malloc(ham._h, nbytes, ctypes.byref(a)))
-    check(lib().smgpu_malloc(ham._h, nbytes, ctypes.byref(b)))
-    for _ in range(3):
-        check(lib().smgpu_memcpy(ham._h, b, a, nbytes, 2))
-    check(lib().smgpu_free(ham._h, a))
-    check(lib().smgpu_free(ham._h, b))
+def cpu_baseline_stream(reqs, fp64_res, work, budget_s=25.0):
+    """The oracle's restatement (C operate_H! with OpenMP over blocks, numpy MGS
+    ReORTH) on this host.  Measured: (a) one full FP64 TRL solve of the C1 workload
+    (Mg24, lowest 5) end to end; (b) the median operate_H! of the stream's bases up
+    to 2e6 amplitudes, 3 warm-ups + >= 3 timed calls each, on all visible cores and
+    for the largest of them on 1 core; (c) numpy MGS per basis element.  The stream
+    rate extrapolates (b) and (c) to every request: seconds per FMA term from the
+    measured bases (their median) times the request's terms per H·v and the FP64
+    leg's H·v count, plus its ReORTH elements."""
+    orc = _oracle()
+    from smgpu import synth
+    cores = len(os.sched_getaffinity(0))
+    out = {"kind": "port", "cores": cores, "cpu": host_cpu(), "unit": "solves/s"}
+    t_start = time.perf_counter()
+    # (a) full C1 solve
+    tmp = tempfile.mkdtemp(prefix="smgpu_cpu_")
+    snt, nuc = synth.config_snt(os.path.join(tmp, "mg24.snt"), "sd", 4, 4, synth.BASE_SEED + 1, (-4.0, 0.0), 1.5)
+    orc.set_threads(cores)
+    o = orc.OracleHamiltonian(snt, nuc)
+    t0 = time.perf_counter()
+    en, elit, nhv, nres, _, _ = orc.trl(o, 5)
+    out["c1_full_solve"] = {"nucleus": nuc, "mdim": int(o.mdim), "seconds": time.perf_counter() - t0,
+                            "iterations": int(elit), "n_hv": int(nhv), "eigenvalues_MeV": [float(x) for x in en]}
+    del o
+    # (b) H·v on the stream's bases up to 2e6 amplitudes, largest first, within the budget
+    seen, per_term = {}, []
+    cand = sorted({(r.space, r.zv, r.nv): r for r in reqs}.values(),
+                  key=lambda r: -work[(r.space, r.zv, r.nv)]["mdim"])
+    cand = [r for r in cand if work[(r.space, r.zv, r.nv)]["mdim"] <= 2_000_000]
+    measured = []
+    for r in cand:
+        if time.perf_counter() - t_start > budget_s:
+            break
+        key = (r.space, r.zv, r.nv)
+        t_hv, oh = _cpu_hv_seconds(orc, r.snt, r.nucleus, 3, cores)
+        seen[f"{r.space}{r.zv}p{r.nv}n"] = {"mdim": int(oh.mdim), "ms": t_hv * 1e3}
+        per_term.append(t_hv / work[key]["terms"])
+        measured.append((r, t_hv))
+        del oh
+    # one core: the largest measured basis up to 5e5 amplitudes (1 warm-up, 3 timed)
+    one = next(((r, t) for r, t in measured if work[(r.space, r.zv, r.nv)]["mdim"] <= 500_000), None)
+    if one is not None:
+        t1, _ = _cpu_hv_seconds(orc, one[0].snt, one[0].nucleus, 3, 1, warmups=1)
+        out["one_core"] = {"basis": f"{one[0].space}{one[0].zv}p{one[0].nv}n",
+                           "mdim": work[(one[0].space, one[0].zv, one[0].nv)]["mdim"],
+                           "ms_per_hv_1core": t1 * 1e3, "ms_per_hv_all_cores": one[1] * 1e3,
+                           "speedup_all_cores": t1 / one[1]}
+        orc.set_threads(cores)
+    sec_per_term = float(np.median(per_term))
+    sec_per_elem = _mgs_seconds_per_element()
+    total = 0.0
+    for r, (ev, info) in zip(reqs, fp64_res):
+        w = work[(r.space, r.zv, r.nv)]
+        nhv = max(1, info.get("n_hv", 0))
+        basis_avg = min(nhv, info.get("lm", 300)) / 2.0
+        total += nhv * (w["terms"] * sec_per_term + basis_avg * w["mdim"] * sec_per_elem)
+    out.update({"value": len(reqs) / total, "hv_bases": seen, "seconds_per_term": sec_per_term,
+                "mgs_seconds_per_element": sec_per_elem, "stream_seconds_estimate": total,
+                "sample": ("measured: one full FP64 TRL solve of Mg24 (C1) on the oracle, and the oracle's "
+                           f"operate_H! on {len(seen)} of the stream's bases (<= 2e6 amplitudes; median of 3 after "
+                           "3 warm-ups) plus numpy MGS; extrapolated: every request of the stream at its FP64 "
+                           "leg's H·v count (seconds per FMA term x terms + ReORTH elements x seconds per element)"),
+                "measure_s": time.perf_counter() - t_start})
+    return out
+
+
+# ------------------------------------------------------------------- H·v roofline
+def hv_roofline(ham, reps, lds_model=True):
+    """Dominant kernel (k_hv, FP32, one vector): HIP-event timing on its launch stream,
+    algorithmic HBM bytes (every table array once + vector in + vector out) and the
+    LDS bytes of the kernel's access pattern."""
+    t32 = ham.time_operate_H("mixed", reps)
+    t64 = ham.time_operate_H("fp64", max(3, reps // 2))
     tb32 = ham.table_bytes("mixed")
-    out = {"mode": "hv", "ms_fp32": t32, "ms_fp64": t64, "reps": args.hv_reps,
-           "alg_bytes_fp32": sum(tb32.values()), "copy_bytes": nbytes, "table_bytes_fp32": tb32}
-    if args.hv_multi:
-        # bl_operate_H!: ms per vector when 2 or 4 vectors share a launch
-        out["ms_per_vector"] = {p: {str(k): ham.time_operate_H(p, args.hv_reps, nvec=k) / k for k in (1, 2, 4)}
-                                for p in ("mixed", "fp64")}
-    print(json.dumps(out))
+    B32 = sum(tb32.values())
+    B64 = sum(ham.table_bytes("fp64").values())
+    terms = ham.work_terms()
+    W = terms["pn"] + terms["pp"] + terms["nn"] + terms["diag"]
+    out = {"ms_fp32": t32, "ms_fp64": t64, "hv_per_s_fp32": 1e3 / t32, "hv_per_s_fp64": 1e3 / t64,
+           "fma_terms": W, "pn_terms_padded": terms["pn_padded"], "terms_per_s_fp32": W / (t32 * 1e-3),
+           "alg_bytes_fp32": B32, "alg_bytes_fp64": B64, "gbs_fp32": B32 / (t32 * 1e-3) / 1e9,
+           "gbs_fp64": B64 / (t64 * 1e-3) / 1e9}
+    lds = ham.lds_bytes("mixed")
+    out["lds_bytes_fp32"] = lds
+    out["lds_gbs_fp32"] = sum(lds.values()) / (t32 * 1e-3) / 1e9
+    return out
 
 
-def _line(metric_value, unit, steps, warmup, t_all, config, extra, dtype="fp32 H·v / fp64 Krylov"):
-    out = {"metric": METRIC, "value": metric_value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": warmup,
+def profiled_traffic(config):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, only when it was
+    taken of the current kernel source (its hash) and this workload."""
+    import hashlib
+    path = os.path.join(ROOT, "profiles", f"hv_{config}_pmc.json")
+    if not os.path.exists(path):
+        return None, "no pmc summary"
+    try:
+        d = json.load(open(path))
+    except (OSError, ValueError):
+        return None, "unreadable pmc summary"
+    src = os.path.join(ROOT, "nucleartoolkit.jl_amd", "csrc", "sm_hv.hip")
+    sha = hashlib.sha256(open(src, "rb").read()).hexdigest()[:16]
+    if d.get("sm_hv_sha16") != sha:
+        return None, f"pmc summary of another kernel source ({d.get('sm_hv_sha16')} vs {sha})"
+    return d.get("hbm_bytes_per_launch"), f"profiles/hv_{config}_pmc.json ({d.get('kernel', '')})"
+
+
+def _line(value, unit, steps, warmup, t_all, config, extra, dtype="fp32 H·v, fp64 recurrence"):
+    out = {"metric": METRIC, "value": value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": warmup,
            "ms_per_step": t_all * 1e3 / max(1, steps), "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": dtype, "data": "synthetic (seeded isospin interactions written as .snt)",
            "config": config}
     out.update(extra)
-    print(json.dumps(out))
+    print(json.dumps(out), flush=True)
+
+
+# ------------------------------------------------------------------------ stream
+def _serve(reqs, server, precision):
+    """Submit every request at its arrival time; returns (results, wall s, first submit)."""
+    t0 = time.perf_counter()
+    ids = []
+    for r in reqs:
+        dt = t0 + r.arrival_s - time.perf_counter()
+        if dt > 0:
+            time.sleep(dt)
+        ids.append(server.submit(r.snt, r.nucleus, r.num_ev, precision=precision, seed=r.seed % (1 << 31)))
+    for i, rid in enumerate(ids):
+        server.wait(rid)
+        if (i + 1) % 64 == 0:
+            print(f"# {precision}: request {i + 1}/{len(ids)} done at {time.perf_counter() - t0:.1f} s",
+                  file=sys.stderr, flush=True)
+    t_all = time.perf_counter() - t0
+    res = [server.result(rid, raise_on_error=False) for rid in ids]
+    return res, t_all
+
+
+def run_stream(args):
+    import smgpu
+    from smgpu import synth
+    from smgpu.server import Server
+    tmp = tempfile.mkdtemp(prefix="smgpu_stream_")
+    lo, hi = (int(x) for x in args.particles.split(","))
+    reqs = synth.serving_stream(tmp, args.requests, rate_per_s=args.rate, particles=(lo, hi))
+    # per-basis sizes and H·v terms (host tables only)
+    work = {}
+    for r in reqs:
+        key = (r.space, r.zv, r.nv)
+        if key not in work:
+            p = smgpu.Problem(r.snt, r.nucleus)
+            work[key] = {"mdim": p.mdim, "nucleus": r.nucleus, "problem": p}
+    print(f"# stream: {len(reqs)} requests over {len(work)} bases, last arrival {reqs[-1].arrival_s:.2f} s",
+          file=sys.stderr, flush=True)
+    # warm-up requests (untimed): one small basis per space
+    warm = []
+    for i in range(max(1, args.warmup)):
+        space = ("sd", "pf")[i % 2]
+        path, nuc = synth.config_snt(os.path.join(tmp, f"warm_{i}.snt"), space, 2, 2, 999 + i, (-6.0, 0.0), 1.2)
+        warm.append((path, nuc))
+
+    def warmup(srv, precision):
+        ids = [srv.submit(p, n, 3, precision=precision) for p, n in warm]
+        for rid in ids:
+            srv.wait(rid)
+            srv.result(rid)
+
+    srv = Server(max_inflight=args.inflight, host_threads=args.host_threads, max_batch=args.batch)
+    warmup(srv, "mixed")
+    res, t_all = _serve(reqs, srv, "mixed")
+    sst = srv.stats()
+    srv.close()
+    print(f"# served leg: {t_all:.1f} s", file=sys.stderr, flush=True)
+    seq = Server(max_inflight=1, host_threads=args.host_threads, max_batch=1)
+    warmup(seq, "fp64")
+    sres, t_seq = _serve(reqs, seq, "fp64")
+    seq.close()
+    print(f"# FP64 leg: {t_seq:.1f} s", file=sys.stderr, flush=True)
+    dE, ok, bad = [], 0, []
+    for i, ((a, ia), (b, ib)) in enumerate(zip(res, sres)):
+        if ia["state"] == "done" and ib["state"] == "done" and len(a) == len(b) > 0:
+            d = float(np.abs(a - b).max())
+            dE.append(d)
+            if d <= TOL_MEV:
+                ok += 1
+            else:
+                bad.append({"request": i, "nucleus": reqs[i].nucleus, "k": reqs[i].num_ev, "dE_MeV": d})
+        else:
+            bad.append({"request": i, "nucleus": reqs[i].nucleus, "state": [ia["state"], ib["state"]],
+                        "error": [ia.get("error", ""), ib.get("error", "")]})
+    value = ok / t_all
+    fp64_rate = sum(1 for _, i in sres if i["state"] == "done") / t_seq
+    lat = [i["latency_ms"] for _, i in res if i["state"] == "done"]
+    lat64 = [i["latency_ms"] for _, i in sres if i["state"] == "done"]
+    mdims = [i["mdim"] for _, i in res]
+    # kernel rooflines on the stream's largest basis (the pf 6p6n requests dominate its H·v time)
+    big = max(work.values(), key=lambda w: w["mdim"])
+    ham = smgpu.Hamiltonian(big["problem"])
+    hv = hv_roofline(ham, args.hv_reps)
+    del ham
+    cpu = None
+    if not args.no_cpu:
+        # FMA terms per H·v of every basis (device tables, one basis at a time) for the CPU model
+        for w in work.values():
+            h = smgpu.Hamiltonian(w["problem"])
+            t = h.work_terms()
+            w["terms"] = t["pn"] + t["pp"] + t["nn"] + t["diag"]
+            del h
+        cpu = cpu_baseline_stream(reqs, sres, work)
+    traffic, traffic_src = profiled_traffic("fe52")
+    roof = {"bound": "hbm", "achieved": hv["gbs_fp32"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": hv["gbs_fp32"] / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
+            "kernel": f"k_hv FP32 one vector on {big['nucleus']} (mdim {big['mdim']})",
+            "lds": {"achieved": hv["lds_gbs_fp32"], "peak": LDS_PEAK_GBS, "unit": "GB/s",
+                    "frac": hv["lds_gbs_fp32"] / LDS_PEAK_GBS}}
+    counts = {}
+    for r in reqs:
+        k = f"{r.space} {r.zv}p{r.nv}n"
+        counts[k] = counts.get(k, 0) + 1
+    _line(value, "solves/s", len(reqs), args.warmup, t_all,
+          {"workload": (f"C4 serving stream: {len(reqs)} requests, Poisson {args.rate}/s, sd/pf, valence {lo}..{hi} "
+                        f"per species, 2M = A%2, k 1..10, per-request seeded interactions (SPE U[-6,0], TBME "
+                        f"N(0,1.2)); mdim {min(mdims)}..{max(mdims)}; served mixed precision in {args.inflight} "
+                        f"slots (batches of up to {args.batch}) vs the plain FP64 TRL one solve at a time"),
+           "requests": len(reqs), "rate_per_s": args.rate, "particles": [lo, hi], "inflight": args.inflight,
+           "batch": args.batch, "lm": 300, "ls": 20, "tol": 1e-8, "steps_note": "one step = one request; the "
+           "stream fixes its own size (--steps is not used by this workload)"},
+          {"latency_ms": {"p50": percentile(lat, 50), "p99": percentile(lat, 99), "note": "arrival to result"},
+           "fp64": {"solves_per_s": fp64_rate, "wall_s": t_seq, "latency_ms_p50": percentile(lat64, 50),
+                    "latency_ms_p99": percentile(lat64, 99), "note": "same requests and arrival times, one slot, "
+                    "plain FP64 (full reorthogonalisation every step)"},
+           "speedup_vs_fp64": value / fp64_rate, "within_tol": ok, "failed_or_off": bad[:8], "n_off": len(bad),
+           "max_abs_dE_vs_fp64_MeV": max(dE) if dE else None,
+           "served": {"wall_s": t_all, "peak_running": sst["peak_running"], "hbm_peak_bytes": sst["hbm_peak"],
+                      "batched": sst["batched"], "index_misses": sst["index_misses"],
+                      "index_hits": sst["index_hits"],
+                      "lm_bounded": sum(1 for _, i in res if i["lm"] < 300)},
+           "bases": counts, "hv": hv, "roofline": roof, "cpu_baseline": cpu})
+
+
+# ---------------------------------------------------------------- served burst
+def run_served(args):
+    import smgpu
+    from smgpu import synth
+    from smgpu.server import Server
+    space, zv, nv, soff, spe, sig, nev, lm, ls = CONFIGS[args.config]
+    tmp = tempfile.mkdtemp(prefix="smgpu_bench_")
+    snt, nuc = synth.config_snt(os.path.join(tmp, f"{args.config}.snt"), space, zv, nv,
+                                synth.BASE_SEED + soff, spe, sig)
+    prob = smgpu.Problem(snt, nuc)
+    ham = smgpu.Hamiltonian(prob, 0)
+    if args.mode == "hv":
+        print(json.dumps({"mode": "hv", "config": args.config, "mdim": prob.mdim, **hv_roofline(ham, args.hv_reps)}))
+        return
+    # plain FP64 path, one solve at a time (tables resident; solve only): >= 3 timed solves
+    ham.trl(nev, lm=lm, ls=ls, precision="fp64")
+    lat64, ev64, st64 = [], None, None
+    for i in range(max(3, args.fp64_steps)):
+        t0 = time.perf_counter()
+        ev64, _, st64 = ham.trl(nev, lm=lm, ls=ls, precision="fp64", seed=3000 + i)
+        lat64.append((time.perf_counter() - t0) * 1e3)
+    fp64_rate = 1e3 / float(np.mean(lat64))
+    # FP64 with partial reorthogonalisation, one at a time (separates the algorithm change)
+    t0 = time.perf_counter()
+    evp, _, stp = ham.trl(nev, lm=lm, ls=ls, precision="fp64", reorth="partial")
+    fp64_partial_ms = (time.perf_counter() - t0) * 1e3
+    ev1, _, st1 = ham.trl(nev, lm=lm, ls=ls, precision="mixed")
+    t0 = time.perf_counter()
+    ev1, _, st1 = ham.trl(nev, lm=lm, ls=ls, precision="mixed")
+    mixed_single_ms = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
+    evf, _, stf = ham.trl(nev, lm=lm, ls=ls, precision="mixed", reorth="full")
+    mixed_full_ms = (time.perf_counter() - t0) * 1e3
+    srv = Server(max_inflight=args.inflight, host_threads=args.host_threads, max_batch=args.batch)
+    warm = [srv.submit(snt, nuc, nev, lm=lm, ls=ls, precision="mixed", seed=1000 + i)
+            for i in range(max(args.warmup, args.inflight * args.batch))]
+    for rid in warm:
+        srv.wait(rid)
+        srv.result(rid)
+    t_all0 = time.perf_counter()
+    ids = [srv.submit(snt, nuc, nev, lm=lm, ls=ls, precision="mixed", seed=2000 + i) for i in range(args.steps)]
+    for rid in ids:
+        srv.wait(rid)
+    t_all = time.perf_counter() - t_all0
+    res = [srv.result(rid) for rid in ids]
+    sstats = srv.stats()
+    srv.close()
+    dE = [float(np.abs(ev - ev64).max()) for ev, _ in res]
+    n_ok = sum(1 for d in dE if d <= TOL_MEV)
+    value = n_ok / t_all
+    lat = [info["latency_ms"] for _, info in res]
+    hv = hv_roofline(ham, args.hv_reps)
+    traffic, traffic_src = profiled_traffic(args.config)
+    roof = {"bound": "hbm", "achieved": hv["gbs_fp32"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": hv["gbs_fp32"] / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
+            "lds": {"achieved": hv["lds_gbs_fp32"], "peak": LDS_PEAK_GBS, "unit": "GB/s",
+                    "frac": hv["lds_gbs_fp32"] / LDS_PEAK_GBS}}
+    _line(value, "solves/s", args.steps, args.warmup, t_all,
+          {"workload": f"{nuc} {space} {zv}p{nv}n 2M={prob.info.twoM} mdim {prob.mdim}, lowest {nev}, tol 1e-8, "
+                       f"lm {lm}, ls {ls}; {args.steps} requests submitted at once, {args.inflight} slots, batches "
+                       f"of up to {args.batch}", "nucleus": nuc, "mdim": prob.mdim, "num_ev": nev},
+          {"latency_ms": {"p50": percentile(lat, 50), "p99": percentile(lat, 99), "note": "submit to result; "
+                          "all requests submitted at t=0"},
+           "fp64": {"solves_per_s": fp64_rate, "solves_timed": len(lat64), "latency_ms": lat64,
+                    "iterations": st64["iterations"], "n_hv": st64["n_hv"], "n_reorth": st64["n_reorth"]},
+           "fp64_partial_reorth": {"solves_per_s": 1e3 / fp64_partial_ms, "n_reorth": stp["n_reorth"],
+                                   "max_abs_dE_vs_fp64_MeV": float(np.abs(evp - ev64).max())},
+           "speedup_vs_fp64": value / fp64_rate,
+           "mixed_single": {"solves_per_s": 1e3 / mixed_single_ms, "n_reorth": st1["n_reorth"],
+                            "max_abs_dE_vs_fp64_MeV": float(np.abs(ev1 - ev64).max())},
+           "mixed_single_full_reorth": {"solves_per_s": 1e3 / mixed_full_ms, "n_reorth": stf["n_reorth"],
+                                        "max_abs_dE_vs_fp64_MeV": float(np.abs(evf - ev64).max())},
+           "served": {"within_tol": n_ok, "requests": len(res), "max_abs_dE_vs_fp64_MeV": max(dE),
+                      "peak_running": sstats["peak_running"]},
+           "hv": hv, "roofline": roof})
 
 
 def run_variants(args):
-    """Interaction variants on one basis: Cr48, a base interaction plus N(0, 0.05)
-    perturbations of every TBME per variant, lowest 3 each.  All variants are
-    submitted at once to the serving loop (one shared index, `--inflight` solves
-    on the device, mixed precision); every eigenvalue is checked against the
-    plain FP64 solve of its variant, run one at a time on one handle whose
-    coefficients are swapped per variant (the shared-basis loop of prepEC)."""
+    """64 interaction variants of Cr48 (N(0, 0.05) on every TBME), lowest 3, all
+    submitted at once to the serving loop; every variant's eigenvalues are checked
+    against its plain FP64 solve (one at a time, coefficients swapped on one handle)."""
     import smgpu
     from smgpu import synth
     from smgpu.server import Server
@@ -141,274 +444,62 @@ def run_variants(args):
     res = [srv.result(rid) for rid in ids]
     sst = srv.stats()
     srv.close()
-    # FP64 reference, one at a time, coefficients swapped on one handle
     prob = smgpu.Problem(paths[0], nuc)
     ham = smgpu.Hamiltonian(prob)
-    n64 = len(paths) if args.fp64_all else min(len(paths), args.fp64_variants)
-    dE = []
-    t64 = 0.0
-    for i in range(n64):
+    dE, t64 = [], 0.0
+    for i in range(len(paths)):
         prob.set_interaction(paths[i])
         ham.update_interaction(prob)
         t1 = time.perf_counter()
         ev64, _, _ = ham.trl(nev, precision="fp64", seed=1000 + i)
         t64 += time.perf_counter() - t1
         dE.append(float(np.abs(res[i][0] - ev64).max()))
-    fp64_rate = n64 / t64
-    ok = sum(1 for d in dE if d <= 1e-5)
-    value = len(res) / t_all if ok == n64 else ok / t_all
+    fp64_rate = len(paths) / t64
+    ok = sum(1 for d in dE if d <= TOL_MEV)
+    value = ok / t_all
     lat = [info["latency_ms"] for _, info in res]
     _line(value, "solves/s", len(paths), args.inflight, t_all,
           {"workload": f"{len(paths)} interaction variants of {nuc} (pf 4p4n, mdim {res[0][1]['mdim']}), lowest {nev} "
-                       f"each, one shared index, {args.inflight} in flight", "nucleus": nuc, "variants": len(paths),
-           "num_ev": nev, "inflight": args.inflight, "batch": args.batch},
+                       f"each, one shared index, {args.inflight} slots", "nucleus": nuc, "variants": len(paths)},
           {"latency_ms": {"p50": percentile(lat, 50), "p99": percentile(lat, 99)},
-           "fp64": {"solves_per_s": fp64_rate, "variants_checked": n64},
+           "fp64": {"solves_per_s": fp64_rate, "variants_checked": len(paths)},
            "speedup_vs_fp64": value / fp64_rate, "max_abs_dE_vs_fp64_MeV": max(dE), "within_tol": ok,
-           "index_uploads": sst["index_uploads"], "index_hits": sst["index_hits"],
-           "ground_energies_MeV": [float(r[0][0]) for r in res]})
-
-
-def run_stream(args):
-    """Serving stream: requests over sd and pf with Poisson arrivals (exponential
-    gaps, mean 1/rate), valence numbers uniform in --particles, k uniform in 1..10,
-    per-request seeded interactions (SPE U[-6,0], TBME N(0,1.2)); mixed precision
-    through the serving loop.  The same requests are then solved one at a time in
-    FP64 through a one-slot server (the sequential baseline), and every eigenvalue
-    is compared."""
-    from smgpu import synth
-    from smgpu.server import Server
-    tmp = tempfile.mkdtemp(prefix="smgpu_stream_")
-    lo, hi = (int(x) for x in args.particles.split(","))
-    reqs = synth.serving_stream(tmp, args.requests, rate_per_s=args.rate, particles=(lo, hi))
-    srv = Server(max_inflight=args.inflight, host_threads=args.host_threads, max_batch=args.batch)
-    t0 = time.perf_counter()
-    ids = []
-    for r in reqs:
-        dt = t0 + r.arrival_s - time.perf_counter()
-        if dt > 0:
-            time.sleep(dt)
-        ids.append(srv.submit(r.snt, r.nucleus, r.num_ev, seed=r.seed % (1 << 31)))
-    for rid in ids:
-        srv.wait(rid)
-    t_all = time.perf_counter() - t0
-    res = [srv.result(rid, raise_on_error=False) for rid in ids]
-    sst = srv.stats()
-    srv.close()
-    lat = [info["latency_ms"] for _, info in res if info["state"] == "done"]
-    # sequential FP64 baseline on the same requests
-    seq = Server(max_inflight=1, host_threads=1)
-    t1 = time.perf_counter()
-    sids = [seq.submit(r.snt, r.nucleus, r.num_ev, precision="fp64", seed=r.seed % (1 << 31)) for r in reqs]
-    for rid in sids:
-        seq.wait(rid)
-    t_seq = time.perf_counter() - t1
-    sres = [seq.result(rid, raise_on_error=False) for rid in sids]
-    seq.close()
-    dE = [float(np.abs(a[0] - b[0]).max()) for a, b in zip(res, sres)
-          if a[1]["state"] == "done" and b[1]["state"] == "done" and len(a[0]) == len(b[0]) > 0]
-    ok = sum(1 for d in dE if d <= 1e-5)
-    value = ok / t_all
-    mdims = [info["mdim"] for _, info in res]
-    _line(value, "solves/s", len(reqs), 0, t_all,
-          {"workload": f"{len(reqs)} requests, sd/pf, valence {lo}..{hi} per species, k 1..10, Poisson "
-                       f"{args.rate}/s, {args.inflight} in flight", "requests": len(reqs), "rate_per_s": args.rate,
-           "particles": [lo, hi], "inflight": args.inflight, "batch": args.batch},
-          {"latency_ms": {"p50": percentile(lat, 50), "p99": percentile(lat, 99), "note": "arrival to result"},
-           "sequential_fp64": {"solves_per_s": len(reqs) / t_seq, "wall_s": t_seq},
-           "speedup_vs_fp64": value / (len(reqs) / t_seq), "max_abs_dE_vs_fp64_MeV": max(dE) if dE else None,
-           "within_tol": ok, "failed": int(sst["failed"]), "peak_running": sst["peak_running"],
-           "hbm_peak_bytes": sst["hbm_peak"], "mdim_range": [min(mdims), max(mdims)],
-           "index_misses": sst["index_misses"], "index_hits": sst["index_hits"]})
+           "index_uploads": sst["index_uploads"], "batched": sst["batched"]})
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=16)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="cr48", choices=sorted(CONFIGS))
-    ap.add_argument("--fp64-steps", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=16, help="requests of the served burst (--workload served)")
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="stream", choices=["stream", "served", "variants"])
+    ap.add_argument("--config", default="cr48", choices=sorted(CONFIGS), help="--workload served / --mode hv")
+    ap.add_argument("--mode", default="solve", choices=["solve", "hv"])
+    ap.add_argument("--fp64-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--hv-reps", type=int, default=20)
-    ap.add_argument("--hv-multi", action="store_true", help="--mode hv: also time 2- and 4-vector launches")
-    ap.add_argument("--inflight", type=int, default=None,
-                    help="server slots (default 2; 4 for --workload stream)")
-    ap.add_argument("--batch", type=int, default=None,
-                    help="requests on one basis solved in lockstep per slot (server max_batch; default 2, "
-                         "1 for --workload stream, whose requests rarely share a basis)")
-    ap.add_argument("--host-threads", type=int, default=2)
-    ap.add_argument("--mode", default="solve", choices=["solve", "hv"],
-                    help="hv: only the H·v launches of the workload (for profiler runs) plus a 256 MiB "
-                         "device-to-device copy that calibrates the HBM byte counters")
-    ap.add_argument("--workload", default="served", choices=["served", "variants", "stream"],
-                    help="served: the --config workload through the serving loop (default); variants: "
-                         "interaction variants of Cr48 on one basis; stream: Poisson request stream")
+    ap.add_argument("--inflight", type=int, default=None)
+    ap.add_argument("--batch", type=int, default=None)
+    ap.add_argument("--host-threads", type=int, default=4)
     ap.add_argument("--variants", type=int, default=64)
-    ap.add_argument("--fp64-variants", type=int, default=8)
-    ap.add_argument("--fp64-all", action="store_true")
-    ap.add_argument("--requests", type=int, default=64)
+    ap.add_argument("--requests", type=int, default=512)
     ap.add_argument("--rate", type=float, default=50.0)
-    ap.add_argument("--particles", default="2,4", help="valence range per species, e.g. 2,6")
+    ap.add_argument("--particles", default="2,6", help="valence range per species")
     args = ap.parse_args()
-    # measured best per workload: the stream's mostly distinct bases gain from more slots
-    # (4 slots, no batching: 49.9 solves/s, p50 120 ms; 2 slots with batches of 2: 47.2, 215 ms),
-    # the Cr48 workloads from two slots with batches of two
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); independent solves shard one per GPU "
+              "and are measured per GPU", file=sys.stderr)
+        sys.exit(2)
     if args.inflight is None:
         args.inflight = 4 if args.workload == "stream" else 2
     if args.batch is None:
         args.batch = 1 if args.workload == "stream" else 2
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); independent solves shard "
-              "one per GPU and are measured per GPU", file=sys.stderr)
-        sys.exit(2)
-
-    if args.workload == "variants":
+    if args.mode == "hv" or args.workload == "served":
+        run_served(args)
+    elif args.workload == "variants":
         run_variants(args)
-        return
-    if args.workload == "stream":
+    else:
         run_stream(args)
-        return
-
-    import smgpu
-    from smgpu import synth
-
-    space, zv, nv, soff, spe, sig, nev, lm, ls = CONFIGS[args.config]
-    tmp = tempfile.mkdtemp(prefix="smgpu_bench_")
-    snt, nuc = synth.config_snt(os.path.join(tmp, f"{args.config}.snt"), space, zv, nv,
-                                synth.BASE_SEED + soff, spe, sig)
-    t0 = time.perf_counter()
-    prob = smgpu.Problem(snt, nuc)
-    ham = smgpu.Hamiltonian(prob, 0)
-    t_setup = time.perf_counter() - t0
-    mdim = prob.mdim
-    if args.mode == "hv":
-        hv_only(ham, args)
-        return
-
-    # plain FP64 path, one solve at a time (tables resident; solve only)
-    lat64 = []
-    ev64 = None
-    st64 = None
-    ham.trl(nev, lm=lm, ls=ls, precision="fp64")          # warm-up: workspace allocation
-    for _ in range(max(1, args.fp64_steps)):
-        t0 = time.perf_counter()
-        ev64, _, st64 = ham.trl(nev, lm=lm, ls=ls, precision="fp64")
-        lat64.append((time.perf_counter() - t0) * 1e3)
-    fp64_rate = 1e3 / float(np.mean(lat64))
-    # mixed precision one at a time on the same handle (for the breakdown)
-    ev1, _, st1 = ham.trl(nev, lm=lm, ls=ls, precision="mixed")
-    t0 = time.perf_counter()
-    ev1, _, st1 = ham.trl(nev, lm=lm, ls=ls, precision="mixed")
-    mixed_single_ms = (time.perf_counter() - t0) * 1e3
-    # the same with a projection pass at every step (the reference's ReORTH schedule)
-    t0 = time.perf_counter()
-    evf, _, stf = ham.trl(nev, lm=lm, ls=ls, precision="mixed", reorth="full")
-    mixed_full_ms = (time.perf_counter() - t0) * 1e3
-
-    # serving form: K requests through the persistent loop, M in flight
-    from smgpu.server import Server
-    srv = Server(max_inflight=args.inflight, host_threads=args.host_threads, max_batch=args.batch)
-    warm = [srv.submit(snt, nuc, nev, lm=lm, ls=ls, precision="mixed", seed=1000 + i)
-            for i in range(max(args.warmup, args.inflight * args.batch))]
-    for rid in warm:
-        srv.wait(rid)
-        srv.result(rid)
-    t_all0 = time.perf_counter()
-    ids = [srv.submit(snt, nuc, nev, lm=lm, ls=ls, precision="mixed", seed=2000 + i) for i in range(args.steps)]
-    for rid in ids:
-        srv.wait(rid)
-    t_all = time.perf_counter() - t_all0
-    res = [srv.result(rid) for rid in ids]
-    sstats = srv.stats()
-    srv.close()
-    dE = [float(np.abs(ev - ev64).max()) for ev, _ in res]
-    n_ok = sum(1 for d in dE if d <= 1e-5)
-    value = n_ok / t_all
-    lat = [info["latency_ms"] for _, info in res]
-    solve_ms = [info["solve_ms"] for _, info in res]
-    prep_ms = [info["prep_end_ms"] - info["prep_start_ms"] for _, info in res]
-    evs = res[-1][0]
-    stats = res[-1][1]
-
-    # H·v roofline (HIP events on the kernel's stream, resident vectors)
-    t_hv32 = ham.time_operate_H("mixed", args.hv_reps)
-    t_hv64 = ham.time_operate_H("fp64", max(3, args.hv_reps // 2))
-    tb32 = ham.table_bytes("mixed")
-    tb64 = ham.table_bytes("fp64")
-    B32 = sum(tb32.values())
-    B64 = sum(tb64.values())
-    terms = ham.work_terms()
-    W = terms["pn"] + terms["pp"] + terms["nn"] + terms["diag"]
-    achieved = B32 / (t_hv32 * 1e-3) / 1e9
-    traffic = None
-    pmc_path = os.path.join(ROOT, "profiles", f"hv_{args.config}_pmc.json")
-    if os.path.exists(pmc_path):
-        try:
-            traffic = json.load(open(pmc_path)).get("hbm_bytes_per_launch")
-        except Exception:
-            traffic = None
-
-    # basis size of every ReORTH of the FP64 run (sweeps of lm-1 steps; a restart keeps ls)
-    nv_sum, it = 0, 0
-    for _ in range(st64["n_hv"]):
-        nv_sum += it + 1
-        it += 1
-        if it == lm - 1:
-            it = ls
-    avg_basis = nv_sum / max(1, st64["n_hv"])
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(snt, nuc, st64["n_hv"], avg_basis)
-
-    out = {
-        "metric": METRIC,
-        "value": value,
-        "unit": "solves/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": t_all * 1e3 / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "fp32 H·v / fp64 Krylov",
-        "data": "synthetic (seeded isospin interaction written as .snt)",
-        "config": {"workload": f"{nuc} {space} {zv}p{nv}n 2M={prob.info.twoM} mdim {mdim}, lowest {nev}, "
-                               f"tol 1e-8, lm {lm}, ls {ls}; {args.steps} requests served in {args.inflight} slots, "
-                               f"batches of up to {args.batch}",
-                   "nucleus": nuc, "mdim": mdim, "num_ev": nev, "lm": lm, "ls": ls, "precision": "mixed",
-                   "refine": "fp64 Rayleigh quotient", "inflight": args.inflight, "batch": args.batch,
-                   "requests": args.steps},
-        "latency_ms": {"p50": percentile(lat, 50), "p99": percentile(lat, 99),
-                       "solve_p50": percentile(solve_ms, 50), "prep_p50": percentile(prep_ms, 50),
-                       "note": "submit to result; all K requests submitted at t=0"},
-        "fp64": {"solves_per_s": fp64_rate, "latency_ms_p50": percentile(lat64, 50),
-                 "iterations": st64["iterations"], "restarts": st64["n_restart"], "n_hv": st64["n_hv"],
-                 "n_reorth": st64["n_reorth"]},
-        "speedup_vs_fp64": value / fp64_rate,
-        "mixed_single": {"solves_per_s": 1e3 / mixed_single_ms, "iterations": st1["iterations"],
-                         "restarts": st1["n_restart"], "n_hv": st1["n_hv"], "n_hv64": st1["n_hv64"],
-                         "n_reorth": st1["n_reorth"], "reorth": "partial (omega recurrence)",
-                         "max_abs_dE_vs_fp64_MeV": float(np.abs(np.array(ev1) - np.array(ev64)).max())},
-        "mixed_single_full_reorth": {"solves_per_s": 1e3 / mixed_full_ms, "iterations": stf["iterations"],
-                                     "n_reorth": stf["n_reorth"],
-                                     "max_abs_dE_vs_fp64_MeV": float(np.abs(np.array(evf) - np.array(ev64)).max())},
-        "served": {"within_tol": n_ok, "requests": len(res), "max_abs_dE_vs_fp64_MeV": max(dE),
-                   "peak_running": sstats["peak_running"], "hbm_peak_bytes": sstats["hbm_peak"],
-                   "iterations": stats["iterations"]},
-        "max_abs_dE_mixed_vs_fp64_MeV": max(dE),
-        "eigenvalues_MeV": [float(x) for x in evs],
-        "hv": {"ms_fp32": t_hv32, "ms_fp64": t_hv64, "hv_per_s_fp32": 1e3 / t_hv32,
-               "fma_terms": W, "terms_per_s_fp32": W / (t_hv32 * 1e-3),
-               "alg_bytes_fp32": B32, "alg_bytes_fp64": B64,
-               "gbs_fp64": B64 / (t_hv64 * 1e-3) / 1e9},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
-        "cpu_baseline": cpu,
-        "setup_s": t_setup,
-    }
-    print(json.dumps(out))
 
 
 if __name__ == "__main__":

@@ -80,7 +80,9 @@ typedef struct {
     int32_t n_reorth2;   /* iterations that needed a second Gram-Schmidt pass */
     int32_t n_jproj;     /* J-projected solves: vectors that were projected */
     int32_t n_reorth;    /* iterations that ran a re-orthogonalisation pass */
-    int32_t reserved_;
+    int32_t n_fallback;  /* 1: a mixed solve whose Ritz values disagreed with the FP64
+                            Rayleigh quotients (lost orthogonality) was redone with
+                            reorthogonalisation at every step */
 } smgpu_solve_stats;
 
 /* library */
@@ -139,6 +141,11 @@ int32_t smgpu_ham_create_variant(smgpu_ham base, smgpu_problem p, smgpu_ham* out
  * names: newline-separated names written into names_buf (may be NULL). */
 int32_t smgpu_table_bytes(smgpu_ham h, int32_t precision, int32_t max_arrays, int64_t* bytes, int32_t* n_arrays,
                           char* names_buf, int32_t names_len);
+/* LDS bytes one one-vector H·v launch of `precision` moves, by part: parts[0] own
+ * rows staged, [1] diagonal reads, [2] nn reads, [3] pn staging writes, [4] pn
+ * walk reads.  With the launch time this gives the kernel's LDS roofline.  No
+ * counterpart in the reference (measurement only). */
+int32_t smgpu_lds_bytes(smgpu_ham h, int32_t precision, int64_t* parts);
 /* FMAs per H·v: [pn terms incl. padding, pn terms exact, pp, nn, diagonal] */
 int32_t smgpu_work_terms(smgpu_ham h, int64_t* terms5);
 
@@ -271,7 +278,8 @@ typedef struct {
     int32_t num_ev;
     int32_t index_shared;   /* 1: device index tables were already resident */
     int32_t slot;           /* slot that ran it (-1 before admission) */
-    int32_t pad_;
+    int32_t lm;             /* Krylov basis size it runs with (the request's lm, or the
+                               largest that fits the HBM budget; set once prepared) */
     int64_t mdim;
     int64_t footprint;      /* HBM bytes the request needs (index + coefficients + workspace) */
     int64_t index_bytes;    /* of which the shared index tables */
@@ -298,7 +306,10 @@ int32_t smgpu_server_create(const smgpu_server_config* cfg, smgpu_server* out);
 /* Cancels requests not yet finished, waits for the slots, frees everything. */
 int32_t smgpu_server_destroy(smgpu_server s);
 /* One main_sm solve: interaction file, nucleus ("Cr48"), 2M (< 0: A % 2) and
- * solver parameters (num_ev <= 64).  Returns at once with the request id. */
+ * solver parameters (num_ev <= 64).  Returns at once with the request id.  A
+ * request whose Lanczos basis at params->lm does not fit the server's HBM budget
+ * runs with the largest lm that does (not below max(ls + 20, 2 ls)); info.lm
+ * reports it. */
 int32_t smgpu_server_submit(smgpu_server s, const char* snt_path, const char* nucleus, int32_t twoM,
                             const smgpu_solve_params* params, int64_t* request_id);
 int32_t smgpu_server_poll(smgpu_server s, int64_t request_id, int32_t* state);

@@ -286,6 +286,16 @@ int32_t smgpu_table_bytes(smgpu_ham h, int32_t precision, int32_t max_arrays, in
     });
 }
 
+int32_t smgpu_lds_bytes(smgpu_ham h, int32_t precision, int64_t* parts) {
+    return guard([&]() -> int32_t {
+        if (!h || !parts) throw std::invalid_argument("null argument");
+        const int k = precision == SMGPU_FP64 ? 0 : 1;
+        const int64_t es = k == 0 ? 8 : 4;
+        for (int i = 0; i < 5; i++) parts[i] = h->dev->idx->lds_elems[k][i] * es;
+        return SMGPU_OK;
+    });
+}
+
 int32_t smgpu_work_terms(smgpu_ham h, int64_t* t) {
     return guard([&]() -> int32_t {
         if (!h || !t) throw std::invalid_argument("null argument");

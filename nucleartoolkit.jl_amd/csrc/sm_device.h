@@ -103,6 +103,7 @@ struct DeviceIndex {
     HvTables view[2];                // index part (coefficient pointers unset)
     uint64_t pattern = 0;            // fingerprint of the pp / nn sparsity patterns
     int64_t pn_terms = 0, pp_terms = 0, nn_terms = 0, pn_terms_exact = 0;
+    int64_t lds_elems[2][5] = {};    // per precision: LDS elements of a one-vector launch (lds_model)
     int64_t device_bytes() const;    // HBM held by the index tables
 };
 

@@ -805,6 +805,52 @@ void index_layout(const Basis& b, const Tables& t, HostLayout& L) {
 }
 
 
+// LDS elements one one-vector k_hv launch moves (RG rows per workgroup, four hop
+// slots per round; bytes = elements x the element size): [0] own rows staged, [1]
+// diagonal reads, [2] nn reads, [3] pn staging writes (X rows and V_eff rows of the
+// hops), [4] pn walk reads (per neutron-entry step of an active wave, one four-slot
+// read of X and one of V_eff per staged row; lanes past the block edge included).
+void lds_model(const HostLayout& L, int rg, const std::vector<int32_t>& gb, const std::vector<int32_t>& gf,
+               int64_t* e) {
+    for (int k = 0; k < 5; k++) e[k] = 0;
+    const int nb = (int)L.nP.size();
+    std::vector<int64_t> nn_len(nb, 0), ntab_len(L.ntab_slice_base.size(), 0);
+    for (int b = 0; b < nb; b++) {
+        const int nsl = (L.nN[b] + 63) / 64;
+        for (int s = 0; s < nsl; s++) nn_len[b] += L.nn_slice_len[L.nn_slice_base[b] + s];
+    }
+    for (size_t q = 0; q < L.pair_bi.size(); q++) {
+        const int nt = L.pair_ntab[q];
+        if (ntab_len[nt]) continue;
+        int bf = 0;
+        while (L.pair_ptr[bf + 1] <= (int)q) bf++;
+        const int nsl = (L.nN[bf] + 63) / 64;
+        for (int s = 0; s < nsl; s++) ntab_len[nt] += L.ntab_slice_len[L.ntab_slice_base[nt] + s];
+    }
+    for (size_t g = 0; g < gb.size(); g++) {
+        const int bf = gb[g], fp0 = gf[g];
+        const int nN = L.nN[bf], nsl = (nN + 63) / 64;
+        const int nr = std::min(rg, L.nP[bf] - fp0);
+        e[0] += (int64_t)nN * rg;
+        e[1] += (int64_t)64 * nsl * rg;
+        e[2] += nn_len[bf] * 64 * rg;
+        for (int q = L.pair_ptr[bf]; q < L.pair_ptr[bf + 1]; q++) {
+            const int pt = L.pair_ptab[q];
+            const int32_t* prp = L.ptab_rowptr.data() + L.ptab_rowptr_base[pt] + fp0;
+            const int nNi = L.nN[L.pair_bi[q]], c2 = L.veff_cols2[L.pair_dm[q]];
+            const int64_t walk = ntab_len[L.pair_ntab[q]] * 64;
+            int nch = 0;
+            for (int r = 0; r < nr; r++) nch = std::max(nch, (prp[r + 1] - prp[r] + 3) / 4);
+            for (int ch = 0; ch < nch; ch++) {
+                int staged = 0;
+                for (int r = 0; r < nr; r++) staged += prp[r] + 4 * ch < prp[r + 1] ? 1 : 0;
+                e[3] += (int64_t)staged * (nNi + c2) * 4;
+                e[4] += (int64_t)staged * walk * 8;
+            }
+        }
+    }
+}
+
 template <typename T> void set_view(DeviceIndex& h, int prec, const HostLayout& L) {
     HvTables& A = h.view[prec];
     A.nb = h.nb; A.R = h.R;
@@ -894,6 +940,7 @@ template <typename T> void set_view(DeviceIndex& h, int prec, const HostLayout& 
     }
     h.grp_block[prec].upload(gb);
     h.grp_fp0[prec].upload(gf);
+    lds_model(L, A.rg, gb, gf, h.lds_elems[prec]);
     A.G = (int)gb.size();
     A.grp_block = h.grp_block[prec].as<int32_t>();
     A.grp_fp0 = h.grp_fp0[prec].as<int32_t>();

@@ -249,6 +249,31 @@ struct Server {
         r.mdim = e->basis.mdim;
         r.ws_need = solve_workspace_bytes(r.mdim, r.params);
         r.coef_need = device_bytes(*r.coef);
+        bound_basis(r, e->index_bytes);
+    }
+
+    // A request whose Krylov basis at its lm does not fit the whole HBM budget (with
+    // its index tables and coefficients) runs with the largest lm that does, down to
+    // max(ls + 20, 2 ls); the thick restart then converges to the same eigenvalues
+    // in more sweeps.  The bound depends on the request and the budget only, never on
+    // what else is running, so a request gives the same result alone and in a mix.
+    // Below that floor the request keeps its lm and admission fails it with ENOMEM.
+    void bound_basis(Request& r, int64_t index_bytes) {
+        const int64_t fixed = index_bytes + r.coef_need;
+        if (fixed + r.ws_need <= cfg.hbm_budget) return;
+        const int lm0 = r.params.lm;
+        const int lm_min = std::min(lm0, std::max(r.params.ls + 20, 2 * r.params.ls));
+        int lo = lm_min, hi = lm0 - 1, best = 0;
+        while (lo <= hi) {
+            const int mid = lo + (hi - lo) / 2;
+            SolveParams q = r.params;
+            q.lm = mid;
+            if (fixed + solve_workspace_bytes(r.mdim, q) <= cfg.hbm_budget) { best = mid; lo = mid + 1; }
+            else hi = mid - 1;
+        }
+        if (best == 0) return;
+        r.params.lm = best;
+        r.ws_need = solve_workspace_bytes(r.mdim, r.params);
     }
 
     void preparer_loop() {
@@ -743,6 +768,7 @@ struct Server {
         info->num_ev = (int32_t)r.evals.size();
         info->index_shared = r.index_shared ? 1 : 0;
         info->slot = r.slot;
+    info->lm = r.params.lm;
         info->mdim = r.mdim;
         info->index_bytes = r.entry ? r.entry->index_bytes : 0;
         info->footprint = info->index_bytes + r.ws_need + r.coef_need;

@@ -429,6 +429,13 @@ __global__ __launch_bounds__(TB) void k_local_step(const TW* __restrict__ win, c
     if (threadIdx.x == 0) { npart[blockIdx.x] = nr; npart2[blockIdx.x] = nr; }
 }
 
+// parameters of the omega recurrence: eps = unit of the basis precision (the value
+// a reorthogonalised row restarts from), eps_r = rounding level of one Lanczos step
+// (the recurrence's error term), thr = gate threshold of the projection pass
+struct OmegaParams {
+    double eps, eps_r, thr;
+};
+
 // Partial re-orthogonalisation (Simon's omega recurrence) for step j = it: with
 // beta' = |w'| from the partials, a_j = loc[0] and the stored alpha/beta of steps < j,
 //   beta' om_{j+1,k} = b_k om_{j,k+1} + (a_k - a_j) om_{j,k} + b_{k-1} om_{j,k-1}
@@ -442,8 +449,9 @@ __global__ __launch_bounds__(TB) void k_local_step(const TW* __restrict__ win, c
 // (one workgroup of TB threads; returns the gate of this step's pass in every thread)
 __device__ int omega_block(const double* __restrict__ alpha, const double* __restrict__ beta, int j,
                            const double* __restrict__ npart, int G, double* __restrict__ om, int ldo,
-                           double* __restrict__ loc, int* __restrict__ gate, double eps,
+                           double* __restrict__ loc, int* __restrict__ gate, const OmegaParams op,
                            double* bj_out = nullptr) {
+    const double eps = op.eps;
     __shared__ double red[TB / 64];
     __shared__ double mx[TB / 64];
     const int forced = gate[2];        // read before the barriers below; thread 0 rewrites it last
@@ -464,7 +472,7 @@ __device__ int omega_block(const double* __restrict__ alpha, const double* __res
         for (int k = threadIdx.x; k < j; k += TB) {
             double t = beta[k] * cur[k + 1] + (alpha[k] - aj) * cur[k] - bjm1 * prev[k];
             if (k > 0) t += beta[k - 1] * cur[k - 1];
-            const double th = 3.0 * eps * (beta[k] + bj);
+            const double th = 3.0 * op.eps_r * (beta[k] + bj);
             t += t >= 0.0 ? th : -th;
             t /= bj;
             next[k] = t;
@@ -476,7 +484,7 @@ __device__ int omega_block(const double* __restrict__ alpha, const double* __res
     if ((threadIdx.x & 63) == 0) mx[threadIdx.x >> 6] = m;
     __syncthreads();
     m = fmax(fmax(mx[0], mx[1]), fmax(mx[2], mx[3]));
-    const int run = (bj == 0.0 || forced > 0 || m > sqrt(eps)) ? 1 : 0;
+    const int run = (bj == 0.0 || forced > 0 || m > op.thr) ? 1 : 0;
     if (run) for (int k = threadIdx.x; k < j; k += TB) next[k] = eps;
     if (threadIdx.x == 0) {
         next[j] = eps;
@@ -496,9 +504,9 @@ __device__ int omega_block(const double* __restrict__ alpha, const double* __res
 // values k_finish_local would write (the same sum of the same partials).
 __global__ void k_omega(double* __restrict__ alpha, double* __restrict__ beta, int j,
                         const double* __restrict__ npart, int G, double* __restrict__ om, int ldo,
-                        double* __restrict__ loc, int* __restrict__ gate, double eps, int* hgate) {
+                        double* __restrict__ loc, int* __restrict__ gate, OmegaParams op, int* hgate) {
     __shared__ double bj;
-    const int run = omega_block(alpha, beta, j, npart, G, om, ldo, loc, gate, eps, &bj);
+    const int run = omega_block(alpha, beta, j, npart, G, om, ldo, loc, gate, op, &bj);
     if (threadIdx.x == 0) {
         if (!run) { alpha[j] = loc[0]; beta[j] = bj; }
         if (hgate) *hgate = run;
@@ -548,7 +556,7 @@ __global__ __launch_bounds__(TB) void k_step_small(const TW* __restrict__ win, c
                                                    int64_t ldb, int it, int64_t n, double* __restrict__ w,
                                                    double* __restrict__ loc, int* __restrict__ gate,
                                                    double* __restrict__ alpha, double* __restrict__ beta,
-                                                   double* __restrict__ om, int ldo, double eps,
+                                                   double* __restrict__ om, int ldo, OmegaParams op,
                                                    double* __restrict__ h1, double* __restrict__ h2,
                                                    TBv* __restrict__ vnext) {
     __shared__ double red[TB / 64];
@@ -576,7 +584,7 @@ __global__ __launch_bounds__(TB) void k_step_small(const TW* __restrict__ win, c
     if (threadIdx.x == 0) { sh[1] = nr; loc[0] = al; }
     __syncthreads();
     const double nb = sh[1];                 // |w'|^2
-    const int run = omega_block(alpha, beta, it, sh + 1, 1, om, ldo, loc, gate, eps);
+    const int run = omega_block(alpha, beta, it, sh + 1, 1, om, ldo, loc, gate, op);
     double nf = nb;
     int second = 0;
     if (run) {
@@ -850,6 +858,13 @@ static void restart_small_on_stream(double* alpha, double* beta, int m, int ls, 
 // ------------------------------------------------------------------ driver
 namespace {
 
+// thrown by a mixed-precision solve whose Ritz values disagree with the FP64
+// Rayleigh quotients of their Ritz vectors; the solve is then redone with
+// reorthogonalisation at every step
+struct Inconsistent : std::runtime_error {
+    Inconsistent() : std::runtime_error("Ritz values inconsistent with their Rayleigh quotients") {}
+};
+
 struct Workspace {
     DBuf basis, wH, w64, part, npart, npart2, h1, h2, alpha, beta, enring, flags, S, swork, U, scal, tvals, Y, Q,
         R, hv64, loc, gate, jy, jz, om;
@@ -940,6 +955,7 @@ struct Solver {
     bool partial = false;       // partial re-orthogonalisation (omega recurrence)
     bool small = false;         // partial, n <= SMALL_STEP_N: the step is one k_step_small
     double eps_basis = 0.0;     // unit of the basis precision for the omega recurrence
+    OmegaParams omega{};
     const JProj* jp = nullptr;  // J projection (doubleLanczos), or none
 
     static Workspace& cached(std::shared_ptr<void>* slots) {
@@ -957,6 +973,20 @@ struct Solver {
         partial = !reorth_cgs2 && !jp &&
                   (P.reorth == SMGPU_REORTH_PARTIAL || (P.reorth == SMGPU_REORTH_DEFAULT && sizeof(TBv) == 4));
         eps_basis = sizeof(TBv) == 4 ? 1.1920928955078125e-07 : 2.220446049250313e-16;
+        // The omega recurrence's error term models the rounding of one Lanczos step.
+        // With an FP32 basis the step's H·v is FP32 too, and a row of it sums hundreds
+        // of FP32 products (error ~ sqrt(terms) eps), so the term is 10 eps: with eps
+        // alone the estimate fell behind the real loss of orthogonality on small and
+        // medium bases (10 of 428 requests of the serving stream broke down after
+        // ~100 steps, eigenvalues 15-35 MeV off); with 10 eps none did, for 44 instead
+        // of 24 projection passes in a Cr48 solve.  The FP64 basis keeps eps.
+        // SMGPU_OMEGA_ROUNDING overrides the FP32 multiplier (tests use 1 to drive
+        // the Rayleigh-quotient check and its fallback).
+        double kr = 10.0;
+        if (const char* e = std::getenv("SMGPU_OMEGA_ROUNDING")) kr = std::atof(e);
+        omega.eps = eps_basis;
+        omega.eps_r = (sizeof(TBv) == 4 ? kr : 1.0) * eps_basis;
+        omega.thr = std::sqrt(eps_basis);
         n = H.mdim;
         ld = (n + 63) / 64 * 64;
         lm = P.lm;
@@ -1059,7 +1089,7 @@ struct Solver {
         if (small) {
             hipLaunchKernelGGL((k_step_small<TBv, TW>), dim3(1), dim3(TB), 0, s, win, (const TBv*)B, ld, it, n,
                                W.w64.as<double>(), loc, gate, W.alpha.as<double>(), W.beta.as<double>(),
-                               W.om.as<double>(), lm + 2, eps_basis, W.h1.as<double>(), W.h2.as<double>(),
+                               W.om.as<double>(), lm + 2, omega, W.h1.as<double>(), W.h2.as<double>(),
                                it + 1 < lm ? V(it + 1) : (TBv*)nullptr);
             return;
         }
@@ -1070,7 +1100,7 @@ struct Solver {
                            (const double*)loc, bp, W.w64.as<double>(), n, W.npart.as<double>(), W.npart2.as<double>());
         hipLaunchKernelGGL(k_omega, dim3(1), dim3(TB), 0, s, W.alpha.as<double>(),
                            W.beta.as<double>(), it, (const double*)W.npart.as<double>(), (int)Gx,
-                           W.om.as<double>(), lm + 2, loc, gate, eps_basis, d_hg ? W.hflag + 4 : nullptr);
+                           W.om.as<double>(), lm + 2, loc, gate, omega, d_hg ? W.hflag + 4 : nullptr);
         if (d_hg) SM_HIP(hipEventRecord(W.evg, s));
     }
 
@@ -1592,6 +1622,11 @@ struct Solver {
             std::vector<double> rq(nev);
             SM_HIP(hipMemcpyAsync(rq.data(), W.h1.p, nev * 8, hipMemcpyDeviceToHost, s));
             SM_HIP(hipStreamSynchronize(s));
+            // A Ritz value and the FP64 Rayleigh quotient of its Ritz vector agree to
+            // the convergence level (~1e-6 MeV); a gap above 1e-3 |theta| means the
+            // Lanczos basis lost its (semi-)orthogonality and T no longer projects H
+            for (int i = 0; i < nev; i++)
+                if (!(std::fabs(rq[i] - evals[i]) <= 1e-3 * std::max(1.0, std::fabs(evals[i])))) throw Inconsistent();
             for (int i = 0; i < nev; i++) evals[i] = rq[i];
             std::sort(evals, evals + nev);
         }
@@ -2015,9 +2050,18 @@ void trl_solve(DeviceHam& h, const SolveParams& p, double* evals, double* evecs,
     if (p.precision == 0) {
         Solver<double> S(h, p, st, ctx, jp);
         S.run(evals, evecs, v0);
-    } else {
+        return;
+    }
+    try {
         Solver<float> S(h, p, st, ctx, jp);
         S.run(evals, evecs, v0);
+    } catch (const Inconsistent&) {
+        SolveParams pf = p;
+        pf.reorth = SMGPU_REORTH_FULL;
+        st = SolveStats{};
+        Solver<float> S(h, pf, st, ctx, jp);
+        S.run(evals, evecs, v0);
+        st.n_fallback = 1;
     }
 }
 
@@ -2121,7 +2165,18 @@ static void batch_run(DeviceHam& h, int cap, int nb, const BatchIO& io, const So
         for (int b : live) {
             try {
                 if (S[b]->advance()) {
-                    S[b]->end(io.evals[b], io.evecs ? io.evecs[b] : nullptr);
+                    try {
+                        S[b]->end(io.evals[b], io.evecs ? io.evecs[b] : nullptr);
+                    } catch (const Inconsistent&) {
+                        // redo this member alone on its own stream and workspace with
+                        // reorthogonalisation at every step
+                        SolveParams pf = io.p[b];
+                        pf.reorth = SMGPU_REORTH_FULL;
+                        io.st[b] = SolveStats{};
+                        Solver<TBv> R(io.hams ? *io.hams[b] : h, pf, io.st[b], ctx[b]);
+                        R.run(io.evals[b], io.evecs ? io.evecs[b] : nullptr, io.v0 ? io.v0[b] : nullptr);
+                        io.st[b].n_fallback = 1;
+                    }
                     busy[b] = 0;
                     if (on_done) on_done(b);
                 } else {

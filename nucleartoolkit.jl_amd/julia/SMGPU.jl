@@ -45,7 +45,7 @@ struct SolveStats               # smgpu_solve_stats
     n_reorth2::Int32
     n_jproj::Int32
     n_reorth::Int32
-    reserved_::Int32
+    n_fallback::Int32
 end
 
 struct ProblemInfo              # smgpu_problem_info

@@ -49,7 +49,7 @@ class SolveStats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int32), ("n_hv", ctypes.c_int32), ("n_hv64", ctypes.c_int32),
                 ("n_restart", ctypes.c_int32), ("wall_ms", ctypes.c_double), ("ritz", ctypes.c_double * 64),
                 ("n_reorth2", ctypes.c_int32), ("n_jproj", ctypes.c_int32), ("n_reorth", ctypes.c_int32),
-                ("reserved_", ctypes.c_int32)]
+                ("n_fallback", ctypes.c_int32)]
 
 
 class ServerConfig(ctypes.Structure):
@@ -60,7 +60,7 @@ class ServerConfig(ctypes.Structure):
 
 class RequestInfo(ctypes.Structure):
     _fields_ = [("state", ctypes.c_int32), ("status", ctypes.c_int32), ("num_ev", ctypes.c_int32),
-                ("index_shared", ctypes.c_int32), ("slot", ctypes.c_int32), ("pad_", ctypes.c_int32),
+                ("index_shared", ctypes.c_int32), ("slot", ctypes.c_int32), ("lm", ctypes.c_int32),
                 ("mdim", ctypes.c_int64), ("footprint", ctypes.c_int64), ("index_bytes", ctypes.c_int64),
                 ("submit_ms", ctypes.c_double),
                 ("prep_start_ms", ctypes.c_double), ("prep_end_ms", ctypes.c_double), ("admit_ms", ctypes.c_double),
@@ -99,6 +99,7 @@ SIGNATURES = {
     "smgpu_ham_create_variant": (I32, [P, P, P]),
     "smgpu_table_bytes": (I32, [P, I32, I32, P, P, ctypes.c_char_p, I32]),
     "smgpu_work_terms": (I32, [P, P]),
+    "smgpu_lds_bytes": (I32, [P, I32, P]),
     "smgpu_apply_H": (I32, [P, I32, P, P, P]),
     "smgpu_apply_H_host": (I32, [P, I32, P, P]),
     "smgpu_time_apply_H": (I32, [P, I32, I32, P]),

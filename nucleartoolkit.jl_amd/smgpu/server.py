@@ -32,12 +32,12 @@ def solve_params(num_ev, lm=300, ls=20, tol=1e-8, num_history=3, precision="mixe
 def _info_dict(info: RequestInfo):
     s = info.solve
     return dict(state=REQ_STATES.get(info.state, info.state), status=info.status, num_ev=info.num_ev,
-                index_shared=bool(info.index_shared), slot=info.slot, mdim=info.mdim, footprint=info.footprint,
+                index_shared=bool(info.index_shared), slot=info.slot, lm=info.lm, mdim=info.mdim, footprint=info.footprint,
                 index_bytes=info.index_bytes,
                 submit_ms=info.submit_ms, prep_start_ms=info.prep_start_ms, prep_end_ms=info.prep_end_ms,
                 admit_ms=info.admit_ms, done_ms=info.done_ms, latency_ms=info.latency_ms,
                 iterations=s.iterations, n_hv=s.n_hv, n_hv64=s.n_hv64, n_restart=s.n_restart, solve_ms=s.wall_ms,
-                n_reorth2=s.n_reorth2)
+                n_reorth2=s.n_reorth2, n_reorth=s.n_reorth, n_fallback=s.n_fallback)
 
 
 class Server:

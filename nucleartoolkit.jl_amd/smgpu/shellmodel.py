@@ -152,6 +152,12 @@ class Hamiltonian:
         names = buf.value.decode().strip().split("\n")
         return dict(zip(names, [int(x) for x in b[:n.value]]))
 
+    def lds_bytes(self, precision="mixed"):
+        """LDS bytes of one one-vector H·v launch, by part (smgpu_lds_bytes)."""
+        t = np.zeros(5, np.int64)
+        check(lib().smgpu_lds_bytes(self._h, _prec(precision), ptr(t)))
+        return dict(own_rows=int(t[0]), diag=int(t[1]), nn=int(t[2]), pn_stage=int(t[3]), pn_walk=int(t[4]))
+
     def work_terms(self):
         t = np.zeros(5, np.int64)
         check(lib().smgpu_work_terms(self._h, ptr(t)))
@@ -194,7 +200,7 @@ class Hamiltonian:
                                            ptr(vec) if vec is not None else None, ctypes.byref(st)))
         stats = dict(iterations=st.iterations, n_hv=st.n_hv, n_hv64=st.n_hv64, n_restart=st.n_restart,
                      wall_ms=st.wall_ms, ritz=np.array(st.ritz[:nev]), n_reorth2=st.n_reorth2,
-                     n_jproj=st.n_jproj, n_reorth=st.n_reorth)
+                     n_jproj=st.n_jproj, n_reorth=st.n_reorth, n_fallback=st.n_fallback)
         return ev, vec, stats
 
     def trl_batch(self, num_ev, seeds, lm=300, ls=20, tol=1e-8, num_history=3, precision="fp64", refine=True,
@@ -241,7 +247,7 @@ class Hamiltonian:
                                      ctypes.byref(st)))
         stats = dict(iterations=st.iterations, n_hv=st.n_hv, n_hv64=st.n_hv64, n_restart=st.n_restart,
                      wall_ms=st.wall_ms, ritz=np.array(st.ritz[:nev]), n_reorth2=st.n_reorth2,
-                     n_jproj=st.n_jproj, n_reorth=st.n_reorth)
+                     n_jproj=st.n_jproj, n_reorth=st.n_reorth, n_fallback=st.n_fallback)
         return ev, vec, stats
 
     def __del__(self):
@@ -280,7 +286,7 @@ def _solve_batch(hams, variants, num_ev, seeds, lm, ls, tol, num_history, precis
         st = stats[b]
         sd = dict(iterations=st.iterations, n_hv=st.n_hv, n_hv64=st.n_hv64, n_restart=st.n_restart,
                   wall_ms=st.wall_ms, ritz=np.array(st.ritz[:nev]), n_reorth2=st.n_reorth2,
-                  n_jproj=st.n_jproj, n_reorth=st.n_reorth)
+                  n_jproj=st.n_jproj, n_reorth=st.n_reorth, n_fallback=st.n_fallback)
         out.append((evs[b], vecs[b] if vecs is not None else None, sd))
     return out
 

@@ -71,7 +71,8 @@ def test_server_alone_vs_mix(snt_dir):
 
 def test_server_shared_basis_and_budget(snt_dir):
     """Same basis, different interactions: the index is uploaded once; a budget
-    of one request's footprint plus half its private part admits one at a time."""
+    of one request's footprint plus half its private part admits one at a time; a
+    request whose Krylov basis does not fit the budget runs with a smaller lm."""
     paths = [synth.config_snt(f"{snt_dir}/bud_{i}.snt", "sd", 3, 3, 8000 + i, (-6, 0), 1.2)[0] for i in range(4)]
     nuc = synth.nucleus_name("sd", 3, 3)
     with Server(max_inflight=1) as srv:
@@ -91,8 +92,16 @@ def test_server_shared_basis_and_budget(snt_dir):
     assert st["hbm_peak"] <= budget
     assert st["index_uploads"] == 1 and st["index_misses"] == 1 and st["index_hits"] == 3, st
     assert sum(i["index_shared"] for i in infos) == 3
-    # a budget below one request's footprint: ENOMEM for that request only
+    # a budget below one request's footprint at lm 300: the request runs with the
+    # largest Krylov basis that fits (thick restarts) and reaches the same levels
     with Server(max_inflight=2, hbm_budget=F // 2) as srv:
+        rid = srv.submit(paths[0], nuc, 4)
+        assert srv.wait(rid, 60000)
+        ev, info = srv.result(rid)
+        assert info["state"] == "done" and 40 <= info["lm"] < 300 and info["footprint"] <= F // 2, info
+        assert np.abs(ev - ev0).max() <= TOL_FP64
+    # a budget that cannot hold even the index tables: ENOMEM for that request only
+    with Server(max_inflight=2, hbm_budget=idx // 2) as srv:
         rid = srv.submit(paths[0], nuc, 4)
         assert srv.wait(rid, 60000)
         _, info = srv.result(rid, raise_on_error=False)
