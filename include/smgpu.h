@@ -142,7 +142,7 @@ int32_t smgpu_ham_create_variant(smgpu_ham base, smgpu_problem p, smgpu_ham* out
 int32_t smgpu_table_bytes(smgpu_ham h, int32_t precision, int32_t max_arrays, int64_t* bytes, int32_t* n_arrays,
                           char* names_buf, int32_t names_len);
 /* LDS bytes one one-vector H·v launch of `precision` moves, by part: parts[0] own
- * rows staged, [1] diagonal reads, [2] nn reads, [3] pn staging writes, [4] pn
+ * rows staged and pn sums moved to natural order, [1] diagonal reads, [2] nn reads, [3] pn staging writes, [4] pn
  * walk reads.  With the launch time this gives the kernel's LDS roofline.  No
  * counterpart in the reference (measurement only). */
 int32_t smgpu_lds_bytes(smgpu_ham h, int32_t precision, int64_t* parts);

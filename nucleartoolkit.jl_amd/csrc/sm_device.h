@@ -76,6 +76,7 @@ struct HvTables {
     const int64_t* ntab_slice_off;
     const int32_t* ntab_slice_len;
     const uint32_t* ntab_ent;         // in | col<<20 ; col = 2*bd + neg, pad col = 2*cols
+    const int32_t* nperm;             // per block (offset en_off): walk slot -> neutron SD
     const int64_t* veff_off;          // per dM
     const int32_t* veff_cols2;        // per dM: 2*(cols+1)
     const void* veff;                 // (T) signed copies: [row][2*bd+neg]
@@ -98,7 +99,7 @@ struct DeviceIndex {
     DBuf grp_block[2], grp_fp0[2];   // row groups per precision (cost order)
     DBuf row_block, block_row0, nP, nN, tdims, en_off, pp_rowptr, pp_col, nn_slice_base, nn_slice_off,
         nn_slice_len, nn_col, pair_ptr, pair_bi, pair_dm, pair_ptab, pair_ntab, ptab_rowptr_base,
-        ptab_rowptr, ptab_ent_base, ptab_ent, ntab_slice_base, ntab_slice_off, ntab_slice_len, ntab_ent,
+        ptab_rowptr, ptab_ent_base, ptab_ent, ntab_slice_base, ntab_slice_off, ntab_slice_len, ntab_ent, nperm,
         veff_off, veff_cols2;
     HvTables view[2];                // index part (coefficient pointers unset)
     uint64_t pattern = 0;            // fingerprint of the pp / nn sparsity patterns

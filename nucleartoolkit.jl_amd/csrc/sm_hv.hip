@@ -3,17 +3,18 @@
 // Output element (bf, fp, fn) (block, proton SD, neutron SD; global index
 // tdims[bf] + fp*nN + fn, the reference's order) is owned by one lane; all
 // contributions are gathered, so no atomics are needed and the result does not
-// depend on the launch shape:
-//   diagonal : (e_p[fp] + e_n[fn]) v[fp,fn]
-//   nn       : sum_k c_k v[fp, col_k]         (row fp of v staged in LDS)
-//   pp       : sum_e c_e v[ip_e, fn]          (coalesced rows of the same block)
+// depend on the launch shape.  In the order they are added:
 //   pn       : for every input block bi coupled by a proton dM, for every proton
 //              hop ip -> fp (operator ac, sign s_p):
 //                sum_{neutron hops in -> fn (operator bd, sign s_n)}
 //                    s_p s_n V_eff[dM][ac][bd] v[bi: ip, in]
 //              The input rows s_p v[ip,:] and the V_eff rows of the hop's
 //              operators are staged in LDS; each lane walks its own neutron hop
-//              list (sliced ELL: 64 consecutive fn per slice, coalesced).
+//              list (sliced ELL: 64 fn per slice, in the block's walk order
+//              nperm, coalesced); the sums then move to the natural order
+//   diagonal : (e_p[fp] + e_n[fn]) v[fp,fn]
+//   nn       : sum_k c_k v[fp, col_k]         (row fp of v staged in LDS)
+//   pp       : sum_e c_e v[ip_e, fn]          (coalesced rows of the same block)
 // The reference's loop nest for the same sums is operate_H!
 // (src/ShellModel/lanczos_methods.jl:153-248).
 #include <algorithm>
@@ -112,94 +113,12 @@ __global__ __launch_bounds__(TPB) void k_hv(HvTables A, HvVecs P) {
     // are never stored) and read the zero padding of the ELL tables.
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int nsl = (nN + 63) >> 6;
-    // diagonal (e_p + e_n) and nn (sliced ELL, entries loaded four at a time)
 #pragma unroll
-    for (int k = 0; k < NPT; k++) {
+    for (int k = 0; k < NPT; k++)
 #pragma unroll
         for (int r = 0; r < RG; r++)
 #pragma unroll
             for (int b = 0; b < NV; b++) acc[k][r][b] = T(0);
-        const int sk = wave + NW * k;
-        if (sk < nsl) {
-            const int fnc = min(tid + k * TPB, nN - 1);
-            T e_n[NC];
-#pragma unroll
-            for (int c = 0; c < NC; c++) e_n[c] = cen(c)[fnc];
-#pragma unroll
-            for (int r = 0; r < RG; r++)
-#pragma unroll
-                for (int b = 0; b < NV; b++)
-                    acc[k][r][b] = (epr[r][PV ? b : 0] + e_n[PV ? b : 0]) * xs[(fnc * RG + r) * NV + b];
-            const int64_t s = nsb + sk;
-            const int64_t off = A.nn_slice_off[s] + (tid & 63);
-            const int len = __builtin_amdgcn_readfirstlane(A.nn_slice_len[s]);
-            int q = 0;
-            for (; q + 4 <= len; q += 4) {
-                T c[4][NC];
-                int col[4];
-#pragma unroll
-                for (int u = 0; u < 4; u++) {
-#pragma unroll
-                    for (int cc = 0; cc < NC; cc++) c[u][cc] = cnn(cc)[off + (int64_t)(q + u) * 64];
-                    col[u] = A.nn_col[off + (int64_t)(q + u) * 64];
-                }
-#pragma unroll
-                for (int u = 0; u < 4; u++)
-#pragma unroll
-                    for (int r = 0; r < RG; r++)
-#pragma unroll
-                        for (int b = 0; b < NV; b++)
-                            acc[k][r][b] += c[u][PV ? b : 0] * xs[(col[u] * RG + r) * NV + b];
-            }
-            for (; q < len; q++) {
-                T c[NC];
-#pragma unroll
-                for (int cc = 0; cc < NC; cc++) c[cc] = cnn(cc)[off + (int64_t)q * 64];
-                const int col = A.nn_col[off + (int64_t)q * 64];
-#pragma unroll
-                for (int r = 0; r < RG; r++)
-#pragma unroll
-                    for (int b = 0; b < NV; b++) acc[k][r][b] += c[PV ? b : 0] * xs[(col * RG + r) * NV + b];
-            }
-        }
-    }
-    // pp: rows of the same block; UP entries (uniform) per step, their row gathers
-    // issued together as unconditional loads
-    {
-        constexpr int UP = NPT * NV <= 2 ? 8 : (NPT * NV <= 4 ? 4 : 2);
-#pragma unroll
-        for (int r = 0; r < RG; r++) {
-            if (r < nr) {
-                const int64_t pe0 = A.pp_rowptr[row0 + r], pe1 = A.pp_rowptr[row0 + r + 1];
-                for (int64_t e = pe0; e < pe1; e += UP) {
-                    T c[UP][NC];
-                    int64_t so[UP];
-#pragma unroll
-                    for (int u = 0; u < UP; u++) {
-                        const int64_t eu = min(e + u, pe1 - 1);
-#pragma unroll
-                        for (int cc = 0; cc < NC; cc++) c[u][cc] = e + u < pe1 ? cpp(cc)[eu] : T(0);
-                        so[u] = bstart + (int64_t)A.pp_col[eu] * nN;
-                    }
-#pragma unroll
-                    for (int k = 0; k < NPT; k++) {
-                        if (wave + NW * k >= nsl) continue;         // wave-uniform
-                        const int fnc = min(tid + k * TPB, nN - 1);
-#pragma unroll
-                        for (int b = 0; b < NV; b++) {
-                            T x[UP];
-#pragma unroll
-                            for (int u = 0; u < UP; u++) x[u] = static_cast<const T*>(P.v[b])[so[u] + fnc];
-                            T a = acc[k][r][b];
-#pragma unroll
-                            for (int u = 0; u < UP; u++) a += c[u][PV ? b : 0] * x[u];
-                            acc[k][r][b] = a;
-                        }
-                    }
-                }
-            }
-        }
-    }
     // pn
     const int q1 = A.pair_ptr[bf + 1];
     for (int q = A.pair_ptr[bf]; q < q1; q++) {
@@ -427,6 +346,118 @@ __global__ __launch_bounds__(TPB) void k_hv(HvTables A, HvVecs P) {
             }
         }
     }
+    // The pn sums were accumulated in the block's walk order (lane slot j of column k
+    // holds neutron SD nperm[j]); move them to the natural order through LDS (the X
+    // region, free after the last round) before the diagonal, nn and pp terms.
+    if (q1 > A.pair_ptr[bf]) {
+        __syncthreads();
+        T* Px = X;
+        const int32_t* pm = A.nperm + A.en_off[bf];
+#pragma unroll
+        for (int k = 0; k < NPT; k++) {
+            const int j = tid + k * TPB;
+            if (j < nN) {
+                const int f = pm[j];
+#pragma unroll
+                for (int r = 0; r < RG; r++)
+#pragma unroll
+                    for (int b = 0; b < NV; b++) Px[(f * RG + r) * NV + b] = acc[k][r][b];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < NPT; k++) {
+            const int j = tid + k * TPB;
+#pragma unroll
+            for (int r = 0; r < RG; r++)
+#pragma unroll
+                for (int b = 0; b < NV; b++) acc[k][r][b] = j < nN ? Px[(j * RG + r) * NV + b] : T(0);
+        }
+    }
+    // diagonal (e_p + e_n) and nn (sliced ELL, entries loaded four at a time)
+#pragma unroll
+    for (int k = 0; k < NPT; k++) {
+        const int sk = wave + NW * k;
+        if (sk < nsl) {
+            const int fnc = min(tid + k * TPB, nN - 1);
+            T e_n[NC];
+#pragma unroll
+            for (int c = 0; c < NC; c++) e_n[c] = cen(c)[fnc];
+#pragma unroll
+            for (int r = 0; r < RG; r++)
+#pragma unroll
+                for (int b = 0; b < NV; b++)
+                    acc[k][r][b] += (epr[r][PV ? b : 0] + e_n[PV ? b : 0]) * xs[(fnc * RG + r) * NV + b];
+            const int64_t s = nsb + sk;
+            const int64_t off = A.nn_slice_off[s] + (tid & 63);
+            const int len = __builtin_amdgcn_readfirstlane(A.nn_slice_len[s]);
+            int q = 0;
+            for (; q + 4 <= len; q += 4) {
+                T c[4][NC];
+                int col[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+#pragma unroll
+                    for (int cc = 0; cc < NC; cc++) c[u][cc] = cnn(cc)[off + (int64_t)(q + u) * 64];
+                    col[u] = A.nn_col[off + (int64_t)(q + u) * 64];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+#pragma unroll
+                    for (int r = 0; r < RG; r++)
+#pragma unroll
+                        for (int b = 0; b < NV; b++)
+                            acc[k][r][b] += c[u][PV ? b : 0] * xs[(col[u] * RG + r) * NV + b];
+            }
+            for (; q < len; q++) {
+                T c[NC];
+#pragma unroll
+                for (int cc = 0; cc < NC; cc++) c[cc] = cnn(cc)[off + (int64_t)q * 64];
+                const int col = A.nn_col[off + (int64_t)q * 64];
+#pragma unroll
+                for (int r = 0; r < RG; r++)
+#pragma unroll
+                    for (int b = 0; b < NV; b++) acc[k][r][b] += c[PV ? b : 0] * xs[(col * RG + r) * NV + b];
+            }
+        }
+    }
+    // pp: rows of the same block; UP entries (uniform) per step, their row gathers
+    // issued together as unconditional loads
+    {
+        constexpr int UP = NPT * NV <= 2 ? 8 : (NPT * NV <= 4 ? 4 : 2);
+#pragma unroll
+        for (int r = 0; r < RG; r++) {
+            if (r < nr) {
+                const int64_t pe0 = A.pp_rowptr[row0 + r], pe1 = A.pp_rowptr[row0 + r + 1];
+                for (int64_t e = pe0; e < pe1; e += UP) {
+                    T c[UP][NC];
+                    int64_t so[UP];
+#pragma unroll
+                    for (int u = 0; u < UP; u++) {
+                        const int64_t eu = min(e + u, pe1 - 1);
+#pragma unroll
+                        for (int cc = 0; cc < NC; cc++) c[u][cc] = e + u < pe1 ? cpp(cc)[eu] : T(0);
+                        so[u] = bstart + (int64_t)A.pp_col[eu] * nN;
+                    }
+#pragma unroll
+                    for (int k = 0; k < NPT; k++) {
+                        if (wave + NW * k >= nsl) continue;         // wave-uniform
+                        const int fnc = min(tid + k * TPB, nN - 1);
+#pragma unroll
+                        for (int b = 0; b < NV; b++) {
+                            T x[UP];
+#pragma unroll
+                            for (int u = 0; u < UP; u++) x[u] = static_cast<const T*>(P.v[b])[so[u] + fnc];
+                            T a = acc[k][r][b];
+#pragma unroll
+                            for (int u = 0; u < UP; u++) a += c[u][PV ? b : 0] * x[u];
+                            acc[k][r][b] = a;
+                        }
+                    }
+                }
+            }
+        }
+    }
 #pragma unroll
     for (int r = 0; r < RG; r++) {
         if (r < nr) {
@@ -629,6 +660,7 @@ struct HostLayout {
     std::vector<uint32_t> ptab_ent;
     std::vector<int64_t> ntab_slice_base, ntab_slice_off;
     std::vector<int32_t> ntab_slice_len;
+    std::vector<int32_t> nperm;        // per block (offset en_off): pn lane slot -> neutron SD
     std::vector<uint32_t> ntab_ent;
     std::vector<int64_t> veff_off;
     std::vector<int32_t> veff_cols2;
@@ -724,7 +756,43 @@ void index_layout(const Basis& b, const Tables& t, HostLayout& L) {
         }
         for (int f = 0; f < nN; f++) L.nn_terms += (m.rowptr[f + 1] - m.rowptr[f]) * (int64_t)L.nP[k];
     }
-    // hop tables: proton tables keep CSR, neutron tables become sliced ELL
+    // The pn walk visits a block's neutron SDs in an order of its own: sorted by their
+    // walk cost summed over the block's input blocks (entries x staging rounds), so
+    // the 64 SDs of a slice have similar entry counts and the wave-uniform walk to the
+    // slice's longest row pads less.  Measured on the tables: padded / exact pn terms
+    // Cr48 1.352 -> 1.232, Mn50 1.292 -> 1.137, Fe52 1.266 -> 1.091.  The kernel
+    // accumulates in this order and moves the sums to the natural order once.  Measured
+    // against the natural order (ms per launch FP32 / FP64): Cr48 0.460 -> 0.450 /
+    // 0.680 -> 0.671, Mn50 5.49 -> 5.32 / 9.13 -> 8.77, Fe52 40.5 -> 39.4 / 78.1 -> 76.0;
+    // sorting only within windows of 128-512 SDs gained less.
+    std::vector<int> block_of_nset(b.sp[1].sets.size(), -1);
+    for (int k = 0; k < b.nb; k++) block_of_nset[b.nset[k]] = k;
+    std::vector<std::vector<int32_t>> perm(b.nb);
+    {
+        std::vector<std::vector<double>> key(b.nb);
+        for (int k = 0; k < b.nb; k++) key[k].assign(L.nN[k], 0.0);
+        for (int bf = 0; bf < b.nb; bf++)
+            for (int bi = 0; bi < b.nb; bi++) {
+                const int dM = b.Mp[bf] - b.Mp[bi];
+                if (t.ob[0].index_of_dM(dM) < 0 || t.ob[1].index_of_dM(-dM) < 0) continue;
+                auto ip = t.hop_index.find({0, b.pset[bi], b.pset[bf]});
+                auto in = t.hop_index.find({1, b.nset[bi], b.nset[bf]});
+                if (ip == t.hop_index.end() || in == t.hop_index.end()) continue;
+                const HopTable& HP = t.hops[ip->second];
+                const HopTable& HN = t.hops[in->second];
+                double rounds = 0;
+                for (size_t r = 0; r + 1 < HP.rowptr.size(); r++) rounds += (double)((HP.rowptr[r + 1] - HP.rowptr[r] + 3) / 4);
+                for (int f = 0; f < L.nN[bf]; f++) key[bf][f] += rounds * (double)(HN.rowptr[f + 1] - HN.rowptr[f]);
+            }
+        for (int k = 0; k < b.nb; k++) {
+            perm[k].resize(L.nN[k]);
+            for (int f = 0; f < L.nN[k]; f++) perm[k][f] = f;
+            std::stable_sort(perm[k].begin(), perm[k].end(), [&](int a, int c) { return key[k][a] > key[k][c]; });
+            L.nperm.insert(L.nperm.end(), perm[k].begin(), perm[k].end());
+        }
+    }
+    // hop tables: proton tables keep CSR, neutron tables become sliced ELL (rows in
+    // the output block's walk order)
     std::vector<int> pt_of(t.hops.size(), -1), nt_of(t.hops.size(), -1);
     for (size_t h = 0; h < t.hops.size(); h++) {
         const HopTable& H = t.hops[h];
@@ -740,18 +808,23 @@ void index_layout(const Basis& b, const Tables& t, HostLayout& L) {
             int kp = t.ob[0].index_of_dM(-H.dM);
             int cols = kp >= 0 ? t.veff_cols[kp] : 0;
             int nf = (int)H.rowptr.size() - 1;
+            const int bfk = block_of_nset[H.fset];
+            if (bfk < 0 || (int)perm[bfk].size() != nf) throw std::logic_error("neutron table without its block");
+            const std::vector<int32_t>& pm = perm[bfk];
             for (int s0 = 0; s0 < nf; s0 += 64) {
                 // at least one row per slice (zero-column entries): the kernel reads a
                 // slice's last row unconditionally when prefetching past its length
                 int len = 1;
-                for (int f = s0; f < std::min(nf, s0 + 64); f++) len = std::max<int>(len, (int)(H.rowptr[f + 1] - H.rowptr[f]));
+                for (int j = s0; j < std::min(nf, s0 + 64); j++)
+                    len = std::max<int>(len, (int)(H.rowptr[pm[j] + 1] - H.rowptr[pm[j]]));
                 L.ntab_slice_off.push_back((int64_t)L.ntab_ent.size());
                 L.ntab_slice_len.push_back(len);
                 for (int q = 0; q < len; q++)
                     for (int lane = 0; lane < 64; lane++) {
-                        int f = s0 + lane;
+                        const int j = s0 + lane;
                         uint32_t ent = (uint32_t)(2 * cols) << 20;         // zero column
-                        if (f < nf && H.rowptr[f] + q < H.rowptr[f + 1]) {
+                        const int f = j < nf ? pm[j] : 0;
+                        if (j < nf && H.rowptr[f] + q < H.rowptr[f + 1]) {
                             uint32_t e = H.ent[H.rowptr[f] + q];
                             uint32_t in = e & 0xFFFFFu, op = (e >> 20) & 0x7FFu, neg = e >> 31;
                             ent = in | ((2 * op + neg) << 20);
@@ -791,9 +864,11 @@ void index_layout(const Basis& b, const Tables& t, HostLayout& L) {
             for (size_t f = 0; f + 1 < HN.rowptr.size(); f++) nsum += HN.rowptr[f + 1] - HN.rowptr[f];
             int64_t npad = 0;
             int nf = (int)HN.rowptr.size() - 1;
+            const std::vector<int32_t>& pm = perm[bf];
             for (int s0 = 0; s0 < nf; s0 += 64) {
                 int len = 1;
-                for (int f = s0; f < std::min(nf, s0 + 64); f++) len = std::max<int>(len, (int)(HN.rowptr[f + 1] - HN.rowptr[f]));
+                for (int j = s0; j < std::min(nf, s0 + 64); j++)
+                    len = std::max<int>(len, (int)(HN.rowptr[pm[j] + 1] - HN.rowptr[pm[j]]));
                 npad += (int64_t)len * std::min(64, nf - s0);
             }
             int64_t psum = (int64_t)HP.ent.size();
@@ -806,7 +881,8 @@ void index_layout(const Basis& b, const Tables& t, HostLayout& L) {
 
 
 // LDS elements one one-vector k_hv launch moves (RG rows per workgroup, four hop
-// slots per round; bytes = elements x the element size): [0] own rows staged, [1]
+// slots per round; bytes = elements x the element size): [0] own rows staged and
+// the pn sums moved from walk order to natural order, [1]
 // diagonal reads, [2] nn reads, [3] pn staging writes (X rows and V_eff rows of the
 // hops), [4] pn walk reads (per neutron-entry step of an active wave, one four-slot
 // read of X and one of V_eff per staged row; lanes past the block edge included).
@@ -831,7 +907,7 @@ void lds_model(const HostLayout& L, int rg, const std::vector<int32_t>& gb, cons
         const int bf = gb[g], fp0 = gf[g];
         const int nN = L.nN[bf], nsl = (nN + 63) / 64;
         const int nr = std::min(rg, L.nP[bf] - fp0);
-        e[0] += (int64_t)nN * rg;
+        e[0] += (int64_t)nN * rg * (L.pair_ptr[bf + 1] > L.pair_ptr[bf] ? 3 : 1);
         e[1] += (int64_t)64 * nsl * rg;
         e[2] += nn_len[bf] * 64 * rg;
         for (int q = L.pair_ptr[bf]; q < L.pair_ptr[bf + 1]; q++) {
@@ -866,6 +942,7 @@ template <typename T> void set_view(DeviceIndex& h, int prec, const HostLayout& 
     A.ptab_ent_base = h.ptab_ent_base.as<int64_t>(); A.ptab_ent = h.ptab_ent.as<uint32_t>();
     A.ntab_slice_base = h.ntab_slice_base.as<int64_t>(); A.ntab_slice_off = h.ntab_slice_off.as<int64_t>();
     A.ntab_slice_len = h.ntab_slice_len.as<int32_t>(); A.ntab_ent = h.ntab_ent.as<uint32_t>();
+    A.nperm = h.nperm.as<int32_t>();
     A.veff_off = h.veff_off.as<int64_t>(); A.veff_cols2 = h.veff_cols2.as<int32_t>(); A.veff = nullptr;
     A.nN_max = std::max(1, L.nN_max);
     A.cols2_max = std::max(2, L.cols2_max);
@@ -1016,6 +1093,7 @@ int64_t device_bytes(const HostIndex& x) {
            vbytes(L.pair_bi) + vbytes(L.pair_dm) + vbytes(L.pair_ptab) + vbytes(L.pair_ntab) +
            vbytes(L.ptab_rowptr_base) + vbytes(L.ptab_rowptr) + vbytes(L.ptab_ent_base) + vbytes(L.ptab_ent) +
            vbytes(L.ntab_slice_base) + vbytes(L.ntab_slice_off) + vbytes(L.ntab_slice_len) + vbytes(L.ntab_ent) +
+           vbytes(L.nperm) +
            vbytes(L.veff_off) + vbytes(L.veff_cols2) + 2 * 2 * rows * (int64_t)sizeof(int32_t);
 }
 
@@ -1064,7 +1142,7 @@ std::shared_ptr<DeviceIndex> upload_index(const HostIndex& x, int device) {
     h->ptab_rowptr_base.upload(L.ptab_rowptr_base); h->ptab_rowptr.upload(L.ptab_rowptr);
     h->ptab_ent_base.upload(L.ptab_ent_base); h->ptab_ent.upload(L.ptab_ent);
     h->ntab_slice_base.upload(L.ntab_slice_base); h->ntab_slice_off.upload(L.ntab_slice_off);
-    h->ntab_slice_len.upload(L.ntab_slice_len); h->ntab_ent.upload(L.ntab_ent);
+    h->ntab_slice_len.upload(L.ntab_slice_len); h->ntab_ent.upload(L.ntab_ent); h->nperm.upload(L.nperm);
     h->veff_off.upload(L.veff_off); h->veff_cols2.upload(L.veff_cols2);
     set_view<double>(*h, 0, L);
     set_view<float>(*h, 1, L);
@@ -1086,7 +1164,7 @@ int64_t DeviceIndex::device_bytes() const {
                           &tdims, &en_off, &pp_rowptr, &pp_col, &nn_slice_base, &nn_slice_off, &nn_slice_len,
                           &nn_col, &pair_ptr, &pair_bi, &pair_dm, &pair_ptab, &pair_ntab, &ptab_rowptr_base,
                           &ptab_rowptr, &ptab_ent_base, &ptab_ent, &ntab_slice_base, &ntab_slice_off,
-                          &ntab_slice_len, &ntab_ent, &veff_off, &veff_cols2})
+                          &ntab_slice_len, &ntab_ent, &nperm, &veff_off, &veff_cols2})
         s += (int64_t)d->cap;
     return s;
 }
@@ -1153,6 +1231,7 @@ std::vector<std::pair<std::string, int64_t>> DeviceHam::table_bytes(int prec) co
                &pair_ntab = I.pair_ntab, &ptab_rowptr_base = I.ptab_rowptr_base, &ptab_rowptr = I.ptab_rowptr,
                &ptab_ent_base = I.ptab_ent_base, &ptab_ent = I.ptab_ent, &ntab_slice_base = I.ntab_slice_base,
                &ntab_slice_off = I.ntab_slice_off, &ntab_slice_len = I.ntab_slice_len, &ntab_ent = I.ntab_ent,
+               &nperm = I.nperm,
                &veff_off = I.veff_off, &veff_cols2 = I.veff_cols2;
     std::vector<std::pair<std::string, int64_t>> v = {
         {"row_block", (int64_t)row_block.bytes}, {"block_row0", (int64_t)block_row0.bytes},
@@ -1171,6 +1250,7 @@ std::vector<std::pair<std::string, int64_t>> DeviceHam::table_bytes(int prec) co
         {"ptab_ent_base", (int64_t)ptab_ent_base.bytes}, {"ptab_ent", (int64_t)ptab_ent.bytes},
         {"ntab_slice_base", (int64_t)ntab_slice_base.bytes}, {"ntab_slice_off", (int64_t)ntab_slice_off.bytes},
         {"ntab_slice_len", (int64_t)ntab_slice_len.bytes}, {"ntab_ent", (int64_t)ntab_ent.bytes},
+        {"nperm", (int64_t)nperm.bytes},
         {"veff_off", (int64_t)veff_off.bytes}, {"veff_cols2", (int64_t)veff_cols2.bytes},
         {"veff", (int64_t)veff[prec].bytes},
         {"vector_in", (int64_t)(mdim * s)}, {"vector_out", (int64_t)(mdim * s)}};
