@@ -18,7 +18,14 @@ tol 1e-8).
                the time from the first arrival to the last result.
   FP64 leg   : the same requests, same arrival times, through a one-slot server
                in plain FP64 (every vector FP64, full reorthogonalisation every
-               step): the plain FP64 TRL one solve at a time.
+               step): the plain FP64 TRL one solve at a time.  It takes about
+               570 s on one MI355X, so it is recorded by `--leg fp64` (eigenvalues,
+               per-request solve times and latencies, source hash) into
+               profiles/stream512_fp64_leg.json; the default run serves the stream
+               live, checks every result against the recorded FP64 eigenvalues,
+               and re-runs a seeded sample of the FP64 solves live (about a minute
+               of solve time) to show the record still holds.  Without a record
+               (or for another stream) the FP64 leg runs live in full.
 Latencies of both legs are arrival-to-result.  Beside them: the H·v kernel's
 timing and rooflines on the stream's largest basis (HIP events on the launch
 stream) and the oracle's CPU restatement of operate_H!/TRL timed on this host.
@@ -107,7 +114,7 @@ def _mgs_seconds_per_element(n=2_000_000, k=16):
     return (time.perf_counter() - t0) / (k * n)
 
 
-def cpu_baseline_stream(reqs, fp64_res, work, budget_s=25.0):
+def cpu_baseline_stream(reqs, fp64_hv, work, budget_s=25.0):
     """The oracle's restatement (C operate_H! with OpenMP over blocks, numpy MGS
     ReORTH) on this host.  Measured: (a) one full FP64 TRL solve of the C1 workload
     (Mg24, lowest 5) end to end; (b) the median operate_H! of the stream's bases up
@@ -158,10 +165,10 @@ def cpu_baseline_stream(reqs, fp64_res, work, budget_s=25.0):
     sec_per_term = float(np.median(per_term))
     sec_per_elem = _mgs_seconds_per_element()
     total = 0.0
-    for r, (ev, info) in zip(reqs, fp64_res):
+    for r, (nhv, lm) in zip(reqs, fp64_hv):
         w = work[(r.space, r.zv, r.nv)]
-        nhv = max(1, info.get("n_hv", 0))
-        basis_avg = min(nhv, info.get("lm", 300)) / 2.0
+        nhv = max(1, nhv)
+        basis_avg = min(nhv, lm) / 2.0
         total += nhv * (w["terms"] * sec_per_term + basis_avg * w["mdim"] * sec_per_elem)
     out.update({"value": len(reqs) / total, "hv_bases": seen, "seconds_per_term": sec_per_term,
                 "mgs_seconds_per_element": sec_per_elem, "stream_seconds_estimate": total,
@@ -193,6 +200,18 @@ def hv_roofline(ham, reps, lds_model=True):
     out["lds_bytes_fp32"] = lds
     out["lds_gbs_fp32"] = sum(lds.values()) / (t32 * 1e-3) / 1e9
     return out
+
+
+def source_hash():
+    """sha256 (16 hex) of the library sources: the kernels and solver a record was taken with."""
+    import hashlib
+    d = os.path.join(ROOT, "nucleartoolkit.jl_amd", "csrc")
+    h = hashlib.sha256()
+    for name in sorted(os.listdir(d)):
+        if name.endswith((".hip", ".cpp", ".h")):
+            h.update(name.encode())
+            h.update(open(os.path.join(d, name), "rb").read())
+    return h.hexdigest()[:16]
 
 
 def profiled_traffic(config):
@@ -271,33 +290,96 @@ def run_stream(args):
             srv.wait(rid)
             srv.result(rid)
 
+    if args.leg == "fp64":
+        seq = Server(max_inflight=1, host_threads=args.host_threads, max_batch=1)
+        warmup(seq, "fp64")
+        sres, t_seq = _serve(reqs, seq, "fp64")
+        seq.close()
+        rec = {"what": "sequential FP64 leg of the C4 serving stream: the same requests and arrival times as the "
+                       "served leg, one slot, plain FP64 TRL", "source_sha16": source_hash(),
+               "requests": len(reqs), "rate_per_s": args.rate, "particles": [lo, hi], "wall_s": t_seq,
+               "host": platform.node(), "date": time.strftime("%Y-%m-%d %H:%M:%S"),
+               "per_request": [{"i": i, "nucleus": r.nucleus, "k": r.num_ev, "state": info["state"],
+                                "ev": [float("%.12g" % x) for x in ev], "solve_ms": round(info["solve_ms"], 3),
+                                "latency_ms": round(info["latency_ms"], 3), "n_hv": info["n_hv"], "lm": info["lm"]}
+                               for i, (r, (ev, info)) in enumerate(zip(reqs, sres))]}
+        with open(args.record, "w") as f:
+            json.dump(rec, f, separators=(",", ":"))
+        done = sum(1 for _, i in sres if i["state"] == "done")
+        print(json.dumps({"leg": "fp64", "record": args.record, "wall_s": t_seq, "done": done,
+                          "solves_per_s": done / t_seq, "source_sha16": rec["source_sha16"]}), flush=True)
+        return
     srv = Server(max_inflight=args.inflight, host_threads=args.host_threads, max_batch=args.batch)
     warmup(srv, "mixed")
     res, t_all = _serve(reqs, srv, "mixed")
     sst = srv.stats()
     srv.close()
     print(f"# served leg: {t_all:.1f} s", file=sys.stderr, flush=True)
-    seq = Server(max_inflight=1, host_threads=args.host_threads, max_batch=1)
-    warmup(seq, "fp64")
-    sres, t_seq = _serve(reqs, seq, "fp64")
-    seq.close()
-    print(f"# FP64 leg: {t_seq:.1f} s", file=sys.stderr, flush=True)
+    # the sequential FP64 leg: the recorded full run of this stream (its eigenvalues check
+    # every served result), and a live re-run of a sample of its requests
+    rec = json.load(open(args.record)) if os.path.exists(args.record) else None
+    if rec is not None and (rec["requests"], rec["rate_per_s"], rec["particles"]) != (len(reqs), args.rate, [lo, hi]):
+        rec = None
+    if rec is None:
+        seq = Server(max_inflight=1, host_threads=args.host_threads, max_batch=1)
+        warmup(seq, "fp64")
+        sres, t_seq = _serve(reqs, seq, "fp64")
+        seq.close()
+        fp64_src = {"source": "measured in this run"}
+        ev64 = [ev for ev, _ in sres]
+        st64 = [i["state"] for _, i in sres]
+        lat64 = [i["latency_ms"] for _, i in sres if i["state"] == "done"]
+        nhv64 = [(i["n_hv"], i["lm"]) for _, i in sres]
+    else:
+        t_seq = rec["wall_s"]
+        pr = rec["per_request"]
+        ev64 = [np.array(q["ev"]) for q in pr]
+        st64 = [q["state"] for q in pr]
+        lat64 = [q["latency_ms"] for q in pr if q["state"] == "done"]
+        nhv64 = [(q["n_hv"], q["lm"]) for q in pr]
+        # live spot check: a seeded sample of the recorded requests, solved again one at a
+        # time in FP64, within about a minute of recorded solve time
+        rng = np.random.default_rng(7)
+        sample, budget = [], 0.0
+        for i in rng.permutation(len(pr)):
+            if budget + pr[i]["solve_ms"] * 1e-3 > args.spot_s or len(sample) >= 24:
+                continue
+            sample.append(int(i))
+            budget += pr[i]["solve_ms"] * 1e-3
+        seq = Server(max_inflight=1, host_threads=args.host_threads, max_batch=1)
+        warmup(seq, "fp64")
+        live = []
+        for i in sorted(sample):
+            r = reqs[i]
+            rid = seq.submit(r.snt, r.nucleus, r.num_ev, precision="fp64", seed=r.seed % (1 << 31))
+            seq.wait(rid)
+            ev, info = seq.result(rid, raise_on_error=False)
+            live.append({"i": i, "nucleus": r.nucleus, "solve_ms": info["solve_ms"],
+                         "recorded_ms": pr[i]["solve_ms"],
+                         "max_abs_dev_MeV": float(np.abs(ev - ev64[i]).max()) if len(ev) == len(ev64[i]) else None})
+        seq.close()
+        ratio = sum(q["solve_ms"] for q in live) / max(1e-9, sum(q["recorded_ms"] for q in live))
+        fp64_src = {"source": f"recorded sequential FP64 run ({os.path.relpath(args.record, ROOT)}, {rec['date']}, "
+                              f"same requests and arrival times)",
+                    "source_sha16_recorded": rec["source_sha16"], "source_sha16_now": source_hash(),
+                    "spot_check": {"requests": len(live), "live_over_recorded_time": ratio,
+                                   "max_abs_dev_MeV": max((q["max_abs_dev_MeV"] or 0.0) for q in live) if live
+                                   else None, "solves": live}}
     dE, ok, bad = [], 0, []
-    for i, ((a, ia), (b, ib)) in enumerate(zip(res, sres)):
-        if ia["state"] == "done" and ib["state"] == "done" and len(a) == len(b) > 0:
-            d = float(np.abs(a - b).max())
+    for i, (a, ia) in enumerate(res):
+        if ia["state"] == "done" and st64[i] == "done" and len(a) == len(ev64[i]) > 0:
+            d = float(np.abs(a - ev64[i]).max())
             dE.append(d)
             if d <= TOL_MEV:
                 ok += 1
             else:
                 bad.append({"request": i, "nucleus": reqs[i].nucleus, "k": reqs[i].num_ev, "dE_MeV": d})
         else:
-            bad.append({"request": i, "nucleus": reqs[i].nucleus, "state": [ia["state"], ib["state"]],
-                        "error": [ia.get("error", ""), ib.get("error", "")]})
+            bad.append({"request": i, "nucleus": reqs[i].nucleus, "state": [ia["state"], st64[i]],
+                        "error": ia.get("error", "")})
     value = ok / t_all
-    fp64_rate = sum(1 for _, i in sres if i["state"] == "done") / t_seq
+    fp64_rate = sum(1 for x in st64 if x == "done") / t_seq
     lat = [i["latency_ms"] for _, i in res if i["state"] == "done"]
-    lat64 = [i["latency_ms"] for _, i in sres if i["state"] == "done"]
     mdims = [i["mdim"] for _, i in res]
     # kernel rooflines on the stream's largest basis (the pf 6p6n requests dominate its H·v time)
     big = max(work.values(), key=lambda w: w["mdim"])
@@ -312,17 +394,27 @@ def run_stream(args):
             t = h.work_terms()
             w["terms"] = t["pn"] + t["pp"] + t["nn"] + t["diag"]
             del h
-        cpu = cpu_baseline_stream(reqs, sres, work)
+        cpu = cpu_baseline_stream(reqs, nhv64, work)
     traffic, traffic_src = profiled_traffic("fe52")
     roof = {"bound": "hbm", "achieved": hv["gbs_fp32"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": hv["gbs_fp32"] / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
             "kernel": f"k_hv FP32 one vector on {big['nucleus']} (mdim {big['mdim']})",
             "lds": {"achieved": hv["lds_gbs_fp32"], "peak": LDS_PEAK_GBS, "unit": "GB/s",
                     "frac": hv["lds_gbs_fp32"] / LDS_PEAK_GBS}}
+    # per basis: requests, and each leg's summed solve time, H·v count and smallest lm
+    fp64_ms = [q["solve_ms"] for q in rec["per_request"]] if rec is not None else [i["solve_ms"] for _, i in sres]
     counts = {}
-    for r in reqs:
+    for r, (_, ia), t64, (nh64, lm64) in zip(reqs, res, fp64_ms, nhv64):
         k = f"{r.space} {r.zv}p{r.nv}n"
-        counts[k] = counts.get(k, 0) + 1
+        c = counts.setdefault(k, {"mdim": work[(r.space, r.zv, r.nv)]["mdim"], "requests": 0, "mixed_solve_s": 0.0,
+                                  "mixed_n_hv": 0, "fp64_solve_s": 0.0, "fp64_n_hv": 0, "fp64_lm": 300})
+        c["requests"] += 1
+        c["mixed_solve_s"] += ia["solve_ms"] * 1e-3
+        c["mixed_n_hv"] += ia["n_hv"]
+        c["fp64_solve_s"] += t64 * 1e-3
+        c["fp64_n_hv"] += nh64
+        c["fp64_lm"] = min(c["fp64_lm"], lm64)
+    counts = dict(sorted(counts.items(), key=lambda kv: -kv[1]["mdim"]))
     _line(value, "solves/s", len(reqs), args.warmup, t_all,
           {"workload": (f"C4 serving stream: {len(reqs)} requests, Poisson {args.rate}/s, sd/pf, valence {lo}..{hi} "
                         f"per species, 2M = A%2, k 1..10, per-request seeded interactions (SPE U[-6,0], TBME "
@@ -334,13 +426,14 @@ def run_stream(args):
           {"latency_ms": {"p50": percentile(lat, 50), "p99": percentile(lat, 99), "note": "arrival to result"},
            "fp64": {"solves_per_s": fp64_rate, "wall_s": t_seq, "latency_ms_p50": percentile(lat64, 50),
                     "latency_ms_p99": percentile(lat64, 99), "note": "same requests and arrival times, one slot, "
-                    "plain FP64 (full reorthogonalisation every step)"},
+                    "plain FP64 (full reorthogonalisation every step)", **fp64_src},
            "speedup_vs_fp64": value / fp64_rate, "within_tol": ok, "failed_or_off": bad[:8], "n_off": len(bad),
            "max_abs_dE_vs_fp64_MeV": max(dE) if dE else None,
            "served": {"wall_s": t_all, "peak_running": sst["peak_running"], "hbm_peak_bytes": sst["hbm_peak"],
                       "batched": sst["batched"], "index_misses": sst["index_misses"],
                       "index_hits": sst["index_hits"],
-                      "lm_bounded": sum(1 for _, i in res if i["lm"] < 300)},
+                      "lm_bounded": sum(1 for _, i in res if i["lm"] < 300),
+                      "fallbacks": sum(1 for _, i in res if i.get("n_fallback", 0))},
            "bases": counts, "hv": hv, "roofline": roof, "cpu_baseline": cpu})
 
 
@@ -485,6 +578,12 @@ def main():
     ap.add_argument("--requests", type=int, default=512)
     ap.add_argument("--rate", type=float, default=50.0)
     ap.add_argument("--particles", default="2,6", help="valence range per species")
+    ap.add_argument("--leg", default="served", choices=["served", "fp64"],
+                    help="stream: fp64 runs only the sequential FP64 leg and writes --record")
+    ap.add_argument("--record", default=os.path.join(ROOT, "profiles", "stream512_fp64_leg.json"),
+                    help="stream: the recorded sequential FP64 leg (read by the served leg, written by --leg fp64)")
+    ap.add_argument("--spot-s", type=float, default=60.0,
+                    help="stream: recorded FP64 solve time re-run live as a spot check")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); independent solves shard one per GPU "
