@@ -357,7 +357,8 @@ namespace {
 struct Op4 { uint64_t a, b, c, d; };
 struct Op4Hash {
     size_t operator()(const std::array<uint64_t, 4>& k) const {
-        uint64_t h = k[0] * 0x9E3779B97F4A7C15ull ^ (k[1] + 0x632BE59BD9B4E019ull + (h << 6));
+        uint64_t h = k[0] * 0x9E3779B97F4A7C15ull;
+        h ^= k[1] + 0x632BE59BD9B4E019ull + (h << 6);
         h ^= k[2] * 0xC2B2AE3D27D4EB4Full + (h >> 3);
         h ^= k[3] * 0x165667B19E3779F9ull + (h << 7);
         return (size_t)h;
