@@ -236,6 +236,21 @@ int32_t smgpu_trbl_solve(smgpu_ham h, smgpu_ham j2, int32_t twoJ, int32_t q, con
  * restart): alpha[nsteps], beta[nsteps] of the projected tridiagonal matrix. */
 int32_t smgpu_lanczos_steps(smgpu_ham h, int32_t precision, int32_t nsteps, const double* v0, double* alpha,
                             double* beta);
+/* smgpu_lanczos_steps without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same inputs,
+ * outputs and results as smgpu_lanczos_steps (the same kernels in the same
+ * order); every device buffer of the call (Krylov basis, work vectors, partial
+ * sums, flags) lives in one static device array of smgpu_lanczos_scratch_bytes()
+ * bytes (64 MiB) that each device holds until the process ends, and everything
+ * runs on the handle's stream.  A size whose buffers do not fit returns
+ * SMGPU_EINVAL (about (nsteps + 8) * mdim * 8 bytes in FP64; mixed precision
+ * stores the basis in FP32).  The array keeps the last call's contents; calls on one
+ * device share it and are serialized by a host mutex inside the library (calls
+ * from separate processes have separate arrays). */
+int32_t smgpu_lanczos_steps_noalloc(smgpu_ham h, int32_t precision, int32_t nsteps, const double* v0,
+                                    double* alpha, double* beta);
+int64_t smgpu_lanczos_scratch_bytes(void);
 
 /* ------------------------------------------------------------------ serving
  * Persistent request loop (no counterpart in the reference, whose main_sm

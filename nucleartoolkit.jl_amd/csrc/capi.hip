@@ -731,6 +731,24 @@ int32_t smgpu_lanczos_steps(smgpu_ham h, int32_t precision, int32_t nsteps, cons
     });
 }
 
+int32_t smgpu_lanczos_steps_noalloc(smgpu_ham h, int32_t precision, int32_t nsteps, const double* v0,
+                                    double* alpha, double* beta) {
+    return guard([&]() -> int32_t {
+        if (!h || !alpha || !beta) throw std::invalid_argument("null argument");
+        if (nsteps < 1) throw std::invalid_argument("need nsteps >= 1");
+        smgpu_solve_params p;
+        smgpu_default_params(&p);
+        p.precision = precision;
+        p.lm = nsteps + 2;
+        p.ls = 1;
+        p.num_ev = 1;
+        lanczos_steps_noalloc(*h->dev, p, nsteps, v0, alpha, beta);
+        return SMGPU_OK;
+    });
+}
+
+int64_t smgpu_lanczos_scratch_bytes(void) { return lanczos_scratch_max(); }
+
 int32_t smgpu_malloc(smgpu_ham h, int64_t bytes, void** dev) {
     return guard([&]() -> int32_t {
         if (!h || !dev || bytes < 0) throw std::invalid_argument("bad argument");

@@ -25,6 +25,7 @@ struct DBuf {
     void* p = nullptr;
     size_t bytes = 0;
     size_t cap = 0;
+    bool owned = true;         // false: a view of memory owned elsewhere (never freed here)
     DBuf() = default;
     DBuf(const DBuf&) = delete;
     DBuf& operator=(const DBuf&) = delete;
@@ -34,6 +35,12 @@ struct DBuf {
     void ensure(size_t n) {    // grow-only reuse
         if (cap < n || !p) alloc(n);
         else bytes = n;
+    }
+    void view(void* ptr, size_t n) {   // point at n bytes of caller-owned device memory
+        release();
+        p = ptr;
+        bytes = cap = n;
+        owned = false;
     }
     template <typename T> void upload(const std::vector<T>& v) {
         ensure(v.size() * sizeof(T));

@@ -31,12 +31,13 @@ namespace smgpu {
 static constexpr size_t HV_LDS_TWO_ROWS = 16 * 1024;  // two-row groups for narrow blocks only
 
 DBuf::~DBuf() {
-    if (p) (void)hipFree(p);
+    if (p && owned) (void)hipFree(p);
 }
 void DBuf::release() {
-    if (p) (void)hipFree(p);
+    if (p && owned) (void)hipFree(p);
     p = nullptr;
     bytes = cap = 0;
+    owned = true;
 }
 void DBuf::alloc(size_t n) {
     release();

@@ -76,5 +76,12 @@ void trbl_solve(DeviceHam& h, int q, const SolveParams& p, double* evals, double
 void restart_small_device(int device, const double* alpha, const double* beta, int m, int ls, double* S,
                           double* alpha_out, double* beta_out);
 void lanczos_steps(DeviceHam& h, const SolveParams& p, int nsteps, const double* v0, double* alpha, double* beta);
+// lanczos_steps with every device buffer in the library's static scratch array (no
+// hipMalloc / hipFree / host-pinned allocation; one stream, the handle's).  Calls on
+// one device are serialized by a host mutex; sizes whose buffers exceed
+// lanczos_scratch_max() bytes are rejected with std::invalid_argument.
+void lanczos_steps_noalloc(DeviceHam& h, const SolveParams& p, int nsteps, const double* v0, double* alpha,
+                           double* beta);
+int64_t lanczos_scratch_max();
 
 }  // namespace smgpu

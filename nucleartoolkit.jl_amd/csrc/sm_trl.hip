@@ -40,11 +40,21 @@
 #include <string>
 #include <thread>
 
+#include <mutex>
+
 #include "dense_eig.h"
 #include "sm_device.h"
 #include "sm_status.h"
 #include "sm_trl.h"
 #include "tridiag.h"
+
+// Device scratch of lanczos_steps_noalloc: the Krylov basis, work vectors, partial
+// sums, flags and the small problems of one call, carved from this array.  Each
+// device the library runs on holds its own copy (LANCZOS_SCRATCH_BYTES) until the
+// process ends.  Host code never uses the array's name as an address: it uses the
+// pointer hipGetSymbolAddress returns for the current device.
+static constexpr size_t LANCZOS_SCRATCH_BYTES = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[LANCZOS_SCRATCH_BYTES];
 
 namespace smgpu {
 
@@ -874,8 +884,10 @@ struct Workspace {
     hipEvent_t evg = nullptr;      // main stream: k_omega has written hflag[4]
     hipEvent_t evm[4] = {};        // main stream: a step's alpha/beta are written
     hipStream_t side = nullptr;    // Ritz values and convergence test, off the H·v stream
+    bool view = false;             // every buffer (hflag included) is a view of device scratch:
+                                   // one stream, no events, nothing freed here
     void host_sync_objects() {
-        if (hflag) return;
+        if (hflag || view) return;
         // coherent: k_ritz writes the convergence flags here directly
         SM_HIP(hipHostMalloc((void**)&hflag, 8 * sizeof(int), hipHostMallocCoherent));
         SM_HIP(hipEventCreateWithFlags(&evg, hipEventDisableTiming));
@@ -895,7 +907,7 @@ struct Workspace {
         for (int q = 0; q < 4; q++) if (evm[q]) (void)hipEventDestroy(evm[q]);
         for (int q = 0; q < 4; q++) if (ev[q]) (void)hipEventDestroy(ev[q]);
         if (evg) (void)hipEventDestroy(evg);
-        if (hflag) (void)hipHostFree(hflag);
+        if (hflag && !view) (void)hipHostFree(hflag);
     }
 };
 
@@ -966,9 +978,14 @@ struct Solver {
 
     // ctx: stream + workspace slots of the caller (a serving worker), or the handle's own
     Solver(DeviceHam& h, const SolveParams& p, SolveStats& stats, const SolveCtx* ctx, const JProj* jproj = nullptr)
-        : H(h), P(p), st(stats), W(cached(ctx ? ctx->ws : h.trl_ws)), jp(jproj) {
-        s = ctx ? ctx->stream : H.stream;
+        : Solver(h, p, stats, cached(ctx ? ctx->ws : h.trl_ws), ctx ? ctx->stream : h.stream, jproj) {
         cancel = ctx ? ctx->cancel : nullptr;
+    }
+    // an explicit workspace and stream (a view workspace runs everything on `stream`)
+    Solver(DeviceHam& h, const SolveParams& p, SolveStats& stats, Workspace& ws, hipStream_t stream,
+           const JProj* jproj = nullptr)
+        : H(h), P(p), st(stats), W(ws), jp(jproj) {
+        s = stream;
         if (const char* e = std::getenv("SMGPU_REORTH")) reorth_cgs2 = std::string(e) == "cgs2";
         partial = !reorth_cgs2 && !jp &&
                   (P.reorth == SMGPU_REORTH_PARTIAL || (P.reorth == SMGPU_REORTH_DEFAULT && sizeof(TBv) == 4));
@@ -1203,6 +1220,8 @@ struct Solver {
                                    W.beta.as<double>() + it, V(it + 1), n);
             SM_HIP(hipEventRecord(step_ready, s));
             rs = s;
+        } else if (W.view) {
+            rs = s;                          // one stream: the test runs in order after the step
         } else {
             SM_HIP(hipEventRecord(W.evm[(step + 1) & 3], s));
             SM_HIP(hipStreamWaitEvent(W.side, W.evm[(step + 1) & 3], 0));
@@ -1220,7 +1239,7 @@ struct Solver {
         // grow-only buffers cached on the handle (no per-solve hipMalloc / memset)
         workspace_plan<TBv>(n, P, [&](DBuf Workspace::*m, int64_t b) { (W.*m).ensure((size_t)b); });
         W.host_sync_objects();
-        SM_HIP(hipStreamSynchronize(W.side));     // a canceled solve may have left tests queued
+        if (W.side) SM_HIP(hipStreamSynchronize(W.side));     // a canceled solve may have left tests queued
         // host sources of the asynchronous copies below live until the stream
         // synchronisation at the end of setup()
         std::vector<double> e0(4 * 64, 1.0e4);
@@ -1257,7 +1276,7 @@ struct Solver {
         setup(v0_in);
         for (int it = 0; it < nsteps; it++) enqueue_iteration(it, it, it == 0 ? 0 : std::min(nev, it));
         SM_HIP(hipStreamSynchronize(s));
-        SM_HIP(hipStreamSynchronize(W.side));
+        if (W.side) SM_HIP(hipStreamSynchronize(W.side));
         SM_HIP(hipMemcpy(alpha_out, W.alpha.p, nsteps * 8, hipMemcpyDeviceToHost));
         SM_HIP(hipMemcpy(beta_out, W.beta.p, nsteps * 8, hipMemcpyDeviceToHost));
     }
@@ -2249,6 +2268,82 @@ void lanczos_steps(DeviceHam& h, const SolveParams& p, int nsteps, const double*
         Solver<float> S(h, p, st, nullptr);
         S.run_steps(nsteps, v0, alpha, beta);
     }
+}
+
+// ------------------------------------------------ lanczos_steps without allocation
+namespace {
+constexpr int MAX_DEVICES = 64;
+constexpr size_t SCRATCH_ALIGN = 256;
+
+struct ScratchSlot {
+    std::mutex mu;          // held for a whole call: calls on one device run one at a time
+    void* ptr = nullptr;    // device address of `scratch` on this device
+};
+ScratchSlot g_scratch[MAX_DEVICES];
+
+// the device address of this device's copy of `scratch`, looked up on the first
+// call on the device (that device current); the caller holds the slot's mutex
+void* scratch_on(int device) {
+    ScratchSlot& sl = g_scratch[device];
+    if (!sl.ptr) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(e) +
+                                     " (hipGetSymbolAddress of the Lanczos scratch)");
+        sl.ptr = p;
+    }
+    return sl.ptr;
+}
+
+// Points every buffer of the solve plan (and the 8-int flag ring k_ritz writes) at
+// consecutive 256-byte aligned pieces of base; returns the bytes used.  base null:
+// only counts.
+template <typename TBv>
+size_t carve(Workspace& W, int64_t n, const SolveParams& p, unsigned char* base) {
+    size_t off = 0;
+    auto take = [&](size_t b) {
+        const size_t at = off;
+        off += (b + SCRATCH_ALIGN - 1) / SCRATCH_ALIGN * SCRATCH_ALIGN;
+        return at;
+    };
+    workspace_plan<TBv>(n, p, [&](DBuf Workspace::*m, int64_t b) {
+        const size_t at = take((size_t)b);
+        if (base) (W.*m).view(base + at, (size_t)b);
+    });
+    const size_t at = take(8 * sizeof(int));
+    if (base) W.hflag = reinterpret_cast<int*>(base + at);
+    return off;
+}
+
+template <typename TBv>
+void steps_in_scratch(DeviceHam& h, const SolveParams& p, int nsteps, const double* v0, double* alpha,
+                      double* beta) {
+    Workspace W;
+    W.view = true;
+    const size_t need = carve<TBv>(W, h.mdim, p, nullptr);
+    if (need > LANCZOS_SCRATCH_BYTES)
+        throw std::invalid_argument("lanczos_steps_noalloc: mdim " + std::to_string(h.mdim) + " with " +
+                                    std::to_string(nsteps) + " steps needs " + std::to_string(need) +
+                                    " bytes of device scratch, above its maximum of " +
+                                    std::to_string(LANCZOS_SCRATCH_BYTES));
+    ScratchSlot& sl = g_scratch[h.device];
+    std::lock_guard<std::mutex> lk(sl.mu);
+    carve<TBv>(W, h.mdim, p, static_cast<unsigned char*>(scratch_on(h.device)));
+    SolveStats st{};
+    Solver<TBv> S(h, p, st, W, h.stream);
+    S.run_steps(nsteps, v0, alpha, beta);
+}
+}  // namespace
+
+int64_t lanczos_scratch_max() { return (int64_t)LANCZOS_SCRATCH_BYTES; }
+
+void lanczos_steps_noalloc(DeviceHam& h, const SolveParams& p, int nsteps, const double* v0, double* alpha,
+                           double* beta) {
+    if (h.device < 0 || h.device >= MAX_DEVICES) throw std::invalid_argument("device index out of range");
+    SM_HIP(hipSetDevice(h.device));
+    if (p.precision == 0) steps_in_scratch<double>(h, p, nsteps, v0, alpha, beta);
+    else steps_in_scratch<float>(h, p, nsteps, v0, alpha, beta);
 }
 
 }  // namespace smgpu

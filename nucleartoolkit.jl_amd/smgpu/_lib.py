@@ -117,6 +117,8 @@ SIGNATURES = {
     "smgpu_trl_solve_batch": (I32, [P, I32, P, P, P, P, P, P]),
     "smgpu_trl_solve_variants": (I32, [P, I32, P, P, P, P, P, P]),
     "smgpu_lanczos_steps": (I32, [P, I32, I32, P, P, P]),
+    "smgpu_lanczos_steps_noalloc": (I32, [P, I32, I32, P, P, P]),
+    "smgpu_lanczos_scratch_bytes": (I64, []),
     "smgpu_malloc": (I32, [P, I64, P]),
     "smgpu_free": (I32, [P, P]),
     "smgpu_memcpy": (I32, [P, P, P, I64, I32]),
