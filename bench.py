@@ -202,6 +202,27 @@ def hv_roofline(ham, reps, lds_model=True):
     return out
 
 
+def calibration_copies(ham, mib=256, reps=3):
+    """Device-to-device copies of a known size on the handle's device: the byte
+    counters of a profiler run (FETCH_SIZE / WRITE_SIZE) are scaled by what they read
+    for these copies (each reads and writes `mib` MiB)."""
+    import ctypes
+    from smgpu._lib import check, lib
+    L = lib()
+    n = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    check(L.smgpu_malloc(ham._h, n, ctypes.byref(a)))
+    check(L.smgpu_malloc(ham._h, n, ctypes.byref(b)))
+    try:
+        for _ in range(reps):
+            check(L.smgpu_memcpy(ham._h, b, a, n, 2))
+        check(L.smgpu_synchronize(ham._h))
+    finally:
+        check(L.smgpu_free(ham._h, a))
+        check(L.smgpu_free(ham._h, b))
+    return {"bytes_read": n, "bytes_written": n, "copies": reps}
+
+
 def source_hash():
     """sha256 (16 hex) of the library sources: the kernels and solver a record was taken with."""
     import hashlib
@@ -449,7 +470,9 @@ def run_served(args):
     prob = smgpu.Problem(snt, nuc)
     ham = smgpu.Hamiltonian(prob, 0)
     if args.mode == "hv":
-        print(json.dumps({"mode": "hv", "config": args.config, "mdim": prob.mdim, **hv_roofline(ham, args.hv_reps)}))
+        out = {"mode": "hv", "config": args.config, "mdim": prob.mdim, **hv_roofline(ham, args.hv_reps)}
+        out["copy_calibration"] = calibration_copies(ham)
+        print(json.dumps(out))
         return
     # plain FP64 path, one solve at a time (tables resident; solve only): >= 3 timed solves
     ham.trl(nev, lm=lm, ls=ls, precision="fp64")

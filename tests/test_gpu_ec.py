@@ -83,3 +83,71 @@ def test_ec_reference_setup_o20(golden):
     eu = smgpu.main_sm(f"{golden}/usdb.snt", "O20", 3, [0])
     assert abs(v3[0] - e3[0]) < 1e-8, (v3[0], e3[0])
     assert np.all(vu >= eu - 1e-8), (vu, eu)
+
+
+def _direct_hmat(V, sntf, nuc, twoM):
+    p = smgpu.Problem(sntf, nuc, twoM=twoM)
+    h = smgpu.Hamiltonian(p)
+    H = V @ h.operate_H_multi(V, "fp64").T
+    return 0.5 * (H + H.T)
+
+
+def test_td_mode_reference_setup_o20(golden, tmp_path):
+    """prepEC "TD" mode on the reference's O20 setup (test/ShellModel_test.jl:40-68):
+    the sample .wav files of tmp_0..4 (2J = 0), then the TD file.  Checked:
+    * the file layout (Dim, Dim(Dim+1)/2 pairs, lTD = 6 SPEs + 158 TBME lines) and
+      N_ij against the sample vectors;
+    * the one-body sum rule: summed over the neutron orbits the densities give
+      N_v <psi_i|psi_j> (N_v = 4), over the proton orbits 0 (Z_v = 0);
+    * H_ij = TD . [SPEs, oTBMEs] against <psi_i|H|psi_j> from the GPU H.v for
+      tmp_3, usdb (mass-scaled TBMEs) and three random interactions on the orbits
+      (1e-10 of max|H|);
+    * solveEC from the TD file equals solveEC from the resident vectors (1e-9 MeV)."""
+    Hs = [f"{golden}/random_snts/tmp_{i}.snt" for i in range(5)]
+    wav = str(tmp_path) + "/"
+    samples = ec.prep_ec(Hs, "O20", 3, tJ=0, save_wav=True, path_to_samplewav=wav)
+    tdf = ec.prep_ec_td(Hs, "O20", 3, 5, 0, path_to_samplewav=wav, tdmatdir=wav + "tdmat/")
+    dim, npair, lTD, Nmat, infos, TDs = ec.read_tdmat(tdf)
+    V = samples.vectors
+    assert dim == V.shape[0] and npair == dim * (dim + 1) // 2 and lTD == 6 + 158
+    assert [x[2] for x in infos] == list(range(1, npair + 1))
+    assert np.abs(Nmat - V @ V.T).max() < 1e-14
+    for (i, j, _), td in zip(infos, TDs):
+        assert abs(td[:3].sum()) < 1e-9
+        assert abs(td[3:6].sum() - 4.0 * Nmat[i - 1, j - 1]) < 1e-9
+    rng = np.random.default_rng(5)
+    p0 = ec.read_snt_params(Hs[0])
+    targets = [Hs[3], f"{golden}/usdb.snt"]
+    for r in range(3):
+        f = ec.write_snt_params(f"{tmp_path}/rand_{r}.snt", p0, rng.uniform(-5, 0, 6), rng.normal(0, 2.0, 158))
+        targets.append(f)
+    for sntf in targets:
+        Htd = ec.ec_hmat(TDs, infos, dim, sntf, "O20")
+        Hdir = _direct_hmat(V, sntf, "O20", 0)
+        err = np.abs(Htd - Hdir).max() / np.abs(Hdir).max()
+        print("TD vs H.v", sntf.split("/")[-1], err, flush=True)
+        assert err < 1e-10
+    got = ec.solve_ec_td(targets[:2], "O20", [(0, 3)], tdmatpath=wav + "tdmat/")
+    ref = ec.solve_ec(targets[:2], "O20", samples, 3)
+    for (_, _, a), (_, _, b) in zip(got, ref):
+        assert np.abs(a - b).max() < 1e-9
+    info_lines = open(wav + "tdmat/wfinfos_O20_J0.dat").read().splitlines()
+    assert len(info_lines) == dim and info_lines[0].startswith("O20_tmp_0_j0.wav nth= 1 TDmatidx  1")
+
+
+def test_td_mode_both_species(snt_dir, tmp_path):
+    """TD densities on a basis with valence protons and neutrons (Mg24, sd 4p4n,
+    mdim 28,503): one-body sum rules for both species and H_ij of an interaction
+    outside the sample set against the GPU H.v."""
+    paths, nuc = _variants(snt_dir, 4)
+    samples = ec.prep_ec(paths[:3], nuc, 2)
+    V = samples.vectors
+    Nmat, TD = ec.transition_densities(V, paths[0], nuc, samples.twoM)
+    p = ec.read_snt_params(paths[0])
+    assert TD.shape == (p.n_params, V.shape[0], V.shape[0])
+    assert np.abs(TD[:3].sum(0) - 4.0 * Nmat).max() < 1e-9
+    assert np.abs(TD[3:6].sum(0) - 4.0 * Nmat).max() < 1e-9
+    MEs = np.concatenate([ec.read_snt_params(paths[3]).spe, ec.scaled_tbmes(ec.read_snt_params(paths[3]), nuc)])
+    Htd = np.einsum("k,kij->ij", MEs, TD)
+    Hdir = _direct_hmat(V, paths[3], nuc, samples.twoM)
+    assert np.abs(Htd - Hdir).max() < 1e-10 * np.abs(Hdir).max()

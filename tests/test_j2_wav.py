@@ -52,3 +52,36 @@ def test_wav_round_trip(tmp_path):
     assert np.array_equal(wav.read_appwav(p, 17), vecs[0])
     with pytest.raises(ValueError):
         wav.read_wav(p, 18)
+
+
+def test_snt_params_and_tdmat_layout(golden, tmp_path):
+    """The TD mode's file handling (CPU): an interaction file rewritten with its
+    mass-scaled TBMEs and massop 0 gives the same host tables as the original;
+    the TD file round-trips in the reference's layout (Int64 header, per pair
+    i, j, Nij, N_ij, TDs) with idx_from_ij's 1-based pair numbering."""
+    import numpy as np
+    import smgpu
+    from smgpu import ec
+    p = ec.read_snt_params(f"{golden}/usdb.snt")
+    assert (p.lp, p.ln, len(p.tbme), p.massop) == (3, 3, 158, 1)
+    f = ec.write_snt_params(str(tmp_path / "usdb_scaled.snt"), p, p.spe, ec.scaled_tbmes(p, "Na23"))
+    a = smgpu.Problem(f"{golden}/usdb.snt", "Na23")
+    b = smgpu.Problem(f, "Na23")
+    for s in (0, 1):
+        xa, xb = a.pair_matrix(s), b.pair_matrix(s)
+        for u, v in zip(xa[:3], xb[:3]):
+            assert np.array_equal(u, v)
+        assert np.abs(xa[3] - xb[3]).max() <= 1e-14 * np.abs(xa[3]).max()
+    rng = np.random.default_rng(3)
+    dim, P = 4, 7
+    N = rng.normal(size=(dim, dim))
+    N = N + N.T
+    TD = rng.normal(size=(P, dim, dim))
+    ec.write_tdmat(str(tmp_path / "x.dat"), N, TD)
+    raw = np.fromfile(str(tmp_path / "x.dat"), dtype="<i8", count=3)
+    assert list(raw) == [dim, 10, P]
+    d, npair, lTD, Nmat, infos, TDs = ec.read_tdmat(str(tmp_path / "x.dat"))
+    assert (d, npair, lTD) == (dim, 10, P)
+    assert infos[:5] == [(1, 1, 1), (1, 2, 2), (1, 3, 3), (1, 4, 4), (2, 2, 5)]
+    for (i, j, _), td in zip(infos, TDs):
+        assert Nmat[i - 1, j - 1] == N[i - 1, j - 1] and np.array_equal(td, TD[:, i - 1, j - 1])
