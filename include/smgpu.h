@@ -34,6 +34,7 @@ extern "C" {
 typedef struct smgpu_problem_s* smgpu_problem;  /* interaction + basis + host tables */
 typedef struct smgpu_ham_s* smgpu_ham;          /* device-resident tables on one GPU */
 typedef struct smgpu_server_s* smgpu_server;    /* persistent serving loop */
+typedef struct smgpu_moments_s* smgpu_moments;  /* M1 / E2 one-body operators on a basis */
 
 typedef struct {
     int32_t Z, N, A, vp, vn, twoM;
@@ -251,6 +252,33 @@ int32_t smgpu_lanczos_steps(smgpu_ham h, int32_t precision, int32_t nsteps, cons
 int32_t smgpu_lanczos_steps_noalloc(smgpu_ham h, int32_t precision, int32_t nsteps, const double* v0,
                                     double* alpha, double* beta);
 int64_t smgpu_lanczos_scratch_bytes(void);
+
+/* ------------------------------------------------------------------ moments
+ * main_sm's calc_moment (shellmodel_main.jl:243-246): the magnetic dipole and
+ * electric quadrupole moments of Ritz vectors.
+ * smgpu_moments_create builds prep_obtr_diag's M = 0 one-body operators
+ * (transit.jl:82-245) on the problem's basis and uploads them to `device`:
+ * gfactors[4] = (g_l proton, g_l neutron, g_s proton, g_s neutron) and
+ * effcharge[2] = (e_p, e_n) as main_sm's keywords (defaults 1, 0, 5.586, -3.826
+ * and 1.5, 0.5); bpar <= 0 takes main_sm's oscillator length (init_ho_by_mass,
+ * transit.jl:49-60, formula 2 for 16 <= A <= 40, shellmodel_main.jl:104-107). */
+int32_t smgpu_moments_create(smgpu_problem p, int32_t device, const double* gfactors, const double* effcharge,
+                             double bpar, smgpu_moments* out);
+int32_t smgpu_moments_destroy(smgpu_moments m);
+/* w = O v with O the M1 (which = 0) or E2 (1) operator (op_obtr_diag,
+ * transit.jl:247-283), FP64 device vectors in the M-scheme order, on `stream`
+ * (NULL: the handle's own) */
+int32_t smgpu_moments_apply(smgpu_moments m, int32_t which, const double* v_dev, double* w_dev, void* stream);
+/* the same on host vectors: nvec vectors of mdim amplitudes each */
+int32_t smgpu_moments_apply_host(smgpu_moments m, int32_t which, int32_t nvec, const double* v, double* w);
+/* eval_moment (transit.jl:702-765) of nvec host vectors (nvec x mdim) with J values
+ * totJs[n] (the state's J, not 2J; J < 0 is skipped): mu[n] in nuclear magnetons and
+ * Q[n] in e fm^2, NaN where the reference prints nothing (J < 0, or a vanishing 3j
+ * symbol at the basis' M, e.g. mu of any state of an M = 0 basis). */
+int32_t smgpu_eval_moment(smgpu_moments m, int32_t nvec, const double* vecs, const double* totJs, double* mu,
+                          double* Q);
+/* the oscillator length in use (fm) */
+int32_t smgpu_moments_bpar(smgpu_moments m, double* bpar);
 
 /* ------------------------------------------------------------------ serving
  * Persistent request loop (no counterpart in the reference, whose main_sm

@@ -33,6 +33,10 @@ struct smgpu_ham_s {
     std::unique_ptr<DeviceHam> dev;
 };
 
+struct smgpu_moments_s {
+    std::unique_ptr<DeviceMoments> dev;
+};
+
 template <typename TA, typename TB>
 __global__ void k_cast(const TA* a, TB* b, int64_t n) {
     for (int64_t x = (int64_t)blockIdx.x * 256 + threadIdx.x; x < n; x += (int64_t)gridDim.x * 256) b[x] = (TB)a[x];
@@ -748,6 +752,76 @@ int32_t smgpu_lanczos_steps_noalloc(smgpu_ham h, int32_t precision, int32_t nste
 }
 
 int64_t smgpu_lanczos_scratch_bytes(void) { return lanczos_scratch_max(); }
+
+int32_t smgpu_moments_create(smgpu_problem p, int32_t device, const double* gfactors, const double* effcharge,
+                             double bpar, smgpu_moments* out) {
+    return guard([&]() -> int32_t {
+        if (!p || !out) throw std::invalid_argument("null argument");
+        *out = nullptr;
+        const double g0[4] = {1.0, 0.0, 5.586, -3.826}, e0[2] = {1.5, 0.5};
+        const double* g = gfactors ? gfactors : g0;
+        const double* e = effcharge ? effcharge : e0;
+        const double b = bpar > 0.0 ? bpar : ho_bpar_main_sm(p->snt.A);
+        MomentOps mo = build_moment_ops(p->snt, p->basis, b, g, e);
+        auto m = std::make_unique<smgpu_moments_s>();
+        m->dev = upload_moments(mo, p->basis, device);
+        *out = m.release();
+        return SMGPU_OK;
+    });
+}
+
+int32_t smgpu_moments_destroy(smgpu_moments m) {
+    return guard([&]() -> int32_t {
+        if (m && m->dev) SM_HIP(hipSetDevice(m->dev->device));
+        delete m;
+        return SMGPU_OK;
+    });
+}
+
+int32_t smgpu_moments_apply(smgpu_moments m, int32_t which, const double* v_dev, double* w_dev, void* stream) {
+    return guard([&]() -> int32_t {
+        if (!m || !v_dev || !w_dev) throw std::invalid_argument("null argument");
+        SM_HIP(hipSetDevice(m->dev->device));
+        moments_apply(*m->dev, which, v_dev, w_dev, stream ? (hipStream_t)stream : m->dev->stream);
+        return SMGPU_OK;
+    });
+}
+
+int32_t smgpu_moments_apply_host(smgpu_moments m, int32_t which, int32_t nvec, const double* v, double* w) {
+    return guard([&]() -> int32_t {
+        if (!m || !v || !w || nvec < 0) throw std::invalid_argument("bad argument");
+        DeviceMoments& d = *m->dev;
+        SM_HIP(hipSetDevice(d.device));
+        const size_t n = (size_t)d.mdim;
+        DBuf dv, dw;
+        dv.alloc(std::max<size_t>(n, 1) * 8);
+        dw.alloc(std::max<size_t>(n, 1) * 8);
+        for (int b = 0; b < nvec; b++) {
+            SM_HIP(hipMemcpy(dv.p, v + b * n, n * 8, hipMemcpyHostToDevice));
+            moments_apply(d, which, dv.as<double>(), dw.as<double>(), d.stream);
+            SM_HIP(hipStreamSynchronize(d.stream));
+            SM_HIP(hipMemcpy(w + b * n, dw.p, n * 8, hipMemcpyDeviceToHost));
+        }
+        return SMGPU_OK;
+    });
+}
+
+int32_t smgpu_eval_moment(smgpu_moments m, int32_t nvec, const double* vecs, const double* totJs, double* mu,
+                          double* Q) {
+    return guard([&]() -> int32_t {
+        if (!m || !vecs || !totJs || !mu || !Q || nvec < 0) throw std::invalid_argument("bad argument");
+        eval_moment(*m->dev, nvec, vecs, totJs, mu, Q);
+        return SMGPU_OK;
+    });
+}
+
+int32_t smgpu_moments_bpar(smgpu_moments m, double* bpar) {
+    return guard([&]() -> int32_t {
+        if (!m || !bpar) throw std::invalid_argument("null argument");
+        *bpar = m->dev->bpar;
+        return SMGPU_OK;
+    });
+}
 
 int32_t smgpu_malloc(smgpu_ham h, int64_t bytes, void** dev) {
     return guard([&]() -> int32_t {

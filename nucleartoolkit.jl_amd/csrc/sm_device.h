@@ -187,4 +187,25 @@ void launch_hv_multi(const DeviceHam& h, int prec, int nvec, const void* v, int6
 // re-upload interaction-dependent coefficients (same basis and index tables)
 void update_coefficients(DeviceHam& h, const Basis& b, const Tables& t);
 
+// Moment operators (M1, E2) of main_sm's calc_moment on the device: per operator
+// and species the symmetric matrices of every M-set, flattened (rows of set s start
+// at rp_base[s] in rowptr, their entries at ent_base[s]).
+struct DeviceMoments {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int64_t mdim = 0;
+    int nb = 0, R = 0, twoM = 0;
+    double bpar = 0.0;
+    DBuf row_block, block_row0, nN, tdims, pset, nset;
+    DBuf rp_base[2][2], ent_base[2][2], rowptr[2][2], col[2][2], val[2][2];   // [op][species]
+    DBuf v, w, part;                       // work vectors of eval_moment (grow-only)
+    ~DeviceMoments();
+};
+std::unique_ptr<DeviceMoments> upload_moments(const MomentOps& mo, const Basis& b, int device);
+// w = O v, O = M1 (which 0) or E2 (1), FP64 device vectors in the M-scheme order
+void moments_apply(const DeviceMoments& m, int which, const double* v, double* w, hipStream_t s);
+// eval_moment (transit.jl:702-765): mu[n], Q[n] of nvec host vectors with J values
+// totJs (NaN where the reference prints nothing: J < 0 or a vanishing 3j symbol)
+void eval_moment(DeviceMoments& m, int nvec, const double* vecs, const double* totJs, double* mu, double* Q);
+
 }  // namespace smgpu

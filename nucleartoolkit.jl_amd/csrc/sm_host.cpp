@@ -677,4 +677,156 @@ Tables build_tables(const Snt& snt, const Basis& b, int nthreads) {
     return t;
 }
 
+// ------------------------------------------------------------------ moments
+double wigner3j(int j1, int j2, int j3, int m1, int m2, int m3) {
+    if (m1 + m2 + m3 != 0 || ((j1 - j2 - m3) & 1)) return 0.0;
+    const int e = (j1 - j2 - m3) / 2;
+    return ((e & 1) ? -1.0 : 1.0) / std::sqrt((double)(j3 + 1)) * clebsch_gordan(j1, m1, j2, m2, j3, -m3);
+}
+
+// Delta(abc)^2 of doubled arguments; false if the triangle or the parity fails
+static bool delta2(int a, int b, int c, __float128* d) {
+    if (((a + b + c) & 1) || a + b < c || a + c < b || b + c < a) return false;
+    *d = qfact((a + b - c) / 2) * qfact((a - b + c) / 2) * qfact((-a + b + c) / 2) / qfact((a + b + c) / 2 + 1);
+    return true;
+}
+
+// Racah's formula in quad precision, rounded once to double
+double wigner6j(int j1, int j2, int j3, int j4, int j5, int j6) {
+    __float128 d[4];
+    if (!delta2(j1, j2, j3, &d[0]) || !delta2(j1, j5, j6, &d[1]) || !delta2(j4, j2, j6, &d[2]) ||
+        !delta2(j4, j5, j3, &d[3]))
+        return 0.0;
+    const int a[4] = {(j1 + j2 + j3) / 2, (j1 + j5 + j6) / 2, (j4 + j2 + j6) / 2, (j4 + j5 + j3) / 2};
+    const int c[3] = {(j1 + j2 + j4 + j5) / 2, (j2 + j3 + j5 + j6) / 2, (j3 + j1 + j6 + j4) / 2};
+    const int t0 = std::max(std::max(a[0], a[1]), std::max(a[2], a[3]));
+    const int t1 = std::min(std::min(c[0], c[1]), c[2]);
+    __float128 s = 0;
+    for (int t = t0; t <= t1; t++) {
+        __float128 den = 1;
+        for (int x : a) den *= qfact(t - x);
+        for (int x : c) den *= qfact(x - t);
+        s += ((t & 1) ? -1 : 1) * qfact(t + 1) / den;
+    }
+    if (s == 0) return 0.0;
+    return (double)(sqrtq(d[0] * d[1] * d[2] * d[3]) * s);
+}
+
+static __float128 dfact(int n) {
+    __float128 r = 1;
+    for (; n > 1; n -= 2) r *= n;
+    return r;
+}
+
+// binomial coefficient with Julia's extension to negative n
+static __float128 binom(int n, int k) {
+    if (k < 0) return 0;
+    if (n < 0) return ((k & 1) ? -1 : 1) * binom(k - n - 1, k);
+    if (k > n) return 0;
+    __float128 r = 1;
+    for (int i = 1; i <= k; i++) r = r * (n - k + i) / i;
+    return r;
+}
+
+double radius_power(int k, int n1, int l1, int n2, int l2, double bpar) {
+    const int ll = l1 + l2 + k;
+    const int ll1 = (l2 - l1 + k) / 2, ll2 = (l1 - l2 + k) / 2;   // truncation toward zero
+    __float128 s = 0;
+    for (int i = std::max({0, n1 - ll1, n2 - ll2}); i <= std::min(n1, n2); i++)
+        s += binom(ll1, n1 - i) * binom(ll2, n2 - i) * dfact(ll + 2 * i + 1) / dfact(2 * i);
+    s *= sqrtq(dfact(2 * n1) / dfact(2 * n1 + 2 * l1 + 1));
+    s *= sqrtq(dfact(2 * n2) / dfact(2 * n2 + 2 * l2 + 1));
+    return (double)s / std::sqrt(std::pow(2.0, k)) * (((n1 - n2) & 1) ? -1.0 : 1.0) * std::pow(bpar, k);
+}
+
+double ho_bpar_main_sm(int A) {
+    const double amass = (938.27231 + 939.56563) / 2, hc = 197.32696;
+    const double a = (double)A;
+    const double hw = (16 <= A && A <= 40) ? 45.0 * std::pow(a, -1.0 / 3.0) - 25.0 * std::pow(a, -2.0 / 3.0)
+                                           : 41.0 * std::pow(a, -1.0 / 3.0);
+    return hc / std::sqrt(hw * amass);
+}
+
+MomentOps build_moment_ops(const Snt& snt, const Basis& b, double bpar, const double* g, const double* ec) {
+    MomentOps mo;
+    mo.bpar = bpar;
+    const double c_mu = std::sqrt(3.0 / (4.0 * M_PI)), c_Q = std::sqrt(5.0 / (16.0 * M_PI));
+    for (int sp = 0; sp < 2; sp++) {
+        const Species& S = b.sp[sp];
+        const std::vector<Orbit>& sps = sp == 0 ? snt.p_sps : snt.n_sps;
+        const double gl = sp == 0 ? g[0] : g[1], gs = sp == 0 ? g[2] : g[3], e = ec[sp];
+        // the M-sets of this species that the basis uses
+        std::vector<char> used(S.sets.size(), 0);
+        for (int k = 0; k < b.nb; k++) used[sp == 0 ? b.pset[k] : b.nset[k]] = 1;
+        // one-body terms (m-state a <- m-state c, coefficient) of each operator
+        std::vector<std::array<int, 2>> terms[2];
+        std::vector<double> coef[2];
+        for (size_t i = 0; i < sps.size(); i++)
+            for (size_t f = i; f < sps.size(); f++) {
+                const Orbit oi = sps[i], of = sps[f];
+                const int ji = oi.j2, jf = of.j2, li = oi.l, lf = of.l;
+                const double ex_r2 = radius_power(2, oi.n, li, of.n, lf, bpar);
+                const double l6j = wigner6j(2 * lf, 2 * li, 2, ji, jf, 1);
+                const double s6j = wigner6j(1, 1, 2, ji, jf, 2 * lf);
+                const double Q3j = wigner3j(ji, 4, jf, 1, 0, -1);
+                const double pfac = std::sqrt((double)((ji + 1) * (jf + 1)));
+                const std::vector<int> kc = mstates_of(S, (int)i), ka = mstates_of(S, (int)f);
+                for (int mode = 0; mode < 2; mode++) {
+                    if (mode == 0 && (oi.n != of.n || li != lf)) continue;
+                    for (int c : kc)
+                        for (int a : ka) {
+                            const int m = S.m2[c];
+                            if (S.m2[a] != m) continue;
+                            double v;
+                            if (mode == 0) {
+                                const int s1 = (jf - m) / 2 + lf + (ji + 3) / 2, s2 = (jf - m) / 2 + lf + (jf + 3) / 2;
+                                const double fac1 = ((s1 & 1) ? -1.0 : 1.0) * l6j * std::sqrt((double)(lf * (lf + 1) * (2 * lf + 1)));
+                                const double fac2 = ((s2 & 1) ? -1.0 : 1.0) * s6j * std::sqrt(1.5);
+                                v = pfac * wigner3j(jf, 2, ji, -m, 0, m) * (fac1 * gl + fac2 * gs) * c_mu;
+                            } else {
+                                if ((li + lf) & 1) continue;      // (1 + (-1)^(li+lf)) = 0
+                                const int sq = jf + (1 - m) / 2;
+                                v = ((sq & 1) ? -2.0 : 2.0) * wigner3j(jf, 4, ji, -m, 0, m) * Q3j * pfac * e * c_Q * ex_r2;
+                            }
+                            if (v == 0.0) continue;
+                            terms[mode].push_back({a, c});
+                            coef[mode].push_back(v);
+                        }
+                }
+            }
+        for (int mode = 0; mode < 2; mode++) {
+            std::vector<PairSym>& out = mo.op[mode][sp];
+            out.assign(S.sets.size(), PairSym{});
+            for (size_t set = 0; set < S.sets.size(); set++) {
+                const auto& F = S.sets[set];
+                PairSym& P = out[set];
+                P.rowptr.assign(F.size() + 1, 0);
+                if (!used[set]) continue;
+                std::vector<std::map<int32_t, double>> rows(F.size());
+                for (size_t q = 0; q < terms[mode].size(); q++) {
+                    const uint64_t ba = bit_of(S, terms[mode][q][0]), bc = bit_of(S, terms[mode][q][1]);
+                    for (size_t ni = 0; ni < F.size(); ni++) {
+                        const uint64_t phi = F[ni];
+                        if (!(phi & bc)) continue;
+                        const uint64_t mid = phi & ~bc;
+                        if (mid & ba) continue;
+                        const int64_t nf = S.find((int)set, mid | ba);
+                        if (nf < 0) continue;
+                        // fermion sign of a+ c on phi (calc_phase_1! with its leading 1)
+                        const int ex = 1 + popc(phi & at_or_above(bc)) + popc(mid & at_or_above(ba));
+                        double w = ((ex & 1) ? -1.0 : 1.0) * coef[mode][q] * ((int64_t)ni == nf ? 0.5 : 1.0);
+                        rows[nf][(int32_t)ni] += w;
+                        rows[ni][(int32_t)nf] += w;
+                    }
+                }
+                for (size_t r = 0; r < F.size(); r++) {
+                    for (auto& kv : rows[r]) { P.col.push_back(kv.first); P.val.push_back(kv.second); }
+                    P.rowptr[r + 1] = (int64_t)P.col.size();
+                }
+            }
+        }
+    }
+    return mo;
+}
+
 }  // namespace smgpu

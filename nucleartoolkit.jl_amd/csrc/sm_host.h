@@ -118,6 +118,26 @@ struct Tables {
 // main_sm gets from prep_J / operate_J!, shellmodel_main.jl:149-157, 226-230).
 Snt j2_operator(const Snt& model);
 
+// ---- moments (transit.jl): the M = 0 components of the M1 and E2 one-body
+// operators (prep_obtr_diag, :82-245) as symmetric matrices over the SDs of each
+// M-set of a species (they conserve M), full rows with merged entries; op_obtr_diag
+// (:247-283) applies each pushed term both ways, diagonal terms at half weight.
+// gfactors = (g_l proton, g_l neutron, g_s proton, g_s neutron), effcharge = (e_p, e_n).
+struct MomentOps {
+    double bpar = 0.0;
+    std::vector<PairSym> op[2][2];    // [0 = M1, 1 = E2][species] per M-set
+};
+MomentOps build_moment_ops(const Snt& snt, const Basis& b, double bpar, const double* gfactors,
+                           const double* effcharge);
+// oscillator length of main_sm (init_ho_by_mass, transit.jl:49-60, with formula 2
+// for 16 <= A <= 40, shellmodel_main.jl:104-107)
+double ho_bpar_main_sm(int A);
+// (j1 j2 j3; m1 m2 m3) and {j1 j2 j3; j4 j5 j6}, doubled arguments
+double wigner3j(int j1, int j2, int j3, int m1, int m2, int m3);
+double wigner6j(int j1, int j2, int j3, int j4, int j5, int j6);
+// <n1 l1|r^k|n2 l2> of oscillator orbits with length bpar (radius_power, transit.jl:62-80)
+double radius_power(int k, int n1, int l1, int n2, int l2, double bpar);
+
 // Interaction-dependent parts only (V_eff, pp, nn, e_p, e_n) can be rebuilt
 // for a new interaction on the same basis with `build_coefficients`.
 Tables build_tables(const Snt& snt, const Basis& b, int nthreads);

@@ -355,6 +355,61 @@ def mdim_print(target_nuc, info, tJ=-1):
     print(s0 + s1, end="")          # the J / En lines follow on the same line, as in main_sm
 
 
+class Moments:
+    """The M1 and E2 one-body operators of main_sm's calc_moment on a problem's basis,
+    on one GPU (prep_obtr_diag, transit.jl:82-245): `apply` is op_obtr_diag
+    (:247-283), `eval` is eval_moment (:702-765)."""
+
+    def __init__(self, problem: Problem, device=0, gfactors=(1.0, 0.0, 5.586, -3.826), effcharge=(1.5, 0.5),
+                 bpar=0.0):
+        g = np.ascontiguousarray(gfactors, dtype=np.float64)
+        e = np.ascontiguousarray(effcharge, dtype=np.float64)
+        if g.shape != (4,) or e.shape != (2,):
+            raise ValueError("gfactors needs 4 values and effcharge 2")
+        h = ctypes.c_void_p()
+        check(lib().smgpu_moments_create(problem._h, int(device), ptr(g), ptr(e), float(bpar), ctypes.byref(h)))
+        self._h = h
+        self.mdim = problem.mdim
+        b = ctypes.c_double()
+        check(lib().smgpu_moments_bpar(h, ctypes.byref(b)))
+        self.bpar = b.value
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().smgpu_moments_destroy(self._h)
+            self._h = None
+
+    def apply(self, V, which):
+        """O V for the rows of V; which = "M1" or "E2"."""
+        V = np.ascontiguousarray(np.atleast_2d(V), dtype=np.float64)
+        W = np.zeros_like(V)
+        check(lib().smgpu_moments_apply_host(self._h, {"M1": 0, "E2": 1}[which], V.shape[0], ptr(V), ptr(W)))
+        return W
+
+    def eval(self, vecs, totJs):
+        """(mu, Q) per state; NaN where eval_moment prints nothing."""
+        V = np.ascontiguousarray(np.atleast_2d(vecs), dtype=np.float64)
+        J = np.ascontiguousarray(totJs, dtype=np.float64)
+        mu = np.zeros(V.shape[0])
+        Q = np.zeros(V.shape[0])
+        check(lib().smgpu_eval_moment(self._h, V.shape[0], ptr(V), ptr(J), ptr(mu), ptr(Q)))
+        return mu, Q
+
+
+def format_moments(mu, Q):
+    """eval_moment's text (transit.jl:730-763): per state 'mu (n=k) ' / 'Q (n=k) '."""
+    out = ""
+    for n, (m, q) in enumerate(zip(mu, Q)):
+        tx = ""
+        if not np.isnan(m):
+            tx += f"mu (n={n + 1}) " + f"{m:12.5f} " + "\t"
+        if not np.isnan(q):
+            tx += f"Q (n={n + 1}) " + f"{q:12.5f} "
+        if tx:
+            out += tx + "\n"
+    return out
+
+
 def _print_levels(ev, totJs, tJ):
     """main_sm's result lines (shellmodel_main.jl:256-266)."""
     if tJ is None:
@@ -368,7 +423,8 @@ def _print_levels(ev, totJs, tJ):
 def main_sm(sntf, target_nuc, num_ev, target_J=(), *, lm=300, ls=20, tol=1e-8, num_history=3,
             precision="fp64", refine=True, device=0, seed=1234, return_stats=False, return_J=False,
             save_wav=False, wav_path=None, in_wf="", q=1, is_block=False, mdimmode=False, verbose=False,
-            is_show=False):
+            is_show=False, calc_moment=False, gfactors=(1.0, 0.0, 5.586, -3.826), effcharge=(1.5, 0.5),
+            return_moments=False):
     """Lowest `num_ev` eigenvalues (MeV) of `target_nuc` with the interaction `sntf`.
 
     Mirrors main_sm (shellmodel_main.jl:89-281): with target_J = [] the plain
@@ -384,7 +440,10 @@ def main_sm(sntf, target_nuc, num_ev, target_J=(), *, lm=300, ls=20, tol=1e-8, n
     prints the dimension line and returns None after the basis (:129-130);
     verbose=True prints main_sm's dimension and J / En / Ex lines; is_show=True
     prints the wall time of the table build, upload and solve (main_sm's
-    TimerOutput summary, :279).
+    TimerOutput summary, :279).  calc_moment=True computes the magnetic dipole and
+    electric quadrupole moments of the states (eval_moment, :242-246, 267-269) with
+    main_sm's gfactors / effcharge keywords and prints them as main_sm does
+    (with verbose) or returns (mu, Q) with return_moments=True.
     """
     if len(target_J) > 1:
         raise ValueError("Multiple J is not supported (main_sm asserts length(target_J) <= 1)")
@@ -402,7 +461,7 @@ def main_sm(sntf, target_nuc, num_ev, target_J=(), *, lm=300, ls=20, tol=1e-8, n
     ham = Hamiltonian(prob, device)
     t2 = time.perf_counter()
     nev = min(int(num_ev), prob.mdim)
-    need_vecs = return_J or save_wav or verbose
+    need_vecs = return_J or save_wav or verbose or calc_moment or return_moments
     v0 = None
     if in_wf:
         from .wav import read_appwav
@@ -428,6 +487,15 @@ def main_sm(sntf, target_nuc, num_ev, target_J=(), *, lm=300, ls=20, tol=1e-8, n
             write_ritzvecs(wav_path, prob.info.twoM, ev, totJs, vecs)
         if return_J:
             out.append(totJs)
+        if calc_moment or return_moments:
+            mom = Moments(prob, device, gfactors, effcharge)
+            mu, Qm = mom.eval(vecs, totJs)
+            del mom
+            tx_mom = format_moments(mu, Qm)
+            if verbose and tx_mom:
+                print(tx_mom)
+            if return_moments:
+                out.append((mu, Qm))
     if is_show:
         print(f"tables {1e3 * (t1 - t0):.1f} ms, upload {1e3 * (t2 - t1):.1f} ms, solve {1e3 * (t3 - t2):.1f} ms "
               f"({st['iterations']} iterations, {st['n_hv']} H.v, {st['n_restart']} restarts)")

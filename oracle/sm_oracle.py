@@ -18,11 +18,15 @@ Parts:
                             lanczos_methods.jl:153-248)
   * `trl`                   lanczos_methods.jl:1-91, 250-307; shellmodel_main.jl:1005-1012
   * `main_sm`               shellmodel_main.jl:89-281 (eigenvalue observable only)
+  * moments                 transit.jl:49-283, 702-765 (init_ho_by_mass, radius_power,
+                            prep_obtr_diag, op_obtr_diag, eval_moment; exact 3j / 6j)
 
 Pinning (tests/test_oracle_*.py): the ckpot Be8 known answers
 (test/ShellModel_test.jl:10-14), the Mg24 dimension (example/log_sample_script.txt:323),
 closed-form dimensions, sympy's Clebsch-Gordan values, and an independent
-Jordan-Wigner brute-force Hamiltonian (tests/brute_force.py).
+Jordan-Wigner brute-force Hamiltonian (tests/brute_force.py).  The moments are pinned
+by the single-particle closed forms (Schmidt values, Q = -(2j-1)/(2j+2) <r^2> e),
+oscillator <r^2> values and 3j / 6j orthogonality (tests/test_moments.py).
 """
 from __future__ import annotations
 
@@ -588,3 +592,209 @@ def main_sm(snt_path, nuc, num_ev, lm=300, ls=20, tol=1e-8):
         return np.zeros(0)
     en, elit, nhv, nres, vks, T = trl(ham, min(num_ev, ham.mdim), lm=lm, ls=ls, tol=tol)
     return en
+
+
+# --------------------------------------------------------------------------- moments
+# transit.jl: init_ho_by_mass :49-60, radius_power :62-80, prep_obtr_diag :82-245,
+# op_obtr_diag :247-283, eval_moment :702-765; main_sm's mass formula :104-107.
+def init_ho_by_mass(A, hwprm=1):
+    """(hw, bpar) of the oscillator for mass number A (transit.jl:49-60)."""
+    amass = (938.27231 + 939.56563) / 2
+    hc = 197.32696
+    if hwprm == 1:
+        hw = 41.0 * A ** (-1.0 / 3.0)
+    else:
+        hw = 45.0 * A ** (-1.0 / 3.0) - 25.0 * A ** (-2.0 / 3.0)
+    return hw, hc / math.sqrt(hw * amass)
+
+
+def bpar_main_sm(A):
+    """main_sm's choice (shellmodel_main.jl:104-107): formula 2 for 16 <= A <= 40."""
+    return init_ho_by_mass(A, 2 if 16 <= A <= 40 else 1)[1]
+
+
+def _dfact(n):
+    r = 1
+    while n > 1:
+        r *= n
+        n -= 2
+    return r
+
+
+def _binom(n, k):
+    """Julia's binomial, including negative n."""
+    if k < 0:
+        return 0
+    if n < 0:
+        return (-1) ** k * _binom(k - n - 1, k)
+    return math.comb(n, k)
+
+
+def radius_power(k, n1, l1, n2, l2, bpar):
+    """<n1 l1|r^k|n2 l2> of oscillator orbits (transit.jl:62-80), exact integers then float."""
+    ll = l1 + l2 + k
+    ll1 = int((l2 - l1 + k) / 2)      # Julia div: truncation toward zero
+    ll2 = int((l1 - l2 + k) / 2)
+    s = Fraction(0)
+    for i in range(max(0, n1 - ll1, n2 - ll2), min(n1, n2) + 1):
+        s += _binom(ll1, n1 - i) * _binom(ll2, n2 - i) * Fraction(_dfact(ll + 2 * i + 1), _dfact(2 * i))
+    s = float(s)
+    s *= math.sqrt(_dfact(2 * n1) / _dfact(2 * n1 + 2 * l1 + 1))
+    s *= math.sqrt(_dfact(2 * n2) / _dfact(2 * n2 + 2 * l2 + 1))
+    return s * (1.0 / math.sqrt(2 ** k)) * (-1) ** (n1 - n2) * bpar ** k
+
+
+def wigner3j(j1, j2, j3, m1, m2, m3):
+    """(j1 j2 j3; m1 m2 m3), doubled arguments: (-1)^(j1-j2-m3) / sqrt(2 j3 + 1) CG."""
+    if m1 + m2 + m3 != 0 or (j1 - j2 - m3) % 2:
+        return 0.0
+    return (-1) ** ((j1 - j2 - m3) // 2) / math.sqrt(j3 + 1) * cg(j1, m1, j2, m2, j3, -m3)
+
+
+def _delta2(a, b, c):
+    """Delta(abc)^2 of doubled arguments (None if the triangle fails)."""
+    if (a + b + c) % 2 or a + b < c or a + c < b or b + c < a:
+        return None
+    return Fraction(_fact((a + b - c) // 2) * _fact((a - b + c) // 2) * _fact((-a + b + c) // 2),
+                    _fact((a + b + c) // 2 + 1))
+
+
+def wigner6j(j1, j2, j3, j4, j5, j6):
+    """{j1 j2 j3; j4 j5 j6}, doubled arguments: Racah's formula in rationals."""
+    ds = [_delta2(j1, j2, j3), _delta2(j1, j5, j6), _delta2(j4, j2, j6), _delta2(j4, j5, j3)]
+    if any(d is None for d in ds):
+        return 0.0
+    a = [(j1 + j2 + j3) // 2, (j1 + j5 + j6) // 2, (j4 + j2 + j6) // 2, (j4 + j5 + j3) // 2]
+    b = [(j1 + j2 + j4 + j5) // 2, (j2 + j3 + j5 + j6) // 2, (j3 + j1 + j6 + j4) // 2]
+    s = Fraction(0)
+    for t in range(max(a), min(b) + 1):
+        den = 1
+        for x in a:
+            den *= _fact(t - x)
+        for x in b:
+            den *= _fact(x - t)
+        s += Fraction((-1) ** t * _fact(t + 1), den)
+    sq = ds[0] * ds[1] * ds[2] * ds[3] * s * s
+    if sq == 0:
+        return 0.0
+    K = 200
+    root = math.isqrt((sq.numerator << (2 * K)) // sq.denominator)
+    return (1 if s > 0 else -1) * float(Fraction(root, 1 << K))
+
+
+def _phase_1(phi, cre, ani):
+    """TF_connectable_1 + calc_phase_1! (shellmodel_main.jl:863-868, 884-894): (ok, sign, new)."""
+    if phi & ani == 0:
+        return False, 0, 0
+    mid = phi & ~ani
+    if mid & cre:
+        return False, 0, 0
+    e = 1 + bin(phi & ~(ani - 1)).count("1")
+    e += bin(mid & ~(cre - 1)).count("1")
+    return True, (-1) ** e, mid | cre
+
+
+def prep_obtr_diag(p_sps, n_sps, msps_p, msps_n, pbits, nbits, bpar, gfactors, eff_charge):
+    """prep_obtr_diag (transit.jl:82-245): M1 and E2 one-body operator lists
+    {"M1": (proton, neutron), "E2": (proton, neutron)}, entries (block, Npi, Npf,
+    phase, fac) with 0-based SD indices; phase means the term enters negated."""
+    c_mu = math.sqrt(3.0 / (4.0 * math.pi))
+    c_Q = math.sqrt(5.0 / (16.0 * math.pi))
+    out = {"M1": ([], []), "E2": ([], [])}
+    for sp, (sps, msps, bits) in enumerate(((p_sps, msps_p, pbits), (n_sps, msps_n, nbits))):
+        gl, gs = (gfactors[0], gfactors[2]) if sp == 0 else (gfactors[1], gfactors[3])
+        ec = eff_charge[sp]
+        L = len(msps)
+        for i in range(len(sps)):
+            ni, li, ji, _ = sps[i]
+            for f in range(i, len(sps)):
+                nf, lf, jf, _ = sps[f]
+                ex_r2 = radius_power(2, ni, li, nf, lf, bpar)
+                l6j = wigner6j(2 * lf, 2 * li, 2, ji, jf, 1)
+                s6j = wigner6j(1, 1, 2, ji, jf, 2 * lf)
+                Q3j = wigner3j(ji, 4, jf, 1, 0, -1)
+                _, mc_s, mc_idx = _possible_mz(sps[i], msps)
+                _, ma_s, ma_idx = _possible_mz(sps[f], msps)
+                pfac = math.sqrt((ji + 1) * (jf + 1))
+                for mode in ("M1", "E2"):
+                    if mode == "M1" and (ni != nf or li != lf):
+                        continue
+                    for ic, mc in enumerate(mc_s):
+                        bit_c = 1 << (L - 1 - mc_idx[ic])
+                        for ia, ma in enumerate(ma_s):
+                            if ma != mc:
+                                continue
+                            bit_a = 1 << (L - 1 - ma_idx[ia])
+                            if mode == "M1":
+                                sign1 = (-1) ** ((jf - ma) // 2 + lf + (ji + 3) // 2)
+                                sign2 = (-1) ** ((jf - ma) // 2 + lf + (jf + 3) // 2)
+                                fac1 = sign1 * l6j * math.sqrt(lf * (lf + 1) * (2 * lf + 1))
+                                fac2 = sign2 * s6j * math.sqrt(1.5)
+                                coeff = pfac * wigner3j(jf, 2, ji, -ma, 0, mc) * (fac1 * gl + fac2 * gs) * c_mu
+                            else:
+                                signQ = (-1) ** (jf + (1 - ma) // 2) * (1 + (-1) ** (li + lf))
+                                if signQ == 0:
+                                    continue
+                                coeff = signQ * wigner3j(jf, 4, ji, -ma, 0, mc) * Q3j * pfac * ec * c_Q * ex_r2
+                                if sp == 1 and coeff == 0.0:
+                                    continue
+                            for bi, blist in enumerate(bits):
+                                where = {b: k for k, b in enumerate(blist)}
+                                for Ni, phi in enumerate(blist):
+                                    ok, sgn, new = _phase_1(phi, bit_a, bit_c)
+                                    if not ok or new not in where:
+                                        continue
+                                    Nf = where[new]
+                                    out[mode][sp].append((bi, Ni, Nf, sgn == -1, coeff * (0.5 if Ni == Nf else 1.0)))
+    return out
+
+
+def op_obtr_diag(p_ops, n_ops, pbits, nbits, tdims, wf):
+    """op_obtr_diag (transit.jl:247-283): twf = O wf (each entry applied both ways)."""
+    wf = np.asarray(wf, dtype=np.float64)
+    twf = np.zeros_like(wf)
+    for bi, Ni, Nf, ph, fac in p_ops:
+        c = -fac if ph else fac
+        lN = len(nbits[bi])
+        a, b = tdims[bi] + Ni * lN, tdims[bi] + Nf * lN
+        twf[b:b + lN] += c * wf[a:a + lN]
+        twf[a:a + lN] += c * wf[b:b + lN]
+    for bi, Ni, Nf, ph, fac in n_ops:
+        c = -fac if ph else fac
+        lN = len(nbits[bi])
+        nP = len(pbits[bi])
+        a = tdims[bi] + Ni + lN * np.arange(nP)
+        b = tdims[bi] + Nf + lN * np.arange(nP)
+        twf[b] += c * wf[a]
+        twf[a] += c * wf[b]
+    return twf
+
+
+def eval_moment(Mtot, Rvecs, totJs, ops, pbits, nbits, tdims):
+    """eval_moment (transit.jl:702-765): per state n, (mu or None, Q or None), and the
+    text main_sm prints.  totJs are J values (J < 0 skipped), Mtot is 2M."""
+    c_mu = math.sqrt(4.0 * math.pi / 3.0)
+    c_Q = math.sqrt(16.0 * math.pi / 5.0)
+    res, tx_mom = [], ""
+    M2 = int(Mtot)
+    for nth, wf in enumerate(Rvecs):
+        t1 = op_obtr_diag(ops["M1"][0], ops["M1"][1], pbits, nbits, tdims, wf)
+        t2 = op_obtr_diag(ops["E2"][0], ops["E2"][1], pbits, nbits, tdims, wf)
+        J2 = int(round(2 * totJs[nth]))
+        mu = Q = None
+        if J2 >= 0:
+            tx = ""
+            sel = wigner3j(J2, 2, J2, -M2, 0, M2)
+            if sel != 0.0:
+                cgv = wigner3j(J2, 2, J2, -J2, 0, J2) / sel
+                mu = (-1) ** ((J2 - M2) // 2) * float(np.dot(t1, wf)) * cgv * c_mu
+                tx += f"mu (n={nth + 1}) " + f"{mu:12.5f} " + "\t"
+            sel = wigner3j(J2, 4, J2, -M2, 0, M2)
+            if sel != 0.0:
+                cgv = wigner3j(J2, 4, J2, -J2, 0, J2) / sel
+                Q = float(np.dot(t2, wf)) * cgv * c_Q * (-1) ** ((J2 - M2) // 2)
+                tx += f"Q (n={nth + 1}) " + f"{Q:12.5f} "
+            if tx:
+                tx_mom += tx + "\n"
+        res.append((mu, Q))
+    return res, tx_mom
