@@ -58,8 +58,17 @@ typedef struct {
     int32_t max_iter;    /* safety limit on Lanczos iterations (0 = 100000) */
     uint64_t seed;       /* start vector seed */
     int32_t reorth;      /* SMGPU_REORTH_*: re-orthogonalisation of the TRL basis */
-    int32_t reserved_;
+    int32_t steps;       /* SMGPU_STEPS_*: how the Lanczos steps of a solve reach the device */
 } smgpu_solve_params;
+
+/* SMGPU_STEPS_GRAPH: the kernels of one Lanczos step (H·v, reorthogonalisation,
+ * Ritz values and convergence test) are captured once per solve as a hipGraph
+ * and replayed every step, reading the iteration from device memory; LAUNCH: each
+ * kernel is launched; AUTO: the library's choice by basis size.  Results are
+ * bit-identical.  (Single solves; lockstep batches and J-projected solves launch.) */
+#define SMGPU_STEPS_AUTO 0
+#define SMGPU_STEPS_LAUNCH 1
+#define SMGPU_STEPS_GRAPH 2
 
 /* Re-orthogonalisation of each new Lanczos vector against the stored basis:
  * DEFAULT is FULL for FP64 (the reference's ReORTH at every step) and PARTIAL for

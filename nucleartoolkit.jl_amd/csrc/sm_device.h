@@ -171,6 +171,9 @@ struct HvVecs {
     const void* pp[4];
     const void* nn[4];
     const void* veff[4];
+    // captured Lanczos steps: v[b] is a basis and the input is v[b] + (*vsel) * vld
+    const int* vsel;
+    int64_t vld;
 };
 // w[b] = H v[b] for nvec arbitrary vectors, in launches of up to four that share
 // the table loads (each result bit-identical to launch_hv)
@@ -184,6 +187,9 @@ void launch_hv_variants(const DeviceHam* const* hams, int prec, int nvec, const 
 // up to four vectors that share the table loads
 void launch_hv_multi(const DeviceHam& h, int prec, int nvec, const void* v, int64_t ldv, void* w, int64_t ldw,
                      hipStream_t s);
+// one-vector H·v of column *vsel of the basis v (leading dimension ld) into w, the
+// column read on the device (captured Lanczos steps)
+void launch_hv_column(const DeviceHam& h, int prec, const void* v, int64_t ld, const int* vsel, void* w, hipStream_t s);
 // re-upload interaction-dependent coefficients (same basis and index tables)
 void update_coefficients(DeviceHam& h, const Basis& b, const Tables& t);
 

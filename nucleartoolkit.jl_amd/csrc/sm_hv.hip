@@ -91,13 +91,17 @@ __global__ __launch_bounds__(TPB) void k_hv(HvTables A, HvVecs P) {
     auto cpp = [&](int b) { return static_cast<const T*>(PV ? P.pp[b] : A.pp_val); };
     auto cnn = [&](int b) { return static_cast<const T*>(PV ? P.nn[b] : A.nn_val); };
     auto cve = [&](int b) { return static_cast<const T*>(PV ? P.veff[b] : A.veff); };
+    const int64_t vshift = P.vsel ? (int64_t)*P.vsel * P.vld : 0;
+    const T* vin[NV];
+#pragma unroll
+    for (int b = 0; b < NV; b++) vin[b] = static_cast<const T*>(P.v[b]) + vshift;
     for (int i = tid; i < nN; i += TPB) {
 #pragma unroll
         for (int r = 0; r < RG; r++)
 #pragma unroll
             for (int b = 0; b < NV; b++)
                 xs[(i * RG + r) * NV + b] =
-                    r < nr ? static_cast<const T*>(P.v[b])[bstart + (int64_t)(fp0 + r) * nN + i] : T(0);
+                    r < nr ? vin[b][bstart + (int64_t)(fp0 + r) * nN + i] : T(0);
     }
     __syncthreads();
 
@@ -194,7 +198,7 @@ __global__ __launch_bounds__(TPB) void k_hv(HvTables A, HvVecs P) {
                     for (int i = tid; i < nNi; i += TPB) {
 #pragma unroll
                         for (int b = 0; b < NV; b++) {
-                            const T* vb = static_cast<const T*>(P.v[b]);
+                            const T* vb = vin[b];
                             T4 x4;
                             x4.x = sg[0] * vb[xo[0] + i];
                             x4.y = sg[1] * vb[xo[1] + i];
@@ -448,7 +452,7 @@ __global__ __launch_bounds__(TPB) void k_hv(HvTables A, HvVecs P) {
                         for (int b = 0; b < NV; b++) {
                             T x[UP];
 #pragma unroll
-                            for (int u = 0; u < UP; u++) x[u] = static_cast<const T*>(P.v[b])[so[u] + fnc];
+                            for (int u = 0; u < UP; u++) x[u] = vin[b][so[u] + fnc];
                             T a = acc[k][r][b];
 #pragma unroll
                             for (int u = 0; u < UP; u++) a += c[u][PV ? b : 0] * x[u];
@@ -561,6 +565,17 @@ void launch_hv(const DeviceHam& h, int prec, const void* v, void* w, hipStream_t
     HvVecs P{};
     P.v[0] = v;
     P.w[0] = w;
+    if (prec == 0) launch_t<double>(h.view[0], 1, false, P, s);
+    else launch_t<float>(h.view[1], 1, false, P, s);
+}
+
+void launch_hv_column(const DeviceHam& h, int prec, const void* v, int64_t ld, const int* vsel, void* w, hipStream_t s) {
+    if (h.idx->R == 0) return;
+    HvVecs P{};
+    P.v[0] = v;
+    P.w[0] = w;
+    P.vsel = vsel;
+    P.vld = ld;
     if (prec == 0) launch_t<double>(h.view[0], 1, false, P, s);
     else launch_t<float>(h.view[1], 1, false, P, s);
 }

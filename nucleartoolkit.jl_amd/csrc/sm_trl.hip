@@ -62,12 +62,56 @@ static constexpr int TB = 256;
 // Largest vector (amplitudes) whose mixed-precision Lanczos step (partial
 // re-orthogonalisation) runs as the one-workgroup k_step_small
 static constexpr int64_t SMALL_STEP_N = 8192;
+// solo solves up to this many amplitudes replay captured steps by default (SMGPU_STEPS_AUTO)
+static constexpr int64_t GRAPH_AUTO_N = 0;
 static constexpr int EPL = 8;                 // elements per lane in the streaming passes
 static constexpr int CHUNK = 64 * EPL;        // x per block in k_dotsw
 static constexpr int UCHUNK = TB * 4;         // x per block in k_updatew
 // a J-filtered Lanczos vector that keeps less than this of its norm was noise
 // outside the J space: the J-space Krylov subspace is exhausted
 static constexpr double J_EXHAUSTED = 1e-3;
+
+// Where one Lanczos iteration stands, kept in device memory so that one captured
+// graph replays every step: the step kernels' *_s forms read the iteration and the
+// ring slots from here, and the last node (k_step_next) moves it to the next step.
+struct StepRitz {
+    int it;      // T[0..it] of the step
+    int nn;      // min(num_ev, it + 1): Ritz values of the step
+    int nprev;   // values of the step before usable as interlacing brackets
+    int ring;    // (step + 1) & 3: the step's slot of the Ritz-value and flag rings
+    int pring;   // step & 3: the slot of the step before
+};
+struct StepDev {
+    int it;      // basis index of the H·v input v_it
+    int nv;      // it + 1: vectors of the projection pass
+    int store;   // it + 1 < lm: the step stores v_{it+1}
+    StepRitz cur;    // this step's Ritz problem
+    StepRitz prev;   // the previous step's, solved beside this step's H·v
+    int prev_valid;  // 0 for the first step of a sweep
+};
+
+__device__ inline void step_at(StepDev* sd, int it, int ring, int nprev, int num_ev, int lm) {
+    sd->it = it;
+    sd->nv = it + 1;
+    sd->store = it + 1 < lm ? 1 : 0;
+    sd->cur.it = it;
+    sd->cur.nn = min(num_ev, it + 1);
+    sd->cur.nprev = nprev;
+    sd->cur.ring = ring;
+    sd->cur.pring = (ring + 3) & 3;
+}
+
+__global__ void k_step_set(StepDev* sd, int it, int step, int nprev, int num_ev, int lm) {
+    step_at(sd, it, (step + 1) & 3, nprev, num_ev, lm);
+    sd->prev_valid = 0;
+}
+
+__global__ void k_step_next(StepDev* sd, int num_ev, int lm) {
+    sd->prev = sd->cur;
+    sd->prev_valid = 1;
+    const int it = sd->it + 1;
+    step_at(sd, it, (sd->cur.ring + 1) & 3, min(num_ev, it), num_ev, lm);
+}
 
 __device__ inline double wave_sum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -122,7 +166,7 @@ __global__ __launch_bounds__(TB) void k_dotsw(const TBv* __restrict__ B, int64_t
 
 // part[blk * ldp] = sum over the block's grid-stride range of a[x] b[x]
 template <typename TA, typename TB2>
-__global__ __launch_bounds__(TB) void k_dot2(const TA* __restrict__ a, const TB2* __restrict__ b, int64_t n,
+__device__ __forceinline__ void dot2_body(const TA* __restrict__ a, const TB2* __restrict__ b, int64_t n,
                                              double* __restrict__ part, int ldp) {
     __shared__ double red[TB / 64];
     double s = 0.0;
@@ -131,9 +175,21 @@ __global__ __launch_bounds__(TB) void k_dot2(const TA* __restrict__ a, const TB2
     s = block_sum(s, red);
     if (threadIdx.x == 0) part[(int64_t)blockIdx.x * ldp] = s;
 }
+template <typename TA, typename TB2>
+__global__ __launch_bounds__(TB) void k_dot2(const TA* __restrict__ a, const TB2* __restrict__ b, int64_t n,
+                                             double* __restrict__ part, int ldp) {
+    dot2_body<TA, TB2>(a, b, n, part, ldp);
+}
+// step-state form: a = base + it * lda
+template <typename TA, typename TB2>
+__global__ __launch_bounds__(TB) void k_dot2_s(const StepDev* __restrict__ sd, const TA* __restrict__ base, int64_t lda,
+                                               const TB2* __restrict__ b, int64_t n, double* __restrict__ part, int ldp) {
+    dot2_body<TA, TB2>(base + (int64_t)sd->it * lda, b, n, part, ldp);
+}
+
 
 // out[j] = sum_g part[g*ldp + j], j < nj
-__global__ __launch_bounds__(TB) void k_reduce(const double* __restrict__ part, int G, int ldp,
+__device__ __forceinline__ void reduce_body(const double* __restrict__ part, int G, int ldp,
                                                double* __restrict__ out, const int* __restrict__ gate) {
     if (gate && *gate == 0) return;
     __shared__ double red[TB / 64];
@@ -143,6 +199,17 @@ __global__ __launch_bounds__(TB) void k_reduce(const double* __restrict__ part, 
     s = block_sum(s, red);
     if (threadIdx.x == 0) out[j] = s;
 }
+__global__ __launch_bounds__(TB) void k_reduce(const double* __restrict__ part, int G, int ldp,
+                                               double* __restrict__ out, const int* __restrict__ gate) {
+    reduce_body(part, G, ldp, out, gate);
+}
+// step-state form over nv = it + 1 outputs: launched with lm blocks, those past nv exit
+__global__ __launch_bounds__(TB) void k_reduce_s(const StepDev* __restrict__ sd, const double* __restrict__ part, int G,
+                                                 int ldp, double* __restrict__ out, const int* __restrict__ gate) {
+    if ((int)blockIdx.x >= sd->nv) return;
+    reduce_body(part, G, ldp, out, gate);
+}
+
 
 // wout[x] = win[x] - sum_j h[j] B[j][x];  part[blk] = sum wout^2 over the block
 template <typename TBv, typename TW>
@@ -221,7 +288,7 @@ template <typename TBv> __device__ inline int64_t gs_x(int64_t seg0, int lane, i
 // w' is also written to wout (FP64) and |w'|^2 partials to npart[blk].
 // gate: skip the launch when *gate == 0 (conditional second pass).
 template <typename TBv, typename TW, bool LOCAL>
-__global__ __launch_bounds__(TB) void k_gs_dots(const TBv* __restrict__ B, int64_t ldb, int nv,
+__device__ __forceinline__ void gs_dots_body(const TBv* __restrict__ B, int64_t ldb, int nv,
                                                 const TW* __restrict__ w, int64_t n, double* __restrict__ part,
                                                 int ldp, const int* __restrict__ gate,
                                                 const TBv* __restrict__ vc, const TBv* __restrict__ vp,
@@ -285,13 +352,35 @@ __global__ __launch_bounds__(TB) void k_gs_dots(const TBv* __restrict__ B, int64
         if (lane == 0) part[(int64_t)blockIdx.x * ldp + j] = s;
     }
 }
+template <typename TBv, typename TW, bool LOCAL>
+__global__ __launch_bounds__(TB) void k_gs_dots(const TBv* __restrict__ B, int64_t ldb, int nv,
+                                                const TW* __restrict__ w, int64_t n, double* __restrict__ part,
+                                                int ldp, const int* __restrict__ gate,
+                                                const TBv* __restrict__ vc, const TBv* __restrict__ vp,
+                                                const double* __restrict__ alpha, const double* __restrict__ bprev,
+                                                double* __restrict__ wout, double* __restrict__ npart) {
+    gs_dots_body<TBv, TW, LOCAL>(B, ldb, nv, w, n, part, ldp, gate, vc, vp, alpha, bprev, wout, npart);
+}
+// step-state form: nv = it + 1; with LOCAL, v_c = B_it, v_p = B_{it-1} and bprev = beta + it - 1
+template <typename TBv, typename TW, bool LOCAL>
+__global__ __launch_bounds__(TB) void k_gs_dots_s(const StepDev* __restrict__ sd, const TBv* __restrict__ B, int64_t ldb,
+                                                  const TW* __restrict__ w, int64_t n, double* __restrict__ part,
+                                                  int ldp, const int* __restrict__ gate,
+                                                  const double* __restrict__ alpha, const double* __restrict__ beta,
+                                                  double* __restrict__ wout, double* __restrict__ npart) {
+    const int it = sd->it;
+    gs_dots_body<TBv, TW, LOCAL>(B, ldb, it + 1, w, n, part, ldp, gate, B + (int64_t)it * ldb,
+                                 it > 0 ? B + (int64_t)(it - 1) * ldb : nullptr, alpha,
+                                 it > 0 ? beta + it - 1 : nullptr, wout, npart);
+}
+
 
 // wout = win - sum_j h[j] B[j]  and, if dots, part = partial B^T wout; npart = |wout|^2 partial.
 // Each wave owns a quarter of the block's segment for the update and then takes
 // vectors j = wave, wave+4, ... for the dot products (its segment rows were
 // just streamed by the block, so this second read hits the caches).
 template <typename TBv, typename TW, bool DOTS>
-__global__ __launch_bounds__(TB) void k_gs_upd(const TBv* __restrict__ B, int64_t ldb, int nv,
+__device__ __forceinline__ void gs_upd_body(const TBv* __restrict__ B, int64_t ldb, int nv,
                                                const double* __restrict__ h, const TW* __restrict__ win,
                                                double* __restrict__ wout, int64_t n, double* __restrict__ part,
                                                int ldp, double* __restrict__ npart, const int* __restrict__ gate) {
@@ -363,6 +452,22 @@ __global__ __launch_bounds__(TB) void k_gs_upd(const TBv* __restrict__ B, int64_
         if (lane == 0) part[(int64_t)blockIdx.x * ldp + j] = s;
     }
 }
+template <typename TBv, typename TW, bool DOTS>
+__global__ __launch_bounds__(TB) void k_gs_upd(const TBv* __restrict__ B, int64_t ldb, int nv,
+                                               const double* __restrict__ h, const TW* __restrict__ win,
+                                               double* __restrict__ wout, int64_t n, double* __restrict__ part,
+                                               int ldp, double* __restrict__ npart, const int* __restrict__ gate) {
+    gs_upd_body<TBv, TW, DOTS>(B, ldb, nv, h, win, wout, n, part, ldp, npart, gate);
+}
+// step-state form: nv = it + 1
+template <typename TBv, typename TW, bool DOTS>
+__global__ __launch_bounds__(TB) void k_gs_upd_s(const StepDev* __restrict__ sd, const TBv* __restrict__ B, int64_t ldb,
+                                                 const double* __restrict__ h, const TW* __restrict__ win,
+                                                 double* __restrict__ wout, int64_t n, double* __restrict__ part,
+                                                 int ldp, double* __restrict__ npart, const int* __restrict__ gate) {
+    gs_upd_body<TBv, TW, DOTS>(B, ldb, sd->nv, h, win, wout, n, part, ldp, npart, gate);
+}
+
 
 // alpha[it] = v.Hv (before any orthogonalisation), beta[it] = |w| after CGS2
 __global__ void k_finish(const double* h1, int it, const double* npart, int G, double* alpha, double* beta) {
@@ -401,7 +506,7 @@ __global__ void k_dgks(const double* npart_after, const double* npart_before, in
 }
 
 // alpha[it] = v_it . H v_it (the local-step coefficient), beta[it] = |w| at the end
-__global__ void k_finish_local(double* loc, int it, const double* npart, int G, double* alpha, double* beta) {
+__device__ __forceinline__ void finish_local_body(double* loc, int it, const double* npart, int G, double* alpha, double* beta) {
     __shared__ double red[TB / 64];
     double s = 0.0;
     for (int g = threadIdx.x; g < G; g += TB) s += npart[g];
@@ -412,6 +517,14 @@ __global__ void k_finish_local(double* loc, int it, const double* npart, int G, 
         loc[3] += loc[4];          // second CGS passes so far (loc[4] = this step's gate)
     }
 }
+__global__ void k_finish_local(double* loc, int it, const double* npart, int G, double* alpha, double* beta) {
+    finish_local_body(loc, it, npart, G, alpha, beta);
+}
+__global__ void k_finish_local_s(const StepDev* __restrict__ sd, double* loc, const double* npart, int G, double* alpha,
+                                 double* beta) {
+    finish_local_body(loc, sd->it, npart, G, alpha, beta);
+}
+
 
 __global__ void k_sqrt1(const double* a, double* b) { b[0] = sqrt(a[0]); }
 
@@ -419,7 +532,7 @@ __global__ void k_sqrt1(const double* a, double* b) { b[0] = sqrt(a[0]); }
 // with |w|^2 partials per segment of GsShape<TBv>::SEG elements into npart and npart2
 // (the same partition as the Gram-Schmidt kernels, so k_finish_local reads either).
 template <typename TBv, typename TW>
-__global__ __launch_bounds__(TB) void k_local_step(const TW* __restrict__ win, const TBv* __restrict__ vc,
+__device__ __forceinline__ void local_step_body(const TW* __restrict__ win, const TBv* __restrict__ vc,
                                                    const TBv* __restrict__ vp, const double* __restrict__ alpha,
                                                    const double* __restrict__ bprev, double* __restrict__ wout,
                                                    int64_t n, double* __restrict__ npart, double* __restrict__ npart2) {
@@ -438,6 +551,25 @@ __global__ __launch_bounds__(TB) void k_local_step(const TW* __restrict__ win, c
     nr = block_sum(nr, red);
     if (threadIdx.x == 0) { npart[blockIdx.x] = nr; npart2[blockIdx.x] = nr; }
 }
+template <typename TBv, typename TW>
+__global__ __launch_bounds__(TB) void k_local_step(const TW* __restrict__ win, const TBv* __restrict__ vc,
+                                                   const TBv* __restrict__ vp, const double* __restrict__ alpha,
+                                                   const double* __restrict__ bprev, double* __restrict__ wout,
+                                                   int64_t n, double* __restrict__ npart, double* __restrict__ npart2) {
+    local_step_body<TBv, TW>(win, vc, vp, alpha, bprev, wout, n, npart, npart2);
+}
+// step-state form: v_c = B_it, v_p = B_{it-1}, bprev = beta + it - 1
+template <typename TBv, typename TW>
+__global__ __launch_bounds__(TB) void k_local_step_s(const StepDev* __restrict__ sd, const TW* __restrict__ win,
+                                                     const TBv* __restrict__ B, int64_t ldb,
+                                                     const double* __restrict__ alpha, const double* __restrict__ beta,
+                                                     double* __restrict__ wout, int64_t n, double* __restrict__ npart,
+                                                     double* __restrict__ npart2) {
+    const int it = sd->it;
+    local_step_body<TBv, TW>(win, B + (int64_t)it * ldb, it > 0 ? B + (int64_t)(it - 1) * ldb : nullptr, alpha,
+                             it > 0 ? beta + it - 1 : nullptr, wout, n, npart, npart2);
+}
+
 
 // parameters of the omega recurrence: eps = unit of the basis precision (the value
 // a reorthogonalised row restarts from), eps_r = rounding level of one Lanczos step
@@ -522,6 +654,15 @@ __global__ void k_omega(double* __restrict__ alpha, double* __restrict__ beta, i
         if (hgate) *hgate = run;
     }
 }
+__global__ void k_omega_s(const StepDev* __restrict__ sd, double* __restrict__ alpha, double* __restrict__ beta,
+                          const double* __restrict__ npart, int G, double* __restrict__ om, int ldo,
+                          double* __restrict__ loc, int* __restrict__ gate, OmegaParams op) {
+    __shared__ double bj;
+    const int j = sd->it;
+    const int run = omega_block(alpha, beta, j, npart, G, om, ldo, loc, gate, op, &bj);
+    if (threadIdx.x == 0 && !run) { alpha[j] = loc[0]; beta[j] = bj; }
+}
+
 
 // One projection pass of w (FP64, in place) against the basis B[0..nv) by one
 // workgroup: h[j] = B_j . w (wave j mod 4, lanes along the vector), then
@@ -562,7 +703,7 @@ __device__ double small_pass(const TBv* __restrict__ B, int64_t ldb, int nv, dou
 // vnext = w / beta[it] (when set).  The same operations as the multi-kernel step
 // (k_dot2 .. k_store), with one launch instead of about a dozen dependent ones.
 template <typename TBv, typename TW>
-__global__ __launch_bounds__(TB) void k_step_small(const TW* __restrict__ win, const TBv* __restrict__ B,
+__device__ __forceinline__ void step_small_body(const TW* __restrict__ win, const TBv* __restrict__ B,
                                                    int64_t ldb, int it, int64_t n, double* __restrict__ w,
                                                    double* __restrict__ loc, int* __restrict__ gate,
                                                    double* __restrict__ alpha, double* __restrict__ beta,
@@ -615,6 +756,29 @@ __global__ __launch_bounds__(TB) void k_step_small(const TW* __restrict__ win, c
     #pragma unroll 4
     for (int64_t x = threadIdx.x; x < n; x += TB) vnext[x] = (TBv)(w[x] * inv);
 }
+template <typename TBv, typename TW>
+__global__ __launch_bounds__(TB) void k_step_small(const TW* __restrict__ win, const TBv* __restrict__ B,
+                                                   int64_t ldb, int it, int64_t n, double* __restrict__ w,
+                                                   double* __restrict__ loc, int* __restrict__ gate,
+                                                   double* __restrict__ alpha, double* __restrict__ beta,
+                                                   double* __restrict__ om, int ldo, OmegaParams op,
+                                                   double* __restrict__ h1, double* __restrict__ h2,
+                                                   TBv* __restrict__ vnext) {
+    step_small_body<TBv, TW>(win, B, ldb, it, n, w, loc, gate, alpha, beta, om, ldo, op, h1, h2, vnext);
+}
+// step-state form: it from sd, vnext = B_{it+1} while it + 1 < lm
+template <typename TBv, typename TW>
+__global__ __launch_bounds__(TB) void k_step_small_s(const StepDev* __restrict__ sd, const TW* __restrict__ win,
+                                                     TBv* __restrict__ B, int64_t ldb, int64_t n, double* __restrict__ w,
+                                                     double* __restrict__ loc, int* __restrict__ gate,
+                                                     double* __restrict__ alpha, double* __restrict__ beta,
+                                                     double* __restrict__ om, int ldo, OmegaParams op,
+                                                     double* __restrict__ h1, double* __restrict__ h2) {
+    const int it = sd->it;
+    step_small_body<TBv, TW>(win, B, ldb, it, n, w, loc, gate, alpha, beta, om, ldo, op, h1, h2,
+                             sd->store ? B + (int64_t)(it + 1) * ldb : nullptr);
+}
+
 
 // omega rows after a thick restart at step j: the kept vectors and v_j are
 // orthonormalised, so om_{j,k} = om_{j-1,k} = eps off the diagonal; the next two
@@ -652,6 +816,19 @@ __global__ __launch_bounds__(TB) void k_store(const double* __restrict__ w, cons
     for (int64_t x = (int64_t)blockIdx.x * TB + threadIdx.x; x < n; x += (int64_t)gridDim.x * TB)
         dst[x] = (TBv)(w[x] * inv);
 }
+// step-state form: V_{it+1} = w / beta[it] while it + 1 < lm
+template <typename TBv>
+__global__ __launch_bounds__(TB) void k_store_s(const StepDev* __restrict__ sd, const double* __restrict__ w,
+                                                const double* __restrict__ beta, TBv* __restrict__ B, int64_t ldb,
+                                                int64_t n) {
+    if (!sd->store) return;
+    const int it = sd->it;
+    const double inv = 1.0 / beta[it];
+    TBv* dst = B + (int64_t)(it + 1) * ldb;
+    for (int64_t x = (int64_t)blockIdx.x * TB + threadIdx.x; x < n; x += (int64_t)gridDim.x * TB)
+        dst[x] = (TBv)(w[x] * inv);
+}
+
 
 template <typename TA, typename TBv>
 __global__ __launch_bounds__(TB) void k_convert(const TA* __restrict__ a, TBv* __restrict__ b, int64_t n, double scale) {
@@ -666,10 +843,10 @@ __global__ __launch_bounds__(TB) void k_convert(const TA* __restrict__ a, TBv* _
 // test of diagonalize_T! (lanczos_methods.jl:250-258: all(prev - out < tol) over
 // num_ev values; entries past the blocks keep prev) and writes its result to
 // *flag and to the host-visible *hflag (ticket: zero between launches).
-__global__ __launch_bounds__(TB) void k_ritz(const double* __restrict__ alpha, const double* __restrict__ beta,
-                                             int n, const double* __restrict__ prev, int nprev,
-                                             double* __restrict__ out, int num_ev, double tol,
-                                             int* __restrict__ flag, int* hflag, int* __restrict__ ticket) {
+__device__ __forceinline__ void ritz_body(const double* __restrict__ alpha, const double* __restrict__ beta,
+                                          int n, const double* __restrict__ prev, int nprev,
+                                          double* __restrict__ out, int num_ev, double tol,
+                                          int* __restrict__ flag, int* hflag, int* __restrict__ ticket, int nblk) {
     __shared__ double sa[1024], sb[1024];
     __shared__ int firstbig;
     __shared__ double slo, shi;
@@ -751,12 +928,12 @@ __global__ __launch_bounds__(TB) void k_ritz(const double* __restrict__ alpha, c
     __shared__ int is_last;
     if (threadIdx.x == 0) {
         __threadfence();
-        is_last = atomicAdd(ticket, 1) == (int)gridDim.x - 1;
+        is_last = atomicAdd(ticket, 1) == nblk - 1;
     }
     __syncthreads();
     if (!is_last || threadIdx.x != 0) return;
     __threadfence();
-    for (int k = (int)gridDim.x; k < num_ev; k++) out[k] = prev[k];
+    for (int k = nblk; k < num_ev; k++) out[k] = prev[k];
     int okc = 1;
     for (int k = 0; k < num_ev; k++) okc &= (prev[k] - out[k] < tol) ? 1 : 0;
     *flag = okc;
@@ -764,6 +941,26 @@ __global__ __launch_bounds__(TB) void k_ritz(const double* __restrict__ alpha, c
     __threadfence_system();
     *ticket = 0;
 }
+__global__ __launch_bounds__(TB) void k_ritz(const double* __restrict__ alpha, const double* __restrict__ beta,
+                                             int n, const double* __restrict__ prev, int nprev,
+                                             double* __restrict__ out, int num_ev, double tol,
+                                             int* __restrict__ flag, int* hflag, int* __restrict__ ticket) {
+    ritz_body(alpha, beta, n, prev, nprev, out, num_ev, tol, flag, hflag, ticket, (int)gridDim.x);
+}
+// step-state form: the Ritz problem of the previous step (T[0..it], its ring slots),
+// nothing for the first step of a sweep; launched with num_ev blocks, those past
+// min(num_ev, it + 1) exit
+__global__ __launch_bounds__(TB) void k_ritz_s(const StepDev* __restrict__ sd, const double* __restrict__ alpha,
+                                               const double* __restrict__ beta, double* __restrict__ enring,
+                                               int num_ev, double tol, int* __restrict__ flags, int* hflag,
+                                               int* __restrict__ ticket) {
+    if (!sd->prev_valid) return;
+    const StepRitz r = sd->prev;
+    if ((int)blockIdx.x >= r.nn) return;
+    ritz_body(alpha, beta, r.it + 1, enring + (int64_t)r.pring * 64, r.nprev, enring + (int64_t)r.ring * 64,
+              num_ev, tol, flags + r.ring, hflag + r.ring, ticket, r.nn);
+}
+
 
 // diagonalize_T! bookkeeping: entries n..num_ev-1 keep the previous values,
 // flag = all(prev - cur < tol)
@@ -877,7 +1074,7 @@ struct Inconsistent : std::runtime_error {
 
 struct Workspace {
     DBuf basis, wH, w64, part, npart, npart2, h1, h2, alpha, beta, enring, flags, S, swork, U, scal, tvals, Y, Q,
-        R, hv64, loc, gate, jy, jz, om;
+        R, hv64, loc, gate, jy, jz, om, stepdev;
     int* hflag = nullptr;          // host ring of convergence flags (coherent; k_ritz writes it),
                                    // hflag[4]: the projection gate of the step (k_omega writes it)
     hipEvent_t ev[4] = {};         // side stream: a step's convergence flag is in hflag
@@ -898,7 +1095,7 @@ struct Workspace {
     int64_t capacity() const {
         int64_t c = 0;
         for (const DBuf* d : {&basis, &wH, &w64, &part, &npart, &npart2, &h1, &h2, &alpha, &beta, &enring, &flags, &S,
-                              &swork, &U, &scal, &tvals, &Y, &Q, &R, &hv64, &loc, &gate, &jy, &jz, &om})
+                              &swork, &U, &scal, &tvals, &Y, &Q, &R, &hv64, &loc, &gate, &jy, &jz, &om, &stepdev})
             c += (int64_t)d->cap;
         return c;
     }
@@ -947,6 +1144,7 @@ static void workspace_plan(int64_t n, const SolveParams& p, F&& f) {
     f(&Workspace::hv64, mixed && p.refine ? nev * ld * 8 : 0);   // FP64 H·v of the Ritz vectors
     f(&Workspace::jy, ld * 8);     // J projection scratch (FP64), small next to the basis
     f(&Workspace::jz, ld * 8);
+    f(&Workspace::stepdev, (int64_t)sizeof(StepDev));   // captured steps: where the iteration stands
 }
 
 template <typename TBv>
@@ -1393,10 +1591,183 @@ struct Solver {
     }
 
     void run(double* evals, double* evecs, const double* v0_in = nullptr) {
+        if (graph_steps()) {
+            run_graph(evals, evecs, v0_in);
+            return;
+        }
         start(v0_in);
         do {
             launch_hv(H, prec_hv, hv_in(), hv_out(), s);
         } while (!advance());
+        end(evals, evecs);
+    }
+
+    // ---- captured steps: one Lanczos step (H·v, the step after it, the Ritz values
+    // and convergence test, the move to the next step) captured once as a hipGraph
+    // and replayed for every step; the kernels read the iteration from W.stepdev.
+    // The host keeps the lag-1 test and runs restarts eagerly, then resets the step
+    // state.  Every replay runs the kernels of the launched step with the same
+    // arguments, so the results are bit-identical to the launched steps.
+    hipGraph_t g_graph = nullptr;
+    hipGraphExec_t g_exec = nullptr;
+    ~Solver() {
+        if (g_exec) (void)hipGraphExecDestroy(g_exec);
+        if (g_graph) (void)hipGraphDestroy(g_graph);
+    }
+
+    bool graph_steps() const {
+        if (jp || reorth_cgs2 || step_ready) return false;
+        int mode = P.steps;
+        if (const char* e = std::getenv("SMGPU_STEPS")) {
+            const std::string v(e);
+            if (v == "graph") mode = SMGPU_STEPS_GRAPH;
+            else if (v == "launch") mode = SMGPU_STEPS_LAUNCH;
+        }
+        if (mode == SMGPU_STEPS_GRAPH) return true;
+        if (mode == SMGPU_STEPS_LAUNCH) return false;
+        return n <= GRAPH_AUTO_N;
+    }
+
+    StepDev* sdev() { return W.stepdev.as<StepDev>(); }
+
+    template <typename TW>
+    void chain_s(const TW* win) {
+        StepDev* sd = sdev();
+        TBv* B = V(0);
+        double* part = W.part.as<double>();
+        double* loc = W.loc.as<double>();
+        int* gate = W.gate.as<int>();
+        double* al = W.alpha.as<double>();
+        double* be = W.beta.as<double>();
+        double* w64 = W.w64.as<double>();
+        if (small) {
+            hipLaunchKernelGGL((k_step_small_s<TBv, TW>), dim3(1), dim3(TB), 0, s, sd, win, B, ld, n, w64, loc, gate, al,
+                               be, W.om.as<double>(), lm + 2, omega, W.h1.as<double>(), W.h2.as<double>());
+            return;
+        }
+        const int G2 = (int)std::min<int64_t>(1024, Gd);
+        hipLaunchKernelGGL((k_dot2_s<TBv, TW>), dim3(G2), dim3(TB), 0, s, sd, (const TBv*)B, ld, win, n, part, ldp);
+        hipLaunchKernelGGL(k_reduce, dim3(1), dim3(TB), 0, s, part, G2, ldp, loc, (const int*)nullptr);
+        const int* g1 = partial ? gate + 1 : nullptr;
+        if (partial) {
+            hipLaunchKernelGGL((k_local_step_s<TBv, TW>), dim3(Gx), dim3(TB), 0, s, sd, win, (const TBv*)B, ld,
+                               (const double*)loc, (const double*)be, w64, n, W.npart.as<double>(),
+                               W.npart2.as<double>());
+            hipLaunchKernelGGL(k_omega_s, dim3(1), dim3(TB), 0, s, sd, al, be, (const double*)W.npart.as<double>(),
+                               (int)Gx, W.om.as<double>(), lm + 2, loc, gate, omega);
+            hipLaunchKernelGGL((k_gs_dots_s<TBv, double, false>), dim3(Gx), dim3(TB), 0, s, sd, (const TBv*)B, ld,
+                               (const double*)w64, n, part, ldp, g1, (const double*)nullptr, (const double*)nullptr,
+                               (double*)nullptr, (double*)nullptr);
+        } else {
+            hipLaunchKernelGGL((k_gs_dots_s<TBv, TW, true>), dim3(Gx), dim3(TB), 0, s, sd, (const TBv*)B, ld, win, n,
+                               part, ldp, (const int*)nullptr, (const double*)loc, (const double*)be, w64,
+                               W.npart2.as<double>());
+        }
+        hipLaunchKernelGGL(k_reduce_s, dim3(lm), dim3(TB), 0, s, sd, part, (int)Gx, ldp, W.h1.as<double>(), g1);
+        hipLaunchKernelGGL((k_gs_upd_s<TBv, double, false>), dim3(Gx), dim3(TB), 0, s, sd, (const TBv*)B, ld,
+                           (const double*)W.h1.as<double>(), (const double*)w64, w64, n, part, ldp,
+                           W.npart.as<double>(), g1);
+        hipLaunchKernelGGL(k_dgks, dim3(1), dim3(TB), 0, s, (const double*)W.npart.as<double>(),
+                           (const double*)W.npart2.as<double>(), (int)Gx, loc, gate, g1);
+        hipLaunchKernelGGL((k_gs_dots_s<TBv, double, false>), dim3(Gx), dim3(TB), 0, s, sd, (const TBv*)B, ld,
+                           (const double*)w64, n, part, ldp, (const int*)gate, (const double*)nullptr,
+                           (const double*)nullptr, (double*)nullptr, (double*)nullptr);
+        hipLaunchKernelGGL(k_reduce_s, dim3(lm), dim3(TB), 0, s, sd, part, (int)Gx, ldp, W.h2.as<double>(),
+                           (const int*)gate);
+        hipLaunchKernelGGL((k_gs_upd_s<TBv, double, false>), dim3(Gx), dim3(TB), 0, s, sd, (const TBv*)B, ld,
+                           (const double*)W.h2.as<double>(), (const double*)w64, w64, n, part, ldp,
+                           W.npart.as<double>(), (const int*)gate);
+        hipLaunchKernelGGL(k_finish_local_s, dim3(1), dim3(TB), 0, s, sd, loc, (const double*)W.npart.as<double>(),
+                           (int)Gx, al, be);
+    }
+
+    // the Ritz values and convergence test of the step in sd->prev (on stream rs)
+    void ritz_prev(hipStream_t rs) {
+        hipLaunchKernelGGL(k_ritz_s, dim3(nev), dim3(TB), 0, rs, (const StepDev*)sdev(),
+                           (const double*)W.alpha.as<double>(), (const double*)W.beta.as<double>(),
+                           W.enring.as<double>(), nev, P.tol, W.flags.as<int>(), W.hflag, W.flags.as<int>() + 4);
+    }
+
+    // Graph of one step: the previous step's Ritz values and convergence test on a
+    // branch beside this step's H·v (as launched steps run them on the side stream),
+    // then the step after H·v, the store of v_{it+1}, and the move to the next step.
+    void capture_step() {
+        StepDev* sd = sdev();
+        SM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+            SM_HIP(hipEventRecord(W.evm[0], s));
+            SM_HIP(hipStreamWaitEvent(W.side, W.evm[0], 0));
+            ritz_prev(W.side);
+            SM_HIP(hipEventRecord(W.evm[1], W.side));
+            launch_hv_column(H, prec_hv, V(0), ld, &sd->it, W.wH.p, s);
+            if (prec_hv == 0) chain_s<double>(W.wH.as<double>());
+            else chain_s<float>(W.wH.as<float>());
+            if (!small)
+                hipLaunchKernelGGL((k_store_s<TBv>), dim3(gstore), dim3(TB), 0, s, sd, (const double*)W.w64.as<double>(),
+                                   (const double*)W.beta.as<double>(), V(0), ld, n);
+            SM_HIP(hipStreamWaitEvent(s, W.evm[1], 0));
+            hipLaunchKernelGGL(k_step_next, dim3(1), dim3(1), 0, s, sd, nev, lm);
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(s, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        SM_HIP(hipStreamEndCapture(s, &g_graph));
+        SM_HIP(hipGraphInstantiate(&g_exec, g_graph, nullptr, nullptr, 0));
+    }
+
+    void step_set(int it, int step, int nprev) {
+        hipLaunchKernelGGL(k_step_set, dim3(1), dim3(1), 0, s, sdev(), it, step, nprev, nev, lm);
+    }
+
+    // The graph of step k holds the convergence test of step k-1, so the host tests
+    // step k-2 after launching step k (one step further behind than launched steps,
+    // which test k-1 after enqueueing k): the device always has the next step
+    // queued.  At the end of a sweep or of the Krylov space the tests of the last two
+    // steps are taken there, with the same outcome as launched steps; a solve that
+    // converges runs one more speculative step, which only writes entries past the
+    // converged ones.
+    void run_graph(double* evals, double* evecs, const double* v0_in) {
+        start(v0_in);
+        W.host_sync_objects();
+        if (!g_exec) capture_step();
+        step_set(d_it, d_step, 0);
+        for (;;) {
+            const int it = d_it, step = d_step;
+            SM_HIP(hipGraphLaunch(g_exec, s));
+            st.n_hv++;
+            SM_HIP(hipEventRecord(W.ev[step & 3], s));          // the test of step - 1 is done
+            bool done = false, restarted = false;
+            if (step - 2 >= d_first && check(step - 2)) {
+                d_conv_step = step - 2; d_elit = it - 1; done = true;
+            }
+            const bool exhausted = it + 1 == n, sweep_end = it + 2 == lm;
+            if (!done && (exhausted || sweep_end)) {
+                ritz_prev(s);                                     // this step's test, now
+                SM_HIP(hipEventRecord(W.ev[(step + 1) & 3], s));
+                if (step - 1 >= d_first && check(step - 1)) {
+                    d_conv_step = step - 1; d_elit = it; done = true;
+                } else if (exhausted) {
+                    d_conv_step = step; d_elit = it + 1; done = true;
+                } else if (check(step)) {
+                    d_conv_step = step; d_elit = lm - 1; done = true;
+                } else {
+                    restart();
+                    st.n_restart++;
+                    d_k = ls;
+                    d_first = step + 1;
+                    d_after_restart = true;
+                    restarted = true;
+                    step_set(ls, step + 1, 0);
+                }
+            }
+            d_conv = done;
+            if (done) break;
+            d_it = restarted ? ls : it + 1;
+            d_step = step + 1;
+            begin_step();
+        }
         end(evals, evecs);
     }
 

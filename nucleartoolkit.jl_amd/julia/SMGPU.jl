@@ -26,7 +26,7 @@ struct SolveParams              # smgpu_solve_params
     max_iter::Int32
     seed::UInt64
     reorth::Int32                # REORTH_DEFAULT / _FULL / _PARTIAL
-    reserved_::Int32
+    steps::Int32                 # STEPS_AUTO / _LAUNCH / _GRAPH (captured Lanczos steps)
 end
 SolveParams(num_ev, lm, ls, num_history, tol, precision, refine, max_iter, seed) =
     SolveParams(num_ev, lm, ls, num_history, tol, precision, refine, max_iter, seed, Int32(0), Int32(0))

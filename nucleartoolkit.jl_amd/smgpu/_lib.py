@@ -42,7 +42,7 @@ class SolveParams(ctypes.Structure):
     _fields_ = [("num_ev", ctypes.c_int32), ("lm", ctypes.c_int32), ("ls", ctypes.c_int32),
                 ("num_history", ctypes.c_int32), ("tol", ctypes.c_double), ("precision", ctypes.c_int32),
                 ("refine", ctypes.c_int32), ("max_iter", ctypes.c_int32), ("seed", ctypes.c_uint64),
-                ("reorth", ctypes.c_int32), ("reserved_", ctypes.c_int32)]
+                ("reorth", ctypes.c_int32), ("steps", ctypes.c_int32)]
 
 
 class SolveStats(ctypes.Structure):

@@ -172,17 +172,20 @@ class Hamiltonian:
         return a, b
 
     def trl(self, num_ev, lm=300, ls=20, tol=1e-8, num_history=3, precision="fp64", refine=True, seed=1234,
-            want_vectors=False, max_iter=0, v0=None, j2=None, twoJ=None, reorth="default"):
+            want_vectors=False, max_iter=0, v0=None, j2=None, twoJ=None, reorth="default", steps="auto"):
         """Thick-restart Lanczos; returns (evals, evecs or None, stats dict).
         With j2 (a J^2 Hamiltonian on this basis) and twoJ: J-projected TRL
         (doubleLanczos), the lowest states with J = twoJ/2.  reorth: "default"
-        (full for fp64, partial for mixed), "full" or "partial"."""
+        (full for fp64, partial for mixed), "full" or "partial".  steps: "graph"
+        replays each Lanczos step as a captured hipGraph, "launch" launches its
+        kernels, "auto" leaves the choice to the library (results bit-identical)."""
         p = SolveParams()
         lib().smgpu_default_params(ctypes.byref(p))
         p.num_ev = int(num_ev); p.lm = int(lm); p.ls = int(ls); p.tol = float(tol)
         p.num_history = int(num_history); p.precision = _prec(precision); p.refine = int(bool(refine))
         p.seed = int(seed); p.max_iter = int(max_iter)
         p.reorth = {"default": 0, "full": 1, "partial": 2}[reorth]
+        p.steps = {"auto": 0, "launch": 1, "graph": 2}[steps]
         nev = min(int(num_ev), self.mdim)
         ev = np.zeros(nev, np.float64)
         vec = np.zeros((nev, self.mdim), np.float64) if want_vectors else None
