@@ -1,0 +1,137 @@
+// Serving loop under host AddressSanitizer: mixed bases and interactions, lockstep
+// batches with joins and variants, cancels from other threads, concurrent submitters,
+// shutdown with work queued.  Checks that equal requests give equal results.
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <mutex>
+#include <random>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "smgpu.h"
+
+struct Job { std::string path, nuc; int nev, prec, lm; uint64_t seed; };
+
+static int run_phase(const char* root, int inflight, int batch, int variants, int nsub, int per, uint64_t seed0) {
+    smgpu_server_config cfg;
+    smgpu_default_server_config(&cfg);
+    cfg.max_inflight = inflight;
+    cfg.max_batch = batch;
+    cfg.batch_variants = variants;
+    cfg.host_threads = 3;
+    smgpu_server srv = nullptr;
+    if (smgpu_server_create(&cfg, &srv) != SMGPU_OK) { printf("create failed: %s\n", smgpu_last_error()); return 1; }
+    std::vector<Job> menu;
+    const char* sd[] = {"sd_0", "sd_1", "sd_2", "sd_3"};
+    const char* sb[] = {"sdb_0", "sdb_1", "sdb_2"};
+    const char* pf[] = {"pf_0", "pf_1"};
+    for (auto f : sd) menu.push_back({std::string(root) + "/" + f + ".snt", "Ne20", 3, 1, 300, 0});
+    for (auto f : sb) menu.push_back({std::string(root) + "/" + f + ".snt", "Mg24", 4, 1, 300, 0});
+    for (auto f : pf) menu.push_back({std::string(root) + "/" + f + ".snt", "Ti45", 3, 1, 300, 0});
+    std::mutex mu;
+    std::vector<std::pair<int64_t, std::string>> ids;    // request id, key
+    std::vector<int64_t> to_cancel;
+    std::vector<std::thread> th;
+    for (int t = 0; t < nsub; t++)
+        th.emplace_back([&, t] {
+            std::mt19937_64 rng(seed0 + t);
+            for (int k = 0; k < per; k++) {
+                Job j = menu[(rng() % 2) ? 4 + rng() % 3 : rng() % menu.size()];
+                j.prec = (rng() % 4 == 0) ? SMGPU_FP64 : SMGPU_MIXED;
+                j.nev = 1 + (int)(rng() % 5);
+                j.lm = (rng() % 5 == 0) ? 40 : 300;
+                j.seed = 1 + rng() % 3;
+                smgpu_solve_params p;
+                smgpu_default_params(&p);
+                p.num_ev = j.nev; p.precision = j.prec; p.lm = j.lm; p.ls = 20; p.seed = j.seed;
+                int64_t id = -1;
+                if (smgpu_server_submit(srv, j.path.c_str(), j.nuc.c_str(), -1, &p, &id) != SMGPU_OK) {
+                    printf("submit failed: %s\n", smgpu_last_error());
+                    continue;
+                }
+                char key[512];
+                snprintf(key, sizeof key, "%s|%s|%d|%d|%d|%llu", j.path.c_str(), j.nuc.c_str(), j.nev, j.prec, j.lm,
+                         (unsigned long long)j.seed);
+                std::lock_guard<std::mutex> lk(mu);
+                ids.push_back({id, key});
+                if (rng() % 7 == 0) to_cancel.push_back(id);
+                if (rng() % 3 == 0) std::this_thread::sleep_for(std::chrono::microseconds(rng() % 3000));
+            }
+        });
+    std::atomic<bool> subs_done{false};
+    std::thread canceller([&] {
+        std::mt19937_64 rng(seed0 * 7 + 1);
+        for (int r = 0; r < 3000; r++) {
+            std::this_thread::sleep_for(std::chrono::microseconds(500 + rng() % 4000));
+            std::vector<int64_t> c;
+            { std::lock_guard<std::mutex> lk(mu); c.swap(to_cancel); }
+            for (int64_t id : c) smgpu_server_cancel(srv, id);
+            if (subs_done && r % 50 == 49) {
+                // cancel a random running or queued request now and then
+                std::lock_guard<std::mutex> lk(mu);
+                if (!ids.empty()) smgpu_server_cancel(srv, ids[rng() % ids.size()].first);
+            }
+            smgpu_server_stats st;
+            smgpu_server_get_stats(srv, &st);
+            if (subs_done && st.completed + st.failed + st.canceled >= st.submitted) break;
+        }
+    });
+    for (auto& x : th) x.join();
+    subs_done = true;
+    canceller.join();
+    std::map<std::string, std::vector<double>> first;
+    int done = 0, canceled = 0, failed = 0, mismatch = 0;
+    for (auto& [id, key] : ids) {
+        smgpu_server_wait(srv, id, 600000.0);
+        double ev[64];
+        smgpu_request_info info;
+        int32_t rc = smgpu_server_result(srv, id, ev, &info);
+        if (info.state == SMGPU_REQ_DONE) {
+            done++;
+            std::vector<double> v(ev, ev + info.num_ev);
+            for (double x : v) if (!std::isfinite(x)) mismatch++;
+            auto it = first.find(key);
+            if (it == first.end()) first[key] = v;
+            else
+                for (size_t i = 0; i < v.size() && i < it->second.size(); i++)
+                    if (std::fabs(v[i] - it->second[i]) > 1e-9) mismatch++;
+        } else if (info.state == SMGPU_REQ_CANCELED) canceled++;
+        else { failed++; printf("request %lld state %d rc %d: %s\n", (long long)id, info.state, rc, smgpu_last_error()); }
+    }
+    smgpu_server_stats st;
+    smgpu_server_get_stats(srv, &st);
+    printf("phase inflight %d batch %d variants %d: %zu requests, done %d canceled %d failed %d mismatch %d, "
+           "batched %lld peak_batch %d peak_running %d\n", inflight, batch, variants, ids.size(), done, canceled, failed,
+           mismatch, (long long)st.batched, st.peak_batch, st.peak_running);
+    fflush(stdout);
+    // shutdown with work still queued
+    for (int k = 0; k < 6; k++) {
+        smgpu_solve_params p;
+        smgpu_default_params(&p);
+        p.num_ev = 3; p.precision = SMGPU_MIXED;
+        int64_t id;
+        smgpu_server_submit(srv, menu[k % menu.size()].path.c_str(), menu[k % menu.size()].nuc.c_str(), -1, &p, &id);
+    }
+    smgpu_server_destroy(srv);
+    return (failed || mismatch) ? 1 : 0;
+}
+
+int main(int argc, char** argv) {
+    const char* root = argc > 1 ? argv[1] : "snt";
+    int bad = 0;
+    const int rounds = argc > 2 ? atoi(argv[2]) : 1;
+    for (int r = 0; r < rounds; r++) {
+        bad |= run_phase(root, 2, 3, 1, 3, 40, 11 + 100 * r);
+        bad |= run_phase(root, 4, 1, 1, 3, 40, 22 + 100 * r);
+        bad |= run_phase(root, 3, 2, 0, 4, 30, 33 + 100 * r);
+        bad |= run_phase(root, 2, 4, 1, 4, 30, 44 + 100 * r);
+        bad |= run_phase(root, 2, 2, 1, 2, 30, 55 + 100 * r);
+    }
+    printf(bad ? "driver: FAILED\n" : "driver: ok\n");
+    return bad;
+}
