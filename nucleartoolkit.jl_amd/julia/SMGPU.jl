@@ -218,9 +218,59 @@ function mdim_print(target_nuc, inf::ProblemInfo, tJ=-1)
 end
 
 # mdimmode=true prints the dimension line and returns nothing (:129-130)
+# M1 / E2 one-body operators of main_sm's calc_moment on a problem's basis
+# (prep_obtr_diag, src/ShellModel/transit.jl:82-245), on one GPU
+mutable struct Moments
+    h::Ptr{Cvoid}
+end
+function Moments(p::Problem; device::Integer=0, gfactors=[1.0, 0.0, 5.586, -3.826], effcharge=[1.5, 0.5],
+                 bpar::Real=0.0)
+    g = Vector{Float64}(gfactors)
+    e = Vector{Float64}(effcharge)
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:smgpu_moments_create, libsmgpu), Int32,
+                (Ptr{Cvoid}, Int32, Ptr{Float64}, Ptr{Float64}, Float64, Ref{Ptr{Cvoid}}),
+                p.h, device, g, e, Float64(bpar), out))
+    m = Moments(out[])
+    finalizer(x -> ccall((:smgpu_moments_destroy, libsmgpu), Int32, (Ptr{Cvoid},), x.h), m)
+    return m
+end
+
+# eval_moment (transit.jl:702-765): mu and Q of the columns of vecs (mdim x n) with J
+# values totJs; NaN where the reference prints nothing
+function eval_moment(m::Moments, vecs::Matrix{Float64}, totJs::Vector{Float64})
+    n = size(vecs, 2)
+    mu = zeros(Float64, n)
+    Q = zeros(Float64, n)
+    check(ccall((:smgpu_eval_moment, libsmgpu), Int32,
+                (Ptr{Cvoid}, Int32, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                m.h, n, vecs, totJs, mu, Q))
+    return mu, Q
+end
+
+# J from <J^2> (J_from_JJ1, shellmodel_main.jl:1105-1113): the J in 0, 1/2, ..., 50 with
+# |J(J+1) - jj| < tol, else -1
+function j_from_jj1(jj::Real; tol=1e-6)
+    for J2 in 0:100
+        hJ = 0.5 * J2
+        abs(hJ * (hJ + 1.0) - jj) < tol && return hJ
+    end
+    return -1.0
+end
+
+# <v|J^2|v> of the columns of vecs with the J^2 operator on the same basis
+function expect_j2(j2::Ham, vecs::Matrix{Float64})
+    n = size(vecs, 2)
+    w = similar(vecs)
+    check(ccall((:smgpu_apply_H_host_multi, libsmgpu), Int32,
+                (Ptr{Cvoid}, Int32, Int32, Ptr{Float64}, Ptr{Float64}), j2.h, FP64, n, vecs, w))
+    return [sum(view(vecs, :, k) .* view(w, :, k)) for k in 1:n]
+end
+
 function main_sm(sntf, target_nuc, num_ev, target_J; num_history=3, lm=300, ls=20, tol=1e-8,
                  precision::Int32=FP64, device::Integer=0, q::Integer=1, is_block::Bool=false,
-                 mdimmode::Bool=false)
+                 mdimmode::Bool=false, calc_moment::Bool=false, gfactors=[1.0, 0.0, 5.586, -3.826],
+                 effcharge=[1.5, 0.5])
     length(target_J) <= 1 || error("Multiple J is not supported")
     tJ = isempty(target_J) ? -1 : Int(target_J[1])
     prob = Problem(sntf, target_nuc; twoM=tJ)
@@ -236,11 +286,25 @@ function main_sm(sntf, target_nuc, num_ev, target_J; num_history=3, lm=300, ls=2
     j2 = tJ < 0 ? nothing : HamJ2(prob; device=device)
     nev = min(num_ev, mdim)
     if is_block
-        en, _, _ = trbl(H, nev, q; lm=lm, ls=ls, tol=tol, num_history=num_history, precision=precision,
-                        j2=j2, twoJ=max(tJ, 0))
+        en, vecs, _ = trbl(H, nev, q; lm=lm, ls=ls, tol=tol, num_history=num_history, precision=precision,
+                           j2=j2, twoJ=max(tJ, 0), vectors=calc_moment)
     else
-        en, _, _ = trl(H, nev; lm=lm, ls=ls, tol=tol, num_history=num_history, precision=precision,
-                       j2=j2, twoJ=max(tJ, 0))
+        en, vecs, _ = trl(H, nev; lm=lm, ls=ls, tol=tol, num_history=num_history, precision=precision,
+                          j2=j2, twoJ=max(tJ, 0), vectors=calc_moment)
+    end
+    if calc_moment
+        # main_sm's moments (shellmodel_main.jl:242-246, 267-269)
+        jop = j2 === nothing ? HamJ2(prob; device=device) : j2
+        totJs = [j_from_jj1(x) for x in expect_j2(jop, vecs)]
+        mu, Q = eval_moment(Moments(prob; device=device, gfactors=gfactors, effcharge=effcharge), vecs, totJs)
+        tx = ""
+        for k in eachindex(mu)
+            t = ""
+            isnan(mu[k]) || (t *= "mu (n=$k) " * @sprintf("%12.5f ", mu[k]) * "\t")
+            isnan(Q[k]) || (t *= "Q (n=$k) " * @sprintf("%12.5f ", Q[k]))
+            t == "" || (tx *= t * "\n")
+        end
+        tx == "" || println(tx)
     end
     return en
 end
