@@ -12,8 +12,9 @@ tol 1e-8).
 
   served leg : the serving loop (smgpu_server_*) with FP32 H·v and FP64
                recurrence / small eigensolves / Rayleigh-quotient refinement,
-               requests submitted at their arrival times, `--inflight` solves
-               overlapping on the device.  `value` = requests whose every
+               requests submitted at their arrival times, `--inflight` slots
+               overlapping on the device, each running up to `--batch` requests
+               on one basis in lockstep.  `value` = requests whose every
                eigenvalue agrees with the FP64 leg within 1e-5 MeV, divided by
                the time from the first arrival to the last result.
   FP64 leg   : the same requests, same arrival times, through a one-slot server
@@ -369,20 +370,40 @@ def run_stream(args):
             budget += pr[i]["solve_ms"] * 1e-3
         seq = Server(max_inflight=1, host_threads=args.host_threads, max_batch=1)
         warmup(seq, "fp64")
+        warmup(seq, "mixed")
         live = []
+
+        def solo(r, **kw):
+            rid = seq.submit(r.snt, r.nucleus, r.num_ev, seed=r.seed % (1 << 31), **kw)
+            seq.wait(rid)
+            return seq.result(rid, raise_on_error=False)
+
         for i in sorted(sample):
             r = reqs[i]
-            rid = seq.submit(r.snt, r.nucleus, r.num_ev, precision="fp64", seed=r.seed % (1 << 31))
-            seq.wait(rid)
-            ev, info = seq.result(rid, raise_on_error=False)
+            ev, info = solo(r, precision="fp64")
+            # the same request one at a time as FP64 with partial reorthogonalisation and as
+            # mixed precision: the speed-up of the served mixed form split into its parts
+            evp, infp = solo(r, precision="fp64", reorth="partial")
+            evm, infm = solo(r, precision="mixed")
+            dev = (lambda a: float(np.abs(a - ev64[i]).max()) if len(a) == len(ev64[i]) else None)
             live.append({"i": i, "nucleus": r.nucleus, "solve_ms": info["solve_ms"],
-                         "recorded_ms": pr[i]["solve_ms"],
-                         "max_abs_dev_MeV": float(np.abs(ev - ev64[i]).max()) if len(ev) == len(ev64[i]) else None})
+                         "recorded_ms": pr[i]["solve_ms"], "max_abs_dev_MeV": dev(ev),
+                         "fp64_partial_ms": infp["solve_ms"], "fp64_partial_dev_MeV": dev(evp),
+                         "mixed_ms": infm["solve_ms"], "mixed_dev_MeV": dev(evm)})
         seq.close()
         ratio = sum(q["solve_ms"] for q in live) / max(1e-9, sum(q["recorded_ms"] for q in live))
+        t_full = sum(q["solve_ms"] for q in live)
+        split = {"fp64_full_s": t_full * 1e-3,
+                 "fp64_partial_s": sum(q["fp64_partial_ms"] for q in live) * 1e-3,
+                 "mixed_one_at_a_time_s": sum(q["mixed_ms"] for q in live) * 1e-3,
+                 "note": "the sampled requests solved one at a time: full FP64 TRL (the baseline), FP64 with "
+                         "partial reorthogonalisation (the algorithm change alone), mixed precision with partial "
+                         "reorthogonalisation (algorithm and precision); the rest of the served speed-up is "
+                         "concurrency and batching"}
         fp64_src = {"source": f"recorded sequential FP64 run ({os.path.relpath(args.record, ROOT)}, {rec['date']}, "
                               f"same requests and arrival times)",
                     "source_sha16_recorded": rec["source_sha16"], "source_sha16_now": source_hash(),
+                    "speedup_split": split,
                     "spot_check": {"requests": len(live), "live_over_recorded_time": ratio,
                                    "max_abs_dev_MeV": max((q["max_abs_dev_MeV"] or 0.0) for q in live) if live
                                    else None, "solves": live}}
@@ -605,7 +626,7 @@ def main():
                     help="stream: fp64 runs only the sequential FP64 leg and writes --record")
     ap.add_argument("--record", default=os.path.join(ROOT, "profiles", "stream512_fp64_leg.json"),
                     help="stream: the recorded sequential FP64 leg (read by the served leg, written by --leg fp64)")
-    ap.add_argument("--spot-s", type=float, default=60.0,
+    ap.add_argument("--spot-s", type=float, default=40.0,
                     help="stream: recorded FP64 solve time re-run live as a spot check")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -615,7 +636,9 @@ def main():
     if args.inflight is None:
         args.inflight = 4 if args.workload == "stream" else 2
     if args.batch is None:
-        args.batch = 1 if args.workload == "stream" else 2
+        # stream: 4 slots of lockstep batches of two (measured 2.36 solves/s against 2.22 with
+        # single solves, 2.30 with batches of three, 2.34 with 3 slots of two)
+        args.batch = 2
     if args.mode == "hv" or args.workload == "served":
         run_served(args)
     elif args.workload == "variants":

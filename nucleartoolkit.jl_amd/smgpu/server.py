@@ -20,12 +20,13 @@ from .shellmodel import _prec
 
 
 def solve_params(num_ev, lm=300, ls=20, tol=1e-8, num_history=3, precision="mixed", refine=True, seed=1234,
-                 max_iter=0):
+                 max_iter=0, reorth="default"):
     p = SolveParams()
     lib().smgpu_default_params(ctypes.byref(p))
     p.num_ev = int(num_ev); p.lm = int(lm); p.ls = int(ls); p.tol = float(tol)
     p.num_history = int(num_history); p.precision = _prec(precision); p.refine = int(bool(refine))
     p.seed = int(seed); p.max_iter = int(max_iter)
+    p.reorth = {"default": 0, "full": 1, "partial": 2}[reorth]
     return p
 
 
