@@ -337,6 +337,11 @@ def run_stream(args):
     sst = srv.stats()
     srv.close()
     print(f"# served leg: {t_all:.1f} s", file=sys.stderr, flush=True)
+    if args.timeline:
+        with open(args.timeline, "w") as f:
+            json.dump([dict(i=k, nucleus=r.nucleus, space=r.space, zv=r.zv, nv=r.nv, k=r.num_ev,
+                            arrival_s=r.arrival_s, ev=[float(x) for x in ev], **info)
+                       for k, (r, (ev, info)) in enumerate(zip(reqs, res))], f)
     # the sequential FP64 leg: the recorded full run of this stream (its eigenvalues check
     # every served result), and a live re-run of a sample of its requests
     rec = json.load(open(args.record)) if os.path.exists(args.record) else None
@@ -626,6 +631,8 @@ def main():
                     help="stream: fp64 runs only the sequential FP64 leg and writes --record")
     ap.add_argument("--record", default=os.path.join(ROOT, "profiles", "stream512_fp64_leg.json"),
                     help="stream: the recorded sequential FP64 leg (read by the served leg, written by --leg fp64)")
+    ap.add_argument("--timeline", default=None,
+                    help="stream: write every served request's record (slot, times, counts) to this JSON file")
     ap.add_argument("--spot-s", type=float, default=40.0,
                     help="stream: recorded FP64 solve time re-run live as a spot check")
     args = ap.parse_args()

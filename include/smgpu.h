@@ -48,9 +48,9 @@ typedef struct {
 } smgpu_problem_info;
 
 typedef struct {
-    int32_t num_ev;      /* lowest eigenpairs wanted (main_sm num_ev) */
-    int32_t lm;          /* Krylov vectors kept (main_sm lm, default 300) */
-    int32_t ls;          /* vectors kept at a thick restart (main_sm ls, default 20) */
+    int32_t num_ev;      /* lowest eigenpairs wanted (main_sm num_ev), 1..64 */
+    int32_t lm;          /* Krylov vectors kept (main_sm lm, default 300), up to 1000 */
+    int32_t ls;          /* vectors kept at a thick restart (main_sm ls, default 20), 1..64, < lm - 1 */
     int32_t num_history; /* Ritz-value history (main_sm num_history, default 3) */
     double tol;          /* convergence: all(en_prev - en_new < tol) (default 1e-8) */
     int32_t precision;   /* SMGPU_FP64 or SMGPU_MIXED */

@@ -934,7 +934,9 @@ __device__ __forceinline__ void ritz_body(const double* __restrict__ alpha, cons
     if (!is_last || threadIdx.x != 0) return;
     __threadfence();
     for (int k = nblk; k < num_ev; k++) out[k] = prev[k];
-    int okc = 1;
+    // converged once all num_ev values stagnate; while T holds fewer than num_ev
+    // (num_ev above ls, right after a restart) the stale ones do not count as converged
+    int okc = nblk >= num_ev ? 1 : 0;
     for (int k = 0; k < num_ev; k++) okc &= (prev[k] - out[k] < tol) ? 1 : 0;
     *flag = okc;
     *hflag = okc;
@@ -1049,6 +1051,21 @@ __global__ __launch_bounds__(TB) void k_rotate(const TBv* __restrict__ B, int64_
     }
 }
 
+// U[k] = sum_j S[k*m + j] B[j] for k < nk (any nk): k_rotate over chunks of outputs whose
+// coefficients fit 64 KiB of LDS and 32 accumulators; each output keeps the summation
+// order of one pass, so the result does not depend on the chunking.
+template <typename TBv, typename TU>
+static void rotate_chunks(const TBv* B, int64_t ldb, int m, const double* S, int nk, TU* U, int64_t ldu, int64_t n,
+                          hipStream_t st) {
+    const int kc = std::max(1, std::min(32, (64 * 1024 / 8) / std::max(1, m)));
+    const int gb = (int)std::min<int64_t>(2048, (n + TB - 1) / TB);
+    for (int k0 = 0; k0 < nk; k0 += kc) {
+        const int kn = std::min(kc, nk - k0);
+        hipLaunchKernelGGL((k_rotate<TBv, TU>), dim3(gb), dim3(TB), (size_t)kn * m * 8, st, B, ldb, m,
+                           S + (int64_t)k0 * m, kn, U + (int64_t)k0 * ldu, ldu, n);
+    }
+}
+
 // restart small problem on a stream: Ritz values, Ritz vectors, rotation to tridiagonal form
 static void restart_small_on_stream(double* alpha, double* beta, int m, int ls, double* S, double* vals,
                                     double* Y, double* Qrev, double* work, hipStream_t st) {
@@ -1108,12 +1125,21 @@ struct Workspace {
     }
 };
 
+// Vectors kept at a thick restart: ls, raised to num_ev when ls is below it.  The
+// reference keeps ls (lanczos_methods.jl:260-307), and with num_ev above ls the
+// states past ls are lost at every restart: the Ritz values then never all settle
+// (or, with the stale-value convergence test of diagonalize_T!, the final Ritz
+// vectors are asked of a T smaller than num_ev).
+static int64_t kept_ls(const SolveParams& p, int64_t n) {
+    return std::max<int64_t>(p.ls, std::min<int64_t>(p.num_ev, n));
+}
+
 // Every device buffer of a solve over mdim amplitudes with parameters p, in the
 // working precision of TBv: f(member, bytes).  setup() allocates exactly these.
 template <typename TBv, typename F>
 static void workspace_plan(int64_t n, const SolveParams& p, F&& f) {
     const int64_t ld = (n + 63) / 64 * 64;
-    const int64_t lm = p.lm, ls = p.ls, nev = std::max<int64_t>(1, std::min<int64_t>(p.num_ev, n));
+    const int64_t lm = p.lm, ls = kept_ls(p, n), nev = std::max<int64_t>(1, std::min<int64_t>(p.num_ev, n));
     const int64_t ldp = lm + 2;
     const int64_t Gd = (n + CHUNK - 1) / CHUNK, Gu = (n + UCHUNK - 1) / UCHUNK;
     const int64_t Gx = (n + GsShape<TBv>::SEG - 1) / GsShape<TBv>::SEG;
@@ -1133,13 +1159,14 @@ static void workspace_plan(int64_t n, const SolveParams& p, F&& f) {
     f(&Workspace::beta, (lm + 1) * 8);
     f(&Workspace::enring, 4 * 64 * 8);
     f(&Workspace::flags, 8 * (int64_t)sizeof(int));   // convergence flags [0,4), k_ritz's ticket [4]
-    f(&Workspace::S, lm * 32 * 8);
-    f(&Workspace::swork, (32 * 5 * lm + 4096) * 8);
+    const int64_t lsw = std::max<int64_t>(32, std::max(ls, nev));   // restart and Ritz small problems
+    f(&Workspace::S, lm * lsw * 8);
+    f(&Workspace::swork, (lsw * 5 * lm + 4096) * 8);
     f(&Workspace::U, ls * ld * (int64_t)sizeof(TBv));
     f(&Workspace::scal, 64 * 8);
     f(&Workspace::tvals, 64 * 8);
-    f(&Workspace::Y, 32 * lm * 8);
-    f(&Workspace::Q, 32 * 32 * 8);
+    f(&Workspace::Y, lsw * lm * 8);
+    f(&Workspace::Q, lsw * lsw * 8);
     f(&Workspace::R, nev * ld * 8);
     f(&Workspace::hv64, mixed && p.refine ? nev * ld * 8 : 0);   // FP64 H·v of the Ritz vectors
     f(&Workspace::jy, ld * 8);     // J projection scratch (FP64), small next to the basis
@@ -1205,7 +1232,7 @@ struct Solver {
         n = H.mdim;
         ld = (n + 63) / 64 * 64;
         lm = P.lm;
-        ls = P.ls;
+        ls = (int)kept_ls(P, n);
         nev = (int)std::min<int64_t>(P.num_ev, n);
         prec_hv = sizeof(TBv) == 4 ? 1 : 0;
         ldp = lm + 2;
@@ -1490,8 +1517,8 @@ struct Solver {
 
     void start(const double* v0_in) {
         d_t0 = std::chrono::steady_clock::now();
-        if (lm < 3 || ls < 1 || ls > 32 || ls >= lm - 1 || nev < 1 || nev > ls)
-            throw std::invalid_argument("need 1 <= num_ev <= ls <= 32 and ls < lm - 1");
+        if (lm < 3 || ls < 1 || ls > 64 || ls >= lm - 1 || nev < 1 || nev > 64)
+            throw std::invalid_argument("need 1 <= num_ev <= 64, 1 <= max(ls, num_ev) <= 64 and max(ls, num_ev) < lm - 1");
         if (lm > 1000) throw std::invalid_argument("lm above 1000 is not supported");
         setup(v0_in);
         if (jp) jfilter(0, 0, true);
@@ -1847,11 +1874,7 @@ struct Solver {
         if (std::getenv("SMGPU_DEBUG_RESTART")) debug_sweep();
         restart_small_on_stream(W.alpha.as<double>(), W.beta.as<double>(), m, ls, W.S.as<double>(),
                                 W.tvals.as<double>(), W.Y.as<double>(), W.Q.as<double>(), W.swork.as<double>(), s);
-        int gb = (int)std::min<int64_t>(2048, (n + TB - 1) / TB);
-        size_t sh = (size_t)ls * m * 8;
-        if (sh > 64 * 1024) throw std::invalid_argument("ls*(lm-1) too large for the restart rotation");
-        hipLaunchKernelGGL((k_rotate<TBv, TBv>), dim3(gb), dim3(TB), sh, s, (const TBv*)V(0), ld, m,
-                           W.S.as<double>(), ls, W.U.as<TBv>(), ld, n);
+        rotate_chunks<TBv, TBv>((const TBv*)V(0), ld, m, W.S.as<double>(), ls, W.U.as<TBv>(), ld, n, s);
         // v_ls = v_{lm-1}; kept vectors from U
         SM_HIP(hipMemcpyAsync(V(ls), V(lm - 1), n * sizeof(TBv), hipMemcpyDeviceToDevice, s));
         SM_HIP(hipMemcpyAsync(V(0), W.U.p, (size_t)ls * ld * sizeof(TBv), hipMemcpyDeviceToDevice, s));
@@ -1976,11 +1999,7 @@ struct Solver {
             SM_HIP(hipStreamSynchronize(s));
         }
         DBuf& R = W.R;
-        int gb = (int)std::min<int64_t>(2048, (n + TB - 1) / TB);
-        size_t sh = (size_t)nev * elit * 8;
-        if (sh > 64 * 1024) throw std::invalid_argument("num_ev*iterations too large for the Ritz assembly");
-        hipLaunchKernelGGL((k_rotate<TBv, double>), dim3(gb), dim3(TB), sh, s, (const TBv*)V(0), ld, elit,
-                           (const double*)Y, nev, R.as<double>(), ld, n);
+        rotate_chunks<TBv, double>((const TBv*)V(0), ld, elit, (const double*)Y, nev, R.as<double>(), ld, n, s);
         const bool refine = prec_hv == 1 && P.refine;
         DBuf& hv64 = W.hv64;
         double* sc = W.scal.as<double>();
@@ -2076,9 +2095,10 @@ struct BlockSolver {
         n = H.mdim;
         ld = (n + 63) / 64 * 64;
         nbl = P.lm / q;
-        lsub = P.ls / q;
-        lsv = q * lsub;
         nev = (int)std::min<int64_t>(P.num_ev, n);
+        lsub = P.ls / q;
+        if (q * lsub < nev) lsub = (nev + q - 1) / q;     // keep at least num_ev (as kept_ls)
+        lsv = q * lsub;
         lmT = q * nbl;
         ldp = lmT + 2;
         prec_hv = sizeof(TBv) == 4 ? 1 : 0;
@@ -2093,11 +2113,10 @@ struct BlockSolver {
 
     void check_args() {
         if (q < 1 || q > 16) throw std::invalid_argument("block size q must be in 1..16");
-        if (lsub < 1 || lsv > 32) throw std::invalid_argument("need q <= ls and q*(ls/q) <= 32");
-        if (nev < 1 || nev > lsv) throw std::invalid_argument("need 1 <= num_ev <= q*(ls/q)");
+        if (lsub < 1 || lsv > 64) throw std::invalid_argument("need q <= ls and q*(ls/q) <= 64");
+        if (nev < 1 || nev > 64) throw std::invalid_argument("need 1 <= num_ev <= 64");
         if (nbl < lsub + 3) throw std::invalid_argument("need lm/q >= ls/q + 3");
         if (lmT > 1000) throw std::invalid_argument("lm above 1000 is not supported");
-        if ((size_t)lsv * lmT * 8 > 64 * 1024) throw std::invalid_argument("ls*lm too large for the restart rotation");
         if (n < q) throw std::invalid_argument("block size larger than the basis");
     }
 
@@ -2111,7 +2130,7 @@ struct BlockSolver {
         h2.alloc((size_t)q * ldp * 8);
         rd.alloc(64 * 8);
         U.alloc((size_t)lsv * ld * sizeof(TBv));
-        Sd.alloc((size_t)lmT * 32 * 8);
+        Sd.alloc((size_t)lmT * std::max(32, std::max(lsv, nev)) * 8);
         R.alloc((size_t)nev * ld * 8);
         if (prec_hv == 1 && P.refine) hv64.alloc((size_t)nev * ld * 8);
         if (jp) jz.alloc((size_t)ld * 8);
@@ -2205,7 +2224,7 @@ struct BlockSolver {
         cur = prev;
         // band reduction while T is banded; the previous values bound these from above
         eig.lowest_values(T.data(), lmT, k, m, cur.data(), prev.data());
-        bool ok = true;
+        bool ok = k >= nev;     // (stale values past k do not count as converged)
         for (int i = 0; i < nev; i++) ok = ok && (prev[i] - cur[i] < P.tol);
         prev = cur;
         return ok;
@@ -2250,9 +2269,7 @@ struct BlockSolver {
             for (int k = 0; k < lsv; k++) t(lsv + i, k) = t(k, lsv + i) = r[(size_t)i * lsv + k];
         // kept Ritz vectors U = V[0..m-1] Y, then the new block moves behind them
         SM_HIP(hipMemcpyAsync(Sd.p, Y.data(), Y.size() * 8, hipMemcpyHostToDevice, s));
-        const int gb = (int)std::min<int64_t>(2048, (n + TB - 1) / TB);
-        hipLaunchKernelGGL((k_rotate<TBv, TBv>), dim3(gb), dim3(TB), (size_t)lsv * m * 8, s, (const TBv*)V(0), ld, m,
-                           (const double*)Sd.as<double>(), lsv, U.as<TBv>(), ld, n);
+        rotate_chunks<TBv, TBv>((const TBv*)V(0), ld, m, (const double*)Sd.as<double>(), lsv, U.as<TBv>(), ld, n, s);
         SM_HIP(hipMemcpyAsync(V(lsv), V(m), (size_t)q * ld * sizeof(TBv), hipMemcpyDeviceToDevice, s));
         SM_HIP(hipMemcpyAsync(V(0), U.p, (size_t)lsv * ld * sizeof(TBv), hipMemcpyDeviceToDevice, s));
         // blocks 1..ls/q re-orthonormalised against the earlier ones (bl_ThickRestart's final sweep)
@@ -2362,14 +2379,13 @@ struct BlockSolver {
     void finish(int k, double* evals, double* evecs) {
         const bool refine = prec_hv == 1 && P.refine;
         if (!evecs && !refine) return;
-        if ((size_t)nev * k * 8 > 64 * 1024) throw std::invalid_argument("num_ev*basis too large for the Ritz assembly");
+        if (k < nev) throw std::invalid_argument("the Krylov space holds fewer states than num_ev");
         eig.reduce(T.data(), lmT, k);
         std::vector<double> vals(nev), Y((size_t)nev * k);
         eig.lowest_pairs(nev, vals.data(), Y.data());
         SM_HIP(hipMemcpyAsync(Sd.p, Y.data(), Y.size() * 8, hipMemcpyHostToDevice, s));
-        const int gb = (int)std::min<int64_t>(2048, (n + TB - 1) / TB);
-        hipLaunchKernelGGL((k_rotate<TBv, double>), dim3(gb), dim3(TB), (size_t)nev * k * 8, s, (const TBv*)V(0), ld,
-                           k, (const double*)Sd.as<double>(), nev, R.as<double>(), ld, n);
+        rotate_chunks<TBv, double>((const TBv*)V(0), ld, k, (const double*)Sd.as<double>(), nev, R.as<double>(), ld, n,
+                                   s);
         double* r = rd.as<double>();
         for (int i = 0; i < nev; i++) {
             double* x = R.as<double>() + (int64_t)i * ld;

@@ -8,8 +8,13 @@
   mdim >= 2) and a lockstep batch.
 * He6 at 2J = 0 (a J space of two states): both levels, and an error when
   three are asked for.
-* A block larger than the basis, num_ev above 64, ls >= lm - 1 and lm above
-  1000 fail with an error; the handle stays usable afterwards.
+* A block larger than the basis, num_ev above 64, ls above 64, ls >= lm - 1 and
+  lm above 1000 fail with an error; the handle stays usable afterwards.
+* Restart and Ritz sizes past one rotation pass (lanczos_methods.jl:250-307 has
+  no limit on ls or num_ev): on a seeded sd 2p2n interaction (Ne20, mdim 640),
+  TRL with ls 40 and num_ev 36, TRL with num_ev 12 above ls 8 (a restart then
+  keeps 12 vectors), and block TRL (q = 2) with ls 40, and with num_ev 12 above
+  ls 6, all give the dense levels (1e-7 MeV FP64, 1e-5 MeV mixed).
 """
 import numpy as np
 import pytest
@@ -55,7 +60,7 @@ def test_argument_limits(golden):
     with pytest.raises(smgpu.SmgpuError):
         smgpu.main_sm(path, "He4", 1, [], is_block=True, q=2)          # q > mdim
     h = smgpu.Hamiltonian(smgpu.Problem(path, "Be8"))
-    for kw in (dict(num_ev=65), dict(num_ev=5, lm=20, ls=19), dict(num_ev=5, lm=1001)):
+    for kw in (dict(num_ev=65), dict(num_ev=5, lm=20, ls=19), dict(num_ev=5, lm=1001), dict(num_ev=5, lm=100, ls=65)):
         with pytest.raises(smgpu.SmgpuError):
             h.trl(**kw)
     ev, _, _ = h.trl(5)
@@ -76,3 +81,32 @@ def test_tiny_j_space(golden, precision):
     assert list(J) == [0.0, 0.0]
     with pytest.raises(smgpu.SmgpuError):
         smgpu.main_sm(path, "He6", 3, target_J=[0], precision=precision, ls=10, lm=60)
+
+
+@pytest.fixture(scope="module")
+def ne20_dense(tmp_path_factory):
+    from smgpu import synth
+    path, nuc = synth.config_snt(str(tmp_path_factory.mktemp("ne20") / "ne20.snt"), "sd", 2, 2, 4242, (-6, 0), 1.2)
+    H, _ = bf.brute_hamiltonian(path, nuc, 10, 10)
+    assert H.shape == (640, 640)
+    return path, nuc, np.linalg.eigvalsh(H)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_restart_and_ritz_beyond_one_pass(ne20_dense, precision):
+    path, nuc, ex = ne20_dense
+    h = smgpu.Hamiltonian(smgpu.Problem(path, nuc))
+    print("case ls40 nev36", precision, flush=True)
+    ev, _, st = h.trl(36, lm=100, ls=40, precision=precision, max_iter=5000)
+    assert st["n_restart"] >= 1, st
+    np.testing.assert_allclose(ev, ex[:36], atol=TOL[precision])
+    print("case nev12 > ls8", precision, flush=True)
+    ev, _, st = h.trl(12, lm=80, ls=8, precision=precision, max_iter=5000)
+    assert st["n_restart"] >= 1, st
+    np.testing.assert_allclose(ev, ex[:12], atol=TOL[precision])
+    print("case block q2 ls40", precision, flush=True)
+    eb = smgpu.main_sm(path, nuc, 20, [], precision=precision, is_block=True, q=2, lm=100, ls=40)
+    np.testing.assert_allclose(eb, ex[:20], atol=TOL[precision])
+    print("case block q2 nev12 > ls6", precision, flush=True)
+    eb = smgpu.main_sm(path, nuc, 12, [], precision=precision, is_block=True, q=2, lm=80, ls=6)
+    np.testing.assert_allclose(eb, ex[:12], atol=TOL[precision])
