@@ -273,11 +273,18 @@ def _serve(reqs, server, precision):
         if dt > 0:
             time.sleep(dt)
         ids.append(server.submit(r.snt, r.nucleus, r.num_ev, precision=precision, seed=r.seed % (1 << 31)))
-    for i, rid in enumerate(ids):
-        server.wait(rid)
-        if (i + 1) % 64 == 0:
-            print(f"# {precision}: request {i + 1}/{len(ids)} done at {time.perf_counter() - t0:.1f} s",
-                  file=sys.stderr, flush=True)
+    # progress in completion order (requests finish out of submission order)
+    last = 0
+    for rid in ids:
+        while not server.wait(rid, 10000.0):
+            done = server.stats()["completed"]
+            print(f"# {precision}: {done}/{len(ids)} done at {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+                  flush=True)
+        done = server.stats()["completed"]
+        if done - last >= 64:
+            last = done
+            print(f"# {precision}: {done}/{len(ids)} done at {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+                  flush=True)
     t_all = time.perf_counter() - t0
     res = [server.result(rid, raise_on_error=False) for rid in ids]
     return res, t_all
@@ -331,7 +338,8 @@ def run_stream(args):
         print(json.dumps({"leg": "fp64", "record": args.record, "wall_s": t_seq, "done": done,
                           "solves_per_s": done / t_seq, "source_sha16": rec["source_sha16"]}), flush=True)
         return
-    srv = Server(max_inflight=args.inflight, host_threads=args.host_threads, max_batch=args.batch)
+    srv = Server(max_inflight=args.inflight, host_threads=args.host_threads, max_batch=args.batch,
+                 large_mdim=args.large_mdim, reserve_cus=args.reserve_cus)
     warmup(srv, "mixed")
     res, t_all = _serve(reqs, srv, "mixed")
     sst = srv.stats()
@@ -468,7 +476,8 @@ def run_stream(args):
                         f"N(0,1.2)); mdim {min(mdims)}..{max(mdims)}; served mixed precision in {args.inflight} "
                         f"slots (batches of up to {args.batch}) vs the plain FP64 TRL one solve at a time"),
            "requests": len(reqs), "rate_per_s": args.rate, "particles": [lo, hi], "inflight": args.inflight,
-           "batch": args.batch, "lm": 300, "ls": 20, "tol": 1e-8, "steps_note": "one step = one request; the "
+           "batch": args.batch, "large_mdim": args.large_mdim, "reserve_cus": args.reserve_cus,
+           "lm": 300, "ls": 20, "tol": 1e-8, "steps_note": "one step = one request; the "
            "stream fixes its own size (--steps is not used by this workload)"},
           {"latency_ms": {"p50": percentile(lat, 50), "p99": percentile(lat, 99), "note": "arrival to result"},
            "fp64": {"solves_per_s": fp64_rate, "wall_s": t_seq, "latency_ms_p50": percentile(lat64, 50),
@@ -631,6 +640,11 @@ def main():
                     help="stream: fp64 runs only the sequential FP64 leg and writes --record")
     ap.add_argument("--record", default=os.path.join(ROOT, "profiles", "stream512_fp64_leg.json"),
                     help="stream: the recorded sequential FP64 leg (read by the served leg, written by --leg fp64)")
+    ap.add_argument("--large-mdim", type=int, default=0,
+                    help="stream: served leg runs requests of at least this dimension one at a time, the rest "
+                         "smallest first (0: arrival order)")
+    ap.add_argument("--reserve-cus", type=int, default=0,
+                    help="stream, with --large-mdim: the large requests run in slot 0 on all CUs but this many")
     ap.add_argument("--timeline", default=None,
                     help="stream: write every served request's record (slot, times, counts) to this JSON file")
     ap.add_argument("--spot-s", type=float, default=40.0,

@@ -322,6 +322,15 @@ typedef struct {
                                its basis (each member with its own coefficients, H·v launches
                                sharing the index loads); 0: identical Hamiltonians only.
                                smgpu_default_server_config sets 1 */
+    int64_t large_mdim;     /* > 0: requests of at least this basis dimension run one solve
+                               (or batch) at a time, oldest first; the others run smallest
+                               basis first (oldest first once they have waited 60 s), and no
+                               request holds up another.  0: arrival order, a request that
+                               does not fit yet bypassed by at most 4 * max_inflight others */
+    int32_t reserve_cus;    /* with large_mdim and max_inflight >= 2: slot 0 runs the large
+                               requests only, on streams that leave this many CUs (at most
+                               half the device) to the other slots; 0: no CU partition */
+    int32_t reserved_;      /* zero */
 } smgpu_server_config;
 
 typedef struct {

@@ -138,6 +138,9 @@ struct Slot {
 
 struct Server {
     smgpu_server_config cfg{};
+    int64_t large_mdim = 0;   // > 0: large-lane admission order (schedule_priority_locked)
+    int reserve_cus = 0;      // > 0: slot 0 is the large lane, on CUs reserve_cus..
+    bool lane_split = false;  // slot 0 takes the large requests only, the other slots the rest
     Clock::time_point t0 = Clock::now();
     std::mutex m;
     std::condition_variable done_cv, prep_cv;
@@ -378,6 +381,7 @@ struct Server {
             int best_n = 1;
             for (auto& s : slots) {
                 if (s->job) continue;
+                if (lane_split && (r->mdim >= large_mdim) != (s->id == 0)) continue;
                 // the largest batch (up to the mates found) whose growth fits the budget
                 int64_t g = coefficient_growth(s->ham, *r->coef) + workspace_growth(s->ctx, r->mdim, r->params);
                 int n = 1;
@@ -429,8 +433,42 @@ struct Server {
         }
     }
 
+    bool large_running() const {
+        for (auto& s : slots)
+            if (s->job && s->job->mdim >= large_mdim) return true;
+        return false;
+    }
+
+    // Large-lane order: requests of at least large_mdim run one solve (batch) at a
+    // time, oldest first, as soon as none holds a slot; the others, oldest first once
+    // they have waited 60 s, else smallest basis first.  No request holds up another.
+    void schedule_priority_locked() {
+        constexpr double AGE_MS = 60000.0;
+        for (bool admitted = true; admitted;) {
+            admitted = false;
+            const double now = now_ms();
+            const bool lane_busy = large_running();
+            std::vector<std::shared_ptr<Request>> order(ready.begin(), ready.end());
+            auto rank = [&](const std::shared_ptr<Request>& r) {
+                if (r->mdim >= large_mdim) return 0;
+                return now - r->t_submit > AGE_MS ? 1 : 2;
+            };
+            std::stable_sort(order.begin(), order.end(), [&](const auto& a, const auto& b) {
+                const int ra = rank(a), rb = rank(b);
+                if (ra != rb) return ra < rb;
+                if (ra == 2 && a->mdim != b->mdim) return a->mdim < b->mdim;
+                return a->t_submit < b->t_submit;
+            });
+            for (auto& r : order) {
+                if (r->mdim >= large_mdim && lane_busy) continue;
+                if (try_admit_locked(r)) { admitted = true; break; }
+            }
+        }
+    }
+
     void schedule_locked() {
         if (stopping) return;
+        if (large_mdim > 0) { schedule_priority_locked(); return; }
         const int max_bypass = 4 * (int)slots.size();
         // An admission removes the request and any batch mates from the ready list,
         // so every admission ends the pass and the scan starts again from the front.
@@ -680,27 +718,50 @@ struct Server {
         if (cfg.keep_bytes <= 0) cfg.keep_bytes = (int64_t)16 << 30;
         if (cfg.max_batch <= 0) cfg.max_batch = 1;
         if (cfg.max_batch > 16) throw std::invalid_argument("max_batch above 16");
+        if (cfg.large_mdim < 0 || cfg.reserve_cus < 0) throw std::invalid_argument("negative large_mdim or reserve_cus");
+        if (cfg.reserve_cus > 0 && (cfg.large_mdim == 0 || cfg.max_inflight < 2))
+            throw std::invalid_argument("reserve_cus needs large_mdim > 0 and max_inflight >= 2");
+        large_mdim = cfg.large_mdim;
+        reserve_cus = cfg.reserve_cus;
         if (cfg.hbm_budget <= 0) {
             size_t fr = 0, tot = 0;
             SM_HIP(hipMemGetInfo(&fr, &tot));
             cfg.hbm_budget = (int64_t)(0.9 * (double)fr);
         }
+        // large lane on a CU partition: slot 0's streams leave the first reserve_cus CUs
+        // to the other slots, whose short kernels then never queue behind a large H·v
+        std::vector<uint32_t> lane_mask;
+        if (reserve_cus > 0) {
+            hipDeviceProp_t prop{};
+            SM_HIP(hipGetDeviceProperties(&prop, cfg.device));
+            const int ncu = prop.multiProcessorCount;
+            if (reserve_cus > ncu / 2) throw std::invalid_argument("reserved CUs above half the device");
+            lane_mask.assign((size_t)(ncu + 31) / 32, 0u);
+            for (int c = reserve_cus; c < ncu; c++) lane_mask[(size_t)c / 32] |= 1u << (c % 32);
+        }
+        auto new_stream = [&](hipStream_t* st, bool lane) {
+            if (lane && !lane_mask.empty())
+                SM_HIP(hipExtStreamCreateWithCUMask(st, (uint32_t)lane_mask.size(), lane_mask.data()));
+            else
+                SM_HIP(hipStreamCreateWithFlags(st, hipStreamNonBlocking));
+        };
         for (int i = 0; i < cfg.max_inflight; i++) {
             auto s = std::make_unique<Slot>();
             s->id = i;
-            SM_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+            new_stream(&s->stream, i == 0);
             s->ctx.stream = s->stream;
             if (cfg.max_batch > 1) {
                 for (int b = 0; b < cfg.max_batch; b++) s->mham.push_back(std::make_unique<DeviceHam>());
-                SM_HIP(hipStreamCreateWithFlags(&s->hv_stream, hipStreamNonBlocking));
+                new_stream(&s->hv_stream, i == 0);
                 for (int b = 1; b < cfg.max_batch; b++) {
                     auto c = std::make_unique<SolveCtx>();
-                    SM_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+                    new_stream(&c->stream, i == 0);
                     s->more.push_back(std::move(c));
                 }
             }
             slots.push_back(std::move(s));
         }
+        lane_split = !lane_mask.empty();
         for (auto& s : slots) {
             Slot* sp = s.get();
             sp->th = std::thread([this, sp] { slot_loop(*sp); });
@@ -837,6 +898,8 @@ void smgpu_default_server_config(smgpu_server_config* cfg) {
     cfg->keep_bytes = 0;
     cfg->max_batch = 1;
     cfg->batch_variants = 1;
+    cfg->large_mdim = 0;
+    cfg->reserve_cus = 0;
 }
 
 int32_t smgpu_server_create(const smgpu_server_config* cfg, smgpu_server* out) {

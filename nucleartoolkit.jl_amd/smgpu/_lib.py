@@ -55,7 +55,8 @@ class SolveStats(ctypes.Structure):
 class ServerConfig(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("max_inflight", ctypes.c_int32), ("hbm_budget", ctypes.c_int64),
                 ("host_threads", ctypes.c_int32), ("table_threads", ctypes.c_int32), ("keep_bytes", ctypes.c_int64),
-                ("max_batch", ctypes.c_int32), ("batch_variants", ctypes.c_int32)]
+                ("max_batch", ctypes.c_int32), ("batch_variants", ctypes.c_int32), ("large_mdim", ctypes.c_int64),
+                ("reserve_cus", ctypes.c_int32), ("reserved_", ctypes.c_int32)]
 
 
 class RequestInfo(ctypes.Structure):

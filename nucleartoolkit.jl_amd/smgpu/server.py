@@ -45,12 +45,15 @@ class Server:
     """smgpu_server handle.  Use as a context manager or call close()."""
 
     def __init__(self, device=0, max_inflight=4, hbm_budget=0, host_threads=2, table_threads=0, keep_bytes=0,
-                 max_batch=1, batch_variants=True):
+                 max_batch=1, batch_variants=True, large_mdim=0, reserve_cus=0):
+        """large_mdim > 0: requests of at least that dimension run one at a time, the others
+        smallest first; reserve_cus > 0 puts the large ones in slot 0 on the other CUs."""
         c = ServerConfig()
         lib().smgpu_default_server_config(ctypes.byref(c))
         c.device = int(device); c.max_inflight = int(max_inflight); c.hbm_budget = int(hbm_budget)
         c.host_threads = int(host_threads); c.table_threads = int(table_threads); c.keep_bytes = int(keep_bytes)
         c.max_batch = int(max_batch); c.batch_variants = int(bool(batch_variants))
+        c.large_mdim = int(large_mdim); c.reserve_cus = int(reserve_cus)
         h = ctypes.c_void_p()
         check(lib().smgpu_server_create(ctypes.byref(c), ctypes.byref(h)))
         self._h = h

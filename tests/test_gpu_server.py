@@ -11,6 +11,10 @@
   budget; a request larger than the budget fails with ENOMEM;
 * cancel of a queued and of a running request; errors (missing file) are
   reported per request; the server keeps serving afterwards.
+* the large lane: with large_mdim and reserve_cus, the requests at or above
+  large_mdim run in slot 0 only (streams on a CU partition) and the others never
+  there, each with the eigenvalues of the request solved alone; invalid
+  combinations fail at create.
 """
 import time
 
@@ -274,3 +278,25 @@ def test_server_interactions_with_other_sparsity_on_one_basis(snt_dir):
         assert info["state"] == "done"
         assert np.abs(ev - a).max() <= TOL_MIX, ev - a
     assert np.abs(alone[0] - alone[1]).max() > 1e-3
+
+
+def test_server_large_lane_on_cu_partition(snt_dir):
+    reqs = _stream(snt_dir)
+    alone = [_alone(p, n, k, s, "mixed") for p, n, k, s in reqs]
+    mdims = sorted(smgpu.Problem(p, n).mdim for p, n, _, _ in reqs)
+    large = mdims[len(mdims) // 2]          # about half the requests are large
+    with pytest.raises(smgpu.SmgpuError):
+        Server(max_inflight=4, reserve_cus=16)                  # needs large_mdim
+    with pytest.raises(smgpu.SmgpuError):
+        Server(max_inflight=4, large_mdim=large, reserve_cus=100000)
+    with Server(max_inflight=4, max_batch=2, large_mdim=large, reserve_cus=16) as srv:
+        ids = [srv.submit(p, n, k, precision="mixed", seed=s) for p, n, k, s in reqs]
+        for rid in ids:
+            assert srv.wait(rid, 120000)
+        res = [srv.result(rid) for rid in ids]
+        st = srv.stats()
+    assert st["completed"] == len(reqs) and st["failed"] == 0
+    for (path, nuc, k, s), a, (ev, info) in zip(reqs, alone, res):
+        print(nuc, info["mdim"], "slot", info["slot"], flush=True)
+        assert (info["slot"] == 0) == (info["mdim"] >= large), (nuc, info["mdim"], info["slot"])
+        assert np.abs(ev - a).max() <= TOL_MIX, (nuc, ev - a)
