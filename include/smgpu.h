@@ -35,6 +35,7 @@ typedef struct smgpu_problem_s* smgpu_problem;  /* interaction + basis + host ta
 typedef struct smgpu_ham_s* smgpu_ham;          /* device-resident tables on one GPU */
 typedef struct smgpu_server_s* smgpu_server;    /* persistent serving loop */
 typedef struct smgpu_moments_s* smgpu_moments;  /* M1 / E2 one-body operators on a basis */
+typedef struct smgpu_transitions_s* smgpu_transitions;  /* M1 / E2 operators between two bases */
 
 typedef struct {
     int32_t Z, N, A, vp, vn, twoM;
@@ -288,6 +289,33 @@ int32_t smgpu_eval_moment(smgpu_moments m, int32_t nvec, const double* vecs, con
                           double* Q);
 /* the oscillator length in use (fm) */
 int32_t smgpu_moments_bpar(smgpu_moments m, double* bpar);
+
+/* -------------------------------------------------------------- transitions
+ * transit_main (transit.jl:561-700): M1 and E2 transitions between the states of
+ * two problems on one interaction and nucleus, the left at 2M_l and the right at
+ * 2M_r <= 2M_l (transit_main puts each at its 2J).  smgpu_transitions_create
+ * builds prep_obtr_nondiag's one-body operators (transit.jl:286-445), the
+ * components mu = (2M_l - 2M_r) / 2 <= 2 of M1 and E2 from the right basis into
+ * the left one, and uploads them to `device`; gfactors, effcharge and bpar as
+ * smgpu_moments_create. */
+int32_t smgpu_transitions_create(smgpu_problem left, smgpu_problem right, int32_t device, const double* gfactors,
+                                 const double* effcharge, double bpar, smgpu_transitions* out);
+int32_t smgpu_transitions_destroy(smgpu_transitions t);
+/* w_left = O v_right with O the M1 (which = 0) or E2 (1) operator (op_obtr_nondiag,
+ * transit.jl:447-499), FP64 device vectors, on `stream` (NULL: the handle's own) */
+int32_t smgpu_transitions_apply(smgpu_transitions t, int32_t which, const double* v_right_dev, double* w_left_dev,
+                                void* stream);
+/* the same on host vectors: nvec right vectors (mdim_r each) to nvec left ones (mdim_l each) */
+int32_t smgpu_transitions_apply_host(smgpu_transitions t, int32_t which, int32_t nvec, const double* v_right,
+                                     double* w_left);
+/* transit_main's reduced matrix elements Mred(l, r) = <l|O|r> / 3j(J_l lam J_r; -M_l mu M_r)
+ * of nl left (nl x mdim_l) and nr right (nr x mdim_r) host vectors with 2J = jl2 and jr2:
+ * m1[l * nr + r] (lam 1, mu_N) and e2[l * nr + r] (lam 2, e fm^2), NaN where the
+ * transition is not allowed or the 3j symbol vanishes.  B(lam; r -> l) = Mred^2 / (jr2 + 1). */
+int32_t smgpu_transit(smgpu_transitions t, int32_t jl2, int32_t jr2, int32_t nl, const double* wl, int32_t nr,
+                      const double* wr, double* m1, double* e2);
+/* 2M_l - 2M_r, the oscillator length in use (fm), and the two basis dimensions */
+int32_t smgpu_transitions_info(smgpu_transitions t, int32_t* mu2, double* bpar, int64_t* mdim_l, int64_t* mdim_r);
 
 /* ------------------------------------------------------------------ serving
  * Persistent request loop (no counterpart in the reference, whose main_sm

@@ -37,6 +37,10 @@ struct smgpu_moments_s {
     std::unique_ptr<DeviceMoments> dev;
 };
 
+struct smgpu_transitions_s {
+    std::unique_ptr<DeviceTransitions> dev;
+};
+
 template <typename TA, typename TB>
 __global__ void k_cast(const TA* a, TB* b, int64_t n) {
     for (int64_t x = (int64_t)blockIdx.x * 256 + threadIdx.x; x < n; x += (int64_t)gridDim.x * 256) b[x] = (TB)a[x];
@@ -819,6 +823,83 @@ int32_t smgpu_moments_bpar(smgpu_moments m, double* bpar) {
     return guard([&]() -> int32_t {
         if (!m || !bpar) throw std::invalid_argument("null argument");
         *bpar = m->dev->bpar;
+        return SMGPU_OK;
+    });
+}
+
+int32_t smgpu_transitions_create(smgpu_problem left, smgpu_problem right, int32_t device, const double* gfactors,
+                                 const double* effcharge, double bpar, smgpu_transitions* out) {
+    return guard([&]() -> int32_t {
+        if (!left || !right || !out) throw std::invalid_argument("null argument");
+        *out = nullptr;
+        if (left->Z != right->Z || left->N != right->N || left->snt.A != right->snt.A ||
+            left->snt.p_sps.size() != right->snt.p_sps.size() || left->snt.n_sps.size() != right->snt.n_sps.size())
+            throw std::invalid_argument("left and right problems of different nuclei or spaces");
+        const double g0[4] = {1.0, 0.0, 5.586, -3.826}, e0[2] = {1.5, 0.5};
+        const double* g = gfactors ? gfactors : g0;
+        const double* e = effcharge ? effcharge : e0;
+        const double b = bpar > 0.0 ? bpar : ho_bpar_main_sm(left->snt.A);
+        TransitionOps to = build_transition_ops(left->snt, left->basis, right->basis, b, g, e);
+        auto t = std::make_unique<smgpu_transitions_s>();
+        t->dev = upload_transitions(to, left->basis, right->basis, device);
+        *out = t.release();
+        return SMGPU_OK;
+    });
+}
+
+int32_t smgpu_transitions_destroy(smgpu_transitions t) {
+    return guard([&]() -> int32_t {
+        if (t && t->dev) SM_HIP(hipSetDevice(t->dev->device));
+        delete t;
+        return SMGPU_OK;
+    });
+}
+
+int32_t smgpu_transitions_apply(smgpu_transitions t, int32_t which, const double* v_dev, double* w_dev, void* stream) {
+    return guard([&]() -> int32_t {
+        if (!t || !v_dev || !w_dev) throw std::invalid_argument("null argument");
+        SM_HIP(hipSetDevice(t->dev->device));
+        transitions_apply(*t->dev, which, v_dev, w_dev, stream ? (hipStream_t)stream : t->dev->stream);
+        return SMGPU_OK;
+    });
+}
+
+int32_t smgpu_transitions_apply_host(smgpu_transitions t, int32_t which, int32_t nvec, const double* v, double* w) {
+    return guard([&]() -> int32_t {
+        if (!t || !v || !w || nvec < 0) throw std::invalid_argument("bad argument");
+        DeviceTransitions& d = *t->dev;
+        SM_HIP(hipSetDevice(d.device));
+        const size_t nr = (size_t)d.mdim_r, nl = (size_t)d.mdim_l;
+        DBuf dv, dw;
+        dv.alloc(std::max<size_t>(nr, 1) * 8);
+        dw.alloc(std::max<size_t>(nl, 1) * 8);
+        for (int b = 0; b < nvec; b++) {
+            SM_HIP(hipMemcpy(dv.p, v + b * nr, nr * 8, hipMemcpyHostToDevice));
+            transitions_apply(d, which, dv.as<double>(), dw.as<double>(), d.stream);
+            SM_HIP(hipStreamSynchronize(d.stream));
+            SM_HIP(hipMemcpy(w + b * nl, dw.p, nl * 8, hipMemcpyDeviceToHost));
+        }
+        return SMGPU_OK;
+    });
+}
+
+int32_t smgpu_transit(smgpu_transitions t, int32_t jl2, int32_t jr2, int32_t nl, const double* wl, int32_t nr,
+                      const double* wr, double* m1, double* e2) {
+    return guard([&]() -> int32_t {
+        if (!t || !m1 || !e2 || nl < 0 || nr < 0 || (nl > 0 && !wl) || (nr > 0 && !wr))
+            throw std::invalid_argument("bad argument");
+        transit(*t->dev, jl2, jr2, nl, wl, nr, wr, m1, e2);
+        return SMGPU_OK;
+    });
+}
+
+int32_t smgpu_transitions_info(smgpu_transitions t, int32_t* mu2, double* bpar, int64_t* mdim_l, int64_t* mdim_r) {
+    return guard([&]() -> int32_t {
+        if (!t) throw std::invalid_argument("null argument");
+        if (mu2) *mu2 = t->dev->mu2;
+        if (bpar) *bpar = t->dev->bpar;
+        if (mdim_l) *mdim_l = t->dev->mdim_l;
+        if (mdim_r) *mdim_r = t->dev->mdim_r;
         return SMGPU_OK;
     });
 }

@@ -214,4 +214,26 @@ void moments_apply(const DeviceMoments& m, int which, const double* v, double* w
 // totJs (NaN where the reference prints nothing: J < 0 or a vanishing 3j symbol)
 void eval_moment(DeviceMoments& m, int nvec, const double* vecs, const double* totJs, double* mu, double* Q);
 
+// transit_main's operators on the device: right basis -> left basis
+struct DeviceTransitions {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int64_t mdim_l = 0, mdim_r = 0;
+    int R = 0, twoM_l = 0, twoM_r = 0, mu2 = 0;
+    double bpar = 0.0;
+    DBuf row_block, block_row0, nN_l, tdims_l, pset_l, nset_l, srcP, srcN, nN_r, tdims_r;
+    DBuf rp_base[2][2], ent_base[2][2], rowptr[2][2], col[2][2], val[2][2];   // [op][species]
+    DBuf vr, wl, left, part;               // work vectors of transit (grow-only)
+    ~DeviceTransitions();
+};
+std::unique_ptr<DeviceTransitions> upload_transitions(const TransitionOps& to, const Basis& left, const Basis& right,
+                                                      int device);
+// w_left = O v_right, O = M1 (which 0) or E2 (1), FP64 device vectors
+void transitions_apply(const DeviceTransitions& t, int which, const double* v, double* w, hipStream_t s);
+// transit_main (transit.jl:561-700): the reduced matrix elements <l|O_mu|r> / 3j(Jl lam Jr; -Ml mu Mr) of
+// nl left and nr right host vectors, [l * nr + r], for M1 (lam 1) and E2 (lam 2); NaN where the
+// transition is not allowed (triangle rule, mu above lam) or the 3j symbol vanishes
+void transit(DeviceTransitions& t, int jl2, int jr2, int nl, const double* wl, int nr, const double* wr, double* m1,
+             double* e2);
+
 }  // namespace smgpu

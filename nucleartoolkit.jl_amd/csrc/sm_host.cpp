@@ -829,4 +829,105 @@ MomentOps build_moment_ops(const Snt& snt, const Basis& b, double bpar, const do
     return mo;
 }
 
+TransitionOps build_transition_ops(const Snt& snt, const Basis& bl, const Basis& br, double bpar, const double* g,
+                                   const double* ec) {
+    TransitionOps to;
+    to.bpar = bpar;
+    to.mu2 = bl.twoM - br.twoM;
+    if (to.mu2 < 0 || (to.mu2 & 1))
+        throw std::invalid_argument("the left basis needs 2M >= the right basis' 2M, of the same parity");
+    const int omu = to.mu2 / 2;
+    if (omu > 2) throw std::invalid_argument("only M1 and E2 transitions: |M_left - M_right| <= 2");
+    for (int sp = 0; sp < 2; sp++)
+        if (bl.sp[sp].Ms != br.sp[sp].Ms || bl.sp[sp].nv != br.sp[sp].nv || bl.sp[sp].L != br.sp[sp].L)
+            throw std::invalid_argument("left and right bases of different nuclei or spaces");
+    const double c_mu = std::sqrt(3.0 / (4.0 * M_PI)), c_Q = std::sqrt(5.0 / (16.0 * M_PI));
+    // right blocks by (proton set, neutron set)
+    std::map<std::pair<int, int>, int> rblock;
+    for (int k = 0; k < br.nb; k++) rblock[{br.pset[k], br.nset[k]}] = k;
+    for (int sp = 0; sp < 2; sp++) {
+        const Species& S = bl.sp[sp];
+        const std::vector<Orbit>& sps = sp == 0 ? snt.p_sps : snt.n_sps;
+        const double gl = sp == 0 ? g[0] : g[1], gs = sp == 0 ? g[2] : g[3], e = ec[sp];
+        // left M-sets of this species whose source block exists in the right basis
+        std::vector<char> used(S.sets.size(), 0);
+        for (int k = 0; k < bl.nb; k++) {
+            const int t = sp == 0 ? bl.pset[k] : bl.nset[k];
+            const int src = S.set_of_M(S.Ms[t] - to.mu2);
+            if (src < 0) continue;
+            const std::pair<int, int> key = sp == 0 ? std::make_pair(src, bl.nset[k]) : std::make_pair(bl.pset[k], src);
+            if (rblock.count(key)) used[t] = 1;
+        }
+        // one-body terms (m-state a <- m-state c, coefficient), m_a - m_c = mu2
+        std::vector<std::array<int, 2>> terms[2];
+        std::vector<double> coef[2];
+        for (size_t i = 0; i < sps.size(); i++)
+            for (size_t f = 0; f < sps.size(); f++) {
+                const Orbit oi = sps[i], of = sps[f];
+                const int ji = oi.j2, jf = of.j2, li = oi.l, lf = of.l;
+                const double ex_r2 = radius_power(2, oi.n, li, of.n, lf, bpar);
+                const double l6j = wigner6j(2 * lf, 2 * li, 2, ji, jf, 1);
+                const double s6j = wigner6j(1, 1, 2, ji, jf, 2 * lf);
+                const double Q3j = wigner3j(ji, 4, jf, 1, 0, -1);
+                const double pfac = std::sqrt((double)((ji + 1) * (jf + 1)));
+                const std::vector<int> kc = mstates_of(S, (int)i), ka = mstates_of(S, (int)f);
+                for (int mode = 0; mode < 2; mode++) {
+                    if (mode == 0 && (oi.n != of.n || li != lf || omu > 1)) continue;
+                    for (int c : kc)
+                        for (int a : ka) {
+                            const int mc = S.m2[c], ma = S.m2[a];
+                            if (ma - mc != to.mu2) continue;
+                            double v;
+                            if (mode == 0) {
+                                const int s1 = (jf - ma) / 2 + lf + (ji + 3) / 2, s2 = (jf - ma) / 2 + lf + (jf + 3) / 2;
+                                const double fac1 = ((s1 & 1) ? -1.0 : 1.0) * l6j * std::sqrt((double)(lf * (lf + 1) * (2 * lf + 1)));
+                                const double fac2 = ((s2 & 1) ? -1.0 : 1.0) * s6j * std::sqrt(1.5);
+                                v = pfac * wigner3j(jf, 2, ji, -ma, to.mu2, mc) * (fac1 * gl + fac2 * gs) * c_mu;
+                            } else {
+                                if ((li + lf) & 1) continue;      // (1 + (-1)^(li+lf)) = 0
+                                const int sq = jf + (1 - ma) / 2;
+                                v = ((sq & 1) ? -2.0 : 2.0) * wigner3j(jf, 4, ji, -ma, to.mu2, mc) * Q3j * pfac * e * c_Q *
+                                    ex_r2;
+                            }
+                            if (v == 0.0) continue;
+                            terms[mode].push_back({a, c});
+                            coef[mode].push_back(v);
+                        }
+                }
+            }
+        for (int mode = 0; mode < 2; mode++) {
+            std::vector<PairSym>& out = to.op[mode][sp];
+            out.assign(S.sets.size(), PairSym{});
+            for (size_t t = 0; t < S.sets.size(); t++) {
+                const auto& F = S.sets[t];
+                PairSym& P = out[t];
+                P.rowptr.assign(F.size() + 1, 0);
+                if (!used[t]) continue;
+                const int src = S.set_of_M(S.Ms[t] - to.mu2);
+                const auto& I = S.sets[src];
+                std::vector<std::map<int32_t, double>> rows(F.size());
+                for (size_t q = 0; q < terms[mode].size(); q++) {
+                    const uint64_t ba = bit_of(S, terms[mode][q][0]), bc = bit_of(S, terms[mode][q][1]);
+                    for (size_t ni = 0; ni < I.size(); ni++) {
+                        const uint64_t phi = I[ni];
+                        if (!(phi & bc)) continue;
+                        const uint64_t mid = phi & ~bc;
+                        if (mid & ba) continue;
+                        const int64_t nf = S.find((int)t, mid | ba);
+                        if (nf < 0) continue;
+                        // fermion sign of a+ c on phi (calc_phase_1! with its leading 1)
+                        const int ex = 1 + popc(phi & at_or_above(bc)) + popc(mid & at_or_above(ba));
+                        rows[nf][(int32_t)ni] += ((ex & 1) ? -1.0 : 1.0) * coef[mode][q];
+                    }
+                }
+                for (size_t r = 0; r < F.size(); r++) {
+                    for (auto& kv : rows[r]) { P.col.push_back(kv.first); P.val.push_back(kv.second); }
+                    P.rowptr[r + 1] = (int64_t)P.col.size();
+                }
+            }
+        }
+    }
+    return to;
+}
+
 }  // namespace smgpu

@@ -129,6 +129,17 @@ struct MomentOps {
 };
 MomentOps build_moment_ops(const Snt& snt, const Basis& b, double bpar, const double* gfactors,
                            const double* effcharge);
+// The M1 and E2 one-body operators of transit_main (prep_obtr_nondiag, transit.jl:286-445):
+// component mu = (2M_left - 2M_right) / 2 of the tensors, from the right basis into the left
+// one (same interaction and nucleus).  op[mode][species][t] holds, for left M-set t, the CSR
+// over its SDs of the source M-set at M_t - mu2 (not symmetric, no halving).
+struct TransitionOps {
+    double bpar = 0.0;
+    int mu2 = 0;                      // 2M_left - 2M_right = m_a - m_c of every a+ c (units of 1/2)
+    std::vector<PairSym> op[2][2];    // [0 = M1, 1 = E2][species] per left M-set
+};
+TransitionOps build_transition_ops(const Snt& snt, const Basis& left, const Basis& right, double bpar,
+                                   const double* gfactors, const double* effcharge);
 // oscillator length of main_sm (init_ho_by_mass, transit.jl:49-60, with formula 2
 // for 16 <= A <= 40, shellmodel_main.jl:104-107)
 double ho_bpar_main_sm(int A);

@@ -1,14 +1,20 @@
 // Magnetic dipole and electric quadrupole moments of Ritz vectors (main_sm's
 // calc_moment; transit.jl: prep_obtr_diag :82-245, op_obtr_diag :247-283,
-// eval_moment :702-765).  The operators are the M = 0 components of one-body
-// tensors, so they keep every SD in its M-set: w[b, ip, n] gathers the proton
-// matrix over the rows ip' of block b and the neutron matrix over its columns n'.
-// One workgroup per proton row, lanes along the neutron index, every output
-// element written once (deterministic); FP64 throughout.
+// eval_moment :702-765) and the M1 / E2 transitions between two sets of them
+// (transit_main :561-700, prep_obtr_nondiag :286-445, op_obtr_nondiag :447-499).
+// The moment operators are the M = 0 components of one-body tensors, so they keep
+// every SD in its M-set: w[b, ip, n] gathers the proton matrix over the rows ip'
+// of block b and the neutron matrix over its columns n'.  The transition
+// operators (component mu) map a right basis at 2M_r into a left one at 2M_l =
+// 2M_r + 2mu: left block (Mp, Mn) gathers its proton term from the right block
+// (Mp - 2mu, Mn) and its neutron term from (Mp, Mn - 2mu).  One workgroup per
+// (left) proton row, lanes along the neutron index, every output element written
+// once (deterministic); FP64 throughout.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
 #include <limits>
+#include <map>
 #include <stdexcept>
 #include <vector>
 
@@ -65,6 +71,56 @@ __global__ __launch_bounds__(MTB) void k_dot_part(const double* __restrict__ x, 
         __syncthreads();
     }
     if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+struct TrView {
+    const int32_t* row_block;      // left rows
+    const int32_t* block_row0;
+    const int32_t* nN;             // per left block
+    const int64_t* tdims;
+    const int32_t* pset;
+    const int32_t* nset;
+    const int32_t* srcP;           // per left block: right block of the proton term (-1: none)
+    const int32_t* srcN;           // per left block: right block of the neutron term (-1: none)
+    const int32_t* nN_r;           // per right block
+    const int64_t* tdims_r;
+    const int64_t* rp_base[2];     // per species and left M-set
+    const int64_t* ent_base[2];
+    const int64_t* rowptr[2];
+    const int32_t* col[2];         // SD index in the source M-set
+    const double* val[2];
+};
+
+__global__ __launch_bounds__(MTB) void k_transition(TrView A, const double* __restrict__ v, double* __restrict__ w) {
+    const int row = blockIdx.x;
+    const int b = A.row_block[row];
+    const int ip = row - A.block_row0[b];
+    const int nN = A.nN[b];
+    const int64_t orow = A.tdims[b] + (int64_t)ip * nN;
+    const int bp = A.srcP[b], bn = A.srcN[b];
+    int64_t pe0 = 0, pe1 = 0, pbase = 0;
+    if (bp >= 0) {
+        const int ps = A.pset[b];
+        const int64_t* prp = A.rowptr[0] + A.rp_base[0][ps];
+        pe0 = A.ent_base[0][ps] + prp[ip];
+        pe1 = A.ent_base[0][ps] + prp[ip + 1];
+        pbase = A.tdims_r[bp];                  // the right block has this block's neutron set (nN columns)
+    }
+    const int64_t* nrp = nullptr;
+    int64_t nb0 = 0, nrow = 0;
+    if (bn >= 0) {
+        const int ns = A.nset[b];
+        nrp = A.rowptr[1] + A.rp_base[1][ns];
+        nb0 = A.ent_base[1][ns];
+        nrow = A.tdims_r[bn] + (int64_t)ip * A.nN_r[bn];   // the same proton SD in the right block
+    }
+    for (int n = threadIdx.x; n < nN; n += MTB) {
+        double acc = 0.0;
+        for (int64_t e = pe0; e < pe1; e++) acc += A.val[0][e] * v[pbase + (int64_t)A.col[0][e] * nN + n];
+        if (bn >= 0)
+            for (int64_t e = nb0 + nrp[n]; e < nb0 + nrp[n + 1]; e++) acc += A.val[1][e] * v[nrow + A.col[1][e]];
+        w[orow + n] = acc;
+    }
 }
 
 DeviceMoments::~DeviceMoments() {
@@ -179,6 +235,155 @@ void eval_moment(DeviceMoments& m, int nvec, const double* vecs, const double* t
         }
         SM_HIP(hipStreamSynchronize(m.stream));
     }
+}
+
+DeviceTransitions::~DeviceTransitions() {
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+static void upload_csr_sets(const std::vector<PairSym>& sets, DBuf& rp_base, DBuf& ent_base, DBuf& rowptr, DBuf& col,
+                            DBuf& val) {
+    std::vector<int64_t> rpb, eb, rp;
+    std::vector<int32_t> c;
+    std::vector<double> v;
+    for (const PairSym& P : sets) {
+        rpb.push_back((int64_t)rp.size());
+        eb.push_back((int64_t)c.size());
+        rp.insert(rp.end(), P.rowptr.begin(), P.rowptr.end());
+        c.insert(c.end(), P.col.begin(), P.col.end());
+        v.insert(v.end(), P.val.begin(), P.val.end());
+    }
+    if (rpb.empty()) rpb.push_back(0);
+    if (eb.empty()) eb.push_back(0);
+    if (rp.empty()) rp.push_back(0);
+    if (c.empty()) { c.push_back(0); v.push_back(0.0); }
+    rp_base.upload(rpb);
+    ent_base.upload(eb);
+    rowptr.upload(rp);
+    col.upload(c);
+    val.upload(v);
+}
+
+std::unique_ptr<DeviceTransitions> upload_transitions(const TransitionOps& to, const Basis& bl, const Basis& br,
+                                                      int device) {
+    SM_HIP(hipSetDevice(device));
+    auto t = std::make_unique<DeviceTransitions>();
+    t->device = device;
+    t->mdim_l = bl.mdim;
+    t->mdim_r = br.mdim;
+    t->twoM_l = bl.twoM;
+    t->twoM_r = br.twoM;
+    t->mu2 = to.mu2;
+    t->bpar = to.bpar;
+    std::map<std::pair<int, int>, int> rblock;
+    for (int k = 0; k < br.nb; k++) rblock[{br.pset[k], br.nset[k]}] = k;
+    std::vector<int32_t> rb, r0, nN, ps(bl.pset.begin(), bl.pset.end()), ns(bl.nset.begin(), bl.nset.end()), sP, sN,
+        nNr;
+    for (int k = 0; k < bl.nb; k++) {
+        const int np = (int)bl.sp[0].sets[bl.pset[k]].size();
+        r0.push_back((int32_t)rb.size());
+        nN.push_back((int32_t)bl.sp[1].sets[bl.nset[k]].size());
+        for (int i = 0; i < np; i++) rb.push_back(k);
+        const int p_src = bl.sp[0].set_of_M(bl.sp[0].Ms[bl.pset[k]] - to.mu2);
+        const int n_src = bl.sp[1].set_of_M(bl.sp[1].Ms[bl.nset[k]] - to.mu2);
+        auto it = p_src >= 0 ? rblock.find({p_src, bl.nset[k]}) : rblock.end();
+        sP.push_back(it == rblock.end() ? -1 : it->second);
+        auto jt = n_src >= 0 ? rblock.find({bl.pset[k], n_src}) : rblock.end();
+        sN.push_back(jt == rblock.end() ? -1 : jt->second);
+    }
+    for (int k = 0; k < br.nb; k++) nNr.push_back((int32_t)br.sp[1].sets[br.nset[k]].size());
+    t->R = (int)rb.size();
+    for (auto* x : {&rb, &r0, &nN, &ps, &ns, &sP, &sN, &nNr})
+        if (x->empty()) x->push_back(0);
+    t->row_block.upload(rb);
+    t->block_row0.upload(r0);
+    t->nN_l.upload(nN);
+    t->tdims_l.upload(bl.tdims);
+    t->pset_l.upload(ps);
+    t->nset_l.upload(ns);
+    t->srcP.upload(sP);
+    t->srcN.upload(sN);
+    t->nN_r.upload(nNr);
+    t->tdims_r.upload(br.tdims);
+    for (int op = 0; op < 2; op++)
+        for (int sp = 0; sp < 2; sp++)
+            upload_csr_sets(to.op[op][sp], t->rp_base[op][sp], t->ent_base[op][sp], t->rowptr[op][sp], t->col[op][sp],
+                            t->val[op][sp]);
+    SM_HIP(hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking));
+    return t;
+}
+
+void transitions_apply(const DeviceTransitions& t, int which, const double* v, double* w, hipStream_t s) {
+    if (which != 0 && which != 1) throw std::invalid_argument("operator: 0 (M1) or 1 (E2)");
+    if (t.R == 0) return;
+    TrView A;
+    A.row_block = t.row_block.as<int32_t>();
+    A.block_row0 = t.block_row0.as<int32_t>();
+    A.nN = t.nN_l.as<int32_t>();
+    A.tdims = t.tdims_l.as<int64_t>();
+    A.pset = t.pset_l.as<int32_t>();
+    A.nset = t.nset_l.as<int32_t>();
+    A.srcP = t.srcP.as<int32_t>();
+    A.srcN = t.srcN.as<int32_t>();
+    A.nN_r = t.nN_r.as<int32_t>();
+    A.tdims_r = t.tdims_r.as<int64_t>();
+    for (int sp = 0; sp < 2; sp++) {
+        A.rp_base[sp] = t.rp_base[which][sp].as<int64_t>();
+        A.ent_base[sp] = t.ent_base[which][sp].as<int64_t>();
+        A.rowptr[sp] = t.rowptr[which][sp].as<int64_t>();
+        A.col[sp] = t.col[which][sp].as<int32_t>();
+        A.val[sp] = t.val[which][sp].as<double>();
+    }
+    hipLaunchKernelGGL(k_transition, dim3(t.R), dim3(MTB), 0, s, A, v, w);
+    SM_HIP(hipGetLastError());
+}
+
+static double dot_on(DBuf& part, hipStream_t st, const double* x, const double* y, int64_t n) {
+    const int G = (int)std::max<int64_t>(1, std::min<int64_t>(1024, (n + MTB - 1) / MTB));
+    part.ensure((size_t)G * 8);
+    hipLaunchKernelGGL(k_dot_part, dim3(G), dim3(MTB), 0, st, x, y, n, part.as<double>());
+    SM_HIP(hipGetLastError());
+    std::vector<double> h(G);
+    SM_HIP(hipMemcpyAsync(h.data(), part.p, (size_t)G * 8, hipMemcpyDeviceToHost, st));
+    SM_HIP(hipStreamSynchronize(st));
+    double s = 0.0;
+    for (double x2 : h) s += x2;
+    return s;
+}
+
+static bool tri_relation(int ja2, int jb2, int jc2) {
+    return std::abs(ja2 - jb2) <= jc2 && jc2 <= ja2 + jb2;
+}
+
+void transit(DeviceTransitions& t, int jl2, int jr2, int nl, const double* wl, int nr, const double* wr, double* m1,
+             double* e2) {
+    SM_HIP(hipSetDevice(t.device));
+    const double nan = std::numeric_limits<double>::quiet_NaN();
+    for (int64_t k = 0; k < (int64_t)nl * nr; k++) { m1[k] = nan; e2[k] = nan; }
+    if (jl2 < 0 || jr2 < 0 || nl <= 0 || nr <= 0 || t.mdim_l == 0 || t.mdim_r == 0) return;
+    const int Ml2 = t.twoM_l, Mr2 = t.twoM_r;
+    // the operators are components mu of rank lam (mu <= lam), and the reference's selection
+    const double sel[2] = {tri_relation(jl2, jr2, 2) && t.mu2 <= 2 ? wigner3j(jl2, 2, jr2, -Ml2, t.mu2, Mr2) : 0.0,
+                           tri_relation(jl2, jr2, 4) ? wigner3j(jl2, 4, jr2, -Ml2, t.mu2, Mr2) : 0.0};
+    if (sel[0] == 0.0 && sel[1] == 0.0) return;
+    t.left.ensure((size_t)nl * t.mdim_l * 8);
+    t.vr.ensure((size_t)t.mdim_r * 8);
+    t.wl.ensure((size_t)t.mdim_l * 8);
+    SM_HIP(hipMemcpyAsync(t.left.p, wl, (size_t)nl * t.mdim_l * 8, hipMemcpyHostToDevice, t.stream));
+    for (int r = 0; r < nr; r++) {
+        SM_HIP(hipMemcpyAsync(t.vr.p, wr + (int64_t)r * t.mdim_r, (size_t)t.mdim_r * 8, hipMemcpyHostToDevice,
+                              t.stream));
+        for (int op = 0; op < 2; op++) {
+            if (sel[op] == 0.0) continue;
+            transitions_apply(t, op, t.vr.as<double>(), t.wl.as<double>(), t.stream);
+            double* out = op == 0 ? m1 : e2;
+            for (int l = 0; l < nl; l++)
+                out[(int64_t)l * nr + r] =
+                    dot_on(t.part, t.stream, t.left.as<double>() + (int64_t)l * t.mdim_l, t.wl.as<double>(), t.mdim_l) /
+                    sel[op];
+        }
+    }
+    SM_HIP(hipStreamSynchronize(t.stream));
 }
 
 }  // namespace smgpu

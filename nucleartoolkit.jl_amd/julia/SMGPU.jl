@@ -248,6 +248,105 @@ function eval_moment(m::Moments, vecs::Matrix{Float64}, totJs::Vector{Float64})
     return mu, Q
 end
 
+# M1 / E2 operators of transit_main between two problems on one interaction and nucleus,
+# the left at 2M_l and the right at 2M_r <= 2M_l (prep_obtr_nondiag, transit.jl:286-445)
+mutable struct Transitions
+    h::Ptr{Cvoid}
+end
+function Transitions(left::Problem, right::Problem; device::Integer=0, gfactors=[1.0, 0.0, 5.586, -3.826],
+                     effcharge=[1.5, 0.5], bpar::Real=0.0)
+    g = Vector{Float64}(gfactors)
+    e = Vector{Float64}(effcharge)
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:smgpu_transitions_create, libsmgpu), Int32,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Int32, Ptr{Float64}, Ptr{Float64}, Float64, Ref{Ptr{Cvoid}}),
+                left.h, right.h, device, g, e, Float64(bpar), out))
+    t = Transitions(out[])
+    finalizer(x -> ccall((:smgpu_transitions_destroy, libsmgpu), Int32, (Ptr{Cvoid},), x.h), t)
+    return t
+end
+
+# reduced matrix elements <l||O||r> of the columns of wl (mdim_l x nl) and wr (mdim_r x nr):
+# (M1, E2), each nl x nr, NaN where the transition is not allowed
+function transit(t::Transitions, jl2::Integer, jr2::Integer, wl::Matrix{Float64}, wr::Matrix{Float64})
+    nl = size(wl, 2); nr = size(wr, 2)
+    m1 = zeros(Float64, nr, nl)          # C row-major [l * nr + r] = Julia column-major (r, l)
+    e2 = zeros(Float64, nr, nl)
+    check(ccall((:smgpu_transit, libsmgpu), Int32,
+                (Ptr{Cvoid}, Int32, Int32, Int32, Ptr{Float64}, Int32, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                t.h, jl2, jr2, nl, wl, nr, wr, m1, e2))
+    return permutedims(m1), permutedims(e2)
+end
+
+# read_wav(all=true) (shellmodel_main.jl:979-1003): energies, 2J values, vectors (mdim x n)
+function read_wav(path::AbstractString, mdim::Integer)
+    open(path, "r") do io
+        n = read(io, Int32); mtot = read(io, Int32)
+        es = [read(io, Float64) for _ in 1:n]
+        jj = [read(io, Int32) for _ in 1:n]
+        V = zeros(Float64, mdim, n)
+        read!(io, V)
+        return es, jj, V, mtot
+    end
+end
+
+# Wigner 3j symbol of doubled arguments (Racah's formula)
+function w3j(j1, j2, j3, m1, m2, m3)
+    (m1 + m2 + m3 != 0 || abs(m1) > j1 || abs(m2) > j2 || abs(m3) > j3) && return 0.0
+    (j3 < abs(j1 - j2) || j3 > j1 + j2 || isodd(j1 + j2 + j3)) && return 0.0
+    f(x) = factorial(big(x))
+    a = (j1 + j2 - j3) ÷ 2; b = (j1 - j2 + j3) ÷ 2; c = (-j1 + j2 + j3) ÷ 2
+    pre = f(a) * f(b) * f(c) // f((j1 + j2 + j3) ÷ 2 + 1) * f((j1 + m1) ÷ 2) * f((j1 - m1) ÷ 2) *
+          f((j2 + m2) ÷ 2) * f((j2 - m2) ÷ 2) * f((j3 + m3) ÷ 2) * f((j3 - m3) ÷ 2)
+    s = big(0) // 1
+    for k in 0:a
+        d = (k, a - k, (j1 - m1) ÷ 2 - k, (j2 + m2) ÷ 2 - k, (j3 - j2 + m1) ÷ 2 + k, (j3 - j1 - m2) ÷ 2 + k)
+        minimum(d) < 0 && continue
+        s += (-1)^k // prod(f.(d))
+    end
+    sgn = isodd((j1 - j2 - m3) ÷ 2) ? -1.0 : 1.0
+    return sgn * Float64(s) * sqrt(Float64(pre))
+end
+
+# transit_main (transit.jl:561-700): M1 and E2 transitions between the states of two .wav
+# files, the left at 2M = 2J = jl2 and the right at jr2 (swapped when jl2 < jr2); prints
+# the reference's lines and returns (Mred M1, Mred E2) as nl x nr matrices
+function transit_main(sntf, target_nuc, jl2, jr2, in_wfs; num_ev_l=100, num_ev_r=100, calc_EM=true,
+                      gfactors=[1.0, 0.0, 5.586, -3.826], eff_charge=[1.5, 0.5], device::Integer=0)
+    length(in_wfs) == 2 || error("input wfs must be specified")
+    widx = [1, 2]
+    if jl2 < jr2
+        jl2, jr2 = jr2, jl2; widx = [2, 1]
+    end
+    (jl2 - jr2) ÷ 2 > 2 && error("Only the M1&E2 are supported now")
+    left = Problem(sntf, target_nuc; twoM=jl2)
+    right = Problem(sntf, target_nuc; twoM=jr2)
+    _, _, Vl, _ = read_wav(in_wfs[widx[1]], info(left).mdim)
+    _, _, Vr, _ = read_wav(in_wfs[widx[2]], info(right).mdim)
+    Vl = Vl[:, 1:min(num_ev_l, size(Vl, 2))]; Vr = Vr[:, 1:min(num_ev_r, size(Vr, 2))]
+    m1, e2 = transit(Transitions(left, right; device=device, gfactors=gfactors, effcharge=eff_charge), jl2, jr2,
+                     Vl, Vr)
+    same = in_wfs[1] == in_wfs[2]
+    for (mode, lam2, red, cmom, tag) in (("M1", 2, m1, sqrt(4pi / 3), "<gm>"), ("E2", 4, e2, sqrt(16pi / 5), "<eQ>"))
+        all(isnan, red) && continue
+        cg = w3j(jr2, lam2, jl2, -jr2, jl2 - jr2, jl2)
+        for r in 1:size(red, 2), l in 1:size(red, 1)
+            (!calc_EM && l != r) && continue
+            Mred = red[l, r]
+            if calc_EM
+                tx = "Mred.($mode) ($l,$r) " * @sprintf("%9.3f ", Mred) * "  "
+                tx *= "B($mode) ($l<=$r) " * @sprintf("%9.3f ", Mred^2 / (jr2 + 1)) * " B($mode) ($l=>$r) " *
+                      @sprintf("%9.3f ", Mred^2 / (jl2 + 1))
+                (same && l == r) && (tx *= " $tag= " * @sprintf("%9.3f ", Mred * cg * cmom))
+            else
+                tx = " $tag= " * @sprintf("%9.3f ", Mred * cg * cmom)
+            end
+            println(tx)
+        end
+    end
+    return m1, e2
+end
+
 # J from <J^2> (J_from_JJ1, shellmodel_main.jl:1105-1113): the J in 0, 1/2, ..., 50 with
 # |J(J+1) - jj| < tol, else -1
 function j_from_jj1(jj::Real; tol=1e-6)

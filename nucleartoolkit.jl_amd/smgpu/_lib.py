@@ -126,6 +126,12 @@ SIGNATURES = {
     "smgpu_moments_apply_host": (I32, [P, I32, I32, P, P]),
     "smgpu_eval_moment": (I32, [P, I32, P, P, P, P]),
     "smgpu_moments_bpar": (I32, [P, P]),
+    "smgpu_transitions_create": (I32, [P, P, I32, P, P, ctypes.c_double, P]),
+    "smgpu_transitions_destroy": (I32, [P]),
+    "smgpu_transitions_apply": (I32, [P, I32, P, P, P]),
+    "smgpu_transitions_apply_host": (I32, [P, I32, I32, P, P]),
+    "smgpu_transit": (I32, [P, I32, I32, I32, P, I32, P, P, P]),
+    "smgpu_transitions_info": (I32, [P, P, P, P, P]),
     "smgpu_malloc": (I32, [P, I64, P]),
     "smgpu_free": (I32, [P, P]),
     "smgpu_memcpy": (I32, [P, P, P, I64, I32]),

@@ -399,6 +399,133 @@ class Moments:
         return mu, Q
 
 
+class Transitions:
+    """The M1 and E2 operators of transit_main between two problems on one interaction
+    and nucleus (prep_obtr_nondiag, transit.jl:286-445): the components mu = (2M_l -
+    2M_r) / 2 from the right basis into the left one, on one GPU.  `apply` is
+    op_obtr_nondiag (:447-499), `transit` the reduced matrix elements of :598-697."""
+
+    def __init__(self, left: Problem, right: Problem, device=0, gfactors=(1.0, 0.0, 5.586, -3.826),
+                 effcharge=(1.5, 0.5), bpar=0.0):
+        g = np.ascontiguousarray(gfactors, dtype=np.float64)
+        e = np.ascontiguousarray(effcharge, dtype=np.float64)
+        if g.shape != (4,) or e.shape != (2,):
+            raise ValueError("gfactors needs 4 values and effcharge 2")
+        h = ctypes.c_void_p()
+        check(lib().smgpu_transitions_create(left._h, right._h, int(device), ptr(g), ptr(e), float(bpar),
+                                             ctypes.byref(h)))
+        self._h = h
+        mu2 = ctypes.c_int32(); b = ctypes.c_double(); ml = ctypes.c_int64(); mr = ctypes.c_int64()
+        check(lib().smgpu_transitions_info(h, ctypes.byref(mu2), ctypes.byref(b), ctypes.byref(ml), ctypes.byref(mr)))
+        self.mu2, self.bpar, self.mdim_l, self.mdim_r = mu2.value, b.value, ml.value, mr.value
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().smgpu_transitions_destroy(self._h)
+            self._h = None
+
+    def apply(self, V, which):
+        """O V for the rows of V (right basis) into the left basis; which = "M1" or "E2"."""
+        V = np.ascontiguousarray(np.atleast_2d(V), dtype=np.float64)
+        if V.shape[1] != self.mdim_r:
+            raise ValueError("vectors of the right basis expected")
+        W = np.zeros((V.shape[0], self.mdim_l))
+        check(lib().smgpu_transitions_apply_host(self._h, {"M1": 0, "E2": 1}[which], V.shape[0], ptr(V), ptr(W)))
+        return W
+
+    def transit(self, jl2, jr2, left_vecs, right_vecs):
+        """(Mred M1, Mred E2), each [n_left, n_right]; NaN where not allowed."""
+        L = np.ascontiguousarray(np.atleast_2d(left_vecs), dtype=np.float64)
+        R = np.ascontiguousarray(np.atleast_2d(right_vecs), dtype=np.float64)
+        if L.shape[1] != self.mdim_l or R.shape[1] != self.mdim_r:
+            raise ValueError("vectors of the left and right bases expected")
+        m1 = np.zeros((L.shape[0], R.shape[0]))
+        e2 = np.zeros((L.shape[0], R.shape[0]))
+        check(lib().smgpu_transit(self._h, int(jl2), int(jr2), L.shape[0], ptr(L), R.shape[0], ptr(R), ptr(m1),
+                                  ptr(e2)))
+        return m1, e2
+
+
+def _w3j(j1, j2, j3, m1, m2, m3):
+    """Wigner 3j symbol of doubled arguments (Racah's formula, exact integers)."""
+    from fractions import Fraction
+    from math import factorial as F
+    if m1 + m2 + m3 != 0 or abs(m1) > j1 or abs(m2) > j2 or abs(m3) > j3:
+        return 0.0
+    if j3 < abs(j1 - j2) or j3 > j1 + j2 or (j1 + j2 + j3) % 2 or (j1 + m1) % 2 or (j2 + m2) % 2 or (j3 + m3) % 2:
+        return 0.0
+    a, b, c = (j1 + j2 - j3) // 2, (j1 - j2 + j3) // 2, (-j1 + j2 + j3) // 2
+    tri = Fraction(F(a) * F(b) * F(c), F((j1 + j2 + j3) // 2 + 1))
+    pre = tri * F((j1 + m1) // 2) * F((j1 - m1) // 2) * F((j2 + m2) // 2) * F((j2 - m2) // 2) * \
+        F((j3 + m3) // 2) * F((j3 - m3) // 2)
+    s = Fraction(0)
+    for k in range(0, a + 1):
+        d = [k, a - k, (j1 - m1) // 2 - k, (j2 + m2) // 2 - k, (j3 - j2 + m1) // 2 + k, (j3 - j1 - m2) // 2 + k]
+        if min(d) < 0:
+            continue
+        den = 1
+        for x in d:
+            den *= F(x)
+        s += Fraction((-1) ** k, den)
+    sign = -1 if ((j1 - j2 - m3) // 2) % 2 else 1
+    return sign * float(s) * float(pre) ** 0.5
+
+
+def transit_main(sntf, target_nuc, jl2, jr2, in_wfs, *, num_ev_l=100, num_ev_r=100, calc_EM=True,
+                 gfactors=(1.0, 0.0, 5.586, -3.826), eff_charge=(1.5, 0.5), device=0, verbose=True):
+    """transit_main (transit.jl:561-700): M1 and E2 transitions between the states of two
+    .wav files, the left one at 2M = 2J = jl2 and the right one at jr2 (the files are
+    swapped when jl2 < jr2, as the reference does).  For every right state r and left state
+    l: Mred = <l||O||r> (reduced matrix element), B(l <= r) = Mred^2 / (2J_r + 1) and
+    B(l => r) = Mred^2 / (2J_l + 1); with one file given twice, <gm> (mu_N) and <eQ> (e fm^2)
+    of each state.  calc_EM=False gives only the latter.  Prints the reference's lines when
+    verbose and returns them as dicts."""
+    from .wav import read_wav
+    if len(in_wfs) != 2:
+        raise ValueError("input wfs must be specified: [left .wav, right .wav]")
+    widx = [0, 1]
+    if jl2 < jr2:
+        jl2, jr2, widx = jr2, jl2, [1, 0]
+    left = Problem(sntf, target_nuc, twoM=jl2)
+    right = Problem(sntf, target_nuc, twoM=jr2)
+    _, _, Vl, _ = read_wav(in_wfs[widx[0]], left.mdim)
+    _, _, Vr, _ = read_wav(in_wfs[widx[1]], right.mdim)
+    Vl, Vr = Vl[:num_ev_l], Vr[:num_ev_r]
+    if (jl2 - jr2) // 2 > 2:
+        raise ValueError("only M1 and E2 transitions are supported")
+    T = Transitions(left, right, device, gfactors, eff_charge)
+    m1, e2 = T.transit(jl2, jr2, Vl, Vr)
+    same = in_wfs[0] == in_wfs[1]
+    omu2 = jl2 - jr2
+    out = []
+    for mode, lam2, red, cmom, tag in (("M1", 2, m1, np.sqrt(4 * np.pi / 3), "<gm>"),
+                                       ("E2", 4, e2, np.sqrt(16 * np.pi / 5), "<eQ>")):
+        if np.all(np.isnan(red)):
+            continue
+        cg = _w3j(jr2, lam2, jl2, -jr2, omu2, jl2)
+        sign = -1.0 if ((jl2 - jl2) // 2) % 2 else 1.0      # (-1)^(J_l - M_l), M_l = J_l
+        for r in range(red.shape[1]):
+            for l in range(red.shape[0]):
+                if not calc_EM and l != r:
+                    continue
+                mr = float(red[l, r])
+                rec = {"mode": mode, "l": l + 1, "r": r + 1, "Mred": mr, "B_l_from_r": mr ** 2 / (jr2 + 1),
+                       "B_r_from_l": mr ** 2 / (jl2 + 1), "moment": None}
+                if (same or not calc_EM) and l == r:
+                    rec["moment"] = mr * cg * cmom * sign
+                if verbose:
+                    if calc_EM:
+                        tx = (f"Mred.({mode}) ({l + 1},{r + 1}) {mr:9.3f}   B({mode}) ({l + 1}<={r + 1}) "
+                              f"{rec['B_l_from_r']:9.3f}  B({mode}) ({l + 1}=>{r + 1}) {rec['B_r_from_l']:9.3f} ")
+                        if rec["moment"] is not None:
+                            tx += f" {tag}= {rec['moment']:9.3f} "
+                    else:
+                        tx = f" {tag}= {rec['moment']:9.3f} "
+                    print(tx)
+                out.append(rec)
+    return out
+
+
 def format_moments(mu, Q):
     """eval_moment's text (transit.jl:730-763): per state 'mu (n=k) ' / 'Q (n=k) '."""
     out = ""

@@ -20,13 +20,19 @@ Parts:
   * `main_sm`               shellmodel_main.jl:89-281 (eigenvalue observable only)
   * moments                 transit.jl:49-283, 702-765 (init_ho_by_mass, radius_power,
                             prep_obtr_diag, op_obtr_diag, eval_moment; exact 3j / 6j)
+  * transitions             transit.jl:286-534, 561-700 (prep_obtr_nondiag, op_obtr_nondiag,
+                            calc_bifs, transit_main)
 
 Pinning (tests/test_oracle_*.py): the ckpot Be8 known answers
 (test/ShellModel_test.jl:10-14), the Mg24 dimension (example/log_sample_script.txt:323),
 closed-form dimensions, sympy's Clebsch-Gordan values, and an independent
 Jordan-Wigner brute-force Hamiltonian (tests/brute_force.py).  The moments are pinned
 by the single-particle closed forms (Schmidt values, Q = -(2j-1)/(2j+2) <r^2> e),
-oscillator <r^2> values and 3j / 6j orthogonality (tests/test_moments.py).
+oscillator <r^2> values and 3j / 6j orthogonality (tests/test_moments.py).  The
+transitions are pinned by single-nucleon matrix elements built independently from
+Clebsch-Gordan expansions, angular-momentum ladder operators and Gaunt integrals (every
+m-component, mu = 0, 1, 2), by their B values' closed form, and by <gm>, <eQ> of one state
+given twice equal to eval_moment's (tests/test_transit.py).
 """
 from __future__ import annotations
 
@@ -798,3 +804,146 @@ def eval_moment(Mtot, Rvecs, totJs, ops, pbits, nbits, tdims):
                 tx_mom += tx + "\n"
         res.append((mu, Q))
     return res, tx_mom
+
+
+def calc_bifs(Mps_l, Mns_l, Mps_r, Mns_r):
+    """calc_bifs (transit.jl:501-534) from the blocks' (Mp, Mn) lists of occ: (p_bifs, n_bifs)
+    as (left block, right block) pairs with equal Mn (proton operators) or equal Mp (neutron)."""
+    p_bifs, n_bifs = [], []
+    for ir in range(len(Mps_r)):
+        for il in range(len(Mps_l)):
+            if Mns_r[ir] == Mns_l[il]:
+                p_bifs.append((il, ir))
+            if Mps_r[ir] == Mps_l[il]:
+                n_bifs.append((il, ir))
+    return p_bifs, n_bifs
+
+
+def prep_obtr_nondiag(jl2, jr2, p_sps, n_sps, msps_p, msps_n, pbits_l, nbits_l, pbits_r, nbits_r, p_bifs, n_bifs,
+                      omu, bpar, gfactors, eff_charge):
+    """prep_obtr_nondiag (transit.jl:286-445): the M1 and E2 operators' component omu from the
+    right basis to the left one, {"M1": (proton, neutron), "E2": (...)} with entries
+    (bl, Nl, br, Nr, phase, fac) (0-based; phase means the term enters negated)."""
+    c_mu = math.sqrt(3.0 / (4.0 * math.pi))
+    c_Q = math.sqrt(5.0 / (16.0 * math.pi))
+    out = {"M1": ([], []), "E2": ([], [])}
+    for sp, (sps, msps, bits_l, bits_r, bifs) in enumerate(((p_sps, msps_p, pbits_l, pbits_r, p_bifs),
+                                                           (n_sps, msps_n, nbits_l, nbits_r, n_bifs))):
+        gl, gs = (gfactors[0], gfactors[2]) if sp == 0 else (gfactors[1], gfactors[3])
+        ec = eff_charge[sp]
+        L = len(msps)
+        for i in range(len(sps)):
+            ni, li, ji, _ = sps[i]
+            for f in range(len(sps)):
+                nf, lf, jf, _ = sps[f]
+                ex_r2 = radius_power(2, ni, li, nf, lf, bpar)
+                l6j = wigner6j(2 * lf, 2 * li, 2, ji, jf, 1)
+                s6j = wigner6j(1, 1, 2, ji, jf, 2 * lf)
+                Q3j = wigner3j(ji, 4, jf, 1, 0, -1)
+                _, mc_s, mc_idx = _possible_mz(sps[i], msps)
+                _, ma_s, ma_idx = _possible_mz(sps[f], msps)
+                for mode in ("M1", "E2"):
+                    if mode == "M1":
+                        if ni != nf or li != lf or (jl2 - jr2) // 2 > 1:
+                            continue
+                    elif (jl2 - jr2) // 2 > 2:
+                        continue
+                    for ic, mc in enumerate(mc_s):
+                        bit_c = 1 << (L - 1 - mc_idx[ic])
+                        for ia, ma in enumerate(ma_s):
+                            if (ma - mc) // 2 != omu:
+                                continue
+                            bit_a = 1 << (L - 1 - ma_idx[ia])
+                            pfac = math.sqrt((ji + 1) * (jf + 1))
+                            if mode == "M1":
+                                sign1 = (-1) ** ((jf - ma) // 2 + lf + (ji + 3) // 2)
+                                sign2 = (-1) ** ((jf - ma) // 2 + lf + (jf + 3) // 2)
+                                fac1 = sign1 * l6j * math.sqrt(lf * (lf + 1) * (2 * lf + 1))
+                                fac2 = sign2 * s6j * math.sqrt(1.5)
+                                coeff = pfac * wigner3j(jf, 2, ji, -ma, 2 * omu, mc) * (fac1 * gl + fac2 * gs) * c_mu
+                            else:
+                                signQ = (-1) ** (jf + (1 - ma) // 2) * (1 + (-1) ** (li + lf))
+                                if signQ == 0:
+                                    continue
+                                coeff = signQ * wigner3j(jf, 4, ji, -ma, 2 * omu, mc) * Q3j * pfac * ec * c_Q * ex_r2
+                            for bl, br in bifs:
+                                where = {b: k for k, b in enumerate(bits_l[bl])}
+                                for Nr, phi in enumerate(bits_r[br]):
+                                    ok, sgn, new = _phase_1(phi, bit_a, bit_c)
+                                    if not ok or new not in where:
+                                        continue
+                                    out[mode][sp].append((bl, where[new], br, Nr, sgn == -1, coeff))
+    return out
+
+
+def op_obtr_nondiag(p_ops, n_ops, pbits_l, nbits_l, tdims_l, pbits_r, nbits_r, tdims_r, wf):
+    """op_obtr_nondiag (transit.jl:447-499): twf (left basis) = O wf (right basis)."""
+    wf = np.asarray(wf, dtype=np.float64)
+    twf = np.zeros(tdims_l[-1])
+    for bl, Nl, br, Nr, ph, fac in p_ops:
+        c = -fac if ph else fac
+        nr_, nl_ = nbits_r[br], nbits_l[bl]
+        where = {b: k for k, b in enumerate(nl_)}
+        for k_r, n_r in enumerate(nr_):
+            k_l = where.get(n_r)
+            if k_l is None:
+                continue
+            twf[tdims_l[bl] + Nl * len(nl_) + k_l] += c * wf[tdims_r[br] + Nr * len(nr_) + k_r]
+    for bl, Nl, br, Nr, ph, fac in n_ops:
+        c = -fac if ph else fac
+        pr_, pl_ = pbits_r[br], pbits_l[bl]
+        lNr, lNl = len(nbits_r[br]), len(nbits_l[bl])
+        where = {b: k for k, b in enumerate(pl_)}
+        for k_r, p_r in enumerate(pr_):
+            k_l = where.get(p_r)
+            if k_l is None:
+                continue
+            twf[tdims_l[bl] + k_l * lNl + Nl] += c * wf[tdims_r[br] + k_r * lNr + Nr]
+    return twf
+
+
+def transitions(snt_path, nuc, jl2, jr2, gfactors=(1.0, 0.0, 5.586, -3.826), eff_charge=(1.5, 0.5), bpar=None):
+    """transit_main's set-up (transit.jl:561-612): both bases (left at 2M = jl2, right at
+    jr2 <= jl2), calc_bifs and prep_obtr_nondiag.  Returns a dict of the pieces."""
+    snt = read_snt(snt_path, nuc)
+    p_sps, n_sps = snt["p_sps"], snt["n_sps"]
+    Z, N, vp, vn = get_zna(nuc, snt["cp"], snt["cn"])
+    A = Z + N
+    msps_p, msps_n = construct_msps(p_sps, n_sps)
+    if bpar is None:
+        bpar = bpar_main_sm(A)
+    omu = (jl2 - jr2) // 2
+    pl, nl, _, _, Mpl, Mnl, tdl = occ(p_sps, msps_p, vp, n_sps, msps_n, vn, jl2)
+    pr, nr, _, _, Mpr, Mnr, tdr = occ(p_sps, msps_p, vp, n_sps, msps_n, vn, jr2)
+    p_bifs, n_bifs = calc_bifs(Mpl, Mnl, Mpr, Mnr)
+    ops = prep_obtr_nondiag(jl2, jr2, p_sps, n_sps, msps_p, msps_n, pl, nl, pr, nr, p_bifs, n_bifs, omu, bpar,
+                            gfactors, eff_charge)
+    return dict(ops=ops, pbits_l=pl, nbits_l=nl, tdims_l=tdl, pbits_r=pr, nbits_r=nr, tdims_r=tdr, omu=omu,
+                jl2=jl2, jr2=jr2, bpar=bpar, msps_p=msps_p, msps_n=msps_n, p_sps=p_sps, n_sps=n_sps)
+
+
+def transit(T, Vl, Vr, same_file=False, calc_EM=True):
+    """transit_main's evaluation (transit.jl:612-697) on the pieces of `transitions`:
+    records (mode, l, r, Mred, B(l<=r), B(l=>r), moment or None), 1-based l and r."""
+    jl2, jr2, omu = T["jl2"], T["jr2"], T["omu"]
+    c_mom = {"M1": math.sqrt(4.0 * math.pi / 3.0), "E2": math.sqrt(16.0 * math.pi / 5.0)}
+    recs = []
+    for mode, lam in (("M1", 1), ("E2", 2)):
+        if not (abs(jl2 - jr2) <= 2 * lam <= jl2 + jr2):
+            continue
+        sel = wigner3j(jl2, 2 * lam, jr2, -jl2, 2 * omu, jr2)
+        signmu = 1                                   # (-1)^(J_l - M_l) with M_l = J_l
+        cgv = wigner3j(jr2, 2 * lam, jl2, -jr2, 2 * omu, jl2)
+        if sel == 0:
+            continue
+        for r, wr in enumerate(Vr):
+            twf = op_obtr_nondiag(T["ops"][mode][0], T["ops"][mode][1], T["pbits_l"], T["nbits_l"], T["tdims_l"],
+                                  T["pbits_r"], T["nbits_r"], T["tdims_r"], wr)
+            for l, wl in enumerate(Vl):
+                if not calc_EM and l != r:
+                    continue
+                Mred = float(np.dot(wl, twf)) / sel
+                mom = Mred * cgv * c_mom[mode] * signmu if ((same_file or not calc_EM) and l == r) else None
+                recs.append((mode, l + 1, r + 1, Mred, Mred ** 2 / (jr2 + 1), Mred ** 2 / (jl2 + 1), mom))
+    return recs
+
