@@ -14,6 +14,10 @@ lockstep batches of two.  Each request is one main_sm eigenvalue solve
   one request at a time): within 1e-5 MeV;
 * the two large bases are admitted on both sides (the FP64 solve of pf 6p6n
   is admitted with its Lanczos basis bounded by the HBM budget).
+
+At the stream's largest basis (pf 6p6n, mdim 109,954,620, too large for the
+oracle) H·v is checked through size-independent properties: <u, H v> = <H u, v>
+and linearity in FP64, and the FP32 H·v within 1e-5 of the FP64 one.
 """
 import numpy as np
 import pytest
@@ -80,3 +84,24 @@ def test_c4_spec_subset_alone_vs_mix_vs_fp64(tmp_path):
         assert im["n_fallback"] == 0
     big = [im["mdim"] for _, (_, im) in zip(reqs, mix)]
     assert 109954620 in big and 44993824 in big
+
+
+def test_hv_properties_fe52(tmp_path):
+    path, nuc = synth.config_snt(str(tmp_path / "fe52.snt"), "pf", 6, 6, 7, (-6.0, 0.0), 1.2)
+    from smgpu import Hamiltonian, Problem
+    p = Problem(path, nuc)
+    assert p.mdim == 109954620
+    h = Hamiltonian(p)
+    rng = np.random.default_rng(52)
+    v = rng.standard_normal(p.mdim)
+    u = rng.standard_normal(p.mdim)
+    w64 = h.operate_H(v, "fp64")
+    hu = h.operate_H(u, "fp64")
+    scale = np.abs(w64).max()
+    sym = abs(u @ w64 - hu @ v) / (np.abs(u) @ np.abs(w64))
+    hsum = h.operate_H(u - 0.5 * v, "fp64")
+    lin = np.abs(hsum - (hu - 0.5 * w64)).max() / scale
+    w32 = h.operate_H(v, "mixed")
+    e32 = np.abs(w32 - w64).max() / scale
+    print("fe52 symmetry", sym, "linearity", lin, "fp32 vs fp64", e32, flush=True)
+    assert sym <= 1e-11 and lin <= 1e-11 and e32 <= 1e-5
