@@ -356,8 +356,10 @@ typedef struct {
                                request holds up another.  0: arrival order, a request that
                                does not fit yet bypassed by at most 4 * max_inflight others */
     int32_t reserve_cus;    /* with large_mdim and max_inflight >= 2: slot 0 runs the large
-                               requests only, on streams that leave this many CUs (at most
-                               half the device) to the other slots; 0: no CU partition */
+                               requests only; while smaller requests are in the server, on
+                               streams that leave this many CUs (at most half the device) to
+                               the other slots, otherwise on the whole device; 0: no CU
+                               partition */
     int32_t reserved_;      /* zero */
 } smgpu_server_config;
 

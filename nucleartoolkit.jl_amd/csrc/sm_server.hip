@@ -88,8 +88,33 @@ struct Request {
     SolveStats st{};
 };
 
+// a slot's streams: its run stream, the batch H·v stream and those of members 1..
+struct StreamSet {
+    hipStream_t run = nullptr, hv = nullptr;
+    std::vector<hipStream_t> members;
+    void synchronize() const {
+        for (hipStream_t x : all()) SM_HIP(hipStreamSynchronize(x));
+    }
+    void destroy() {
+        for (hipStream_t x : all()) (void)hipStreamDestroy(x);
+        run = hv = nullptr;
+        members.clear();
+    }
+    std::vector<hipStream_t> all() const {
+        std::vector<hipStream_t> v;
+        if (run) v.push_back(run);
+        if (hv) v.push_back(hv);
+        for (hipStream_t x : members) if (x) v.push_back(x);
+        return v;
+    }
+};
+
 struct Slot {
     int id = 0;
+    // sets[0]: the slot's streams (slot 0 of a CU partition: on the lane's CUs);
+    // sets[1]: slot 0 of a CU partition only, the same on the whole device
+    StreamSet sets[2];
+    int bound_set = 0;
     hipStream_t stream = nullptr;
     SolveCtx ctx;             // one request, or member 0 of a batch
     // batches: the H·v stream and members 1.. (each its own stream and workspace)
@@ -128,6 +153,17 @@ struct Slot {
     void drop_index() {
         ham.idx.reset();
         for (auto& h : mham) h->idx.reset();
+    }
+    // runs the next job on stream set k, once the work queued on the current set has drained
+    void use_streams(int k) {
+        if (k == bound_set || !sets[k].run) return;
+        sets[bound_set].synchronize();
+        const StreamSet& n = sets[k];
+        stream = n.run;
+        ctx.stream = n.run;
+        hv_stream = n.hv;
+        for (size_t b = 0; b < more.size(); b++) more[b]->stream = n.members[b];
+        bound_set = k;
     }
     bool release = false;     // free the buffers before waiting again
     std::condition_variable cv;
@@ -646,6 +682,18 @@ struct Server {
         });
     }
 
+    // CU partition: whether a request below large_mdim is queued, being prepared (its size
+    // still unknown), ready or running, so that the large lane keeps to its CUs (server mutex)
+    bool smaller_work_locked() const {
+        for (const auto& kv : reqs) {
+            const Request& q = *kv.second;
+            if (q.state == SMGPU_REQ_QUEUED || q.state == SMGPU_REQ_PREPARING) return true;
+            if (q.state == SMGPU_REQ_READY && q.mdim < large_mdim) return true;
+            if (q.state == SMGPU_REQ_RUNNING && q.slot != 0) return true;
+        }
+        return false;
+    }
+
     // requests running in a slot (server mutex)
     static int running_in(const Slot& s) {
         if (!s.job) return 0;
@@ -658,15 +706,18 @@ struct Server {
         (void)hipSetDevice(cfg.device);
         for (;;) {
             std::shared_ptr<Request> r;
+            int set = 0;
             {
                 std::unique_lock<std::mutex> lk(m);
                 s.cv.wait(lk, [&] { return stopping || s.job; });
                 if (!s.job) return;   // stopping
                 r = s.job;
+                if (s.sets[1].run && !smaller_work_locked()) set = 1;
             }
             int32_t code = SMGPU_OK;
             std::string msg;
             try {
+                s.use_streams(set);
                 // (a batch member's own cancel flag is checked at its first step)
                 if (r->cancel && cfg.max_batch == 1) throw Canceled();
                 run_job(s, *r);
@@ -745,21 +796,35 @@ struct Server {
             else
                 SM_HIP(hipStreamCreateWithFlags(st, hipStreamNonBlocking));
         };
+        auto new_set = [&](StreamSet& set, bool lane) {
+            new_stream(&set.run, lane);
+            if (cfg.max_batch > 1) {
+                new_stream(&set.hv, lane);
+                set.members.assign((size_t)cfg.max_batch - 1, nullptr);
+                for (auto& x : set.members) new_stream(&x, lane);
+            }
+        };
         for (int i = 0; i < cfg.max_inflight; i++) {
             auto s = std::make_unique<Slot>();
             s->id = i;
-            new_stream(&s->stream, i == 0);
-            s->ctx.stream = s->stream;
+            slots.push_back(std::move(s));
+            Slot& sl = *slots.back();
+            new_set(sl.sets[0], i == 0);
+            // slot 0 of a CU partition also gets whole-device streams: a CU-masked queue
+            // slows a large solve by about 14% even with 4 CUs reserved, so the lane runs
+            // on the partition only while smaller requests are in the server
+            if (i == 0 && !lane_mask.empty()) new_set(sl.sets[1], false);
+            sl.stream = sl.sets[0].run;
+            sl.ctx.stream = sl.stream;
             if (cfg.max_batch > 1) {
-                for (int b = 0; b < cfg.max_batch; b++) s->mham.push_back(std::make_unique<DeviceHam>());
-                new_stream(&s->hv_stream, i == 0);
+                for (int b = 0; b < cfg.max_batch; b++) sl.mham.push_back(std::make_unique<DeviceHam>());
+                sl.hv_stream = sl.sets[0].hv;
                 for (int b = 1; b < cfg.max_batch; b++) {
                     auto c = std::make_unique<SolveCtx>();
-                    new_stream(&c->stream, i == 0);
-                    s->more.push_back(std::move(c));
+                    c->stream = sl.sets[0].members[(size_t)b - 1];
+                    sl.more.push_back(std::move(c));
                 }
             }
-            slots.push_back(std::move(s));
         }
         lane_split = !lane_mask.empty();
         for (auto& s : slots) {
@@ -788,9 +853,8 @@ struct Server {
         for (auto& s : slots) {
             s->drop_index();
             s->release_workspaces();
-            if (s->stream) (void)hipStreamDestroy(s->stream);
-            if (s->hv_stream) (void)hipStreamDestroy(s->hv_stream);
-            for (auto& c : s->more) if (c->stream) (void)hipStreamDestroy(c->stream);
+            s->sets[0].destroy();
+            s->sets[1].destroy();
         }
         cache.clear();
     }

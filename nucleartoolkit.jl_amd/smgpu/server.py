@@ -47,7 +47,8 @@ class Server:
     def __init__(self, device=0, max_inflight=4, hbm_budget=0, host_threads=2, table_threads=0, keep_bytes=0,
                  max_batch=1, batch_variants=True, large_mdim=0, reserve_cus=0):
         """large_mdim > 0: requests of at least that dimension run one at a time, the others
-        smallest first; reserve_cus > 0 puts the large ones in slot 0 on the other CUs."""
+        smallest first; reserve_cus > 0 puts the large ones in slot 0, on the other CUs while
+        smaller requests are in the server."""
         c = ServerConfig()
         lib().smgpu_default_server_config(ctypes.byref(c))
         c.device = int(device); c.max_inflight = int(max_inflight); c.hbm_budget = int(hbm_budget)

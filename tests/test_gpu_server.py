@@ -283,20 +283,40 @@ def test_server_interactions_with_other_sparsity_on_one_basis(snt_dir):
 def test_server_large_lane_on_cu_partition(snt_dir):
     reqs = _stream(snt_dir)
     alone = [_alone(p, n, k, s, "mixed") for p, n, k, s in reqs]
-    mdims = sorted(smgpu.Problem(p, n).mdim for p, n, _, _ in reqs)
-    large = mdims[len(mdims) // 2]          # about half the requests are large
+    md = [smgpu.Problem(p, n).mdim for p, n, _, _ in reqs]
+    large = sorted(md)[len(md) // 2]        # about half the requests are large
     with pytest.raises(smgpu.SmgpuError):
         Server(max_inflight=4, reserve_cus=16)                  # needs large_mdim
     with pytest.raises(smgpu.SmgpuError):
         Server(max_inflight=4, large_mdim=large, reserve_cus=100000)
-    with Server(max_inflight=4, max_batch=2, large_mdim=large, reserve_cus=16) as srv:
+
+    def serve_all(srv):
         ids = [srv.submit(p, n, k, precision="mixed", seed=s) for p, n, k, s in reqs]
         for rid in ids:
             assert srv.wait(rid, 120000)
-        res = [srv.result(rid) for rid in ids]
+        return [srv.result(rid) for rid in ids]
+
+    with Server(max_inflight=4, max_batch=2, large_mdim=large, reserve_cus=16) as srv:
+        res = serve_all(srv)                # slot 0 on the CU partition while smaller requests run
+        # the large requests one at a time, nothing smaller in the server: slot 0 on its
+        # whole-device streams; then the mix again, back on the partition
+        solo = []
+        for (p, n, k, s), m, a in zip(reqs, md, alone):
+            if m >= large:
+                rid = srv.submit(p, n, k, precision="mixed", seed=s)
+                assert srv.wait(rid, 120000)
+                ev, info = srv.result(rid)
+                assert info["slot"] == 0
+                solo.append((n, ev, a))
+        res2 = serve_all(srv)
         st = srv.stats()
-    assert st["completed"] == len(reqs) and st["failed"] == 0
-    for (path, nuc, k, s), a, (ev, info) in zip(reqs, alone, res):
-        print(nuc, info["mdim"], "slot", info["slot"], flush=True)
+    assert st["completed"] == 2 * len(reqs) + len(solo) and st["failed"] == 0
+    assert len(solo) >= 2
+    for n, ev, a in solo:
+        assert np.abs(ev - a).max() <= TOL_MIX, (n, ev - a)
+    for (path, nuc, k, s), a, (ev, info), (ev2, info2) in zip(reqs, alone, res, res2):
+        print(nuc, info["mdim"], "slot", info["slot"], info2["slot"], flush=True)
         assert (info["slot"] == 0) == (info["mdim"] >= large), (nuc, info["mdim"], info["slot"])
+        assert (info2["slot"] == 0) == (info2["mdim"] >= large), (nuc, info2["mdim"], info2["slot"])
         assert np.abs(ev - a).max() <= TOL_MIX, (nuc, ev - a)
+        assert np.abs(ev2 - a).max() <= TOL_MIX, (nuc, ev2 - a)
