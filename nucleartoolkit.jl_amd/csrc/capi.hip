@@ -832,8 +832,15 @@ int32_t smgpu_transitions_create(smgpu_problem left, smgpu_problem right, int32_
     return guard([&]() -> int32_t {
         if (!left || !right || !out) throw std::invalid_argument("null argument");
         *out = nullptr;
+        auto same_orbits = [](const std::vector<Orbit>& a, const std::vector<Orbit>& b) {
+            if (a.size() != b.size()) return false;
+            for (size_t i = 0; i < a.size(); i++)
+                if (a[i].n != b[i].n || a[i].l != b[i].l || a[i].j2 != b[i].j2 || a[i].tz != b[i].tz) return false;
+            return true;
+        };
+        // the right basis' m-state bits must name the same orbits as the left's
         if (left->Z != right->Z || left->N != right->N || left->snt.A != right->snt.A ||
-            left->snt.p_sps.size() != right->snt.p_sps.size() || left->snt.n_sps.size() != right->snt.n_sps.size())
+            !same_orbits(left->snt.p_sps, right->snt.p_sps) || !same_orbits(left->snt.n_sps, right->snt.n_sps))
             throw std::invalid_argument("left and right problems of different nuclei or spaces");
         const double g0[4] = {1.0, 0.0, 5.586, -3.826}, e0[2] = {1.5, 0.5};
         const double* g = gfactors ? gfactors : g0;

@@ -186,3 +186,22 @@ def test_mu0_equals_moment_operators(golden):
         mom = {x[0]: x[6] for x in orc.transit(T, [v], [v], same_file=True)}
         res, _ = orc.eval_moment(2, [v], [1.0], diag, h.pbits, h.nbits, h.tdims)
         assert abs(mom["M1"] - res[0][0]) < 1e-12 and abs(mom["E2"] - res[0][1]) < 1e-12
+
+
+def test_create_rejects_mismatched_problems(golden, tmp_path):
+    """smgpu_transitions_create checks its problems before any device work: a right problem
+    read from an interaction file that lists the same orbit counts in another order, a right
+    basis of higher 2M, and |M_left - M_right| above 2 are errors."""
+    import smgpu
+    snt = f"{golden}/usdb.snt"
+    lines = open(snt).read().splitlines()
+    # swap the proton orbits 2 (0d3/2) and 3 (0d5/2): same counts, other m-state bits
+    lines[2], lines[3] = lines[3].replace("   3   0   2   5", "   2   0   2   5"), lines[2].replace("   2   0   2   3", "   3   0   2   3")
+    swapped = tmp_path / "usdb_swapped.snt"
+    swapped.write_text("\n".join(lines) + "\n")
+    p0, p2, p6 = (smgpu.Problem(snt, "Ne20", twoM=m) for m in (0, 2, 6))
+    for left, right, msg in ((p2, smgpu.Problem(str(swapped), "Ne20", twoM=0), "different nuclei or spaces"),
+                             (p2, smgpu.Problem(snt, "Ne20", twoM=4), "left basis needs 2M >="),
+                             (p6, p0, "only M1 and E2")):
+        with pytest.raises(smgpu.SmgpuError, match=msg):
+            smgpu.Transitions(left, right, 0, G, EC)
