@@ -528,16 +528,24 @@ static size_t hv_lds_bytes(const HvTables& A, int nv, bool pv, size_t tsize) {
             (size_t)A.nN_max * A.rg * nv) * tsize;
 }
 
-// Widest instantiated vector count <= nv whose workgroup LDS stays within 32 KiB.
-// Measured per-vector times (FP32 / FP64): Cr48 (14.5 KiB per vector) 0.49 -> 0.34 ms
-// with two vectors and 0.36 with four (48 KiB); Mg24 0.044 -> 0.017 ms with four;
-// Mn50 and Cr48 FP64 (over 32 KiB at two) gain nothing and lose occupancy.
+// Widest instantiated vector count <= nv whose workgroup LDS stays within 32 KiB, and
+// for FP32 on 512-thread shapes two vectors within 80 KiB (two workgroups per CU, the
+// occupancy the one-vector launch has there).  Measured per-vector times (FP32 / FP64):
+// Cr48 (14.5 KiB per vector) 0.49 -> 0.34 ms with two vectors and 0.36 with four
+// (48 KiB); Mg24 0.044 -> 0.017 ms with four.  Five-neutron pf blocks (1,651 wide,
+// 512 threads, about 66 KiB at two vectors), FP32, one vector -> two of one
+// Hamiltonian / two interaction variants: Cr49 1.53 -> 1.07 / 1.24 ms, Mn50 5.30 ->
+// 4.05 / 4.58, Fe51 14.95 -> 11.74 / 13.08.  Slower: the same in FP64 (Mn50 8.66 ->
+// 10.97, 132 KiB), six-neutron blocks (3,952 wide, 158 KiB: Fe52 39.4 -> 43.9, Mn51
+// 13.8 -> 15.3), Fe50 FP64 at two (5.91 -> 6.66).  Results are bit-identical either way.
 static constexpr size_t HV_LDS_SHARED = 32 * 1024;
+static constexpr size_t HV_LDS_SHARED_WIDE = 80 * 1024;
 template <typename T> static int hv_width(const HvTables& A, int nv, bool pv) {
     for (int c : {4, 2}) {
-        if (c > nv) continue;
-        if (hv_fn<T>(A.npt, A.rg, A.pch, c, pv, A.tpb) && hv_lds_bytes(A, c, pv, sizeof(T)) <= HV_LDS_SHARED)
-            return c;
+        if (c > nv || !hv_fn<T>(A.npt, A.rg, A.pch, c, pv, A.tpb)) continue;
+        const size_t lds = hv_lds_bytes(A, c, pv, sizeof(T));
+        if (lds <= HV_LDS_SHARED) return c;
+        if (c == 2 && sizeof(T) == 4 && A.tpb == 512 && lds <= HV_LDS_SHARED_WIDE) return c;
     }
     return 1;
 }

@@ -10,7 +10,9 @@ its wide-block loop, NPT = 8) and on small bases with wide neutron blocks.
   ARPACK, make_golden.py): FP64 <= 1e-7 MeV, mixed <= 1e-5 MeV.
 * Wide neutron blocks on small bases (Sc46 pf 1p5n: widest block 1,651, NPT 8;
   Sc47 pf 1p6n: 3,952, NPT 16; Ti48 pf 2p6n): H·v against the oracle in both
-  precisions and TRL eigenvalues against ARPACK over the oracle's operate_H!.
+  precisions and TRL eigenvalues against ARPACK over the oracle's operate_H!;
+  two-vector launches (one Hamiltonian, two interactions) and a lockstep solve of
+  two interactions equal the single-vector ones bit for bit.
 """
 import json
 
@@ -93,6 +95,22 @@ def test_wide_blocks_hv_and_trl(snt_dir, space, zv, nv, npt):
         e32 = np.abs(h.operate_H(v, "mixed") - ref).max() / scale
         print(nuc, "widest", widest, "mdim", p.mdim, "err", e64, e32, flush=True)
         assert e64 <= 1e-12 and e32 <= 1e-5
+    # two vectors per launch, of one Hamiltonian and of two interactions on the basis (the
+    # 512-thread FP32 shapes of five-neutron blocks run two-vector launches), equal single
+    # launches bit for bit, and so does a lockstep solve of the two interactions
+    path2, _ = synth.config_snt(f"{snt_dir}/wide_{zv}{nv}_b.snt", space, zv, nv,
+                                synth.BASE_SEED + 41 + 10 * zv + nv, (-6, 0), 1.2)
+    hv = h.variant(smgpu.Problem(path2, nuc))
+    V = rng.standard_normal((2, p.mdim))
+    for prec in ("fp64", "mixed"):
+        W1 = np.stack([h.operate_H(V[0], prec), h.operate_H(V[1], prec)])
+        assert np.array_equal(h.operate_H_multi(V, prec), W1), prec
+        Wv = smgpu.operate_H_variants([h, hv], V, prec)
+        assert np.array_equal(Wv[0], W1[0]) and np.array_equal(Wv[1], hv.operate_H(V[1], prec)), prec
+    out = smgpu.trl_variants([h, hv], 4, [1, 2], precision="mixed")
+    for hh, sd, (ev2, _, st2) in zip((h, hv), (1, 2), out):
+        ev1, _, st1 = hh.trl(4, precision="mixed", seed=sd)
+        assert np.array_equal(ev2, ev1) and st2["n_hv"] == st1["n_hv"], (sd, ev2 - ev1)
     op = LinearOperator((o.mdim, o.mdim), matvec=lambda x: o.operate_H(np.ascontiguousarray(x).ravel()),
                         dtype=np.float64)
     ev = np.sort(eigsh(op, k=4, which="SA", tol=1e-13, v0=np.ones(o.mdim))[0])
