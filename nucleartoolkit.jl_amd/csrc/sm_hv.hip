@@ -550,6 +550,21 @@ template <typename T> static int hv_width(const HvTables& A, int nv, bool pv) {
     return 1;
 }
 
+// Two vectors of one Hamiltonian where the basis' own 512-thread FP32 shape cannot take
+// them (six-neutron pf blocks, 3,952 wide: 158 KiB at two vectors): the same tables at
+// 1024 threads (columns fn = tid + 1024 k, the same 64-lane slices), one 16-wave
+// workgroup per CU, the waves per CU of the one-vector launch (72 VGPRs).  Measured per
+// vector, bit-identical: Mn51 13.79 -> 11.74 ms, Fe52 39.42 -> 33.20.  Not for
+// interaction variants: their per-vector V_eff reads cancel the gain (Mn51 13.90, Fe52
+// 38.80).
+template <typename T> static bool wide_pair(const HvTables& A, HvTables& B) {
+    if (sizeof(T) != 4 || A.tpb != 512) return false;
+    B = A;
+    B.tpb = 1024;
+    B.npt = (A.nN_max + 1023) / 1024;
+    return hv_fn<T>(B.npt, B.rg, B.pch, 2, false, 1024) && hv_lds_bytes(B, 2, false, sizeof(T)) <= 160 * 1024;
+}
+
 template <typename T>
 static void launch_t(const HvTables& A, int nv, bool pv, const HvVecs& P, hipStream_t s) {
     dim3 grid(A.G), block(A.tpb);
@@ -591,10 +606,16 @@ void launch_hv_column(const DeviceHam& h, int prec, const void* v, int64_t ld, c
 template <typename T>
 static void launch_set_t(const HvTables& A, int nvec, const void* const* v, void* const* w, hipStream_t s) {
     for (int b = 0; b < nvec;) {
-        const int c = hv_width<T>(A, nvec - b, false);
+        int c = hv_width<T>(A, nvec - b, false);
+        HvTables B;
+        const HvTables* L = &A;
+        if (c == 1 && nvec - b >= 2 && wide_pair<T>(A, B)) {
+            c = 2;
+            L = &B;
+        }
         HvVecs P{};
         for (int j = 0; j < c; j++) { P.v[j] = v[b + j]; P.w[j] = w[b + j]; }
-        launch_t<T>(A, c, false, P, s);
+        launch_t<T>(*L, c, false, P, s);
         b += c;
     }
 }
@@ -615,7 +636,9 @@ static void launch_variants_t(const DeviceHam* const* hams, int prec, int nvec, 
         int run = 1;
         while (b + run < nvec && hams[b + run] == hams[b]) run++;
         if (run >= 2) {
-            const int c = hv_width<T>(A, run, false);
+            int c = hv_width<T>(A, run, false);
+            HvTables B;
+            if (c == 1 && wide_pair<T>(A, B)) c = 2;
             launch_set_t<T>(hams[b]->view[prec], c, v + b, w + b, s);
             b += c;
             continue;

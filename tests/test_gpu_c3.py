@@ -12,7 +12,8 @@ its wide-block loop, NPT = 8) and on small bases with wide neutron blocks.
   Sc47 pf 1p6n: 3,952, NPT 16; Ti48 pf 2p6n): H·v against the oracle in both
   precisions and TRL eigenvalues against ARPACK over the oracle's operate_H!;
   two-vector launches (one Hamiltonian, two interactions) and a lockstep solve of
-  two interactions equal the single-vector ones bit for bit.
+  two interactions equal the single-vector ones bit for bit; block Lanczos with
+  q = 2 (two-vector H·V) against the same ARPACK levels.
 """
 import json
 
@@ -118,3 +119,7 @@ def test_wide_blocks_hv_and_trl(snt_dir, space, zv, nv, npt):
         en, _, st = h.trl(4, precision=prec)
         print(nuc, prec, en - ev, st["iterations"], flush=True)
         assert np.abs(en - ev).max() < tol
+    # block Lanczos, q = 2: its H·V runs two-vector launches (six-neutron blocks: at 1024 threads)
+    en, _, st = h.trbl(4, 2, precision="mixed")
+    print(nuc, "trbl q=2 mixed", en - ev, st["iterations"], flush=True)
+    assert np.abs(en - ev).max() < 1e-5
