@@ -168,8 +168,10 @@ int32_t smgpu_apply_H(smgpu_ham h, int32_t precision, const void* v_dev, void* w
 int32_t smgpu_apply_H_host(smgpu_ham h, int32_t precision, const double* v, double* w);
 /* H·V on nvec vectors at once (bl_operate_H!, lanczos_methods.jl:435-548): w_b = H v_b
  * for v_b = v_dev + b*ldv, w_b = w_dev + b*ldw (elements; ld >= mdim when nvec > 1).
- * Launches of up to four vectors share every table load; each vector's result is
- * bit-identical to smgpu_apply_H on it. */
+ * Launches of up to four vectors share every table load where the vectors' LDS fits
+ * the workgroup's occupancy (up to four on narrow neutron blocks, two on wide FP32
+ * ones, else one per launch); each vector's result is bit-identical to smgpu_apply_H
+ * on it. */
 int32_t smgpu_apply_H_multi(smgpu_ham h, int32_t precision, int32_t nvec, const void* v_dev, int64_t ldv,
                             void* w_dev, int64_t ldw, void* stream);
 /* Host-buffer convenience: v and w are nvec x mdim row-major FP64. */
@@ -177,9 +179,9 @@ int32_t smgpu_apply_H_host_multi(smgpu_ham h, int32_t precision, int32_t nvec, c
 /* Timing of smgpu_apply_H_multi on resident vectors: milliseconds per call (all nvec). */
 int32_t smgpu_time_apply_H_multi(smgpu_ham h, int32_t precision, int32_t nvec, int32_t reps, double* ms);
 /* w_b = H_b v_b for handles that share one index (smgpu_ham_create_variant; a handle
- * may repeat): vectors of one handle share launches of up to four, vectors of
- * different handles are paired into launches that share the index loads and entry
- * decodes with per-vector coefficients.  Each result is bit-identical to
+ * may repeat): vectors of one handle share launches as in smgpu_apply_H_multi,
+ * vectors of different handles are paired, where their LDS fits, into launches that
+ * share the index loads and entry decodes with per-vector coefficients.  Each result is bit-identical to
  * smgpu_apply_H(hs[b], ...) on v_b.  Device pointers in the launch precision; no
  * counterpart in the reference (one interaction per operate_H! call). */
 int32_t smgpu_apply_H_variants(const smgpu_ham* hs, int32_t precision, int32_t nvec, const void* const* v_dev,
