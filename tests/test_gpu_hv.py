@@ -2,7 +2,7 @@
 
 Tolerances: FP64 H·v relative max-norm error <= 1e-12; FP32 H·v <= 1e-5 of
 |H v|_inf; eigenvalues vs oracle / dense <= 1e-7 MeV (FP64 path) and
-<= 1e-5 MeV (mixed path, the target tolerance of the north-star metric).
+<= 1e-5 MeV (mixed path, the tolerance at which served solves are compared with FP64).
 """
 import numpy as np
 import pytest
